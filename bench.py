@@ -1,0 +1,304 @@
+#!/usr/bin/env python
+"""Throughput of numbagg's hot path on one MI355X: GB/s and % of HBM peak.
+
+A *step* is one pass of each headline function over its synthetic batch, with
+inputs already resident in HBM:
+
+* move_mean  float32 (256, 2_000_000), window=1000, min_count=500, random walk
+  + 1e4 offset with 2% NaN;
+* group_nansum float32 (64, 4_000_000) with 5% NaN, int32 labels uniform over
+  1000 labels;
+* ffill float32 (1024, 1_000_000), limit=None, 30% NaN in geometric runs
+  (NaN runs mean 50).
+
+`value` is the whole step's algorithmic bytes (each input read once, each output
+written once) divided by its time. The roofline object covers the step's
+dominant kernel, timed with HIP events on the stream the kernels are launched
+on. `cpu_baseline` times the C restatement of the reference kernels (OpenMP over
+rows, the stand-in for numba's parallel target) on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--all] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+DEV = "cuda"
+METRIC = "GB/s & % HBM peak (move_mean, group_nansum, ffill); ragged-batched vs plain x"
+
+
+# --------------------------------------------------------------------------
+# synthetic inputs (generated on the GPU, seeded)
+# --------------------------------------------------------------------------
+def gen_walk(torch, rows, n, seed, p_nan=0.02, offset=1e4, chunk=16):
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    out = torch.empty((rows, n), dtype=torch.float32, device=DEV)
+    for r0 in range(0, rows, chunk):
+        r1 = min(rows, r0 + chunk)
+        steps = torch.randn((r1 - r0, n), generator=g, device=DEV, dtype=torch.float64)
+        w = torch.cumsum(steps, dim=1).add_(offset)
+        nan = torch.rand((r1 - r0, n), generator=g, device=DEV) < p_nan
+        w[nan] = float("nan")
+        out[r0:r1] = w.to(torch.float32)
+        del steps, w, nan
+    return out
+
+
+def gen_runs(torch, rows, n, seed, mean_nan=50.0, mean_valid=116.7, chunk=64):
+    """N(0,1) values with NaN runs of geometric length (30% NaN overall)."""
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    out = torch.randn((rows, n), generator=g, device=DEV, dtype=torch.float32)
+    k = int(n / (mean_nan + mean_valid) * 1.3) + 64
+    for r0 in range(0, rows, chunk):
+        r1 = min(rows, r0 + chunk)
+        m = r1 - r0
+        u1 = torch.rand((m, k), generator=g, device=DEV, dtype=torch.float64)
+        u2 = torch.rand((m, k), generator=g, device=DEV, dtype=torch.float64)
+        nan_l = torch.floor(torch.log(u1) / math.log(1 - 1 / mean_nan)).long() + 1
+        val_l = torch.floor(torch.log(u2) / math.log(1 - 1 / mean_valid)).long() + 1
+        lengths = torch.stack([val_l, nan_l], dim=2).reshape(m, 2 * k)
+        ends = torch.cumsum(lengths, dim=1)
+        shift = (torch.rand((m, 1), generator=g, device=DEV) * (mean_nan + mean_valid)).long()
+        ends = ends - shift
+        starts_nan = ends[:, 0::2]  # a NaN run starts where a valid run ends
+        ends_nan = ends[:, 1::2]
+        delta = torch.zeros((m, n + 1), dtype=torch.int16, device=DEV)
+        ones = torch.ones_like(starts_nan, dtype=torch.int16)
+        delta.scatter_add_(1, starts_nan.clamp(0, n), ones)
+        delta.scatter_add_(1, ends_nan.clamp(0, n), -ones)
+        mask = torch.cumsum(delta[:, :n], dim=1, dtype=torch.int16) > 0
+        out[r0:r1][mask] = float("nan")
+        del u1, u2, nan_l, val_l, lengths, ends, delta, mask
+    return out
+
+
+def gen_group(torch, rows, n, num_labels, seed, p_nan=0.05, zipf=None):
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    v = torch.randn((rows, n), generator=g, device=DEV, dtype=torch.float32)
+    v[torch.rand((rows, n), generator=g, device=DEV) < p_nan] = float("nan")
+    if zipf is None:
+        lab = torch.randint(0, num_labels, (n,), generator=g, device=DEV, dtype=torch.int32)
+    else:
+        p = torch.arange(1, num_labels + 1, device=DEV, dtype=torch.float64) ** (-zipf)
+        lab = torch.multinomial(p / p.sum(), n, replacement=True, generator=g).to(torch.int32)
+    return v, lab
+
+
+# --------------------------------------------------------------------------
+# workloads
+# --------------------------------------------------------------------------
+class Workload:
+    def __init__(self, name, config, nbytes, run, cpu=None, kernel=None):
+        self.name, self.config, self.bytes, self.run, self.cpu = name, config, nbytes, run, cpu
+        self.kernel = kernel or name
+
+
+def has_entry(nb_lib, name):
+    try:
+        getattr(nb_lib, name)
+        return True
+    except AttributeError:
+        return False
+
+
+def build_workloads(torch, nb, lib, want_all):
+    wl = []
+    if has_entry(lib, "nbg_move_mean"):
+        x = gen_walk(torch, 256, 2_000_000, seed=11)
+
+        def run_mm(x=x):
+            return nb.move_mean(x, window=1000, min_count=500)
+
+        def cpu_mm(x=x):
+            xs = x[:8].cpu().numpy()
+            return ("move_mean", xs, dict(window=1000, min_count=500)), xs.nbytes * 2
+
+        wl.append(Workload("move_mean", {"shape": [256, 2_000_000], "dtype": "f32", "window": 1000, "min_count": 500},
+                           x.numel() * 8, run_mm, cpu_mm, kernel="move_kernel"))
+    if has_entry(lib, "nbg_group_nansum"):
+        v, lab = gen_group(torch, 64, 4_000_000, 1000, seed=12)
+
+        def run_gs(v=v, lab=lab):
+            return nb.group_nansum(v, lab, num_labels=1000, axis=-1)
+
+        def cpu_gs(v=v, lab=lab):
+            vs = v[:8].cpu().numpy()
+            return ("group_nansum", vs, dict(labels=lab.cpu().numpy(), num_labels=1000)), vs.nbytes + lab.numel() * 4
+
+        wl.append(Workload("group_nansum", {"shape": [64, 4_000_000], "dtype": "f32", "num_labels": 1000, "labels": "uniform int32"},
+                           v.numel() * 4 + lab.numel() * 4 + 64 * 1000 * 4, run_gs, cpu_gs, kernel="group_kernel"))
+    x = gen_runs(torch, 1024, 1_000_000, seed=13)
+
+    def run_ff(x=x):
+        return nb.ffill(x)
+
+    def cpu_ff(x=x):
+        xs = x[:64].cpu().numpy()
+        return ("ffill", xs, dict(limit=None)), xs.nbytes * 2
+
+    wl.append(Workload("ffill", {"shape": [1024, 1_000_000], "dtype": "f32", "limit": None, "nan": "30% geometric runs"},
+                       x.numel() * 8, run_ff, cpu_ff, kernel="fill_kernel"))
+    return wl
+
+
+def time_cpu(spec, budget_s=4.0):
+    """Median wall time of the oracle restatement on the sample (>=3 runs)."""
+    from oracle import restated
+
+    func, arr, kw = spec
+    if func.startswith("group_"):
+        call = lambda: restated.group(func, arr, kw["labels"], kw["num_labels"])  # noqa: E731
+    elif func in ("ffill", "bfill"):
+        call = lambda: restated.fill(func, arr, kw.get("limit"))  # noqa: E731
+    else:
+        call = lambda: restated.move(func, arr, kw["window"], kw["min_count"])  # noqa: E731
+    call()  # warm-up
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while len(times) < 3 or (time.perf_counter() < t_end and len(times) < 15):
+        t0 = time.perf_counter()
+        call()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--all", action="store_true", help="also time every hot-path function")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(
+            "bench.py measures one GPU: every kernel here works on one GPU's share and "
+            "no multi-GPU path is built (see DESIGN.md for how it would shard).",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from numbagg_amd import _build, _lib
+
+    if _build.library_is_stale():
+        import __graft_entry__
+
+        __graft_entry__.build()
+    import numbagg_amd as nb
+
+    lib = _lib.load()
+    print(f"# device: {torch.cuda.get_device_name(0)}", file=sys.stderr, flush=True)
+    wl = build_workloads(torch, nb, lib, args.all)
+    torch.cuda.synchronize()
+    print(f"# inputs ready: {[w.name for w in wl]}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        for w in wl:
+            w.run()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = {w.name: [] for w in wl}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for w in wl:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            out = w.run()
+            b.record(stream)
+            ev[w.name].append((a, b))
+            del out
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = {}
+    for w in wl:
+        ms = float(np.mean([a.elapsed_time(b) for a, b in ev[w.name]]))
+        gbs = w.bytes / (ms * 1e-3) / 1e9
+        per[w.name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                       "bytes": w.bytes, "config": w.config}
+        print(f"# {w.name}: {ms:.3f} ms  {gbs:.0f} GB/s  {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak", file=sys.stderr, flush=True)
+
+    ms_step = wall * 1e3 / args.steps
+    total_bytes = sum(w.bytes for w in wl)
+    value = total_bytes / (ms_step * 1e-3) / 1e9
+    dom = max(wl, key=lambda w: per[w.name]["ms"])
+    dom_ach = per[dom.name]["GB/s"]
+
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        omp = int(os.environ.get("OMP_NUM_THREADS", threads))
+        cpu_per = {}
+        tot_b, tot_t = 0, 0.0
+        for w in wl:
+            if w.cpu is None:
+                continue
+            spec, nbytes = w.cpu()
+            t = time_cpu(spec)
+            cpu_per[w.name] = round(nbytes / t / 1e9, 2)
+            tot_b += nbytes
+            tot_t += t
+            print(f"# cpu {w.name}: {nbytes / t / 1e9:.2f} GB/s", file=sys.stderr, flush=True)
+        cpu = {
+            "value": round(tot_b / tot_t / 1e9, 3) if tot_t else None,
+            "unit": "GB/s",
+            "cores": min(omp, threads),
+            "kind": "port",
+            "sample": "C restatement of the numbagg kernels (numba unavailable), OpenMP over rows, "
+            "on the first 8 rows of the move_mean and group_nansum inputs and 64 rows of the ffill input",
+            "per_function": cpu_per,
+        }
+
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded torch RNG on the GPU)",
+        "config": {
+            "workload": "headline step: " + " + ".join(w.name for w in wl),
+            "functions": {w.name: w.config for w in wl},
+            "hbm_peak_GBs": HBM_PEAK_GBS,
+            "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+        },
+        "per_function": per,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom.kernel,
+            "function": dom.name,
+            "achieved": dom_ach,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom_ach / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

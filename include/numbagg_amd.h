@@ -1,0 +1,68 @@
+/*
+ * numbagg_amd — C ABI of the MI355X (gfx950) backend for numbagg's hot path.
+ *
+ * The reference (numbagg) has no native code: each public function is a
+ * numba-compiled NumPy gufunc whose per-row loop lives in a Python kernel
+ * (numbagg/moving.py, moving_exp.py, grouped.py, funcs.py) and whose argument
+ * handling lives in the decorator classes of numbagg/decorators.py. Each entry
+ * point below replaces one such gufunc: the caller (the Python host layer in
+ * numbagg_amd/, which re-states the decorator semantics) hands over device
+ * pointers for a row-major (rows, n) view whose core axis is contiguous, plus
+ * the scalar arguments the gufunc received.
+ *
+ * Conventions shared by every entry point:
+ *   - pointers are device pointers; rows/n/ld_* are element counts; ld_* is
+ *     the distance between consecutive rows (>= n for rows > 1);
+ *   - `stream` is a hipStream_t (NULL = the default stream); calls are
+ *     asynchronous with respect to the host and never allocate device memory;
+ *   - `ws`/`ws_bytes` is caller-owned scratch sized by the matching
+ *     *_workspace_bytes() query (its contents need not be initialised);
+ *   - the return value is NBG_OK or an NBG_ERR_* code; nbg_last_error() then
+ *     returns a message (thread-local).
+ *   - dtype codes: NBG_F32/NBG_F64 for values, NBG_I32/NBG_I64 also accepted
+ *     where the reference compiles integer loops, NBG_I32/NBG_I64 for labels.
+ */
+#ifndef NUMBAGG_AMD_H
+#define NUMBAGG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBG_OK 0
+#define NBG_ERR_INVALID 1  /* invalid argument (shape, window, dtype, workspace) */
+#define NBG_ERR_HIP 2      /* HIP runtime error */
+#define NBG_ERR_INJECTED 3 /* failure forced through NBG_INJECT_ERROR (tests) */
+
+#define NBG_F32 0
+#define NBG_F64 1
+#define NBG_I32 2
+#define NBG_I64 3
+
+/* ---- library ----------------------------------------------------------- */
+const char* nbg_version(void);
+const char* nbg_last_error(void);
+/* Number of bounded device-side waits that expired since the last reset on
+ * the current device. Always 0 unless a cross-workgroup protocol was broken. */
+int nbg_spin_timeouts(unsigned int* count);
+int nbg_reset_spin_timeouts(void);
+
+/* ---- fill (numbagg/funcs.py:294-326 bfill/ffill; ndfill.__call__ at
+ *      numbagg/decorators.py:465-487) ------------------------------------- */
+/* limit: the reference's `lives_remaining` budget (>= 0; None -> n). */
+size_t nbg_fill_workspace_bytes(int64_t rows, int64_t n, int dtype);
+int nbg_ffill(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+              int64_t ld_x, int64_t ld_out, int64_t limit, void* ws,
+              size_t ws_bytes, void* stream);
+int nbg_bfill(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+              int64_t ld_x, int64_t ld_out, int64_t limit, void* ws,
+              size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUMBAGG_AMD_H */

@@ -1,0 +1,91 @@
+"""numbagg_amd: numbagg's hot path on AMD MI355X (gfx950) HIP kernels.
+
+Same public names, signatures and argument semantics as numbagg for the grouped
+reductions, the moving-window and exponentially weighted scans, and
+ffill/bfill (numbagg/__init__.py). NumPy inputs are computed on the GPU and
+returned as NumPy arrays; torch tensors on the GPU stay on the GPU.
+"""
+
+from .funcs import bfill, ffill
+from .grouped import (
+    group_nanargmax,
+    group_nanargmin,
+    group_nancount,
+    group_nanfirst,
+    group_nanlast,
+    group_nanmean,
+    group_nanstd,
+    group_nansum,
+    group_nanvar,
+)
+from .moving import move_corr, move_cov, move_mean, move_std, move_sum, move_var
+from .moving_exp import (
+    move_exp_nancorr,
+    move_exp_nancount,
+    move_exp_nancov,
+    move_exp_nanmean,
+    move_exp_nanstd,
+    move_exp_nansum,
+    move_exp_nanvar,
+)
+
+# Family lists restricted to the accelerated functions (numbagg/__init__.py:64-127).
+GROUPED_FUNCS = [
+    group_nanargmax,
+    group_nanargmin,
+    group_nancount,
+    group_nanfirst,
+    group_nanlast,
+    group_nanmean,
+    group_nanstd,
+    group_nansum,
+    group_nanvar,
+]
+
+MOVE_EXP_FUNCS = [
+    move_exp_nancorr,
+    move_exp_nancount,
+    move_exp_nancov,
+    move_exp_nanmean,
+    move_exp_nanstd,
+    move_exp_nansum,
+    move_exp_nanvar,
+]
+
+MOVE_FUNCS = [move_corr, move_cov, move_mean, move_std, move_sum, move_var]
+
+OTHER_FUNCS = [bfill, ffill]
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "__version__",
+    "bfill",
+    "ffill",
+    "group_nanargmax",
+    "group_nanargmin",
+    "group_nancount",
+    "group_nanfirst",
+    "group_nanlast",
+    "group_nanmean",
+    "group_nanstd",
+    "group_nansum",
+    "group_nanvar",
+    "move_corr",
+    "move_cov",
+    "move_exp_nancorr",
+    "move_exp_nancount",
+    "move_exp_nancov",
+    "move_exp_nanmean",
+    "move_exp_nanstd",
+    "move_exp_nansum",
+    "move_exp_nanvar",
+    "move_mean",
+    "move_std",
+    "move_sum",
+    "move_var",
+    "GROUPED_FUNCS",
+    "MOVE_EXP_FUNCS",
+    "MOVE_FUNCS",
+    "OTHER_FUNCS",
+]

@@ -1,0 +1,95 @@
+"""ctypes binding of ``libnumbagg_amd.so`` (the C ABI in include/numbagg_amd.h).
+
+The product path has no CPU fallback: if the HIP library is missing or no ROCm
+GPU is visible, every call raises ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import re
+import threading
+
+from ._build import LIB_PATH, ROOT_DIR
+
+HEADER = ROOT_DIR / "include" / "numbagg_amd.h"
+
+NBG_F32, NBG_F64, NBG_I32, NBG_I64 = 0, 1, 2, 3
+
+_c_i64 = ctypes.c_int64
+_c_int = ctypes.c_int
+_c_ptr = ctypes.c_void_p
+_c_size = ctypes.c_size_t
+_c_dbl = ctypes.c_double
+
+# Signatures of every entry point, mirroring include/numbagg_amd.h.
+_SIGNATURES: dict[str, tuple[object, list[object]]] = {
+    "nbg_version": (ctypes.c_char_p, []),
+    "nbg_last_error": (ctypes.c_char_p, []),
+    "nbg_spin_timeouts": (_c_int, [ctypes.POINTER(ctypes.c_uint)]),
+    "nbg_reset_spin_timeouts": (_c_int, []),
+    "nbg_fill_workspace_bytes": (_c_size, [_c_i64, _c_i64, _c_int]),
+    "nbg_ffill": (
+        _c_int,
+        [_c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_ptr, _c_size, _c_ptr],
+    ),
+    "nbg_bfill": (
+        _c_int,
+        [_c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_ptr, _c_size, _c_ptr],
+    ),
+}
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+def header_symbols(header: pathlib.Path = HEADER) -> list[str]:
+    """Names of the functions declared in the public header."""
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbg_[a-z0-9_]+)\s*\(", text)))
+
+
+def load() -> ctypes.CDLL:
+    """Load (once) and type the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"numbagg_amd: HIP library {LIB_PATH} is not built; "
+                    "run `python -c 'import __graft_entry__ as g; g.build()'`"
+                )
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (restype, argtypes) in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = restype
+                fn.argtypes = argtypes
+            _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    msg = load().nbg_last_error()
+    return msg.decode() if msg else ""
+
+
+def call(name: str, *args) -> None:
+    """Call an NBG entry point and raise ``RuntimeError`` on a non-zero status."""
+    fn = getattr(load(), name)
+    rc = fn(*args)
+    if rc != 0:
+        raise RuntimeError(f"{name} failed (status {rc}): {last_error()}")
+
+
+def query(name: str, *args) -> int:
+    return int(getattr(load(), name)(*args))
+
+
+def spin_timeouts() -> int:
+    c = ctypes.c_uint(0)
+    call("nbg_spin_timeouts", ctypes.byref(c))
+    return int(c.value)

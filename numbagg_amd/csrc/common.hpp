@@ -1,0 +1,151 @@
+// Shared device/host helpers for the numbagg_amd HIP kernels (gfx950 / CDNA4).
+//
+// Everything here is written for 64-wide wavefronts. Kernels are bandwidth-bound
+// streaming scans and reductions; no MFMA is used anywhere in this library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/numbagg_amd.h"
+
+namespace nbg {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Error reporting (host side). Each C entry point returns an NBG_* status code
+// and, on failure, records a message retrievable with nbg_last_error().
+// ---------------------------------------------------------------------------
+int set_error(int code, const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+int injected_error(const char* entry);  // NBG_INJECT_ERROR=<entry> forces a failure
+int device_capacity_blocks(const void* kernel, int block, size_t lds_bytes);
+
+#define NBG_HIP_TRY(expr)                                         \
+  do {                                                            \
+    hipError_t _e = (expr);                                       \
+    if (_e != hipSuccess) return ::nbg::check_hip(_e, #expr);     \
+  } while (0)
+
+// Device address of this device's counter of bounded spin-waits that expired
+// (should always stay 0). Kernels that wait on other workgroups take it as an
+// argument; the host reads it through nbg_spin_timeouts().
+unsigned int* spin_counter_ptr();
+
+// ---------------------------------------------------------------------------
+// Numerics helpers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool is_nan(T v) { return v != v; }
+
+template <typename T> __device__ __forceinline__ T canonical_nan();
+template <> __device__ __forceinline__ float canonical_nan<float>() { return __builtin_nanf(""); }
+template <> __device__ __forceinline__ double canonical_nan<double>() { return __builtin_nan(""); }
+
+// Error-free transformations (Knuth TwoSum / Dekker Fast2Sum). The library is
+// compiled with -ffp-contract=off so the compiler cannot fuse these.
+template <typename F>
+__device__ __forceinline__ void two_sum(F a, F b, F& s, F& e) {
+  s = a + b;
+  F bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+template <typename F>
+__device__ __forceinline__ void fast_two_sum(F a, F b, F& s, F& e) {
+  s = a + b;
+  e = b - (s - a);
+}
+
+// Compensated ("double-F") accumulator: value = hi + lo with |lo| <= ulp(hi)/2.
+// For F=float this is the FP32 TwoSum accumulator used for first moments of
+// float32 data; for F=double it is double-double, used for float64 data.
+template <typename F>
+struct Comp {
+  F hi, lo;
+  __device__ __forceinline__ static Comp zero() { return Comp{F(0), F(0)}; }
+  __device__ __forceinline__ void add(F x) {
+    F s, e;
+    two_sum(hi, x, s, e);
+    e += lo;
+    fast_two_sum(s, e, hi, lo);
+  }
+  __device__ __forceinline__ void add(const Comp& o) {
+    F s, e, t, f;
+    two_sum(hi, o.hi, s, e);
+    two_sum(lo, o.lo, t, f);
+    e += t;
+    fast_two_sum(s, e, s, e);
+    e += f;
+    fast_two_sum(s, e, hi, lo);
+  }
+  __device__ __forceinline__ void sub(const Comp& o) {
+    Comp n{-o.hi, -o.lo};
+    add(n);
+  }
+  __device__ __forceinline__ double to_double() const { return (double)hi + (double)lo; }
+};
+
+// ---------------------------------------------------------------------------
+// Wave-level primitives.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+__device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
+
+template <typename T>
+__device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, d, kWave); }
+template <typename T>
+__device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, d, kWave); }
+template <typename T>
+__device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, kWave); }
+
+// ---------------------------------------------------------------------------
+// Agent-scope status words for cross-workgroup (decoupled look-back) protocols.
+// Relaxed agent-scope atomics bypass the non-coherent per-XCD caches, so a word
+// published by a workgroup on one XCD is visible to a workgroup on another.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned int ld_acquire(const unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(unsigned int* p, unsigned int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_agent_f64(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent_f64(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded spin: every wait in this library ends. ~2^24 sleeps of 64 cycles is
+// well over a second; a wait that long means the protocol was broken, and the
+// kernel records it in g_spin_timeouts instead of hanging the device.
+constexpr unsigned int kSpinLimit = 1u << 24;
+
+__device__ __forceinline__ void spin_pause() { __builtin_amdgcn_s_sleep(1); }
+
+// Zero-fill kernel used to reset look-back status words and ticket counters.
+// (A kernel rather than a memset node so the call stays graph-capture safe.)
+__global__ void zero_words_kernel(unsigned long long* p, int64_t nwords);
+int launch_zero(void* p, size_t bytes, hipStream_t s);
+
+// Widest vector (in elements, up to 16 bytes) that every row of a (rows, ld)
+// view can be loaded with: the base must be aligned, and for more than one row
+// so must the row stride.
+template <typename T>
+inline int pick_vec(const void* p, int64_t ld, int64_t rows) {
+  constexpr int maxv = 16 / (int)sizeof(T);
+  for (int v = maxv; v > 1; v >>= 1) {
+    if (((uintptr_t)p % (v * sizeof(T))) == 0 && (rows <= 1 || (ld % v) == 0)) return v;
+  }
+  return 1;
+}
+
+}  // namespace nbg

@@ -1,0 +1,363 @@
+// ffill / bfill for gfx950.
+//
+// Reference semantics (numbagg/funcs.py:294-326): walking the row in scan order
+// (ascending for ffill, descending for bfill), a valid value resets a
+// `lives_remaining = limit` budget and becomes the current value; each NaN costs
+// one life, and once the budget is spent the current value becomes NaN. In closed
+// form: out[i] = x[j] where j is the last valid index at or before i in scan
+// order, provided i - j <= limit (distance in scan order); otherwise NaN (the
+// canonical quiet NaN, since the reference stores `np.nan`). Integer rows are
+// copied through by the host layer (isnan is false for integers).
+//
+// Design: a row is cut into tiles of 256 threads x U x VEC elements. Each
+// workgroup takes a ticket from its row's counter, so tiles of a row start in
+// scan order and a tile only ever waits on tiles that are already running
+// (decoupled look-back, deadlock-free whatever the residency). Blocks are
+// interleaved over rows (blockIdx % rows), so a tile's predecessor was dispatched
+// `rows` blocks earlier and has normally published its prefix already.
+//
+// Inside a tile, loads are wave-striped vectors (every load instruction of a wave
+// covers 64*VEC contiguous elements); the carry "last valid (offset, value)" is
+// scanned per 64*VEC row with wave shuffles, across waves through LDS and across
+// tiles through one 64-bit status word per tile:
+//   bits 63..62  flag (1 = aggregate: tile holds no valid value;
+//                      2 = inclusive prefix)
+//   bits 61..0   1 + scan-order index of the last valid value (0 = none)
+#include "common.hpp"
+
+namespace nbg {
+namespace {
+
+constexpr int kFillWaves = 4;
+constexpr int kFillThreads = kFillWaves * kWave;
+
+constexpr unsigned long long kFlagA = 1ull << 62;
+constexpr unsigned long long kFlagP = 2ull << 62;
+constexpr unsigned long long kIdxMask = (1ull << 62) - 1;
+
+template <typename T>
+struct alignas(16) VecT;
+template <>
+struct alignas(16) VecT<float> { float v[4]; };
+template <>
+struct alignas(16) VecT<double> { double v[2]; };
+
+// Running "last valid value" carry: scan-order offset (or -1) plus the value.
+template <typename T>
+struct Last {
+  int so;
+  T val;
+};
+
+template <bool REV, typename T>
+__device__ __forceinline__ Last<T> wave_incl_last(Last<T> a) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    int oso = REV ? shfl_down(a.so, d) : shfl_up(a.so, d);
+    T oval = REV ? shfl_down(a.val, d) : shfl_up(a.val, d);
+    bool in = REV ? (lane + d < kWave) : (lane >= d);
+    if (in && oso > a.so) {
+      a.so = oso;
+      a.val = oval;
+    }
+  }
+  return a;
+}
+
+template <typename T, int VEC, int U, bool REV, bool LOOKBACK>
+__global__ __launch_bounds__(kFillThreads) void fill_kernel(
+    const T* __restrict__ x, T* __restrict__ out, int64_t rows, int64_t n, int64_t ld_x,
+    int64_t ld_out, int64_t limit, int64_t tiles_per_row, unsigned int* __restrict__ counters,
+    unsigned long long* __restrict__ status, unsigned int* __restrict__ timeouts) {
+  constexpr int TILE = kFillThreads * U * VEC;
+  __shared__ int s_ticket;
+  __shared__ int s_wso[kFillWaves];
+  __shared__ T s_wval[kFillWaves];
+  __shared__ long long s_carry;  // scan-order absolute index of the carry, -1 = none
+  __shared__ T s_carry_val;
+  __shared__ int s_first_valid;
+
+  const int64_t row = (int64_t)(blockIdx.x % (uint64_t)rows);
+  int64_t ticket = 0;
+  if (LOOKBACK) {
+    if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&counters[row], 1u);
+    __syncthreads();
+    ticket = s_ticket;
+  } else {
+    ticket = (int64_t)(blockIdx.x / (uint64_t)rows);
+  }
+  // Tiles are aligned to the row start; bfill walks them from the last one.
+  const int64_t tile = REV ? (tiles_per_row - 1 - ticket) : ticket;
+  const int64_t p0 = tile * TILE;
+  // Scan-order coordinate of the tile's first element (may be < 0 for the
+  // partial last tile in bfill; only real elements are ever addressed).
+  const int64_t s_base = REV ? (n - p0 - TILE) : p0;
+
+  const T* xr = x + row * ld_x;
+  T* orow = out + row * ld_out;
+  const int w = wave_id();
+  const int l = lane_id();
+  const bool full = (p0 + TILE <= n);
+
+  T vals[U][VEC];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e0 = ((w * U + u) * kWave + l) * VEC;
+    if (full && VEC > 1) {
+      VecT<T> t = *reinterpret_cast<const VecT<T>*>(xr + p0 + e0);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) vals[u][v] = t.v[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const int64_t p = p0 + e0 + v;
+        vals[u][v] = (p < n) ? xr[p] : canonical_nan<T>();
+      }
+    }
+  }
+
+  // Scan-order offset inside the tile of element (u, v) of this lane.
+  auto so_of = [&](int u, int v) -> int {
+    const int e = ((w * U + u) * kWave + l) * VEC + v;
+    return REV ? (TILE - 1 - e) : e;
+  };
+
+  // Per-slot exclusive carries (from earlier lanes / slots of this wave).
+  Last<T> cin[U];
+  Last<T> wcarry{-1, T(0)};
+#pragma unroll
+  for (int i = 0; i < U; ++i) {
+    const int u = REV ? (U - 1 - i) : i;  // scan order over slots
+    Last<T> agg{-1, T(0)};
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      if (!is_nan(vals[u][v])) {
+        const int so = so_of(u, v);
+        if (so > agg.so) {
+          agg.so = so;
+          agg.val = vals[u][v];
+        }
+      }
+    }
+    Last<T> incl = wave_incl_last<REV>(agg);
+    Last<T> excl;
+    excl.so = REV ? shfl_down(incl.so, 1) : shfl_up(incl.so, 1);
+    excl.val = REV ? shfl_down(incl.val, 1) : shfl_up(incl.val, 1);
+    const bool first_lane = REV ? (l == kWave - 1) : (l == 0);
+    if (first_lane || excl.so < 0) excl = wcarry;
+    cin[u] = excl;
+    Last<T> tail;
+    tail.so = shfl(incl.so, REV ? 0 : kWave - 1);
+    tail.val = shfl(incl.val, REV ? 0 : kWave - 1);
+    if (tail.so >= 0) wcarry = tail;
+  }
+  if (l == 0) {
+    s_wso[w] = wcarry.so;
+    s_wval[w] = wcarry.val;
+  }
+  // The owner of scan offset 0 reports whether that element is valid.
+  if (REV ? (w == kFillWaves - 1 && l == kWave - 1) : (w == 0 && l == 0))
+    s_first_valid = !is_nan(vals[REV ? U - 1 : 0][REV ? VEC - 1 : 0]);
+  __syncthreads();
+
+  // Carry from earlier waves (scan order) and the tile aggregate.
+  Last<T> wave_in{-1, T(0)};
+  Last<T> tile_agg{-1, T(0)};
+#pragma unroll
+  for (int i = 0; i < kFillWaves; ++i) {
+    const int ww = REV ? (kFillWaves - 1 - i) : i;
+    if (s_wso[ww] >= 0) {
+      if ((REV && ww > w) || (!REV && ww < w)) {
+        wave_in.so = s_wso[ww];
+        wave_in.val = s_wval[ww];
+      }
+      tile_agg.so = s_wso[ww];
+      tile_agg.val = s_wval[ww];
+    }
+  }
+
+  // Cross-tile carry (decoupled look-back), by thread 0.
+  if (threadIdx.x == 0) {
+    long long carry = -1;
+    if (LOOKBACK) {
+      unsigned long long* srow = status + row * tiles_per_row;
+      // Status is indexed by scan-order ticket.
+      if (tile_agg.so >= 0) {
+        st_agent(&srow[ticket], kFlagP | (unsigned long long)(s_base + tile_agg.so + 1));
+      } else if (ticket > 0) {
+        st_agent(&srow[ticket], kFlagA);
+      } else {
+        st_agent(&srow[ticket], kFlagP);
+      }
+      // The carry matters unless the tile's first element in scan order is valid.
+      if (ticket > 0 && !s_first_valid) {
+        int64_t j = ticket - 1;
+        while (true) {
+          unsigned long long st;
+          unsigned int spins = 0;
+          while (((st = ld_agent(&srow[j])) >> 62) == 0) {
+            spin_pause();
+            if (++spins > kSpinLimit) {
+              atomicAdd(timeouts, 1u);
+              st = kFlagP;
+              break;
+            }
+          }
+          if ((st >> 62) == 2) {
+            carry = (long long)(st & kIdxMask) - 1;
+            break;
+          }
+          if (j == 0) break;
+          --j;
+        }
+        if (tile_agg.so < 0) st_agent(&srow[ticket], kFlagP | (unsigned long long)(carry + 1));
+      }
+    }
+    s_carry = carry;
+    if (carry >= 0) {
+      const int64_t pc = REV ? (n - 1 - carry) : carry;
+      s_carry_val = xr[pc];
+    }
+  }
+  __syncthreads();
+  const long long carry = s_carry;
+  const T carry_val = (carry >= 0) ? s_carry_val : T(0);
+
+  // Outputs.
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    // Best carry-in for this slot: tile carry < earlier waves < earlier lanes/slots.
+    long long run_abs = carry;
+    T run_val = carry_val;
+    if (wave_in.so >= 0) {
+      run_abs = s_base + wave_in.so;
+      run_val = wave_in.val;
+    }
+    if (cin[u].so >= 0) {
+      run_abs = s_base + cin[u].so;
+      run_val = cin[u].val;
+    }
+    T res[VEC];
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) {
+      const int v = REV ? (VEC - 1 - i) : i;
+      const int64_t s_abs = s_base + so_of(u, v);
+      if (!is_nan(vals[u][v])) {
+        run_abs = s_abs;
+        run_val = vals[u][v];
+      }
+      res[v] = (run_abs >= 0 && (s_abs - run_abs) <= limit) ? run_val : canonical_nan<T>();
+    }
+    const int e0 = ((w * U + u) * kWave + l) * VEC;
+    if (full && VEC > 1) {
+      VecT<T> t;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) t.v[v] = res[v];
+      *reinterpret_cast<VecT<T>*>(orow + p0 + e0) = t;
+    } else {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const int64_t p = p0 + e0 + v;
+        if (p < n) orow[p] = res[v];
+      }
+    }
+  }
+}
+
+template <typename T>
+int fill_impl(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int64_t ld_x,
+              int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, hipStream_t s) {
+  const T* x = static_cast<const T*>(xv);
+  T* out = static_cast<T*>(ov);
+  constexpr int TILE = kFillThreads * 64 / (int)sizeof(T);  // 16 elements x 4 B or 8 x 8 B per thread
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  const int64_t blocks = rows * tiles;
+  if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "fill: too many tiles (%lld)", (long long)blocks);
+  const bool lookback = tiles > 1;
+  unsigned int* counters = nullptr;
+  unsigned long long* status = nullptr;
+  if (lookback) {
+    size_t need = nbg_fill_workspace_bytes(rows, n, sizeof(T) == 4 ? NBG_F32 : NBG_F64);
+    if (ws == nullptr || ws_bytes < need)
+      return set_error(NBG_ERR_INVALID, "fill: workspace too small (%zu < %zu)", ws_bytes, need);
+    status = static_cast<unsigned long long*>(ws);
+    counters = reinterpret_cast<unsigned int*>(status + rows * tiles);
+    int rc = launch_zero(ws, need, s);
+    if (rc) return rc;
+  }
+  unsigned int* to = spin_counter_ptr();
+  if (to == nullptr) return set_error(NBG_ERR_HIP, "fill: cannot resolve device counter");
+  const int vx = pick_vec<T>(x, ld_x, rows), vo = pick_vec<T>(out, ld_out, rows);
+  const int vec = vx < vo ? vx : vo;
+  constexpr int MAXV = 16 / (int)sizeof(T);
+  constexpr int UV = 64 / (int)sizeof(T) / MAXV;  // slots per thread at full vector width
+  dim3 grid((unsigned)blocks), block(kFillThreads);
+#define NBG_FILL_LAUNCH(V, U)                                                                       \
+  do {                                                                                            \
+    if (rev) {                                                                                    \
+      if (lookback)                                                                               \
+        hipLaunchKernelGGL((fill_kernel<T, V, U, true, true>), grid, block, 0, s, x, out, rows, n, \
+                           ld_x, ld_out, limit, tiles, counters, status, to);                     \
+      else                                                                                        \
+        hipLaunchKernelGGL((fill_kernel<T, V, U, true, false>), grid, block, 0, s, x, out, rows,  \
+                           n, ld_x, ld_out, limit, tiles, counters, status, to);                  \
+    } else {                                                                                      \
+      if (lookback)                                                                               \
+        hipLaunchKernelGGL((fill_kernel<T, V, U, false, true>), grid, block, 0, s, x, out, rows,  \
+                           n, ld_x, ld_out, limit, tiles, counters, status, to);                  \
+      else                                                                                        \
+        hipLaunchKernelGGL((fill_kernel<T, V, U, false, false>), grid, block, 0, s, x, out, rows, \
+                           n, ld_x, ld_out, limit, tiles, counters, status, to);                  \
+    }                                                                                             \
+  } while (0)
+  if (vec == MAXV)
+    NBG_FILL_LAUNCH(MAXV, UV);
+  else
+    NBG_FILL_LAUNCH(1, UV * MAXV);
+#undef NBG_FILL_LAUNCH
+  return check_hip(hipGetLastError(), "fill_kernel launch");
+}
+
+int fill_entry(const char* name, bool rev, int dtype, const void* x, void* out, int64_t rows,
+               int64_t n, int64_t ld_x, int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes,
+               void* stream) {
+  if (int rc = injected_error(name)) return rc;
+  if (rows < 0 || n < 0) return set_error(NBG_ERR_INVALID, "%s: negative shape", name);
+  if (limit < 0) return set_error(NBG_ERR_INVALID, "%s: `limit` must be positive: %lld", name, (long long)limit);
+  if (rows == 0 || n == 0) return NBG_OK;
+  if (x == nullptr || out == nullptr) return set_error(NBG_ERR_INVALID, "%s: null pointer", name);
+  if (rows > 1 && (ld_x < n || ld_out < n)) return set_error(NBG_ERR_INVALID, "%s: row stride < n", name);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case NBG_F32: return fill_impl<float>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
+    case NBG_F64: return fill_impl<double>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
+    default: return set_error(NBG_ERR_INVALID, "%s: unsupported dtype code %d", name, dtype);
+  }
+}
+
+}  // namespace
+}  // namespace nbg
+
+extern "C" {
+
+size_t nbg_fill_workspace_bytes(int64_t rows, int64_t n, int dtype) {
+  const int64_t tile = (dtype == NBG_F64) ? nbg::kFillThreads * 8 : nbg::kFillThreads * 16;
+  const int64_t tiles = (n + tile - 1) / tile;
+  if (rows <= 0 || tiles <= 1) return 0;
+  return (size_t)(rows * tiles) * 8 + (size_t)((rows * 4 + 7) / 8) * 8;
+}
+
+int nbg_ffill(int dtype, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
+              int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, void* stream) {
+  return nbg::fill_entry("nbg_ffill", false, dtype, x, out, rows, n, ld_x, ld_out, limit, ws,
+                         ws_bytes, stream);
+}
+
+int nbg_bfill(int dtype, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
+              int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, void* stream) {
+  return nbg::fill_entry("nbg_bfill", true, dtype, x, out, rows, n, ld_x, ld_out, limit, ws,
+                         ws_bytes, stream);
+}
+
+}  // extern "C"

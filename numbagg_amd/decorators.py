@@ -1,0 +1,473 @@
+"""Host-side operator classes: numbagg's decorator semantics over HIP kernels.
+
+Each class restates the argument handling of one numbagg decorator
+(numbagg/decorators.py) — defaults, validation, error types and messages, axis
+handling, dtype selection — and then, instead of dispatching to a numba gufunc,
+normalises the operands to a (rows, n) device view with a contiguous core axis
+and calls the matching ``nbg_*`` entry point of the C ABI.
+
+The decorated Python functions carry only the public name and docstring; their
+bodies are never executed. There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import functools
+import inspect
+import logging
+import os
+from collections.abc import Callable
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _device as dev
+from . import _lib
+
+logger = logging.getLogger(__name__)
+
+
+def _loop_dtype(dtypes: list[np.dtype], func_name: str) -> np.dtype:
+    """The float loop NumPy's gufunc type resolution would pick.
+
+    numbagg registers a float32 loop then a float64 loop (e.g. moving.py:12-14);
+    NumPy takes the first loop every operand can be cast to safely.
+    """
+    for loop in (np.dtype(np.float32), np.dtype(np.float64)):
+        if all(np.can_cast(dt, loop, casting="safe") for dt in dtypes):
+            return loop
+    raise TypeError(
+        f"ufunc '{func_name}' not supported for the input types "
+        f"{tuple(str(d) for d in dtypes)}, and the inputs could not be safely "
+        "coerced to any supported types according to the casting rule ''safe''"
+    )
+
+
+def _shape(a) -> tuple[int, ...]:
+    if isinstance(a, torch.Tensor):
+        return tuple(a.shape)
+    return np.shape(a)
+
+
+def _is_int_scalar(v) -> bool:
+    return isinstance(v, (int, np.integer)) and not isinstance(v, (bool, np.bool_))
+
+
+class _Op:
+    """Common base: name/doc copying and the ``numbagg.<name>`` repr."""
+
+    def __init__(self, func: Callable[..., Any]) -> None:
+        self.func = func
+        functools.wraps(func)(self)
+        self.__signature__ = inspect.signature(self.__call__)
+        self.entry = "nbg_" + func.__name__
+
+    def __repr__(self) -> str:
+        return f"numbagg.{self.__name__}"  # type: ignore[attr-defined]
+
+    @classmethod
+    def wrap(cls, *args, **kwargs):
+        """Decorate a documentation stub (numbagg/decorators.py:122-127)."""
+        return lambda func: cls(func, *args, **kwargs)
+
+    @staticmethod
+    def _no_extra(kwargs: dict) -> None:
+        if kwargs:
+            raise TypeError(f"unsupported keyword arguments: {sorted(kwargs)}")
+
+
+def _single_axis(axis, arr: tuple, func, capital: bool):
+    """Axis-tuple rules shared by ndmove/ndmoveexp (decorators.py:318-332, :385-396)."""
+    if isinstance(axis, tuple):
+        if axis == ():
+            if len(arr) > 1:
+                raise ValueError(
+                    "`axis` cannot be an empty tuple when passing more than one array; "
+                    "since we default to returning the input."
+                )
+            return None
+        if len(axis) > 1:
+            word = "Only" if capital else "only"
+            raise ValueError(f"{word} one axis can be passed to {func}; got {axis}")
+        (axis,) = axis
+    return axis
+
+
+def _broadcast_rows(tensors: list[torch.Tensor], axes: list[int]):
+    """Move each operand's core axis last and broadcast the outer dimensions.
+
+    Returns (list of (rows, n) views with ld, outer_shape, n).
+    """
+    moved = [t.movedim(ax, -1) for t, ax in zip(tensors, axes)]
+    n = moved[0].shape[-1]
+    for m in moved[1:]:
+        if m.shape[-1] != n:
+            raise ValueError(
+                f"core dimension mismatch: operands have core lengths {[x.shape[-1] for x in moved]}"
+            )
+    outer = torch.broadcast_shapes(*[tuple(m.shape[:-1]) for m in moved])
+    views = []
+    for m in moved:
+        if tuple(m.shape[:-1]) != tuple(outer):
+            m = m.expand(*outer, n)
+        views.append(dev.rows_view(m))
+    return views, tuple(outer), n
+
+
+class ndmove(_Op):
+    """Moving-window operator (numbagg/decorators.py:275-341).
+
+    Calls ``nbg_<name>(dtype, inputs..., out, rows, n, ld_inputs..., ld_out,
+    window, min_count, stream)``.
+    """
+
+    def __init__(self, func: Callable[..., Any], nargs: int = 1) -> None:
+        super().__init__(func)
+        self.nargs = nargs
+
+    def __call__(
+        self,
+        *arr,
+        window: int,
+        min_count: int | None = None,
+        axis: int | tuple[int, ...] = -1,
+        **kwargs,
+    ):
+        self._no_extra(kwargs)
+        if min_count is None:
+            min_count = window
+        elif min_count < 0:
+            raise ValueError(f"min_count must be positive: {min_count}")
+        axis = _single_axis(axis, arr, self.func, capital=False)
+        if axis is None:
+            return arr[0]
+        if len(arr) != self.nargs:
+            raise TypeError(f"{self.__name__} takes {self.nargs} array(s), got {len(arr)}")
+        if not 0 < window <= _shape(arr[0])[axis]:
+            raise ValueError(f"window not in valid range: {window}")
+        if not _is_int_scalar(window) or not _is_int_scalar(min_count):
+            raise TypeError(
+                f"ufunc '{self.__name__}' not supported for the input types: window and "
+                "min_count must be integers"
+            )
+        as_tensor = any(dev.is_tensor(a) for a in arr)
+        loop = _loop_dtype([dev.np_dtype(a) for a in arr], self.__name__)
+        tensors = [dev.to_device(a, loop) for a in arr]
+        views, outer, n = _broadcast_rows(tensors, [axis] * len(tensors))
+        rows = views[0][1]
+        out = torch.empty((rows, n), dtype=dev.torch_dtype(loop), device=tensors[0].device)
+        if rows and n:
+            args = [ctypes.c_int(dev.nbg_code(out))]
+            args += [dev.ptr(v[0]) for v in views]
+            args += [dev.ptr(out), rows, n]
+            args += [v[3] for v in views]
+            args += [n, int(window), int(min_count), dev.stream_handle()]
+            _lib.call(self.entry, *args)
+        # NumPy places the output's core dimension at `axis` as well.
+        res = out.view(*outer, n).movedim(-1, axis)
+        return dev.to_output(res, as_tensor)
+
+
+class ndmoveexp(_Op):
+    """Exponentially weighted moving operator (numbagg/decorators.py:344-414).
+
+    ``alpha`` may be a scalar, a 1-D array along the moving axis (shared by all
+    rows) or an N-D array laid out like the input. Calls
+    ``nbg_<name>(x_dtype, dtype, inputs..., alpha, ld_alpha, alpha_scalar, out,
+    rows, n, ld_inputs..., ld_out, min_weight, ws, ws_bytes, stream)``.
+    """
+
+    def __init__(self, func: Callable[..., Any], nargs: int = 1) -> None:
+        super().__init__(func)
+        self.nargs = nargs
+
+    def __call__(
+        self,
+        *arr,
+        alpha,
+        min_weight: float = 0,
+        axis: int | tuple[int, ...] = -1,
+        **kwargs,
+    ):
+        self._no_extra(kwargs)
+        axis = _single_axis(axis, arr, self.func, capital=True)
+        if axis is None:
+            return arr[0]
+        if len(arr) != self.nargs:
+            raise TypeError(f"{self.__name__} takes {self.nargs} array(s), got {len(arr)}")
+        n_axis = _shape(arr[0])[axis]
+        as_tensor = any(dev.is_tensor(a) for a in arr)
+
+        alpha_is_array = isinstance(alpha, (np.ndarray, torch.Tensor))
+        if not alpha_is_array:
+            # np.broadcast_to(alpha, n) in the reference: the dtype of the scalar
+            # (a Python float is float64) takes part in the loop selection.
+            alpha_dtype = np.asarray(alpha).dtype
+            alpha_mode = "scalar"
+        else:
+            alpha_dtype = dev.np_dtype(alpha)
+            alpha_mode = "row" if alpha.ndim == 1 else "full"
+        dtypes = [dev.np_dtype(a) for a in arr] + [alpha_dtype]
+        if isinstance(min_weight, np.generic):
+            dtypes.append(min_weight.dtype)
+        loop = _loop_dtype(dtypes, self.__name__)
+        mw = float(np.asarray(min_weight, dtype=loop))
+
+        in_dtypes = {dev.np_dtype(a) for a in arr}
+        # float32 data on the float64 loop is widened inside the kernel.
+        fused = loop == np.float64 and in_dtypes == {np.dtype(np.float32)}
+        x_dtype = np.dtype(np.float32) if fused else loop
+        tensors = [dev.to_device(a, x_dtype) for a in arr]
+        views, outer, n = _broadcast_rows(tensors, [axis] * len(tensors))
+        rows = views[0][1]
+        device = tensors[0].device
+
+        alpha_ptr, ld_alpha, alpha_scalar, alpha_keep = ctypes.c_void_p(0), 0, 0.0, None
+        if alpha_mode == "scalar":
+            alpha_scalar = float(np.asarray(alpha, dtype=loop))
+        elif alpha_mode == "row":
+            if alpha.shape[0] != n_axis:
+                raise ValueError(
+                    f"{self.__name__}: alpha has length {alpha.shape[0]} but the core "
+                    f"dimension has length {n_axis}"
+                )
+            alpha_keep = dev.to_device(alpha, loop).contiguous()
+            alpha_ptr, ld_alpha = dev.ptr(alpha_keep), 0
+        else:
+            a_t = dev.to_device(alpha, loop).movedim(axis, -1)
+            if a_t.shape[-1] != n:
+                raise ValueError(f"{self.__name__}: alpha core dimension mismatch")
+            alpha_keep = a_t.expand(*outer, n).contiguous().view(max(rows, 1), n)
+            alpha_ptr, ld_alpha = dev.ptr(alpha_keep), n
+
+        out = torch.empty((rows, n), dtype=dev.torch_dtype(loop), device=device)
+        if rows and n:
+            code_in = _lib.NBG_F32 if x_dtype == np.float32 else _lib.NBG_F64
+            ws_bytes = _lib.query("nbg_move_exp_workspace_bytes", rows, n, code_in)
+            ws, ws_bytes = dev.workspace(ws_bytes, device)
+            args = [ctypes.c_int(code_in), ctypes.c_int(dev.nbg_code(out))]
+            args += [dev.ptr(v[0]) for v in views]
+            args += [alpha_ptr, ld_alpha, ctypes.c_double(alpha_scalar), dev.ptr(out), rows, n]
+            args += [v[3] for v in views]
+            args += [n, ctypes.c_double(mw), dev.ptr(ws), ws_bytes, dev.stream_handle()]
+            _lib.call(self.entry, *args)
+        del alpha_keep
+        # NumPy places the output's core dimension at `axis` as well.
+        res = out.view(*outer, n).movedim(-1, axis)
+        return dev.to_output(res, as_tensor)
+
+
+class ndfill(_Op):
+    """Forward/backward fill operator (numbagg/decorators.py:417-487).
+
+    Float rows run the HIP kernel; integer rows have no NaN, so the reference
+    copies them through unchanged, and so does this class (on the device).
+    Calls ``nbg_<name>(dtype, x, out, rows, n, ld_x, ld_out, limit, ws,
+    ws_bytes, stream)``.
+    """
+
+    def __call__(self, arr, *, limit: None | int = None, axis: int = -1, **kwargs):
+        self._no_extra(kwargs)
+        shape = _shape(arr)
+        if limit is None:
+            limit = shape[axis]
+        if limit < 0:
+            raise ValueError(f"`limit` must be positive: {limit}")
+        dt = dev.np_dtype(arr)
+        if not np.issubdtype(dt, np.number):
+            raise TypeError(f"Unsupported dtype for fill operation: {dt}")
+        as_tensor = dev.is_tensor(arr)
+        if not np.issubdtype(dt, np.floating):
+            t = dev.to_device(arr)
+            return dev.to_output(t.clone(), as_tensor)
+        # float16 rows are filled through float32, which round-trips exactly.
+        work = np.dtype(np.float32) if dt == np.float16 else dt
+        t = dev.to_device(arr, work)
+        moved = t.movedim(axis, -1)
+        v2, rows, n, ld = dev.rows_view(moved)
+        out = torch.empty((rows, n), dtype=t.dtype, device=t.device)
+        if rows and n:
+            ws_bytes = _lib.query("nbg_fill_workspace_bytes", rows, n, dev.nbg_code(t))
+            ws, ws_bytes = dev.workspace(ws_bytes, t.device)
+            _lib.call(
+                self.entry,
+                ctypes.c_int(dev.nbg_code(t)),
+                dev.ptr(v2),
+                dev.ptr(out),
+                rows,
+                n,
+                ld,
+                n,
+                int(limit),
+                dev.ptr(ws),
+                ws_bytes,
+                dev.stream_handle(),
+            )
+        res = out.view(moved.shape).movedim(-1, axis)
+        if work != dt:
+            res = res.to(dev.torch_dtype(dt))
+        return dev.to_output(res, as_tensor)
+
+
+# Group-by strategy codes of the C ABI (nbg_group_*).
+GROUP_STRATEGY_AUTO = 0
+GROUP_STRATEGY_LDS = 1
+GROUP_STRATEGY_GLOBAL = 2
+
+
+def _group_strategy() -> int:
+    """Strategy override from NBG_GROUP_STRATEGY ("lds" / "global"), else auto."""
+    v = os.environ.get("NBG_GROUP_STRATEGY", "").lower()
+    return {"lds": GROUP_STRATEGY_LDS, "global": GROUP_STRATEGY_GLOBAL}.get(v, GROUP_STRATEGY_AUTO)
+
+
+class groupndreduce(_Op):
+    """Grouped reduction operator (numbagg/decorators.py:490-674).
+
+    Supports the reference's three axis modes (None, int, tuple). Values are
+    reduced into ``(*broadcast_shape, num_labels)`` by
+    ``nbg_<name>(vdtype, values, ldtype, labels, out, rows, n, ld_v, num_labels,
+    [ddof,] strategy, ws, ws_bytes, stream)``. Labels < 0 are skipped; labels
+    >= num_labels (undefined behaviour in the reference) are skipped as well.
+    """
+
+    def __init__(
+        self,
+        func: Callable[..., Any],
+        *,
+        supports_ddof: bool = False,
+        supports_bool: bool = True,
+        supports_ints: bool = True,
+    ) -> None:
+        self.supports_bool = supports_bool
+        self.supports_ints = supports_ints
+        self.supports_ddof = supports_ddof
+        super().__init__(func)
+
+    def __call__(
+        self,
+        values,
+        labels,
+        *,
+        ddof: int = 1,
+        num_labels: int | None = None,
+        axis: int | tuple[int, ...] | None = None,
+    ):
+        as_tensor = dev.is_tensor(values)
+        if not dev.is_tensor(values):
+            values = np.asarray(values)
+        if not dev.is_tensor(labels):
+            labels = np.asarray(labels)
+        ldt = dev.np_dtype(labels)
+        if ldt.kind not in "i":
+            raise TypeError(
+                "labels must be an integer array; it's expected to have already been "
+                "factorized with a function such as `pd.factorize`"
+            )
+        vdt = dev.np_dtype(values)
+        vshape = tuple(values.shape)
+        lshape = tuple(labels.shape)
+        if vdt == np.bool_:
+            if not self.supports_bool:
+                func_name = getattr(self.func, "__name__", "<unknown>")
+                raise TypeError(
+                    f"{func_name} does not support boolean input. Convert to a numeric type first."
+                )
+            vdt = np.dtype(np.int32)
+        if num_labels is None:
+            if dev.is_tensor(labels):
+                num_labels = int(labels.max().item()) + 1
+            else:
+                num_labels = int(np.max(labels)) + 1
+        if np.issubdtype(vdt, np.integer) and not self.supports_ints:
+            vdt = np.dtype(np.float64)
+        result_dtype = vdt
+
+        ndim = len(vshape)
+        if axis is None:
+            if vshape != lshape:
+                raise ValueError(
+                    "axis required if values and labels have different "
+                    f"shapes: {vshape} vs {lshape}"
+                )
+            core_axes = list(range(ndim))
+        elif isinstance(axis, (int, np.integer)):
+            ax = int(axis)
+            if lshape != (vshape[ax],):
+                raise ValueError(
+                    "values must have same shape along axis as labels: "
+                    f"{(vshape[ax],)} vs {lshape}"
+                )
+            core_axes = [ax % ndim]
+        else:
+            values_shape = tuple(vshape[ax] for ax in axis)
+            if lshape != values_shape:
+                raise ValueError(
+                    f"values must have same shape along axis as labels: {values_shape} vs {lshape}"
+                )
+            core_axes = [ax % ndim for ax in axis]
+        broadcast_shape = tuple(s for i, s in enumerate(vshape) if i not in core_axes)
+
+        kdt = self._kernel_value_dtype(result_dtype)
+        v = dev.to_device(values, kdt)
+        if v.dtype == torch.bool:
+            v = v.to(torch.int32)
+        v = v.movedim(core_axes, list(range(ndim - len(core_axes), ndim)))
+        n = int(np.prod([vshape[a] for a in core_axes], dtype=np.int64))
+        rows = int(np.prod(broadcast_shape, dtype=np.int64))
+        v2 = v.reshape(rows, n)
+        if v2.stride(-1) != 1 or (rows > 1 and v2.stride(0) < n):
+            v2 = v2.contiguous()
+        ld_v = int(v2.stride(0)) if rows > 1 else n
+        lab = dev.to_device(labels)
+        lab = lab.to(torch.int32 if lab.dtype in (torch.int8, torch.int16) else lab.dtype)
+        lab = lab.reshape(n).contiguous()
+
+        out = torch.empty((rows, num_labels), dtype=dev.torch_dtype(kdt), device=v2.device)
+        if rows and num_labels:
+            strategy = _group_strategy()
+            ws_bytes = _lib.query(
+                "nbg_group_workspace_bytes", dev.nbg_code(v2), rows, n, num_labels, strategy
+            )
+            ws, ws_bytes = dev.workspace(ws_bytes, v2.device)
+            args = [
+                ctypes.c_int(dev.nbg_code(v2)),
+                dev.ptr(v2),
+                ctypes.c_int(dev.nbg_code(lab)),
+                dev.ptr(lab),
+                dev.ptr(out),
+                rows,
+                n,
+                ld_v,
+                int(num_labels),
+            ]
+            if self.supports_ddof:
+                args.append(int(ddof))
+            args += [ctypes.c_int(strategy), dev.ptr(ws), ws_bytes, dev.stream_handle()]
+            _lib.call(self.entry, *args)
+        res = out.view(*broadcast_shape, num_labels)
+        if kdt != result_dtype:
+            res = res.to(dev.torch_dtype(result_dtype))
+        return dev.to_output(res, as_tensor)
+
+    @staticmethod
+    def _kernel_value_dtype(dt: np.dtype) -> np.dtype:
+        """Value dtype the kernels run: float32/float64/int32/int64.
+
+        Narrow integers are widened to int32 (wrap-around sums and index outputs
+        are narrowed back afterwards, which reproduces the reference's modular
+        integer arithmetic); float16 runs in float32.
+        """
+        dt = np.dtype(dt)
+        if dt in (np.dtype(np.float32), np.dtype(np.float64), np.dtype(np.int32), np.dtype(np.int64)):
+            return dt
+        if dt == np.float16:
+            return np.dtype(np.float32)
+        if dt.kind in "iub" and dt.itemsize <= 4 and dt != np.uint32:
+            return np.dtype(np.int32)
+        if dt == np.uint32:
+            return np.dtype(np.int64)
+        raise TypeError(f"numbagg_amd: unsupported values dtype {dt}")
+

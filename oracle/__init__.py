@@ -1,0 +1,257 @@
+"""CPU oracle for numbagg's hot path — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this package, and only as the checker (or the timed CPU baseline). The
+numbagg_amd product never imports it.
+
+Two layers:
+
+* ``restated``: ctypes bindings of ``nbg_oracle.c``, a plain-C restatement of
+  the reference kernels with numba's accumulator types and operation order
+  (each C function cites the numbagg file:line it follows).
+* ``exact``: extended-precision / exact oracles used to state float error
+  bounds (long-double window and recurrence evaluation in C; NumPy closed forms
+  for fill and grouped reductions).
+
+All functions take (rows, n) NumPy arrays and return NumPy arrays.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+ORACLE_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "_build" / "libnbg_oracle.so"
+
+_lib: ctypes.CDLL | None = None
+
+
+def build() -> pathlib.Path:
+    """Compile the C oracle (make -C oracle)."""
+    proc = subprocess.run(["make", "-C", str(ORACLE_DIR)], capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{proc.stdout}\n{proc.stderr}")
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (ORACLE_DIR / "nbg_oracle.c").stat().st_mtime:
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+    return _lib
+
+
+_SFX = {np.dtype(np.float32): "f32", np.dtype(np.float64): "f64", np.dtype(np.int32): "i32", np.dtype(np.int64): "i64"}
+
+
+def _p(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _c2(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a)
+    if a.ndim == 1:
+        a = a[None, :]
+    assert a.ndim == 2
+    return a
+
+
+I64 = ctypes.c_int64
+
+
+class restated:
+    """The reference kernels restated in C (same types and operation order)."""
+
+    @staticmethod
+    def move(func: str, x: np.ndarray, window: int, min_count: int | None = None, y: np.ndarray | None = None):
+        x = _c2(x)
+        mc = window if min_count is None else min_count
+        out = np.empty_like(x)
+        rows, n = x.shape
+        sfx = _SFX[x.dtype]
+        fn = getattr(lib(), f"orc_{func}_{sfx}")
+        if y is None:
+            fn(_p(x), _p(out), I64(rows), I64(n), I64(n), I64(n), I64(window), I64(mc))
+        else:
+            y = _c2(y).astype(x.dtype, copy=False)
+            fn(_p(x), _p(y), _p(out), I64(rows), I64(n), I64(n), I64(n), I64(n), I64(window), I64(mc))
+        return out
+
+    @staticmethod
+    def move_exp(func: str, x: np.ndarray, alpha, min_weight: float = 0.0, y: np.ndarray | None = None):
+        """alpha: scalar, (n,) row shared by all rows, or (rows, n); in x's dtype."""
+        x = _c2(x)
+        rows, n = x.shape
+        al = np.asarray(alpha, dtype=x.dtype)
+        if al.ndim == 0:
+            al = np.full((1, n), al, dtype=x.dtype)
+            lda = 0
+        elif al.ndim == 1:
+            al = al[None, :]
+            lda = 0
+        else:
+            lda = n
+        al = np.ascontiguousarray(al)
+        out = np.empty_like(x)
+        sfx = _SFX[x.dtype]
+        fn = getattr(lib(), f"orc_{func}_{sfx}")
+        mw = ctypes.c_float(min_weight) if x.dtype == np.float32 else ctypes.c_double(min_weight)
+        if y is None:
+            fn(_p(x), _p(al), _p(out), I64(rows), I64(n), I64(n), I64(lda), I64(n), mw)
+        else:
+            y = _c2(y).astype(x.dtype, copy=False)
+            fn(_p(x), _p(y), _p(al), _p(out), I64(rows), I64(n), I64(n), I64(n), I64(lda), I64(n), mw)
+        return out
+
+    @staticmethod
+    def fill(func: str, x: np.ndarray, limit: int | None = None):
+        x = _c2(x)
+        rows, n = x.shape
+        lim = n if limit is None else limit
+        out = np.empty_like(x)
+        getattr(lib(), f"orc_{func}_{_SFX[x.dtype]}")(_p(x), _p(out), I64(rows), I64(n), I64(n), I64(n), I64(lim))
+        return out
+
+    @staticmethod
+    def group(func: str, values: np.ndarray, labels: np.ndarray, num_labels: int, ddof: int | None = None):
+        v = _c2(values)
+        lab = np.ascontiguousarray(labels, dtype=np.int64).ravel()
+        rows, n = v.shape
+        assert lab.shape == (n,)
+        out = np.empty((rows, num_labels), dtype=v.dtype)
+        fn = getattr(lib(), f"orc_{func}_{_SFX[v.dtype]}")
+        args = [_p(v), _p(lab), _p(out), I64(rows), I64(n), I64(n), I64(num_labels)]
+        if ddof is not None:
+            args.append(I64(ddof))
+        fn(*args)
+        return out
+
+
+_MOVE_KIND = {"move_sum": 0, "move_mean": 1, "move_var": 2, "move_std": 3, "move_cov": 4, "move_corr": 5}
+_EXP_KIND = {
+    "move_exp_nancount": 0,
+    "move_exp_nanmean": 1,
+    "move_exp_nansum": 2,
+    "move_exp_nanvar": 3,
+    "move_exp_nanstd": 4,
+    "move_exp_nancov": 5,
+    "move_exp_nancorr": 6,
+}
+
+
+class exact:
+    """Extended-precision oracles; results are float64."""
+
+    @staticmethod
+    def move(func: str, x: np.ndarray, window: int, min_count: int | None = None, y: np.ndarray | None = None):
+        x = _c2(x).astype(np.float64)
+        rows, n = x.shape
+        mc = window if min_count is None else min_count
+        out = np.empty_like(x)
+        yp = ctypes.c_void_p(0)
+        if y is not None:
+            y = _c2(y).astype(np.float64)
+            yp = _p(y)
+        lib().orc_exact_move(
+            ctypes.c_int(_MOVE_KIND[func]), _p(x), yp, _p(out), I64(rows), I64(n), I64(window), I64(mc)
+        )
+        return out
+
+    @staticmethod
+    def move_exp(func: str, x: np.ndarray, alpha, min_weight: float = 0.0, y: np.ndarray | None = None, loop_dtype=None):
+        """alpha/min_weight are rounded to ``loop_dtype`` (x's dtype by default) first."""
+        x0 = _c2(x)
+        loop = np.dtype(loop_dtype or x0.dtype)
+        x = x0.astype(np.float64)
+        rows, n = x.shape
+        al = np.asarray(np.asarray(alpha, dtype=loop), dtype=np.float64)
+        if al.ndim == 0:
+            al, lda = np.full((1, n), al), 0
+        elif al.ndim == 1:
+            al, lda = al[None, :], 0
+        else:
+            lda = n
+        al = np.ascontiguousarray(al)
+        mw = float(np.asarray(min_weight, dtype=loop))
+        out = np.empty_like(x)
+        yp = ctypes.c_void_p(0)
+        if y is not None:
+            y = _c2(y).astype(np.float64)
+            yp = _p(y)
+        lib().orc_exact_move_exp(
+            ctypes.c_int(_EXP_KIND[func]), _p(x), yp, _p(al), I64(lda), _p(out), I64(rows), I64(n),
+            ctypes.c_double(mw),
+        )
+        return out
+
+    @staticmethod
+    def fill(func: str, x: np.ndarray, limit: int | None = None) -> np.ndarray:
+        """Closed form: out[i] = x[j], j the last valid index at or before i in
+        scan order, when i - j <= limit; canonical NaN otherwise."""
+        x = _c2(x)
+        rows, n = x.shape
+        lim = n if limit is None else limit
+        xs = x[:, ::-1] if func == "bfill" else x
+        idx = np.arange(n)
+        valid = ~np.isnan(xs)
+        last = np.where(valid, idx[None, :], -1)
+        last = np.maximum.accumulate(last, axis=1)
+        ok = (last >= 0) & ((idx[None, :] - last) <= lim)
+        out = np.full_like(xs, np.nan)
+        rr = np.repeat(np.arange(rows)[:, None], n, axis=1)
+        out[ok] = xs[rr[ok], last[ok]]
+        return out[:, ::-1].copy() if func == "bfill" else out
+
+    @staticmethod
+    def group(func: str, values: np.ndarray, labels: np.ndarray, num_labels: int, ddof: int = 1) -> np.ndarray:
+        """float64 closed forms (two-pass variance) per (row, label)."""
+        v = _c2(values)
+        lab = np.asarray(labels, dtype=np.int64).ravel()
+        rows, n = v.shape
+        out = np.full((rows, num_labels), np.nan)
+        keep = (lab >= 0) & (lab < num_labels)
+        for r in range(rows):
+            vr = v[r].astype(np.float64)
+            ok = keep & ~np.isnan(vr)
+            lr, xr = lab[ok], vr[ok]
+            cnt = np.bincount(lr, minlength=num_labels).astype(np.float64)
+            if func == "group_nancount":
+                out[r] = cnt
+                continue
+            s = np.bincount(lr, weights=xr, minlength=num_labels)
+            if func == "group_nansum":
+                out[r] = s
+            elif func == "group_nanmean":
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    out[r] = np.where(cnt > 0, s / cnt, np.nan)
+            elif func in ("group_nanvar", "group_nanstd"):
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    m = s / np.where(cnt > 0, cnt, 1)
+                    m2 = np.bincount(lr, weights=(xr - m[lr]) ** 2, minlength=num_labels)
+                    den = cnt - ddof
+                    var = np.where(den > 0, m2 / np.where(den > 0, den, 1), np.nan)
+                out[r] = np.sqrt(var) if func == "group_nanstd" else var
+            elif func in ("group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast"):
+                pos = np.nonzero(ok)[0]
+                for k in range(num_labels):
+                    sel = pos[lr == k]
+                    if sel.size == 0:
+                        continue
+                    vals = vr[sel]
+                    if func == "group_nanargmax":
+                        out[r, k] = sel[np.argmax(vals)]
+                    elif func == "group_nanargmin":
+                        out[r, k] = sel[np.argmin(vals)]
+                    elif func == "group_nanfirst":
+                        out[r, k] = vr[sel[0]]
+                    else:
+                        out[r, k] = vr[sel[-1]]
+            else:
+                raise ValueError(func)
+        return out

@@ -1,0 +1,101 @@
+"""Generate the golden fixtures in tests/golden/ (run once; outputs are committed).
+
+The reference numbagg checks its hot path against pandas on fixed inputs
+(numbagg/test/conftest.py:230-550, test_moving.py:29-53, test_moving_exp.py:19-31,
+test_funcs.py:38-53, test_grouped.py:137-166). This script evaluates the same
+pandas expressions on the same deterministic inputs and stores inputs and
+outputs as plain arrays (no pickles):
+
+* array: np.random.RandomState(0).rand(3, 500) with values <= 0.1 set to NaN
+  (conftest.py:581-592), and (1, 500) for the grouped comparisons;
+* two-array functions use (a, a**2 + 1) (conftest.py:73-77);
+* grouped labels: np.random.seed(0); randint(0, 12) (conftest.py:164-172).
+
+Usage: python tests/golden/make_golden.py
+"""
+
+from __future__ import annotations
+
+import pathlib
+import warnings
+
+import numpy as np
+import pandas as pd
+
+OUT = pathlib.Path(__file__).resolve().parent / "pandas_fixtures.npz"
+
+
+def fixture_array(shape):
+    a = np.random.RandomState(0).rand(*shape)
+    return np.where(a > 0.1, a, np.nan)
+
+
+def df_of(a):
+    return pd.DataFrame(a).T if a.ndim == 2 else pd.DataFrame(a)
+
+
+def main() -> None:
+    res: dict[str, np.ndarray] = {}
+    a = fixture_array((3, 500))
+    res["array_3x500"] = a
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        a2 = a**2 + 1
+    res["array2_3x500"] = a2
+
+    # rolling: windows {10, 50} x min_count {None, 0, 1, 3, window}
+    for window in (10, 50):
+        for mc in (None, 0, 1, 3, "window"):
+            m = window if mc == "window" else mc
+            tag = f"w{window}_mc{'None' if mc is None else m}"
+            r = df_of(a).rolling(window=window, min_periods=m)
+            res[f"move_mean_{tag}"] = r.mean().T.values
+            res[f"move_sum_{tag}"] = r.sum().T.values
+            res[f"move_std_{tag}"] = r.std().T.values
+            res[f"move_var_{tag}"] = r.var().T.values
+            res[f"move_cov_{tag}"] = r.cov(df_of(a2)).T.values
+            res[f"move_corr_{tag}"] = r.corr(df_of(a2)).T.values
+
+    # exponentially weighted: alpha {0.5, 0.1}
+    for alpha in (0.5, 0.1):
+        tag = f"a{alpha}"
+        e = df_of(a).ewm(alpha=alpha)
+        res[f"move_exp_nanmean_{tag}"] = e.mean().T.values
+        res[f"move_exp_nanvar_{tag}"] = e.var().T.values
+        res[f"move_exp_nanstd_{tag}"] = e.std().T.values
+        res[f"move_exp_nansum_{tag}"] = e.sum().T.values
+        res[f"move_exp_nancount_{tag}"] = df_of(a).notnull().ewm(alpha=alpha).sum().T.values
+        res[f"move_exp_nancorr_{tag}"] = e.corr(df_of(a2)).T.values
+        res[f"move_exp_nancov_{tag}"] = e.cov(df_of(a2)).T.values
+
+    # fill: limit {1, 3, None}
+    for limit in (1, 3, None):
+        res[f"ffill_l{limit}"] = df_of(a).ffill(limit=limit).T.values
+        res[f"bfill_l{limit}"] = df_of(a).bfill(limit=limit).T.values
+
+    # grouped: (1, 500) array, labels randint(0, 12) seeded with 0
+    g = fixture_array((1, 500))
+    res["group_array_1x500"] = g
+    np.random.seed(0)
+    labels = np.random.randint(0, 12, size=500)
+    res["group_labels"] = labels
+    codes, uniques = pd.factorize(labels, sort=True)
+    res["group_codes"] = codes
+    gb = df_of(g).groupby(labels)
+    res["group_nansum"] = gb.sum().T.values
+    res["group_nanmean"] = gb.mean().T.values
+    res["group_nancount"] = gb.count().T.values
+    res["group_nanfirst"] = gb.first().T.values
+    res["group_nanlast"] = gb.last().T.values
+    res["group_nanargmax"] = gb.idxmax().T.values.astype(np.float64)
+    res["group_nanargmin"] = gb.idxmin().T.values.astype(np.float64)
+    for ddof in (0, 1, 5):
+        res[f"group_nanvar_ddof{ddof}"] = gb.var(ddof=ddof).T.values
+        res[f"group_nanstd_ddof{ddof}"] = gb.std(ddof=ddof).T.values
+
+    np.savez_compressed(OUT, **res)
+    print(f"wrote {OUT} ({len(res)} arrays, pandas {pd.__version__})")
+
+
+if __name__ == "__main__":
+    main()

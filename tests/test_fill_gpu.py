@@ -1,0 +1,81 @@
+"""GPU parity of ffill/bfill: bit-exact against the oracle (closed form and the
+reference restatement), through the public API and the C ABI."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from ._data import bits as _bits
+from ._data import runs_array
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("func", ["ffill", "bfill"])
+@pytest.mark.parametrize("limit", [None, 0, 1, 16, 5000])
+@pytest.mark.parametrize("shape", [(3, 500), (7, 4097), (2, 50_001), (1, 200_000), (300, 333)])
+def test_fill_bit_exact(gpu_lib, dtype, func, limit, shape):
+    import numbagg_amd as nb
+    from oracle import exact, restated
+
+    x = runs_array(*shape, seed=shape[1], dtype=dtype)
+    got = getattr(nb, func)(x, limit=limit)
+    assert got.dtype == x.dtype and got.shape == x.shape
+    np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
+    np.testing.assert_array_equal(_bits(got), _bits(restated.fill(func, x, limit)))
+
+
+@pytest.mark.parametrize("func", ["ffill", "bfill"])
+def test_fill_long_nan_runs_cross_many_tiles(gpu_lib, func):
+    # All-NaN stretches longer than several tiles force the look-back to walk
+    # past tiles that hold no valid value.
+    import numbagg_amd as nb
+    from oracle import exact
+
+    x = np.full((4, 60_000), np.nan, dtype=np.float32)
+    x[0, 5] = 1.5
+    x[1, 59_990] = -2.0
+    x[2, 30_000] = 3.0
+    for limit in (None, 10_000, 40_000):
+        got = getattr(nb, func)(x, limit=limit)
+        np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
+
+
+def test_fill_reference_pandas_vectors(gpu_lib, golden):
+    import numbagg_amd as nb
+
+    a = golden["array_3x500"]
+    for func in ("ffill", "bfill"):
+        for limit in (1, 3, None):
+            np.testing.assert_array_equal(getattr(nb, func)(a, limit=limit), golden[f"{func}_l{limit}"])
+
+
+def test_fill_axes_strides_and_dtypes(gpu_lib):
+    import torch
+
+    import numbagg_amd as nb
+    from oracle import exact
+
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((6, 7, 900))
+    x[rng.random(x.shape) < 0.4] = np.nan
+    for axis in (0, 1, 2, -1):
+        got = nb.ffill(x, limit=3, axis=axis)
+        xm = np.moveaxis(x, axis, -1)
+        want = exact.fill("ffill", xm.reshape(-1, xm.shape[-1]), 3).reshape(xm.shape)
+        np.testing.assert_array_equal(got, np.moveaxis(want, -1, axis))
+    # non-contiguous / unaligned rows on the device (scalar-load kernel path)
+    t = torch.from_numpy(x.astype(np.float32)).cuda()[:, :, 1:]
+    got = nb.bfill(t, limit=2).cpu().numpy()
+    want = exact.fill("bfill", x.astype(np.float32)[:, :, 1:].reshape(-1, 899), 2).reshape(6, 7, 899)
+    np.testing.assert_array_equal(got, want)
+    # float16 goes through float32 exactly; integers are copied through
+    h = x.astype(np.float16)
+    got = nb.ffill(h)
+    assert got.dtype == np.float16
+    np.testing.assert_array_equal(got, exact.fill("ffill", h.reshape(-1, 900).astype(np.float32)).reshape(h.shape).astype(np.float16))
+    i = rng.integers(-5, 5, size=(4, 33)).astype(np.int32)
+    np.testing.assert_array_equal(nb.bfill(i, limit=1), i)
+
