@@ -61,6 +61,87 @@ int nbg_bfill(int dtype, const void* x, void* out, int64_t rows, int64_t n,
               int64_t ld_x, int64_t ld_out, int64_t limit, void* ws,
               size_t ws_bytes, void* stream);
 
+/* ---- moving windows (numbagg/moving.py:12-275; ndmove.__call__ at
+ *      numbagg/decorators.py:305-341) ------------------------------------- */
+/* out[i] = statistic of the valid inputs in [i-window+1, i] (pairwise-complete
+ * for cov/corr), NaN when fewer than the effective min_count (sum: min_count;
+ * mean/corr: max(min_count,1); std/var/cov: max(min_count,2)).
+ * dtype: NBG_F32 or NBG_F64 for inputs and output; 0 < window <= n. */
+int nbg_move_sum(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                 int64_t ld_x, int64_t ld_out, int64_t window,
+                 int64_t min_count, void* stream);     /* moving.py:58-99 */
+int nbg_move_mean(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                  int64_t ld_x, int64_t ld_out, int64_t window,
+                  int64_t min_count, void* stream);    /* moving.py:12-55 */
+int nbg_move_std(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                 int64_t ld_x, int64_t ld_out, int64_t window,
+                 int64_t min_count, void* stream);     /* moving.py:123-151 */
+int nbg_move_var(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                 int64_t ld_x, int64_t ld_out, int64_t window,
+                 int64_t min_count, void* stream);     /* moving.py:154-181 */
+int nbg_move_cov(int dtype, const void* a, const void* b, void* out,
+                 int64_t rows, int64_t n, int64_t ld_a, int64_t ld_b,
+                 int64_t ld_out, int64_t window, int64_t min_count,
+                 void* stream);                        /* moving.py:184-220 */
+int nbg_move_corr(int dtype, const void* a, const void* b, void* out,
+                  int64_t rows, int64_t n, int64_t ld_a, int64_t ld_b,
+                  int64_t ld_out, int64_t window, int64_t min_count,
+                  void* stream);                       /* moving.py:223-275 */
+
+/* ---- exponentially weighted windows (numbagg/moving_exp.py:12-335;
+ *      ndmoveexp.__call__ at numbagg/decorators.py:374-414) --------------- */
+/* State is float64, as in the reference. `dtype` is the loop dtype (NBG_F32 or
+ * NBG_F64): alpha values, min_weight comparison and output use it. `x_dtype`
+ * is the input dtype: equal to dtype, or NBG_F32 with dtype NBG_F64 (float32
+ * data on the float64 loop, widened in the kernel, which is what NumPy's loop
+ * selection does for float32 data with a Python-float alpha).
+ * alpha: NULL -> the scalar alpha_scalar for every position; otherwise an
+ * array in the loop dtype with row stride ld_alpha (0 = one row shared by all
+ * rows). */
+size_t nbg_move_exp_workspace_bytes(int64_t rows, int64_t n, int x_dtype);
+int nbg_move_exp_nancount(int x_dtype, int dtype, const void* x,
+                          const void* alpha, int64_t ld_alpha,
+                          double alpha_scalar, void* out, int64_t rows,
+                          int64_t n, int64_t ld_x, int64_t ld_out,
+                          double min_weight, void* ws, size_t ws_bytes,
+                          void* stream); /* moving_exp.py:12-38 */
+int nbg_move_exp_nanmean(int x_dtype, int dtype, const void* x,
+                         const void* alpha, int64_t ld_alpha,
+                         double alpha_scalar, void* out, int64_t rows,
+                         int64_t n, int64_t ld_x, int64_t ld_out,
+                         double min_weight, void* ws, size_t ws_bytes,
+                         void* stream); /* moving_exp.py:41-72 */
+int nbg_move_exp_nansum(int x_dtype, int dtype, const void* x,
+                        const void* alpha, int64_t ld_alpha,
+                        double alpha_scalar, void* out, int64_t rows,
+                        int64_t n, int64_t ld_x, int64_t ld_out,
+                        double min_weight, void* ws, size_t ws_bytes,
+                        void* stream); /* moving_exp.py:75-103 */
+int nbg_move_exp_nanvar(int x_dtype, int dtype, const void* x,
+                        const void* alpha, int64_t ld_alpha,
+                        double alpha_scalar, void* out, int64_t rows,
+                        int64_t n, int64_t ld_x, int64_t ld_out,
+                        double min_weight, void* ws, size_t ws_bytes,
+                        void* stream); /* moving_exp.py:106-156 */
+int nbg_move_exp_nanstd(int x_dtype, int dtype, const void* x,
+                        const void* alpha, int64_t ld_alpha,
+                        double alpha_scalar, void* out, int64_t rows,
+                        int64_t n, int64_t ld_x, int64_t ld_out,
+                        double min_weight, void* ws, size_t ws_bytes,
+                        void* stream); /* moving_exp.py:159-224 */
+int nbg_move_exp_nancov(int x_dtype, int dtype, const void* a, const void* b,
+                        const void* alpha, int64_t ld_alpha,
+                        double alpha_scalar, void* out, int64_t rows,
+                        int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out,
+                        double min_weight, void* ws, size_t ws_bytes,
+                        void* stream); /* moving_exp.py:227-273 */
+int nbg_move_exp_nancorr(int x_dtype, int dtype, const void* a, const void* b,
+                         const void* alpha, int64_t ld_alpha,
+                         double alpha_scalar, void* out, int64_t rows,
+                         int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out,
+                         double min_weight, void* ws, size_t ws_bytes,
+                         void* stream); /* moving_exp.py:276-335 */
+
 #ifdef __cplusplus
 }
 #endif

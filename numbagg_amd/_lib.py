@@ -40,6 +40,31 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     ),
 }
 
+_MOVE1 = (_c_int, [_c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_ptr])
+_MOVE2 = (
+    _c_int,
+    [_c_int, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_ptr],
+)
+for _name in ("nbg_move_sum", "nbg_move_mean", "nbg_move_std", "nbg_move_var"):
+    _SIGNATURES[_name] = _MOVE1
+for _name in ("nbg_move_cov", "nbg_move_corr"):
+    _SIGNATURES[_name] = _MOVE2
+_SIGNATURES["nbg_move_exp_workspace_bytes"] = (_c_size, [_c_i64, _c_i64, _c_int])
+_EXP1 = (
+    _c_int,
+    [_c_int, _c_int, _c_ptr, _c_ptr, _c_i64, _c_dbl, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_dbl,
+     _c_ptr, _c_size, _c_ptr],
+)
+_EXP2 = (
+    _c_int,
+    [_c_int, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_dbl, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64,
+     _c_i64, _c_dbl, _c_ptr, _c_size, _c_ptr],
+)
+for _name in ("nancount", "nanmean", "nansum", "nanvar", "nanstd"):
+    _SIGNATURES[f"nbg_move_exp_{_name}"] = _EXP1
+for _name in ("nancov", "nancorr"):
+    _SIGNATURES[f"nbg_move_exp_{_name}"] = _EXP2
+
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
 

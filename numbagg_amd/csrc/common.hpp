@@ -131,6 +131,10 @@ constexpr unsigned int kSpinLimit = 1u << 24;
 
 __device__ __forceinline__ void spin_pause() { __builtin_amdgcn_s_sleep(1); }
 
+// LDS index with one padding slot per 16 elements: threads that each own 16
+// consecutive elements then hit distinct banks on every strided access.
+__host__ __device__ __forceinline__ int64_t pad16(int64_t e) { return e + (e >> 4); }
+
 // Zero-fill kernel used to reset look-back status words and ticket counters.
 // (A kernel rather than a memset node so the call stays graph-capture safe.)
 __global__ void zero_words_kernel(unsigned long long* p, int64_t nwords);

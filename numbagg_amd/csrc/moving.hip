@@ -1,0 +1,576 @@
+// Moving-window statistics (move_sum / mean / std / var / cov / corr) for gfx950.
+//
+// Reference semantics (numbagg/moving.py:12-275): out[i] is a statistic of the
+// valid observations in the window [i-window+1, i] (pairwise-complete for the
+// two-input kernels), or NaN when their count is below the effective
+// min_count (sum: min_count; mean, corr: max(min_count, 1); std, var, cov:
+// max(min_count, 2)). The reference carries running float64 sums and adds /
+// removes one observation per step; here every window is evaluated from
+// compensated sums, so results sit closer to the exact value while the NaN
+// mask (a pure function of the valid counts) is the same.
+//
+// Design (no cross-workgroup communication): a workgroup owns an output tile
+// [t0, t0+T) of one row, T = 256 threads x K items. It stages the input
+// segment [t0-window, t0+T) in LDS (padded one slot per 16 to keep the
+// per-thread strided reads conflict-free). Thread t owns outputs
+// [t0+tK, t0+tK+K):
+//   1. delta_t = stats(head chunk) - stats(tail chunk), where the tail chunk is
+//      the K inputs `window` positions earlier;
+//   2. W0 = stats of the halo [t0-window, t0) (block reduction) = the window
+//      ending at t0-1;
+//   3. an exclusive block scan of delta_t gives each thread the window ending
+//      just before its chunk; K sequential add/remove steps produce its outputs.
+// For windows too long for LDS the tail and halo are read from global memory.
+//
+// Accumulators: first moments of float32 data for move_sum/mean are FP32
+// TwoSum pairs; the moment kernels (var/std/cov/corr) work on data shifted by a
+// per-tile constant with float64 first and second moments; float64 data uses
+// double-double throughout (TwoProd for squares and cross products).
+#include "common.hpp"
+
+namespace nbg {
+namespace {
+
+constexpr int kMoveThreads = 256;
+
+enum MoveKind { kSum = 0, kMean = 1, kVar = 2, kStd = 3, kCov = 4, kCorr = 5 };
+
+// ---- scalar accumulators ---------------------------------------------------
+// Plain float64 accumulator with the same interface as Comp<F>.
+struct D {
+  double v;
+  __device__ __forceinline__ static D zero() { return D{0.0}; }
+  __device__ __forceinline__ void add(double x) { v += x; }
+  __device__ __forceinline__ void add(const D& o) { v += o.v; }
+  __device__ __forceinline__ void sub(const D& o) { v -= o.v; }
+  __device__ __forceinline__ double to_double() const { return v; }
+};
+
+__device__ __forceinline__ D shfl_up_acc(const D& a, int d) { return D{shfl_up(a.v, d)}; }
+template <typename F>
+__device__ __forceinline__ Comp<F> shfl_up_acc(const Comp<F>& a, int d) {
+  return Comp<F>{shfl_up(a.hi, d), shfl_up(a.lo, d)};
+}
+
+// a*b added to an accumulator: exact product via fma for double-double.
+__device__ __forceinline__ void add_prod(D& acc, double a, double b) { acc.v += a * b; }
+__device__ __forceinline__ void add_prod(Comp<double>& acc, double a, double b) {
+  double p = a * b;
+  double e = fma(a, b, -p);
+  acc.add(p);
+  acc.add(e);
+}
+__device__ __forceinline__ void sub_prod(D& acc, double a, double b) { acc.v -= a * b; }
+__device__ __forceinline__ void sub_prod(Comp<double>& acc, double a, double b) {
+  double p = a * b;
+  double e = fma(a, b, -p);
+  acc.add(-p);
+  acc.add(-e);
+}
+
+// ---- per-kind window statistics ---------------------------------------------
+template <typename T, int KIND>
+struct Traits;
+
+// First-moment-only kinds (sum, mean).
+template <typename T>
+struct SumStats {
+  using A = Comp<T>;  // FP32 TwoSum pair for float32, double-double for float64
+  int c;
+  A s;
+  __device__ __forceinline__ static SumStats zero() { return SumStats{0, A::zero()}; }
+  __device__ __forceinline__ void add_obs(T a, T, T, T) {
+    c += 1;
+    s.add(a);
+  }
+  __device__ __forceinline__ void sub_obs(T a, T, T, T) {
+    c -= 1;
+    s.add(-a);
+  }
+  __device__ __forceinline__ void merge(const SumStats& o) {
+    c += o.c;
+    s.add(o.s);
+  }
+  __device__ __forceinline__ void unmerge(const SumStats& o) {
+    c -= o.c;
+    s.sub(o.s);
+  }
+  __device__ __forceinline__ SumStats up(int d) const {
+    return SumStats{shfl_up(c, d), shfl_up_acc(s, d)};
+  }
+};
+
+// Moment kinds on shifted data: y = x - k.
+template <typename T, int NM>  // NM: 1 = (s1, s11), 2 = (s1, s2, s12[, s11, s22])
+struct MomStats {
+  using A = typename std::conditional<sizeof(T) == 4, D, Comp<double>>::type;
+  int c;
+  A s1, s2, s11, s22, s12;
+  __device__ __forceinline__ static MomStats zero() {
+    return MomStats{0, A::zero(), A::zero(), A::zero(), A::zero(), A::zero()};
+  }
+  // a, b: observations; ka, kb: shifts.
+  __device__ __forceinline__ void add_obs(T a, T b, T ka, T kb) {
+    const double ya = (double)a - (double)ka;
+    c += 1;
+    s1.add(ya);
+    add_prod(s11, ya, ya);
+    if (NM == 2) {
+      const double yb = (double)b - (double)kb;
+      s2.add(yb);
+      add_prod(s12, ya, yb);
+      add_prod(s22, yb, yb);
+    }
+  }
+  __device__ __forceinline__ void sub_obs(T a, T b, T ka, T kb) {
+    const double ya = (double)a - (double)ka;
+    c -= 1;
+    s1.add(-ya);
+    sub_prod(s11, ya, ya);
+    if (NM == 2) {
+      const double yb = (double)b - (double)kb;
+      s2.add(-yb);
+      sub_prod(s12, ya, yb);
+      sub_prod(s22, yb, yb);
+    }
+  }
+  __device__ __forceinline__ void merge(const MomStats& o) {
+    c += o.c;
+    s1.add(o.s1);
+    s11.add(o.s11);
+    if (NM == 2) {
+      s2.add(o.s2);
+      s12.add(o.s12);
+      s22.add(o.s22);
+    }
+  }
+  __device__ __forceinline__ void unmerge(const MomStats& o) {
+    c -= o.c;
+    s1.sub(o.s1);
+    s11.sub(o.s11);
+    if (NM == 2) {
+      s2.sub(o.s2);
+      s12.sub(o.s12);
+      s22.sub(o.s22);
+    }
+  }
+  __device__ __forceinline__ MomStats up(int d) const {
+    MomStats r;
+    r.c = shfl_up(c, d);
+    r.s1 = shfl_up_acc(s1, d);
+    r.s11 = shfl_up_acc(s11, d);
+    if (NM == 2) {
+      r.s2 = shfl_up_acc(s2, d);
+      r.s12 = shfl_up_acc(s12, d);
+      r.s22 = shfl_up_acc(s22, d);
+    } else {
+      r.s2 = r.s12 = r.s22 = A::zero();
+    }
+    return r;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T mean_of(const Comp<T>& s, int c);
+template <>
+__device__ __forceinline__ float mean_of<float>(const Comp<float>& s, int c) {
+  return (float)(s.to_double() / (double)c);
+}
+template <>
+__device__ __forceinline__ double mean_of<double>(const Comp<double>& s, int c) {
+  const double dc = (double)c;
+  const double q = s.hi / dc;
+  const double r = fma(-q, dc, s.hi) + s.lo;  // exact remainder of hi, plus lo
+  return q + r / dc;
+}
+
+template <typename T>
+struct Traits<T, kSum> {
+  using S = SumStats<T>;
+  static constexpr int NARGS = 1;
+  static constexpr bool SHIFT = false;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < mc) return canonical_nan<T>();
+    return (T)s.s.to_double();
+  }
+};
+template <typename T>
+struct Traits<T, kMean> {
+  using S = SumStats<T>;
+  static constexpr int NARGS = 1;
+  static constexpr bool SHIFT = false;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
+    return mean_of<T>(s.s, s.c);
+  }
+};
+template <typename T, int KIND>
+struct VarTraits {
+  using S = MomStats<T, 1>;
+  static constexpr int NARGS = 1;
+  static constexpr bool SHIFT = true;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
+    const double c = (double)s.c;
+    const double m1 = s.s1.to_double();
+    const double var = (s.s11.to_double() - m1 * m1 / c) / (c - 1.0);
+    return (T)(KIND == kStd ? sqrt(var) : var);
+  }
+};
+template <typename T>
+struct Traits<T, kVar> : VarTraits<T, kVar> {};
+template <typename T>
+struct Traits<T, kStd> : VarTraits<T, kStd> {};
+template <typename T>
+struct Traits<T, kCov> {
+  using S = MomStats<T, 2>;
+  static constexpr int NARGS = 2;
+  static constexpr bool SHIFT = true;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
+    const double c = (double)s.c;
+    return (T)((s.s12.to_double() - s.s1.to_double() * s.s2.to_double() / c) / (c - 1.0));
+  }
+};
+template <typename T>
+struct Traits<T, kCorr> {
+  using S = MomStats<T, 2>;
+  static constexpr int NARGS = 2;
+  static constexpr bool SHIFT = true;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
+    const double r = 1.0 / (double)s.c;
+    const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
+    const double va = s.s11.to_double() * r - ma * ma;
+    const double vb = s.s22.to_double() * r - mb * mb;
+    const double cv = s.s12.to_double() * r - ma * mb;
+    const double vv = va * vb;
+    return vv > 0 ? (T)(cv / sqrt(vv)) : canonical_nan<T>();
+  }
+};
+
+// ---- block-level helpers ------------------------------------------------------
+constexpr int kMoveWaves = kMoveThreads / kWave;
+
+template <typename S>
+struct BlockShared {
+  S wave_tot[kMoveWaves];
+};
+
+// Inclusive wave scan of S (sum-like merge).
+template <typename S>
+__device__ __forceinline__ S wave_incl_scan(S v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    S o = v.up(d);
+    if (lane >= d) {
+      o.merge(v);
+      v = o;
+    }
+  }
+  return v;
+}
+
+// Exclusive block scan; also returns the block total.
+template <typename S>
+__device__ __forceinline__ S block_excl_scan(S v, S& total, BlockShared<S>& sh) {
+  S incl = wave_incl_scan(v);
+  if (lane_id() == kWave - 1) sh.wave_tot[wave_id()] = incl;
+  __syncthreads();
+  S prefix = S::zero();
+  total = S::zero();
+#pragma unroll
+  for (int w = 0; w < kMoveWaves; ++w) {
+    if (w < wave_id()) prefix.merge(sh.wave_tot[w]);
+    total.merge(sh.wave_tot[w]);
+  }
+  __syncthreads();
+  // exclusive = prefix + (incl - own)
+  S excl = incl;
+  excl.unmerge(v);
+  prefix.merge(excl);
+  return prefix;
+}
+
+template <typename S>
+__device__ __forceinline__ S block_reduce(S v, BlockShared<S>& sh) {
+  S total;
+  (void)block_excl_scan(v, total, sh);
+  return total;
+}
+
+
+template <typename T>
+struct alignas(16) Vec16 {
+  T v[16 / sizeof(T)];
+};
+
+// ---- the kernel -------------------------------------------------------------------
+template <typename T, int KIND, bool TAIL_LDS, int VEC>
+__global__ __launch_bounds__(kMoveThreads) void move_kernel(
+    const T* __restrict__ xa, const T* __restrict__ xb, T* __restrict__ out, int64_t rows,
+    int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
+    int64_t tiles_per_row) {
+  using Tr = Traits<T, KIND>;
+  using S = typename Tr::S;
+  constexpr int K = 64 / (int)sizeof(T);  // 16 float32 / 8 float64 outputs per thread
+  constexpr int TILE = kMoveThreads * K;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ BlockShared<S> sh;
+  __shared__ T s_shift[2];
+
+  // Consecutive blocks walk consecutive tiles of a row (neighbouring tiles
+  // share their halo through L2).
+  const int64_t row = (int64_t)(blockIdx.x / (uint64_t)tiles_per_row);
+  const int64_t tile = (int64_t)(blockIdx.x % (uint64_t)tiles_per_row);
+  const int64_t t0 = tile * TILE;
+  const T* ra = xa + row * ld_a;
+  const T* rb = (Tr::NARGS == 2) ? xb + row * ld_b : nullptr;
+  T* ro = out + row * ld_out;
+  const int tid = threadIdx.x;
+
+  // Segment staged in LDS: [s0a, t0 + TILE) (TAIL_LDS) or [t0, t0 + TILE).
+  const int64_t seg_begin = TAIL_LDS ? (t0 - window > 0 ? t0 - window : 0) : t0;
+  const int64_t s0a = seg_begin & ~(int64_t)(VEC - 1);
+  const int64_t seg_len = t0 + TILE - s0a;
+  const int64_t lds_elems = pad16(seg_len) + 16;
+  T* la = reinterpret_cast<T*>(smem_raw);
+  T* lb = la + lds_elems;
+
+  // 1. stage the segment (wave-striped vector loads, NaN past the row end).
+  for (int64_t e = (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
+    const int64_t p = s0a + e;
+    if (VEC > 1 && p + VEC <= n) {
+      Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(ra + p);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
+      if (Tr::NARGS == 2) {
+        Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rb + p);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const int64_t q = p + v;
+        la[pad16(e + v)] = (q < n) ? ra[q] : canonical_nan<T>();
+        if (Tr::NARGS == 2) lb[pad16(e + v)] = (q < n) ? rb[q] : canonical_nan<T>();
+      }
+    }
+  }
+  if (Tr::SHIFT && tid < 2) s_shift[tid] = T(0);
+  __syncthreads();
+
+  // Per-tile shifts for the moment kinds: the first valid value among the first
+  // 256 staged inputs (any constant works; one near the data keeps the float64
+  // moments small).
+  T ka = T(0), kb = T(0);
+  if (Tr::SHIFT) {
+    __shared__ int s_first[2];
+    if (tid < 2) s_first[tid] = 0x7fffffff;
+    __syncthreads();
+    if (tid < seg_len) {
+      if (!is_nan(la[pad16(tid)])) atomicMin(&s_first[0], tid);
+      if (Tr::NARGS == 2 && !is_nan(lb[pad16(tid)])) atomicMin(&s_first[1], tid);
+    }
+    __syncthreads();
+    ka = s_first[0] != 0x7fffffff ? la[pad16(s_first[0])] : T(0);
+    if (Tr::NARGS == 2) kb = s_first[1] != 0x7fffffff ? lb[pad16(s_first[1])] : T(0);
+  }
+
+  auto valid = [&](T a, T b) -> bool {
+    return (Tr::NARGS == 2) ? !(is_nan(a) || is_nan(b)) : !is_nan(a);
+  };
+
+  // 2. halo statistics W0 = window ending at t0-1.
+  S w0 = S::zero();
+  {
+    const int64_t h_begin = t0 - window > 0 ? t0 - window : 0;
+    for (int64_t p = h_begin + tid; p < t0; p += kMoveThreads) {
+      T a, b = T(0);
+      if (TAIL_LDS) {
+        a = la[pad16(p - s0a)];
+        if (Tr::NARGS == 2) b = lb[pad16(p - s0a)];
+      } else {
+        a = ra[p];
+        if (Tr::NARGS == 2) b = rb[p];
+      }
+      if (valid(a, b)) w0.add_obs(a, b, ka, kb);
+    }
+  }
+  w0 = block_reduce(w0, sh);
+
+  // 3. head/tail chunks and the per-thread delta. Chunks are re-read from LDS
+  //    (or global memory for long windows) rather than held in registers.
+  const int64_t c0 = t0 + (int64_t)tid * K;  // first output position of this thread
+  const int64_t eh = TAIL_LDS ? c0 - s0a : (int64_t)tid * K;  // its LDS element index
+  auto head = [&](int j, T& a, T& b) {
+    a = la[pad16(eh + j)];
+    b = (Tr::NARGS == 2) ? lb[pad16(eh + j)] : T(0);
+  };
+  auto tail = [&](int j, T& a, T& b) {
+    const int64_t pt = c0 + j - window;
+    if (pt < 0 || pt >= n) {
+      a = canonical_nan<T>();
+      b = canonical_nan<T>();
+    } else if (TAIL_LDS) {
+      a = la[pad16(pt - s0a)];
+      b = (Tr::NARGS == 2) ? lb[pad16(pt - s0a)] : T(0);
+    } else {
+      a = ra[pt];
+      b = (Tr::NARGS == 2) ? rb[pt] : T(0);
+    }
+  };
+  S delta = S::zero();
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    T a, b;
+    head(j, a, b);
+    if (valid(a, b)) delta.add_obs(a, b, ka, kb);
+    tail(j, a, b);
+    if (valid(a, b)) delta.sub_obs(a, b, ka, kb);
+  }
+  S total;
+  S st = block_excl_scan(delta, total, sh);
+  st.merge(w0);  // window ending at c0 - 1
+
+  // 4. sequential add/remove over the thread's chunk.
+  T res[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    T a, b;
+    head(j, a, b);
+    if (valid(a, b)) st.add_obs(a, b, ka, kb);
+    tail(j, a, b);
+    if (valid(a, b)) st.sub_obs(a, b, ka, kb);
+    res[j] = Tr::out(st, min_count);
+  }
+  if (c0 + K <= n && VEC > 1) {
+#pragma unroll
+    for (int j = 0; j < K; j += VEC) {
+      Vec16<T> vo;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) vo.v[v] = res[j + v];
+      *reinterpret_cast<Vec16<T>*>(ro + c0 + j) = vo;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (c0 + j < n) ro[c0 + j] = res[j];
+  }
+}
+
+// Largest window whose halo is staged in LDS: the padded segment of
+// TILE + window elements per input must fit a 60 KB budget.
+template <typename T, int NARGS>
+constexpr int64_t lds_window_limit() {
+  constexpr int K = 64 / (int)sizeof(T);
+  constexpr int64_t budget = 60 * 1024;
+  return budget * 16 / (17 * (int64_t)sizeof(T) * NARGS) - kMoveThreads * K - 64;
+}
+
+template <typename T, int KIND>
+int move_launch(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64_t ld_a,
+                int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count, hipStream_t s) {
+  using Tr = Traits<T, KIND>;
+  constexpr int K = 64 / (int)sizeof(T);
+  constexpr int TILE = kMoveThreads * K;
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  const int64_t blocks = rows * tiles;
+  if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move: too many tiles");
+  const bool tail_lds = window <= lds_window_limit<T, Tr::NARGS>();
+  int vec = pick_vec<T>(a, ld_a, rows);
+  if (Tr::NARGS == 2) {
+    int vb = pick_vec<T>(b, ld_b, rows);
+    vec = vb < vec ? vb : vec;
+  }
+  {
+    int vo = pick_vec<T>(out, ld_out, rows);
+    vec = vo < vec ? vo : vec;
+  }
+  constexpr int MAXV = 16 / (int)sizeof(T);
+  const int v_use = (vec == MAXV) ? MAXV : 1;
+  const int64_t seg_len = tail_lds ? (TILE + window + MAXV) : TILE;
+  const size_t lds = (size_t)(pad16(seg_len) + 16) * sizeof(T) * Tr::NARGS;
+  dim3 grid((unsigned)blocks), block(kMoveThreads);
+#define NBG_MOVE_LAUNCH(TL, V)                                                                   \
+  hipLaunchKernelGGL((move_kernel<T, KIND, TL, V>), grid, block, lds, s, a, b, out, rows, n,     \
+                     ld_a, ld_b, ld_out, window, min_count, tiles)
+  if (tail_lds) {
+    if (v_use == MAXV) NBG_MOVE_LAUNCH(true, MAXV);
+    else NBG_MOVE_LAUNCH(true, 1);
+  } else {
+    if (v_use == MAXV) NBG_MOVE_LAUNCH(false, MAXV);
+    else NBG_MOVE_LAUNCH(false, 1);
+  }
+#undef NBG_MOVE_LAUNCH
+  return check_hip(hipGetLastError(), "move_kernel launch");
+}
+
+template <int KIND>
+int move_entry(const char* name, int dtype, const void* a, const void* b, void* out, int64_t rows,
+               int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window,
+               int64_t min_count, void* stream) {
+  if (int rc = injected_error(name)) return rc;
+  if (rows < 0 || n < 0) return set_error(NBG_ERR_INVALID, "%s: negative shape", name);
+  if (min_count < 0) return set_error(NBG_ERR_INVALID, "%s: min_count must be positive: %lld", name, (long long)min_count);
+  if (!(window > 0 && window <= n)) return set_error(NBG_ERR_INVALID, "%s: window not in valid range: %lld", name, (long long)window);
+  if (rows == 0 || n == 0) return NBG_OK;
+  constexpr int NARGS = (KIND == kCov || KIND == kCorr) ? 2 : 1;
+  if (a == nullptr || out == nullptr || (NARGS == 2 && b == nullptr))
+    return set_error(NBG_ERR_INVALID, "%s: null pointer", name);
+  if (rows > 1 && (ld_a < n || ld_out < n || (NARGS == 2 && ld_b < n)))
+    return set_error(NBG_ERR_INVALID, "%s: row stride < n", name);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case NBG_F32:
+      return move_launch<float, KIND>((const float*)a, (const float*)b, (float*)out, rows, n, ld_a,
+                                      ld_b, ld_out, window, min_count, s);
+    case NBG_F64:
+      return move_launch<double, KIND>((const double*)a, (const double*)b, (double*)out, rows, n,
+                                       ld_a, ld_b, ld_out, window, min_count, s);
+    default:
+      return set_error(NBG_ERR_INVALID, "%s: unsupported dtype code %d", name, dtype);
+  }
+}
+
+}  // namespace
+}  // namespace nbg
+
+extern "C" {
+
+int nbg_move_sum(int dtype, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
+                 int64_t ld_out, int64_t window, int64_t min_count, void* stream) {
+  return nbg::move_entry<nbg::kSum>("nbg_move_sum", dtype, x, nullptr, out, rows, n, ld_x, 0,
+                                    ld_out, window, min_count, stream);
+}
+int nbg_move_mean(int dtype, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
+                  int64_t ld_out, int64_t window, int64_t min_count, void* stream) {
+  return nbg::move_entry<nbg::kMean>("nbg_move_mean", dtype, x, nullptr, out, rows, n, ld_x, 0,
+                                     ld_out, window, min_count, stream);
+}
+int nbg_move_var(int dtype, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
+                 int64_t ld_out, int64_t window, int64_t min_count, void* stream) {
+  return nbg::move_entry<nbg::kVar>("nbg_move_var", dtype, x, nullptr, out, rows, n, ld_x, 0,
+                                    ld_out, window, min_count, stream);
+}
+int nbg_move_std(int dtype, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
+                 int64_t ld_out, int64_t window, int64_t min_count, void* stream) {
+  return nbg::move_entry<nbg::kStd>("nbg_move_std", dtype, x, nullptr, out, rows, n, ld_x, 0,
+                                    ld_out, window, min_count, stream);
+}
+int nbg_move_cov(int dtype, const void* a, const void* b, void* out, int64_t rows, int64_t n,
+                 int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
+                 void* stream) {
+  return nbg::move_entry<nbg::kCov>("nbg_move_cov", dtype, a, b, out, rows, n, ld_a, ld_b, ld_out,
+                                    window, min_count, stream);
+}
+int nbg_move_corr(int dtype, const void* a, const void* b, void* out, int64_t rows, int64_t n,
+                  int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
+                  void* stream) {
+  return nbg::move_entry<nbg::kCorr>("nbg_move_corr", dtype, a, b, out, rows, n, ld_a, ld_b,
+                                     ld_out, window, min_count, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,509 @@
+// Exponentially weighted moving statistics (move_exp_nan{count,mean,sum,var,std,
+// cov,corr}) for gfx950.
+//
+// Reference semantics (numbagg/moving_exp.py:12-335): per position i, with
+// decay d_i = 1 - alpha_i, every float64 state component is multiplied by d_i
+// (sum_weight_2 by d_i**2), then a valid observation adds its contribution
+// (x, 1, alpha_i, x*x, ...); the output is a function of the state, NaN when the
+// decayed weight is below min_weight (and for var/cov/corr when the bias
+// correction is not positive).
+//
+// The recurrence is affine with a diagonal linear part, s_i = D_i*s_{i-1} + b_i,
+// so a chunk of positions acts on the state as one map (P, P2, B) with
+// P = prod d, P2 = prod d*d, B = the chunk's state started from zero. Design:
+//   * a workgroup owns a tile of 256 threads x K positions of one row, staged in
+//     LDS; each thread builds the map of its K positions;
+//   * an exclusive block scan composes the maps of earlier threads;
+//   * the tile's incoming state comes from a decoupled look-back over earlier
+//     tiles of the row (tile order is fixed by per-row tickets, so a tile only
+//     waits on tiles that are already running), which publish their aggregate
+//     map first and their inclusive state as soon as it is known;
+//   * each thread re-runs its K positions from its exact incoming state, with the
+//     reference's operation order (state *= decay; state += contribution).
+// Squares and cross products of float32 inputs are formed in float64 (exact),
+// where the reference rounds them to float32 first.
+#include "common.hpp"
+
+namespace nbg {
+namespace {
+
+constexpr int kExpThreads = 256;
+constexpr int kExpWaves = kExpThreads / kWave;
+constexpr int kMaxNS = 8;
+constexpr int kPayload = 2 * (kMaxNS + 3);  // doubles per tile: aggregate map + inclusive state
+
+enum ExpKind { kECount = 0, kEMean = 1, kESum = 2, kEVar = 3, kEStd = 4, kECov = 5, kECorr = 6 };
+
+template <int KIND>
+struct ExpSpec;
+// Component layout per kind. SW2: index of sum_weight_2 (decays by d*d), or -1.
+template <>
+struct ExpSpec<kECount> {  // count, weight
+  static constexpr int NS = 2, SW2 = -1, NARGS = 1;
+};
+template <>
+struct ExpSpec<kEMean> {  // numer, denom, weight
+  static constexpr int NS = 3, SW2 = -1, NARGS = 1;
+};
+template <>
+struct ExpSpec<kESum> {  // numer, weight (+ seen flag)
+  static constexpr int NS = 2, SW2 = -1, NARGS = 1;
+};
+template <>
+struct ExpSpec<kEVar> {  // sum_x2, sum_x, sum_w, sum_w2, weight
+  static constexpr int NS = 5, SW2 = 3, NARGS = 1;
+};
+template <>
+struct ExpSpec<kEStd> : ExpSpec<kEVar> {};
+template <>
+struct ExpSpec<kECov> {  // s1, s2, s12, sw, sw2, weight
+  static constexpr int NS = 6, SW2 = 4, NARGS = 2;
+};
+template <>
+struct ExpSpec<kECorr> {  // s1, s2, s12, sw, sw2, weight, s11, s22
+  static constexpr int NS = 8, SW2 = 4, NARGS = 2;
+};
+
+template <int NS>
+struct Map {
+  double p, p2;
+  double b[NS];
+  int seen;
+  __device__ __forceinline__ static Map identity() {
+    Map m;
+    m.p = 1.0;
+    m.p2 = 1.0;
+#pragma unroll
+    for (int c = 0; c < NS; ++c) m.b[c] = 0.0;
+    m.seen = 0;
+    return m;
+  }
+};
+
+// Apply `later` after `earlier`.
+template <int NS, int SW2>
+__device__ __forceinline__ Map<NS> compose(const Map<NS>& earlier, const Map<NS>& later) {
+  Map<NS> r;
+  r.p = later.p * earlier.p;
+  r.p2 = later.p2 * earlier.p2;
+#pragma unroll
+  for (int c = 0; c < NS; ++c) r.b[c] = (c == SW2 ? later.p2 : later.p) * earlier.b[c] + later.b[c];
+  r.seen = earlier.seen | later.seen;
+  return r;
+}
+
+template <int NS>
+__device__ __forceinline__ Map<NS> shfl_up_map(const Map<NS>& m, int d) {
+  Map<NS> r;
+  r.p = shfl_up(m.p, d);
+  r.p2 = shfl_up(m.p2, d);
+#pragma unroll
+  for (int c = 0; c < NS; ++c) r.b[c] = shfl_up(m.b[c], d);
+  r.seen = shfl_up(m.seen, d);
+  return r;
+}
+
+// One step of the reference recurrence on state s (components per ExpSpec).
+template <int KIND>
+__device__ __forceinline__ void exp_step(double* s, int& seen, double a, double b, double alpha) {
+  using Sp = ExpSpec<KIND>;
+  const double d = 1.0 - alpha;
+  const double d2 = d * d;
+#pragma unroll
+  for (int c = 0; c < Sp::NS; ++c) s[c] *= (c == Sp::SW2 ? d2 : d);
+  const bool valid = (Sp::NARGS == 2) ? !(is_nan(a) || is_nan(b)) : !is_nan(a);
+  if (!valid) return;
+  seen = 1;
+  if (KIND == kECount) {
+    s[0] += 1.0;
+    s[1] += alpha;
+  } else if (KIND == kEMean) {
+    s[0] += a;
+    s[1] += 1.0;
+    s[2] += alpha;
+  } else if (KIND == kESum) {
+    s[0] += a;
+    s[1] += alpha;
+  } else if (KIND == kEVar || KIND == kEStd) {
+    s[0] += a * a;
+    s[1] += a;
+    s[2] += 1.0;
+    s[3] += 1.0;
+    s[4] += alpha;
+  } else {
+    s[0] += a;
+    s[1] += b;
+    s[2] += a * b;
+    s[3] += 1.0;
+    s[4] += 1.0;
+    s[5] += alpha;
+    if (KIND == kECorr) {
+      s[6] += a * a;
+      s[7] += b * b;
+    }
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ double exp_out(const double* s, int seen, double mw) {
+  const double nan = canonical_nan<double>();
+  if (KIND == kECount) return s[1] >= mw ? s[0] : nan;
+  if (KIND == kEMean) return s[2] >= mw ? s[0] / s[1] : nan;
+  if (KIND == kESum) return (s[1] >= mw && seen) ? s[0] : nan;
+  if (KIND == kEVar || KIND == kEStd) {
+    const double m = s[1] / s[2];
+    const double var_biased = s[0] / s[2] - m * m;
+    const double bias = 1.0 - s[3] / (s[2] * s[2]);
+    if (!(s[4] >= mw && bias > 0)) return nan;
+    const double v = var_biased / bias;
+    return KIND == kEStd ? sqrt(v) : v;
+  }
+  const double bias = 1.0 - s[4] / (s[3] * s[3]);
+  if (KIND == kECov) {
+    const double cov_biased = (s[2] - s[0] * s[1] / s[3]) / s[3];
+    return (s[5] >= mw && bias > 0) ? cov_biased / bias : nan;
+  }
+  // kECorr
+  const double cov = s[2] - s[0] * s[1] / s[3];
+  const double v1 = s[6] - s[0] * s[0] / s[3];
+  const double v2 = s[7] - s[1] * s[1] / s[3];
+  if (!(s[5] >= mw && bias > 0)) return nan;
+  const double den = sqrt(v1 * v2);
+  return den > 0 ? cov / den : nan;
+}
+
+
+struct ExpArgs {
+  const void* xa;
+  const void* xb;
+  const void* alpha;  // nullptr: scalar alpha
+  int64_t ld_alpha;   // 0: one alpha row shared by all rows
+  double alpha_scalar;
+  void* out;
+  int64_t rows, n, ld_a, ld_b, ld_out;
+  double min_weight;
+  int64_t tiles_per_row;
+  unsigned int* flags;
+  unsigned int* counters;
+  double* payload;
+  unsigned int* timeouts;
+};
+
+template <typename TX, typename TA, int KIND>
+__global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
+  using Sp = ExpSpec<KIND>;
+  constexpr int NS = Sp::NS;
+  constexpr int K = 64 / (int)sizeof(TX);
+  constexpr int TILE = kExpThreads * K;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ Map<NS> s_wave[kExpWaves];
+  __shared__ double s_in[NS];
+  __shared__ int s_in_seen;
+  __shared__ int s_ticket;
+
+  const TX* xa = static_cast<const TX*>(A.xa);
+  const TX* xb = static_cast<const TX*>(A.xb);
+  const TA* al = static_cast<const TA*>(A.alpha);
+  TA* out = static_cast<TA*>(A.out);
+
+  const int64_t row = (int64_t)(blockIdx.x % (uint64_t)A.rows);
+  if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&A.counters[row], 1u);
+  __syncthreads();
+  const int64_t tile = s_ticket;
+  const int64_t t0 = tile * TILE;
+  const TX* ra = xa + row * A.ld_a;
+  const TX* rb = (Sp::NARGS == 2) ? xb + row * A.ld_b : nullptr;
+  const TA* rl = al ? al + row * A.ld_alpha : nullptr;
+  TA* ro = out + row * A.ld_out;
+  const int tid = threadIdx.x;
+  const int64_t n = A.n;
+
+  // Stage the tile (coalesced across the block; NaN past the row end).
+  TX* la = reinterpret_cast<TX*>(smem_raw);
+  TX* lb = la + pad16(TILE) + 16;
+  const bool full = t0 + TILE <= n;
+  constexpr int VEC = 16 / (int)sizeof(TX);
+  const bool vec_ok = full && (((uintptr_t)(ra + t0)) % 16 == 0) &&
+                      (Sp::NARGS == 1 || ((uintptr_t)(rb + t0)) % 16 == 0);
+  if (vec_ok) {
+    struct alignas(16) V { TX v[VEC]; };
+    for (int e = tid * VEC; e < TILE; e += kExpThreads * VEC) {
+      V va = *reinterpret_cast<const V*>(ra + t0 + e);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
+      if (Sp::NARGS == 2) {
+        V vb = *reinterpret_cast<const V*>(rb + t0 + e);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
+      }
+    }
+  } else {
+    for (int e = tid; e < TILE; e += kExpThreads) {
+      const int64_t p = t0 + e;
+      la[pad16(e)] = p < n ? ra[p] : canonical_nan<TX>();
+      if (Sp::NARGS == 2) lb[pad16(e)] = p < n ? rb[p] : canonical_nan<TX>();
+    }
+  }
+  __syncthreads();
+
+  const int64_t e0 = (int64_t)tid * K;
+  auto alpha_at = [&](int64_t p) -> double {
+    return rl ? (double)rl[p] : (double)(TA)A.alpha_scalar;
+  };
+
+  // 1. this thread's map.
+  Map<NS> m;
+  {
+    double s[NS];
+#pragma unroll
+    for (int c = 0; c < NS; ++c) s[c] = 0.0;
+    int seen = 0;
+    double p = 1.0, p2 = 1.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int64_t pos = t0 + e0 + j;
+      if (pos >= n) break;
+      const double alpha = alpha_at(pos);
+      const double d = 1.0 - alpha;
+      p *= d;
+      p2 *= d * d;
+      const double a = (double)la[pad16(e0 + j)];
+      const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
+      exp_step<KIND>(s, seen, a, b, alpha);
+    }
+    m.p = p;
+    m.p2 = p2;
+#pragma unroll
+    for (int c = 0; c < NS; ++c) m.b[c] = s[c];
+    m.seen = seen;
+  }
+
+  // 2. exclusive block scan of maps (composition in position order).
+  const int lane = lane_id(), wave = wave_id();
+  Map<NS> incl = m;
+#pragma unroll
+  for (int dd = 1; dd < kWave; dd <<= 1) {
+    Map<NS> o = shfl_up_map(incl, dd);
+    if (lane >= dd) incl = compose<NS, Sp::SW2>(o, incl);
+  }
+  Map<NS> excl = shfl_up_map(incl, 1);
+  if (lane == 0) excl = Map<NS>::identity();
+  if (lane == kWave - 1) s_wave[wave] = incl;
+  __syncthreads();
+  Map<NS> wprefix = Map<NS>::identity();
+  for (int w = 0; w < wave; ++w) wprefix = compose<NS, Sp::SW2>(wprefix, s_wave[w]);
+  excl = compose<NS, Sp::SW2>(wprefix, excl);
+
+  // 3. decoupled look-back for the tile's incoming state (thread 0).
+  if (tid == 0) {
+    Map<NS> agg = Map<NS>::identity();
+    for (int w = 0; w < kExpWaves; ++w) agg = compose<NS, Sp::SW2>(agg, s_wave[w]);
+    unsigned int* frow = A.flags + row * A.tiles_per_row;
+    double* prow = A.payload + row * A.tiles_per_row * kPayload;
+    double sin[NS];
+#pragma unroll
+    for (int c = 0; c < NS; ++c) sin[c] = 0.0;
+    int sin_seen = 0;
+    if (tile > 0) {
+      // publish the aggregate map
+      double* pl = prow + tile * kPayload;
+      st_agent_f64(pl + 0, agg.p);
+      st_agent_f64(pl + 1, agg.p2);
+#pragma unroll
+      for (int c = 0; c < NS; ++c) st_agent_f64(pl + 2 + c, agg.b[c]);
+      st_agent_f64(pl + 2 + NS, (double)agg.seen);
+      st_release(&frow[tile], 1u);
+      // walk back, composing aggregates until an inclusive state is found
+      Map<NS> acc = Map<NS>::identity();  // maps of tiles (j, tile) composed
+      int64_t j = tile - 1;
+      while (true) {
+        unsigned int f;
+        unsigned int spins = 0;
+        while ((f = ld_acquire(&frow[j])) == 0u) {
+          spin_pause();
+          if (++spins > kSpinLimit) {
+            atomicAdd(A.timeouts, 1u);
+            break;
+          }
+        }
+        const double* pj = prow + j * kPayload;
+        if (f == 2u) {
+          double sj[NS];
+#pragma unroll
+          for (int c = 0; c < NS; ++c) sj[c] = ld_agent_f64(pj + kMaxNS + 3 + c);
+          const int seen_j = (int)ld_agent_f64(pj + kMaxNS + 3 + NS);
+#pragma unroll
+          for (int c = 0; c < NS; ++c) sin[c] = (c == Sp::SW2 ? acc.p2 : acc.p) * sj[c] + acc.b[c];
+          sin_seen = seen_j | acc.seen;
+          break;
+        }
+        if (f == 0u) break;  // expired wait (counted above): continue from zero state
+        Map<NS> mj;
+        mj.p = ld_agent_f64(pj + 0);
+        mj.p2 = ld_agent_f64(pj + 1);
+#pragma unroll
+        for (int c = 0; c < NS; ++c) mj.b[c] = ld_agent_f64(pj + 2 + c);
+        mj.seen = (int)ld_agent_f64(pj + 2 + NS);
+        acc = compose<NS, Sp::SW2>(mj, acc);
+        if (j == 0) {
+#pragma unroll
+          for (int c = 0; c < NS; ++c) sin[c] = acc.b[c];
+          sin_seen = acc.seen;
+          break;
+        }
+        --j;
+      }
+    }
+    // publish the inclusive state
+    double* pl = prow + tile * kPayload;
+#pragma unroll
+    for (int c = 0; c < NS; ++c)
+      st_agent_f64(pl + kMaxNS + 3 + c, (c == Sp::SW2 ? agg.p2 : agg.p) * sin[c] + agg.b[c]);
+    st_agent_f64(pl + kMaxNS + 3 + NS, (double)(sin_seen | agg.seen));
+    st_release(&frow[tile], 2u);
+#pragma unroll
+    for (int c = 0; c < NS; ++c) s_in[c] = sin[c];
+    s_in_seen = sin_seen;
+  }
+  __syncthreads();
+
+  // 4. re-run this thread's positions from its incoming state.
+  double s[NS];
+#pragma unroll
+  for (int c = 0; c < NS; ++c) s[c] = (c == Sp::SW2 ? excl.p2 : excl.p) * s_in[c] + excl.b[c];
+  int seen = s_in_seen | excl.seen;
+  TA res[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int64_t pos = t0 + e0 + j;
+    if (pos < n) {
+      const double alpha = alpha_at(pos);
+      const double a = (double)la[pad16(e0 + j)];
+      const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
+      exp_step<KIND>(s, seen, a, b, alpha);
+      res[j] = (TA)exp_out<KIND>(s, seen, A.min_weight);
+    }
+  }
+  constexpr int OV = 16 / (int)sizeof(TA);  // outputs per 16-byte store
+  if (full && ((uintptr_t)(ro + t0 + e0)) % 16 == 0) {
+    struct alignas(16) V { TA v[OV]; };
+#pragma unroll
+    for (int j = 0; j < K; j += OV) {
+      V vo;
+#pragma unroll
+      for (int v = 0; v < OV; ++v) vo.v[v] = res[j + v];
+      *reinterpret_cast<V*>(ro + t0 + e0 + j) = vo;
+    }
+  } else if (full) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) ro[t0 + e0 + j] = res[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (t0 + e0 + j < n) ro[t0 + e0 + j] = res[j];
+  }
+}
+
+template <typename TX>
+constexpr int64_t exp_tile() {
+  return kExpThreads * (64 / (int64_t)sizeof(TX));
+}
+
+template <typename TX, typename TA, int KIND>
+int exp_launch(ExpArgs a, void* ws, size_t ws_bytes, hipStream_t s) {
+  const int64_t tiles = (a.n + exp_tile<TX>() - 1) / exp_tile<TX>();
+  const int64_t blocks = a.rows * tiles;
+  if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move_exp: too many tiles");
+  const size_t need = nbg_move_exp_workspace_bytes(a.rows, a.n, sizeof(TX) == 4 ? NBG_F32 : NBG_F64);
+  if (ws == nullptr || ws_bytes < need)
+    return set_error(NBG_ERR_INVALID, "move_exp: workspace too small (%zu < %zu)", ws_bytes, need);
+  // layout: flags[rows*tiles] u32 | counters[rows] u32 | (8-aligned) payload
+  const size_t flag_bytes = (size_t)(a.rows * tiles + a.rows) * 4;
+  const size_t zero_bytes = (flag_bytes + 255) / 256 * 256;
+  a.flags = static_cast<unsigned int*>(ws);
+  a.counters = a.flags + a.rows * tiles;
+  a.payload = reinterpret_cast<double*>(static_cast<char*>(ws) + zero_bytes);
+  a.tiles_per_row = tiles;
+  a.timeouts = spin_counter_ptr();
+  if (a.timeouts == nullptr) return set_error(NBG_ERR_HIP, "move_exp: cannot resolve device counter");
+  if (int rc = launch_zero(ws, zero_bytes, s)) return rc;
+  constexpr int NARGS = ExpSpec<KIND>::NARGS;
+  const size_t lds = (size_t)(pad16(exp_tile<TX>()) + 16) * sizeof(TX) * NARGS;
+  hipLaunchKernelGGL((exp_kernel<TX, TA, KIND>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
+  return check_hip(hipGetLastError(), "exp_kernel launch");
+}
+
+template <int KIND>
+int exp_entry(const char* name, int x_dtype, int dtype, const void* xa, const void* xb,
+              const void* alpha, int64_t ld_alpha, double alpha_scalar, void* out, int64_t rows,
+              int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, double min_weight, void* ws,
+              size_t ws_bytes, void* stream) {
+  if (int rc = injected_error(name)) return rc;
+  if (rows < 0 || n < 0) return set_error(NBG_ERR_INVALID, "%s: negative shape", name);
+  if (rows == 0 || n == 0) return NBG_OK;
+  constexpr int NARGS = ExpSpec<KIND>::NARGS;
+  if (xa == nullptr || out == nullptr || (NARGS == 2 && xb == nullptr))
+    return set_error(NBG_ERR_INVALID, "%s: null pointer", name);
+  if (rows > 1 && (ld_a < n || ld_out < n || (NARGS == 2 && ld_b < n)))
+    return set_error(NBG_ERR_INVALID, "%s: row stride < n", name);
+  if (alpha != nullptr && ld_alpha != 0 && ld_alpha < n)
+    return set_error(NBG_ERR_INVALID, "%s: alpha row stride must be 0 or >= n", name);
+  ExpArgs a{};
+  a.xa = xa;
+  a.xb = xb;
+  a.alpha = alpha;
+  a.ld_alpha = ld_alpha;
+  a.alpha_scalar = alpha_scalar;
+  a.out = out;
+  a.rows = rows;
+  a.n = n;
+  a.ld_a = ld_a;
+  a.ld_b = ld_b;
+  a.ld_out = ld_out;
+  a.min_weight = min_weight;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (x_dtype == NBG_F32 && dtype == NBG_F32) return exp_launch<float, float, KIND>(a, ws, ws_bytes, s);
+  if (x_dtype == NBG_F64 && dtype == NBG_F64) return exp_launch<double, double, KIND>(a, ws, ws_bytes, s);
+  if (x_dtype == NBG_F32 && dtype == NBG_F64) return exp_launch<float, double, KIND>(a, ws, ws_bytes, s);
+  return set_error(NBG_ERR_INVALID, "%s: unsupported dtype pair (%d, %d)", name, x_dtype, dtype);
+}
+
+}  // namespace
+}  // namespace nbg
+
+extern "C" {
+
+size_t nbg_move_exp_workspace_bytes(int64_t rows, int64_t n, int x_dtype) {
+  const int64_t tile = (x_dtype == NBG_F64) ? nbg::exp_tile<double>() : nbg::exp_tile<float>();
+  const int64_t tiles = (n + tile - 1) / tile;
+  if (rows <= 0 || tiles <= 0) return 0;
+  const size_t flag_bytes = (size_t)(rows * tiles + rows) * 4;
+  const size_t zero_bytes = (flag_bytes + 255) / 256 * 256;
+  return zero_bytes + (size_t)(rows * tiles) * nbg::kPayload * sizeof(double);
+}
+
+#define NBG_EXP1(NAME, KIND)                                                                      \
+  int NAME(int x_dtype, int dtype, const void* x, const void* alpha, int64_t ld_alpha,           \
+           double alpha_scalar, void* out, int64_t rows, int64_t n, int64_t ld_x, int64_t ld_out, \
+           double min_weight, void* ws, size_t ws_bytes, void* stream) {                          \
+    return nbg::exp_entry<KIND>(#NAME, x_dtype, dtype, x, nullptr, alpha, ld_alpha, alpha_scalar, \
+                                out, rows, n, ld_x, 0, ld_out, min_weight, ws, ws_bytes, stream); \
+  }
+#define NBG_EXP2(NAME, KIND)                                                                     \
+  int NAME(int x_dtype, int dtype, const void* a, const void* b, const void* alpha,             \
+           int64_t ld_alpha, double alpha_scalar, void* out, int64_t rows, int64_t n,           \
+           int64_t ld_a, int64_t ld_b, int64_t ld_out, double min_weight, void* ws,             \
+           size_t ws_bytes, void* stream) {                                                      \
+    return nbg::exp_entry<KIND>(#NAME, x_dtype, dtype, a, b, alpha, ld_alpha, alpha_scalar, out, \
+                                rows, n, ld_a, ld_b, ld_out, min_weight, ws, ws_bytes, stream);  \
+  }
+
+NBG_EXP1(nbg_move_exp_nancount, nbg::kECount)
+NBG_EXP1(nbg_move_exp_nanmean, nbg::kEMean)
+NBG_EXP1(nbg_move_exp_nansum, nbg::kESum)
+NBG_EXP1(nbg_move_exp_nanvar, nbg::kEVar)
+NBG_EXP1(nbg_move_exp_nanstd, nbg::kEStd)
+NBG_EXP2(nbg_move_exp_nancov, nbg::kECov)
+NBG_EXP2(nbg_move_exp_nancorr, nbg::kECorr)
+
+}  // extern "C"

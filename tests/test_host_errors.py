@@ -24,3 +24,31 @@ def test_repr_and_names():
     assert nb.group_nanvar.supports_ddof and not nb.group_nansum.supports_ddof
     assert not nb.group_nanmean.supports_ints and nb.group_nansum.supports_ints
     assert not nb.group_nanvar.supports_bool
+
+
+def test_move_errors():
+    # numbagg/test/test_moving.py:168-177
+    array = np.random.RandomState(0).rand(3, 500)
+    with pytest.raises(TypeError):
+        nb.move_mean(array, window=0.5)
+    with pytest.raises(ValueError):
+        nb.move_mean(array, window=-1)
+    with pytest.raises(ValueError, match="window not in valid range"):
+        nb.move_mean(array, window=array.shape[-1] + 1)
+    with pytest.raises(ValueError, match="min_count must be positive"):
+        nb.move_mean(array, window=1, min_count=-1)
+    x = np.arange(10.0)
+    assert nb.move_mean(x, window=2, axis=()) is x
+    with pytest.raises(ValueError, match="only one axis"):
+        nb.move_mean(x[None], window=2, axis=(0, 1))
+    with pytest.raises(ValueError, match="empty tuple"):
+        nb.move_corr(x, x, window=2, axis=())
+
+
+def test_move_exp_errors():
+    x = np.arange(10.0)
+    assert nb.move_exp_nanmean(x, alpha=0.5, axis=()) is x
+    with pytest.raises(ValueError, match="Only one axis"):
+        nb.move_exp_nanmean(x[None], alpha=0.5, axis=(0, 1))
+    with pytest.raises(ValueError, match="empty tuple"):
+        nb.move_exp_nancorr(x, x, alpha=0.5, axis=())
