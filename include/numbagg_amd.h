@@ -142,6 +142,63 @@ int nbg_move_exp_nancorr(int x_dtype, int dtype, const void* a, const void* b,
                          double min_weight, void* ws, size_t ws_bytes,
                          void* stream); /* moving_exp.py:276-335 */
 
+/* ---- grouped reductions (numbagg/grouped.py:7-208; groupndreduce.__call__
+ *      at numbagg/decorators.py:558-674) ---------------------------------- */
+/* values: (rows, n) with row stride ld_v; labels: (n,) shared by all rows,
+ * int32 or int64; out: (rows, num_labels) contiguous, in the values dtype.
+ * Labels < 0 are skipped, and so are labels >= num_labels.
+ * vdtype: NBG_F32/NBG_F64, or NBG_I32/NBG_I64 for sum, count, argmax,
+ * argmin, first and last (mean/var/std take floats; the host widens integer
+ * input to float64 as the reference does).
+ * strategy: per-label accumulators privatised in LDS, or global atomics. */
+#define NBG_GROUP_AUTO 0
+#define NBG_GROUP_LDS 1
+#define NBG_GROUP_GLOBAL 2
+size_t nbg_group_workspace_bytes(int vdtype, int64_t rows, int64_t n,
+                                 int64_t num_labels, int strategy);
+int nbg_group_nansum(int vdtype, const void* values, int ldtype,
+                     const void* labels, void* out, int64_t rows, int64_t n,
+                     int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                     size_t ws_bytes, void* stream); /* grouped.py:30-40 */
+int nbg_group_nanmean(int vdtype, const void* values, int ldtype,
+                      const void* labels, void* out, int64_t rows, int64_t n,
+                      int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                      size_t ws_bytes, void* stream); /* grouped.py:7-27 */
+int nbg_group_nancount(int vdtype, const void* values, int ldtype,
+                       const void* labels, void* out, int64_t rows, int64_t n,
+                       int64_t ld_v, int64_t num_labels, int strategy,
+                       void* ws, size_t ws_bytes,
+                       void* stream); /* grouped.py:43-51 */
+int nbg_group_nanargmax(int vdtype, const void* values, int ldtype,
+                        const void* labels, void* out, int64_t rows, int64_t n,
+                        int64_t ld_v, int64_t num_labels, int strategy,
+                        void* ws, size_t ws_bytes,
+                        void* stream); /* grouped.py:54-75 */
+int nbg_group_nanargmin(int vdtype, const void* values, int ldtype,
+                        const void* labels, void* out, int64_t rows, int64_t n,
+                        int64_t ld_v, int64_t num_labels, int strategy,
+                        void* ws, size_t ws_bytes,
+                        void* stream); /* grouped.py:78-92 */
+int nbg_group_nanfirst(int vdtype, const void* values, int ldtype,
+                       const void* labels, void* out, int64_t rows, int64_t n,
+                       int64_t ld_v, int64_t num_labels, int strategy,
+                       void* ws, size_t ws_bytes,
+                       void* stream); /* grouped.py:95-110 */
+int nbg_group_nanlast(int vdtype, const void* values, int ldtype,
+                      const void* labels, void* out, int64_t rows, int64_t n,
+                      int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                      size_t ws_bytes, void* stream); /* grouped.py:113-121 */
+int nbg_group_nanvar(int vdtype, const void* values, int ldtype,
+                     const void* labels, void* out, int64_t rows, int64_t n,
+                     int64_t ld_v, int64_t num_labels, int64_t ddof,
+                     int strategy, void* ws, size_t ws_bytes,
+                     void* stream); /* grouped.py:148-176 */
+int nbg_group_nanstd(int vdtype, const void* values, int ldtype,
+                     const void* labels, void* out, int64_t rows, int64_t n,
+                     int64_t ld_v, int64_t num_labels, int64_t ddof,
+                     int strategy, void* ws, size_t ws_bytes,
+                     void* stream); /* grouped.py:179-208 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,20 @@ for _name in ("nancount", "nanmean", "nansum", "nanvar", "nanstd"):
     _SIGNATURES[f"nbg_move_exp_{_name}"] = _EXP1
 for _name in ("nancov", "nancorr"):
     _SIGNATURES[f"nbg_move_exp_{_name}"] = _EXP2
+_SIGNATURES["nbg_group_workspace_bytes"] = (_c_size, [_c_int, _c_i64, _c_i64, _c_i64, _c_int])
+_GROUP = (
+    _c_int,
+    [_c_int, _c_ptr, _c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_int, _c_ptr, _c_size, _c_ptr],
+)
+_GROUP_DDOF = (
+    _c_int,
+    [_c_int, _c_ptr, _c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_int, _c_ptr, _c_size,
+     _c_ptr],
+)
+for _name in ("nansum", "nanmean", "nancount", "nanargmax", "nanargmin", "nanfirst", "nanlast"):
+    _SIGNATURES[f"nbg_group_{_name}"] = _GROUP
+for _name in ("nanvar", "nanstd"):
+    _SIGNATURES[f"nbg_group_{_name}"] = _GROUP_DDOF
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None

@@ -1,0 +1,594 @@
+// Grouped reductions (group_nan{sum,mean,count,argmax,argmin,first,last,var,std})
+// for gfx950.
+//
+// Reference semantics (numbagg/grouped.py:7-208): values (rows, n) share one
+// label vector (n,); for every row, each valid value (not NaN) whose label is
+// >= 0 is reduced into out[row, label]. Labels >= num_labels are skipped (the
+// reference leaves them undefined). Empty labels give 0 (sum, count), NaN
+// (mean, var, std, arg*, first, last; integer outputs get the minimum integer,
+// which is what numba's NaN-to-integer store produces).
+//
+// Design:
+//   * a workgroup reduces one (row, chunk) of values. With the LDS strategy its
+//     per-label accumulators live in LDS (privatised: conflicts only between
+//     the lanes of one workgroup) and are flushed once per chunk with global
+//     atomics; with the global strategy (label counts too large for LDS) every
+//     element updates the global accumulators directly;
+//   * a finalize kernel turns accumulators into the output dtype.
+// Accumulators: float sums in float64 (rounded once to the output dtype),
+// exact integer counts, 64-bit packed (orderable key, ~index) maxima for
+// float32/int32 argmax/argmin so that the first index wins ties in one pass,
+// key-then-index passes for 64-bit values, index min/max for first/last, and a
+// corrected two-pass variance (per-label means, then sums of deviations).
+#include "common.hpp"
+
+namespace nbg {
+namespace {
+
+constexpr int kGroupThreads = 256;
+constexpr int kGroupUnroll = 4;
+
+enum GKind {
+  kGSum = 0,
+  kGMean = 1,
+  kGCount = 2,
+  kGArgMax = 3,
+  kGArgMin = 4,
+  kGFirst = 5,
+  kGLast = 6,
+  kGVar = 7,
+  kGStd = 8,
+};
+
+// Internal accumulation passes.
+enum GPass {
+  kPSum = 0,      // s (f64 or i64) [+ count]
+  kPCount = 1,    // count
+  kPPacked = 2,   // packed (key32, ~idx32) max
+  kPKey = 3,      // 64-bit key max
+  kPKeyIdx = 4,   // min index among elements whose key equals the max
+  kPFirst = 5,    // min index
+  kPLast = 6,     // max index + 1
+  kPDev = 7,      // deviations from the label mean: sum d, sum d^2
+};
+
+template <typename T>
+__device__ __forceinline__ bool gvalid(T v) {
+  return !is_nan(v);
+}
+
+// Order-preserving maps to unsigned integers (argmax uses them directly, argmin
+// their complement). -0.0 is folded onto +0.0: the reference compares values
+// with `>`/`<`, under which the two zeros tie.
+__device__ __forceinline__ uint32_t okey32(float v) {
+  v = v + 0.0f;
+  uint32_t b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ uint32_t okey32(int32_t v) { return (uint32_t)v ^ 0x80000000u; }
+__device__ __forceinline__ uint64_t okey64(double v) {
+  v = v + 0.0;
+  uint64_t b = (uint64_t)__double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+// int64 values are compared as float64, like the reference (its running
+// extreme is a float64 array).
+__device__ __forceinline__ uint64_t okey64(int64_t v) { return okey64((double)v); }
+__device__ __forceinline__ uint64_t okey64(float v) { return okey64((double)v); }
+__device__ __forceinline__ uint64_t okey64(int32_t v) { return okey64((double)v); }
+
+template <typename TV>
+struct SumAcc {
+  using type = double;
+};
+// Integer sums wrap modulo 2^64 and are narrowed at the end, which reproduces
+// the reference's wrapping integer adds.
+template <>
+struct SumAcc<int32_t> {
+  using type = unsigned long long;
+};
+template <>
+struct SumAcc<int64_t> {
+  using type = unsigned long long;
+};
+
+struct GArgs {
+  const void* values;
+  const void* labels;
+  void* out;
+  int64_t rows, n, ld_v, num_labels, ddof;
+  int64_t chunk, chunks_per_row;
+  // global accumulators (rows x num_labels each)
+  void* acc0;
+  void* acc1;
+  void* acc2;           // variance finalize: counts
+  const double* means;  // kPDev: per (row, label) mean
+  int argmin;           // kPPacked / kPKey / kPKeyIdx: complement keys
+};
+
+__device__ __forceinline__ void g_atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void g_atomic_add(unsigned long long* p, unsigned long long v) {
+  atomicAdd(p, v);
+}
+
+// One accumulation pass over a (row, chunk). LDS: per-label accumulators.
+template <typename TV, typename TL, int PASS, bool LDS>
+__global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
+  using SA = typename SumAcc<TV>::type;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t L = A.num_labels;
+  const int64_t row = (int64_t)(blockIdx.x / (uint64_t)A.chunks_per_row);
+  const int64_t ck = (int64_t)(blockIdx.x % (uint64_t)A.chunks_per_row);
+  const int64_t c0 = ck * A.chunk;
+  const int64_t c1 = (c0 + A.chunk < A.n) ? c0 + A.chunk : A.n;
+  const TV* vr = static_cast<const TV*>(A.values) + row * A.ld_v;
+  const TL* lab = static_cast<const TL*>(A.labels);
+  const int tid = threadIdx.x;
+
+  // Global accumulators of this row.
+  SA* g_s = static_cast<SA*>(A.acc0) + row * L;
+  unsigned long long* g_u0 = static_cast<unsigned long long*>(A.acc0) + row * L;
+  unsigned long long* g_u1 = static_cast<unsigned long long*>(A.acc1) + row * L;
+  double* g_d0 = static_cast<double*>(A.acc0) + row * L;
+  double* g_d1 = static_cast<double*>(A.acc1) + row * L;
+  const double* g_mean = A.means ? A.means + row * L : nullptr;
+
+  // LDS accumulators.
+  SA* l_s = reinterpret_cast<SA*>(smem_raw);
+  unsigned int* l_c = reinterpret_cast<unsigned int*>(smem_raw + (PASS == kPSum ? L * sizeof(SA) : 0));
+  unsigned long long* l_u = reinterpret_cast<unsigned long long*>(smem_raw);
+  unsigned int* l_u32 = reinterpret_cast<unsigned int*>(smem_raw);
+  double* l_d0 = reinterpret_cast<double*>(smem_raw);
+  double* l_d1 = l_d0 + L;
+  double* l_mean = l_d0 + 2 * L;
+  unsigned long long* l_key = l_u + L;  // kPKeyIdx: staged per-label maxima
+  if (LDS) {
+    for (int64_t l = tid; l < L; l += kGroupThreads) {
+      if constexpr (PASS == kPSum) {
+        l_s[l] = SA(0);
+        l_c[l] = 0u;
+      } else if constexpr (PASS == kPCount) {
+        l_u32[l] = 0u;
+      } else if constexpr (PASS == kPPacked || PASS == kPKey || PASS == kPLast) {
+        l_u[l] = 0ull;
+      } else if constexpr (PASS == kPKeyIdx) {
+        l_u[l] = ~0ull;
+        l_key[l] = g_u0[l];
+      } else if constexpr (PASS == kPFirst) {
+        l_u[l] = ~0ull;
+      } else if constexpr (PASS == kPDev) {
+        l_d0[l] = 0.0;
+        l_d1[l] = 0.0;
+        l_mean[l] = g_mean[l];
+      }
+    }
+    __syncthreads();
+  }
+
+  auto update = [&](TV v, TL lbl, int64_t i) {
+    if (lbl < 0 || (int64_t)lbl >= L) return;
+    if (!gvalid(v)) return;
+    const int64_t l = (int64_t)lbl;
+    if constexpr (PASS == kPSum) {
+      if (LDS) {
+        atomicAdd(&l_s[l], (SA)v);  // ds_add_f64 / ds_add_u64
+        atomicAdd(&l_c[l], 1u);
+      } else {
+        g_atomic_add(&g_s[l], (SA)v);
+        atomicAdd(&g_u1[l], 1ull);
+      }
+    } else if constexpr (PASS == kPCount) {
+      if (LDS) atomicAdd(&l_u32[l], 1u);
+      else atomicAdd(&g_u0[l], 1ull);
+    } else if constexpr (PASS == kPPacked) {
+      uint32_t k = okey32(v);
+      if (A.argmin) k = ~k;
+      const unsigned long long w = ((unsigned long long)k << 32) | (unsigned long long)(~(uint32_t)i);
+      if (LDS) atomicMax(&l_u[l], w);
+      else atomicMax(&g_u0[l], w);
+    } else if constexpr (PASS == kPKey) {
+      uint64_t k = okey64(v);
+      if (A.argmin) k = ~k;
+      if (LDS) atomicMax(&l_u[l], (unsigned long long)k);
+      else atomicMax(&g_u0[l], (unsigned long long)k);
+    } else if constexpr (PASS == kPKeyIdx) {
+      uint64_t k = okey64(v);
+      if (A.argmin) k = ~k;
+      if (k == (LDS ? l_key[l] : g_u0[l])) {
+        if (LDS) atomicMin(&l_u[l], (unsigned long long)i);
+        else atomicMin(&g_u1[l], (unsigned long long)i);
+      }
+    } else if constexpr (PASS == kPFirst) {
+      if (LDS) atomicMin(&l_u[l], (unsigned long long)i);
+      else atomicMin(&g_u0[l], (unsigned long long)i);
+    } else if constexpr (PASS == kPLast) {
+      if (LDS) atomicMax(&l_u[l], (unsigned long long)(i + 1));
+      else atomicMax(&g_u0[l], (unsigned long long)(i + 1));
+    } else if constexpr (PASS == kPDev) {
+      const double d = (double)v - (LDS ? l_mean[l] : g_mean[l]);
+      if (LDS) {
+        atomicAdd(&l_d0[l], d);
+        atomicAdd(&l_d1[l], d * d);
+      } else {
+        unsafeAtomicAdd(&g_d0[l], d);
+        unsafeAtomicAdd(&g_d1[l], d * d);
+      }
+    }
+  };
+
+  // Stream the chunk: every thread keeps kGroupUnroll vectors of values and
+  // labels in flight; full super-steps first, then a coalesced remainder.
+  constexpr int VV = 16 / (int)sizeof(TV);
+  constexpr int VL = 16 / (int)sizeof(TL);
+  constexpr int VE = VV < VL ? VV : VL;  // elements per vector step
+  constexpr int64_t STEP = (int64_t)kGroupThreads * VE;
+  constexpr int64_t SUPER = STEP * kGroupUnroll;
+  const bool aligned = (((uintptr_t)(vr + c0)) % 16 == 0) && (((uintptr_t)(lab + c0)) % 16 == 0);
+  const int64_t nfull = aligned ? (c1 - c0) / SUPER : 0;
+  {
+    struct alignas(16) V { TV v[VE]; };
+    struct alignas(VE * sizeof(TL)) W { TL v[VE]; };
+    for (int64_t k = 0; k < nfull; ++k) {
+      const int64_t base = c0 + k * SUPER + (int64_t)tid * VE;
+      V vv[kGroupUnroll];
+      W ll[kGroupUnroll];
+#pragma unroll
+      for (int u = 0; u < kGroupUnroll; ++u) {
+        vv[u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+        ll[u] = *reinterpret_cast<const W*>(lab + base + u * STEP);
+      }
+#pragma unroll
+      for (int u = 0; u < kGroupUnroll; ++u)
+#pragma unroll
+        for (int e = 0; e < VE; ++e) update(vv[u].v[e], ll[u].v[e], base + u * STEP + e);
+    }
+  }
+  for (int64_t j = c0 + nfull * SUPER + tid; j < c1; j += kGroupThreads) update(vr[j], lab[j], j);
+
+  if (LDS) {
+    __syncthreads();
+    for (int64_t l = tid; l < L; l += kGroupThreads) {
+      if constexpr (PASS == kPSum) {
+        const unsigned int c = l_c[l];
+        if (c) {
+          g_atomic_add(&g_s[l], l_s[l]);
+          atomicAdd(&g_u1[l], (unsigned long long)c);
+        }
+      } else if constexpr (PASS == kPCount) {
+        if (l_u32[l]) atomicAdd(&g_u0[l], (unsigned long long)l_u32[l]);
+      } else if constexpr (PASS == kPPacked || PASS == kPKey || PASS == kPLast) {
+        if (l_u[l]) atomicMax(&g_u0[l], l_u[l]);
+      } else if constexpr (PASS == kPKeyIdx) {
+        if (l_u[l] != ~0ull) atomicMin(&g_u1[l], l_u[l]);
+      } else if constexpr (PASS == kPFirst) {
+        if (l_u[l] != ~0ull) atomicMin(&g_u0[l], l_u[l]);
+      } else if constexpr (PASS == kPDev) {
+        if (l_d1[l] != 0.0 || l_d0[l] != 0.0) {
+          unsafeAtomicAdd(&g_d0[l], l_d0[l]);
+          unsafeAtomicAdd(&g_d1[l], l_d1[l]);
+        }
+      }
+    }
+  }
+}
+
+// Initialise accumulators: acc0/acc1 words set to the pass identities.
+__global__ void group_init_kernel(unsigned long long* acc0, unsigned long long init0,
+                                  unsigned long long* acc1, unsigned long long init1,
+                                  int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    acc0[i] = init0;
+    if (acc1) acc1[i] = init1;
+  }
+}
+
+template <typename TV>
+__device__ __forceinline__ TV missing_value() {
+  if (sizeof(TV) == 4 && (TV)0.5 != 0) return (TV)canonical_nan<float>();
+  if ((TV)0.5 != 0) return (TV)canonical_nan<double>();
+  return (TV)(sizeof(TV) == 4 ? (TV)0x80000000u : (TV)0x8000000000000000ull);
+}
+
+// Turn accumulators into outputs. `means` mode writes per-label means for the
+// second variance pass instead of outputs.
+template <typename TV, int KIND>
+__global__ void group_finalize_kernel(GArgs A, double* means_out) {
+  using SA = typename SumAcc<TV>::type;
+  const int64_t total = A.rows * A.num_labels;
+  TV* out = static_cast<TV*>(A.out);
+  const TV* vals = static_cast<const TV*>(A.values);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = q / A.num_labels;
+    if (means_out) {
+      const double s = static_cast<const double*>(A.acc0)[q];
+      const unsigned long long c = static_cast<const unsigned long long*>(A.acc1)[q];
+      means_out[q] = c ? s / (double)c : 0.0;
+      continue;
+    }
+    TV r;
+    if constexpr (KIND == kGSum) {
+      r = (TV)static_cast<const SA*>(A.acc0)[q];  // integers: modular narrowing
+    } else if constexpr (KIND == kGMean) {
+      const double s = static_cast<const double*>(A.acc0)[q];
+      const unsigned long long c = static_cast<const unsigned long long*>(A.acc1)[q];
+      r = c ? (TV)(s / (double)c) : missing_value<TV>();
+    } else if constexpr (KIND == kGCount) {
+      r = (TV)static_cast<const unsigned long long*>(A.acc0)[q];
+    } else if constexpr (KIND == kGArgMax || KIND == kGArgMin) {
+      const unsigned long long w = static_cast<const unsigned long long*>(A.acc0)[q];
+      const unsigned long long idx1 = static_cast<const unsigned long long*>(A.acc1)[q];
+      if (A.acc1 == nullptr) {  // packed (key32, ~idx32)
+        r = w ? (TV)(double)(uint32_t)(~(uint32_t)(w & 0xffffffffull)) : missing_value<TV>();
+      } else {
+        r = (idx1 != ~0ull) ? (TV)(double)idx1 : missing_value<TV>();
+      }
+    } else if constexpr (KIND == kGFirst) {
+      const unsigned long long i = static_cast<const unsigned long long*>(A.acc0)[q];
+      r = (i != ~0ull) ? vals[row * A.ld_v + (int64_t)i] : missing_value<TV>();
+    } else if constexpr (KIND == kGLast) {
+      const unsigned long long i = static_cast<const unsigned long long*>(A.acc0)[q];
+      r = i ? vals[row * A.ld_v + (int64_t)(i - 1)] : missing_value<TV>();
+    } else {  // var / std from (count, sum d, sum d^2)
+      const unsigned long long c = static_cast<const unsigned long long*>(A.acc2)[q];
+      const double d1 = static_cast<const double*>(A.acc0)[q];
+      const double d2 = static_cast<const double*>(A.acc1)[q];
+      const double den = (double)((long long)c - A.ddof);
+      if (!(den > 0)) {
+        r = missing_value<TV>();
+      } else {
+        const double m2 = d2 - d1 * d1 / (double)c;
+        const double var = m2 / den;
+        r = (TV)(KIND == kGStd ? sqrt(var) : var);
+      }
+    }
+    out[q] = r;
+  }
+}
+
+template <typename TV, typename TL, int PASS, bool LDS>
+int launch_pass(GArgs a, size_t lds, hipStream_t s) {
+  const int64_t blocks = a.rows * a.chunks_per_row;
+  hipLaunchKernelGGL((group_pass_kernel<TV, TL, PASS, LDS>), dim3((unsigned)blocks),
+                     dim3(kGroupThreads), LDS ? lds : 0, s, a);
+  return check_hip(hipGetLastError(), "group_pass_kernel launch");
+}
+
+template <typename TV, typename TL, int PASS>
+int run_pass(GArgs a, bool lds, hipStream_t s) {
+  size_t bytes = 0;
+  const int64_t L = a.num_labels;
+  switch (PASS) {
+    case kPSum: bytes = L * (sizeof(typename SumAcc<TV>::type) + 4); break;
+    case kPCount: bytes = L * 4; break;
+    case kPDev: bytes = L * 24; break;
+    case kPKeyIdx: bytes = L * 16; break;
+    default: bytes = L * 8; break;
+  }
+  if (lds) return launch_pass<TV, TL, PASS, true>(a, bytes, s);
+  return launch_pass<TV, TL, PASS, false>(a, 0, s);
+}
+
+int launch_init(void* acc0, unsigned long long init0, void* acc1, unsigned long long init1,
+                int64_t count, hipStream_t s) {
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(group_init_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                     (unsigned long long*)acc0, init0, (unsigned long long*)acc1, init1, count);
+  return check_hip(hipGetLastError(), "group_init_kernel launch");
+}
+
+template <typename TV, int KIND>
+int launch_finalize(GArgs a, double* means_out, hipStream_t s) {
+  const int64_t total = a.rows * a.num_labels;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((group_finalize_kernel<TV, KIND>), dim3((unsigned)blocks), dim3(256), 0, s, a,
+                     means_out);
+  return check_hip(hipGetLastError(), "group_finalize_kernel launch");
+}
+
+constexpr bool float_only_kind(int kind) { return kind == kGMean || kind == kGVar || kind == kGStd; }
+
+// LDS budget for privatised accumulators (bytes per workgroup).
+constexpr int64_t kGroupLdsBudget = 48 * 1024;
+
+int64_t lds_bytes_per_label(int kind, int vdtype) {
+  switch (kind) {
+    case kGSum: return 12;
+    case kGMean: return 12;
+    case kGCount: return 4;
+    case kGVar:
+    case kGStd: return 24;
+    case kGArgMax:
+    case kGArgMin: return 16;
+    default: return 8;
+  }
+  (void)vdtype;
+}
+
+// Workspace layout: acc0 | acc1 | acc2 | acc3 (rows x num_labels x 8 bytes each).
+size_t group_ws_bytes(int64_t rows, int64_t L) { return (size_t)(4 * rows * L) * 8; }
+
+template <typename TV, typename TL, int KIND>
+int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) {
+  const int64_t L = a.num_labels;
+  const size_t need = group_ws_bytes(a.rows, L);
+  if (ws == nullptr || ws_bytes < need)
+    return set_error(NBG_ERR_INVALID, "group: workspace too small (%zu < %zu)", ws_bytes, need);
+  const int64_t per = (int64_t)a.rows * L;
+  unsigned long long* acc0 = static_cast<unsigned long long*>(ws);
+  unsigned long long* acc1 = acc0 + per;
+  unsigned long long* acc2 = acc1 + per;
+  unsigned long long* acc3 = acc2 + per;
+  a.acc2 = nullptr;
+  bool lds = L * lds_bytes_per_label(KIND, 0) <= kGroupLdsBudget;
+  if (strategy == NBG_GROUP_LDS && !lds)
+    return set_error(NBG_ERR_INVALID, "group: %lld labels do not fit the LDS strategy", (long long)L);
+  if (strategy == NBG_GROUP_GLOBAL) lds = false;
+
+  // Chunking: about 8 workgroups per CU over all rows, 4096-aligned chunks.
+  const int64_t target_blocks = 2048;
+  int64_t chunks = (target_blocks + a.rows - 1) / a.rows;
+  int64_t chunk = (a.n + chunks - 1) / chunks;
+  if (chunk < 8192) chunk = 8192;
+  chunk = (chunk + 4095) / 4096 * 4096;
+  a.chunk = chunk;
+  a.chunks_per_row = (a.n + chunk - 1) / chunk;
+  if (a.rows * a.chunks_per_row > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "group: grid too large");
+  a.means = nullptr;
+  a.argmin = (KIND == kGArgMin);
+  int rc = 0;
+  const bool packed = a.n < 0xffffffffLL;  // indices fit the low 32 bits
+  switch (KIND) {
+    case kGSum:
+    case kGMean:
+      if ((rc = launch_init(acc0, 0ull, acc1, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = acc1;
+      if ((rc = run_pass<TV, TL, kPSum>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
+    case kGCount:
+      if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = nullptr;
+      if ((rc = run_pass<TV, TL, kPCount>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
+    case kGArgMax:
+    case kGArgMin:
+      if constexpr (sizeof(TV) == 4) {
+        if (packed) {
+          if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
+          a.acc0 = acc0;
+          a.acc1 = nullptr;
+          if ((rc = run_pass<TV, TL, kPPacked>(a, lds, s))) return rc;
+          return launch_finalize<TV, KIND>(a, nullptr, s);
+        }
+      }
+      if ((rc = launch_init(acc0, 0ull, acc1, ~0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = acc1;
+      if ((rc = run_pass<TV, TL, kPKey>(a, lds, s))) return rc;
+      if ((rc = run_pass<TV, TL, kPKeyIdx>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
+    case kGFirst:
+      if ((rc = launch_init(acc0, ~0ull, nullptr, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = nullptr;
+      if ((rc = run_pass<TV, TL, kPFirst>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
+    case kGLast:
+      if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = nullptr;
+      if ((rc = run_pass<TV, TL, kPLast>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
+    default: {  // var / std: per-label means, then sums of deviations
+      if ((rc = launch_init(acc0, 0ull, acc1, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = acc1;
+      if ((rc = run_pass<TV, TL, kPSum>(a, lds, s))) return rc;
+      double* means = reinterpret_cast<double*>(acc2);
+      if ((rc = launch_finalize<TV, KIND>(a, means, s))) return rc;
+      GArgs b = a;
+      b.means = means;
+      b.acc0 = acc0;  // sum of deviations (overwrites the sums)
+      b.acc1 = acc3;  // sum of squared deviations
+      b.acc2 = acc1;  // counts from the first pass
+      if ((rc = launch_init(acc0, 0ull, acc3, 0ull, per, s))) return rc;
+      if ((rc = run_pass<TV, TL, kPDev>(b, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(b, nullptr, s);
+    }
+  }
+}
+
+template <int KIND>
+int group_entry(const char* name, int vdtype, const void* values, int ldtype, const void* labels,
+                void* out, int64_t rows, int64_t n, int64_t ld_v, int64_t num_labels, int64_t ddof,
+                int strategy, void* ws, size_t ws_bytes, void* stream) {
+  if (int rc = injected_error(name)) return rc;
+  if (rows < 0 || n < 0 || num_labels < 0) return set_error(NBG_ERR_INVALID, "%s: negative shape", name);
+  if (strategy < NBG_GROUP_AUTO || strategy > NBG_GROUP_GLOBAL)
+    return set_error(NBG_ERR_INVALID, "%s: unknown strategy %d", name, strategy);
+  if (rows == 0 || num_labels == 0) return NBG_OK;
+  if (values == nullptr && n > 0) return set_error(NBG_ERR_INVALID, "%s: null values", name);
+  if ((labels == nullptr && n > 0) || out == nullptr) return set_error(NBG_ERR_INVALID, "%s: null pointer", name);
+  if (rows > 1 && ld_v < n) return set_error(NBG_ERR_INVALID, "%s: row stride < n", name);
+  const bool float_only = (KIND == kGMean || KIND == kGVar || KIND == kGStd);
+  if (float_only && vdtype != NBG_F32 && vdtype != NBG_F64)
+    return set_error(NBG_ERR_INVALID, "%s: values must be float32 or float64", name);
+  GArgs a{};
+  a.values = values;
+  a.labels = labels;
+  a.out = out;
+  a.rows = rows;
+  a.n = n;
+  a.ld_v = ld_v;
+  a.num_labels = num_labels;
+  a.ddof = ddof;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define NBG_G(TV, TL) return group_impl<TV, TL, KIND>(a, strategy, ws, ws_bytes, s)
+  if (ldtype == NBG_I32) {
+    switch (vdtype) {
+      case NBG_F32: NBG_G(float, int32_t);
+      case NBG_F64: NBG_G(double, int32_t);
+      case NBG_I32: if constexpr (!float_only_kind(KIND)) NBG_G(int32_t, int32_t); break;
+      case NBG_I64: if constexpr (!float_only_kind(KIND)) NBG_G(int64_t, int32_t); break;
+    }
+  } else if (ldtype == NBG_I64) {
+    switch (vdtype) {
+      case NBG_F32: NBG_G(float, int64_t);
+      case NBG_F64: NBG_G(double, int64_t);
+      case NBG_I32: if constexpr (!float_only_kind(KIND)) NBG_G(int32_t, int64_t); break;
+      case NBG_I64: if constexpr (!float_only_kind(KIND)) NBG_G(int64_t, int64_t); break;
+    }
+  } else {
+    return set_error(NBG_ERR_INVALID, "%s: labels must be int32 or int64 (code %d)", name, ldtype);
+  }
+#undef NBG_G
+  return set_error(NBG_ERR_INVALID, "%s: unsupported values dtype code %d", name, vdtype);
+}
+
+}  // namespace
+}  // namespace nbg
+
+extern "C" {
+
+size_t nbg_group_workspace_bytes(int vdtype, int64_t rows, int64_t n, int64_t num_labels,
+                                 int strategy) {
+  (void)vdtype;
+  (void)n;
+  (void)strategy;
+  if (rows <= 0 || num_labels <= 0) return 0;
+  return nbg::group_ws_bytes(rows, num_labels);
+}
+
+#define NBG_GROUP(NAME, KIND)                                                                  \
+  int NAME(int vdtype, const void* values, int ldtype, const void* labels, void* out,          \
+           int64_t rows, int64_t n, int64_t ld_v, int64_t num_labels, int strategy, void* ws,  \
+           size_t ws_bytes, void* stream) {                                                    \
+    return nbg::group_entry<KIND>(#NAME, vdtype, values, ldtype, labels, out, rows, n, ld_v,   \
+                                  num_labels, 0, strategy, ws, ws_bytes, stream);              \
+  }
+#define NBG_GROUP_DDOF(NAME, KIND)                                                             \
+  int NAME(int vdtype, const void* values, int ldtype, const void* labels, void* out,          \
+           int64_t rows, int64_t n, int64_t ld_v, int64_t num_labels, int64_t ddof,            \
+           int strategy, void* ws, size_t ws_bytes, void* stream) {                            \
+    return nbg::group_entry<KIND>(#NAME, vdtype, values, ldtype, labels, out, rows, n, ld_v,   \
+                                  num_labels, ddof, strategy, ws, ws_bytes, stream);           \
+  }
+
+NBG_GROUP(nbg_group_nansum, nbg::kGSum)
+NBG_GROUP(nbg_group_nanmean, nbg::kGMean)
+NBG_GROUP(nbg_group_nancount, nbg::kGCount)
+NBG_GROUP(nbg_group_nanargmax, nbg::kGArgMax)
+NBG_GROUP(nbg_group_nanargmin, nbg::kGArgMin)
+NBG_GROUP(nbg_group_nanfirst, nbg::kGFirst)
+NBG_GROUP(nbg_group_nanlast, nbg::kGLast)
+NBG_GROUP_DDOF(nbg_group_nanvar, nbg::kGVar)
+NBG_GROUP_DDOF(nbg_group_nanstd, nbg::kGStd)
+
+}  // extern "C"

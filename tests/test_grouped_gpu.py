@@ -1,0 +1,170 @@
+"""GPU parity of the grouped reductions.
+
+count, argmax, argmin, first and last are bit-exact against the oracles (C
+restatement and closed form). Float sums/means are compared to the float64
+oracle: float32 outputs within 1 float32 ULP of the float64 result (float64
+accumulation, one final rounding), float64 outputs within 1e-13 relative of
+the sum of |x|; var/std within 1e-12 relative (float64 corrected two-pass).
+Both group-by strategies (LDS-privatised and global atomics) are covered.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose, assert_array_equal
+
+pytestmark = pytest.mark.gpu
+
+EXACT_FUNCS = ["group_nancount", "group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast"]
+FLOAT_FUNCS = ["group_nansum", "group_nanmean", "group_nanvar", "group_nanstd"]
+
+
+def _data(rows, n, L, dtype, seed, zipf=False, p_nan=0.05):
+    rng = np.random.default_rng(seed)
+    if np.issubdtype(dtype, np.floating):
+        v = rng.standard_normal((rows, n)).astype(dtype)
+        v[rng.random(v.shape) < p_nan] = np.nan
+    else:
+        v = rng.integers(-1000, 1000, size=(rows, n)).astype(dtype)
+    if zipf:
+        p = np.arange(1, L + 1, dtype=np.float64) ** -1.1
+        lab = rng.choice(L, size=n, p=p / p.sum()).astype(np.int32)
+    else:
+        lab = rng.integers(-1, L, size=n).astype(np.int32)
+    return v, lab
+
+
+@pytest.fixture(params=["lds", "global"])
+def strategy(request, monkeypatch):
+    monkeypatch.setenv("NBG_GROUP_STRATEGY", request.param)
+    return request.param
+
+
+@pytest.mark.parametrize("func", EXACT_FUNCS)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32, np.int64])
+@pytest.mark.parametrize("shape,L,zipf", [((3, 1000), 12, False), ((4, 300_000), 1000, True), ((2, 70_001), 5000, False)])
+def test_group_exact_outputs(gpu_lib, strategy, func, dtype, shape, L, zipf):
+    import numbagg_amd as nb
+    from oracle import restated
+
+    v, lab = _data(*shape, L, dtype, seed=shape[1] + L, zipf=zipf)
+    if np.issubdtype(dtype, np.floating):
+        v[:, 10:40] = 0.25  # ties
+        v[0, 100] = -0.0
+        v[0, 101] = 0.0
+    got = getattr(nb, func)(v, lab, num_labels=L, axis=-1)
+    want = restated.group(func, v, lab.astype(np.int64), L)
+    assert got.dtype == v.dtype
+    assert_array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("func", FLOAT_FUNCS)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,L,zipf", [((3, 1000), 12, False), ((4, 300_000), 1000, True), ((2, 70_001), 9000, False)])
+def test_group_float_outputs(gpu_lib, strategy, func, dtype, shape, L, zipf):
+    import numbagg_amd as nb
+    from oracle import exact
+
+    v, lab = _data(*shape, L, dtype, seed=shape[1] + 3 * L, zipf=zipf)
+    got = getattr(nb, func)(v, lab, num_labels=L, axis=-1)
+    want = exact.group(func, v, lab, L)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    if func in ("group_nansum", "group_nanmean") and dtype == np.float32:
+        e = want[ok].astype(np.float32)
+        ulp = np.spacing(np.abs(e)).astype(np.float64)
+        err = np.abs(got[ok].astype(np.float64) - want[ok])
+        # float64 accumulation of float32 values is within 1e-15 of sum|x|; after
+        # rounding once that is at most 1 ULP off the rounded float64 oracle.
+        sabs = exact.group("group_nansum", np.abs(v), lab, L)[ok]
+        cnt = exact.group("group_nancount", v, lab, L)[ok]
+        scale = sabs / (cnt if func == "group_nanmean" else 1.0)
+        assert np.all(err <= ulp + 1e-14 * scale)
+    elif func in ("group_nansum", "group_nanmean"):
+        sabs = exact.group("group_nansum", np.abs(v), lab, L)[ok]
+        cnt = exact.group("group_nancount", v, lab, L)[ok]
+        scale = sabs / (cnt if func == "group_nanmean" else 1.0)
+        assert np.all(np.abs(got[ok] - want[ok]) <= 1e-13 * scale + 1e-300)
+    else:
+        rtol = 2e-7 if dtype == np.float32 else 1e-12
+        assert_allclose(got[ok], want[ok], rtol=rtol)
+
+
+def test_group_ddof(gpu_lib):
+    import numbagg_amd as nb
+    from oracle import exact
+
+    v, lab = _data(2, 5000, 30, np.float64, seed=1)
+    for ddof in (0, 1, 5):
+        for func in ("group_nanvar", "group_nanstd"):
+            got = getattr(nb, func)(v, lab, num_labels=30, axis=-1, ddof=ddof)
+            assert_allclose(got, exact.group(func, v, lab, 30, ddof=ddof), rtol=1e-12)
+
+
+def test_group_reference_pandas_vectors(gpu_lib, golden):
+    import numbagg_amd as nb
+
+    g = golden["group_array_1x500"]
+    codes = golden["group_codes"]
+    for func in ["group_nansum", "group_nanmean", "group_nancount", "group_nanfirst", "group_nanlast", "group_nanargmax", "group_nanargmin"]:
+        assert_allclose(getattr(nb, func)(g, codes, num_labels=12, axis=-1), golden[func])
+    for ddof in (0, 1, 5):
+        assert_allclose(nb.group_nanvar(g, codes, num_labels=12, axis=-1, ddof=ddof), golden[f"group_nanvar_ddof{ddof}"])
+        assert_allclose(nb.group_nanstd(g, codes, num_labels=12, axis=-1, ddof=ddof), golden[f"group_nanstd_ddof{ddof}"])
+
+
+def test_group_known_answers_and_axes(gpu_lib):
+    import numbagg_amd as nb
+
+    values = np.array([[1.0, 5.0, 3.0], [2.0, 4.0, 6.0]])
+    labels = np.array([[0, 0, 1], [1, 2, 2]])
+    assert_allclose(nb.group_nanargmax(values, labels), [1.0, 2.0, 5.0])
+    assert_allclose(nb.group_nanargmin(values, labels), [0.0, 3.0, 4.0])
+    v3 = np.arange(24.0).reshape(2, 3, 4)
+    l3 = np.array([[0, 0, 1, 1], [0, 1, 1, 2], [2, 2, 2, 2]])
+    r = nb.group_nanargmax(v3, l3, axis=(1, 2))
+    assert r.shape == (2, 3)
+    assert_allclose(r[0], [4.0, 6.0, 11.0])
+    assert_allclose(nb.group_nanargmin(v3, l3, axis=(1, 2))[0], [0.0, 2.0, 7.0])
+    # empty groups
+    v = np.array([np.nan, np.nan, np.nan])
+    g = np.array([0, 1, 2])
+    np.testing.assert_equal(nb.group_nansum(v, g), [0, 0, 0])
+    assert np.all(np.isnan(nb.group_nanargmin(v, g)))
+    assert np.all(np.isnan(nb.group_nanmean(v, g)))
+    # no valid labels
+    r = nb.group_nansum(np.array([1.0, 2.0, 3.0, 4.0]), np.array([-1, -1, -1, -1]), num_labels=1)
+    assert_allclose(r, [0.0])
+    # integer values, int mean and bool mean (test_grouped.py:359-394, :606-612)
+    assert_allclose(nb.group_nancount(np.array([1, 2, 3, 4, 5], dtype=np.int32), np.array([0, 0, 0, 1, 1], dtype=np.int32)), [3, 2])
+    assert_allclose(nb.group_nanmean(np.array([1, 2, 3, 4, 5, 6], dtype=np.int32), np.array([0, 0, 0, 1, 1, 2], dtype=np.int32)), [2.0, 4.5, 6.0])
+    assert_allclose(nb.group_nanmean(np.array([True, True, False]), np.array([0, 0, 0])), [2 / 3])
+    # 1-D labels along an axis (test_grouped.py:264-303)
+    values2d = np.arange(25.0).reshape(5, 5)
+    assert_allclose(nb.group_nansum(values2d, np.arange(5), axis=1), values2d)
+    assert_allclose(nb.group_nansum(values2d, np.arange(5), axis=0), values2d.T)
+    got = nb.group_nansum(values2d, [0, 0, -1, 1, 1], axis=1)
+    assert_allclose(got, np.stack([values2d[:, :2].sum(1), values2d[:, 3:].sum(1)], axis=-1))
+    # narrow label dtypes and wrap-around integer sums
+    for ldt in (np.int8, np.int16, np.int32, np.int64):
+        assert_allclose(nb.group_nansum(np.arange(5.0), np.arange(5, dtype=ldt)), np.arange(5.0))
+    big = np.full(10, 2**30, dtype=np.int32)
+    assert nb.group_nansum(big, np.zeros(10, dtype=np.int32))[0] == np.int32(np.int64(10 * 2**30) % 2**32 - 2**32)
+
+
+def test_group_zipf_bench_shape(gpu_lib):
+    """A scaled-down version of the (64, 4M) 1000-label benchmark, uniform and
+    Zipf(1.1) labels, checked for every function."""
+    import numbagg_amd as nb
+    from oracle import exact, restated
+
+    for zipf in (False, True):
+        v, lab = _data(4, 1_000_000, 1000, np.float32, seed=7, zipf=zipf)
+        for func in EXACT_FUNCS:
+            got = getattr(nb, func)(v, lab, num_labels=1000, axis=-1)
+            assert_array_equal(got.view(np.uint32), restated.group(func, v, lab, 1000).view(np.uint32))
+        for func in FLOAT_FUNCS:
+            got = getattr(nb, func)(v, lab, num_labels=1000, axis=-1)
+            want = exact.group(func, v, lab, 1000)
+            assert_allclose(got, want, rtol=1e-6, atol=1e-6)

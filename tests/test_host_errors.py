@@ -52,3 +52,18 @@ def test_move_exp_errors():
         nb.move_exp_nanmean(x[None], alpha=0.5, axis=(0, 1))
     with pytest.raises(ValueError, match="empty tuple"):
         nb.move_exp_nancorr(x, x, alpha=0.5, axis=())
+
+
+def test_group_errors():
+    values2d = np.arange(25.0).reshape(5, 5)
+    labels = np.arange(5)
+    with pytest.raises(ValueError, match="axis required"):
+        nb.group_nansum(values2d, labels)
+    with pytest.raises(ValueError, match="must have same shape"):
+        nb.group_nansum(values2d, labels[:4], axis=0)
+    with pytest.raises(ValueError, match="must have same shape"):
+        nb.group_nansum(values2d, labels[:4], axis=(0,))
+    with pytest.raises(TypeError, match="labels must be an integer array"):
+        nb.group_nansum(values2d, labels.astype(float), axis=1)
+    with pytest.raises(TypeError, match="does not support boolean input"):
+        nb.group_nanvar(np.array([True, False]), np.array([0, 0]), num_labels=1)
