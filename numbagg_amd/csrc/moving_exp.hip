@@ -64,6 +64,10 @@ struct ExpSpec<kECorr> {  // s1, s2, s12, sw, sw2, weight, s11, s22
   static constexpr int NS = 8, SW2 = 4, NARGS = 2;
 };
 
+// `seen` counts valid observations, saturating at 2: nansum needs one (NaN before
+// the first value), var/std/cov/corr need two (with a single observation the
+// bias correction 1 - sw2/sw^2 is exactly zero, though rounding can leave it a
+// few ULPs either side of zero).
 template <int NS>
 struct Map {
   double p, p2;
@@ -88,7 +92,7 @@ __device__ __forceinline__ Map<NS> compose(const Map<NS>& earlier, const Map<NS>
   r.p2 = later.p2 * earlier.p2;
 #pragma unroll
   for (int c = 0; c < NS; ++c) r.b[c] = (c == SW2 ? later.p2 : later.p) * earlier.b[c] + later.b[c];
-  r.seen = earlier.seen | later.seen;
+  r.seen = min(2, earlier.seen + later.seen);
   return r;
 }
 
@@ -113,7 +117,7 @@ __device__ __forceinline__ void exp_step(double* s, int& seen, double a, double 
   for (int c = 0; c < Sp::NS; ++c) s[c] *= (c == Sp::SW2 ? d2 : d);
   const bool valid = (Sp::NARGS == 2) ? !(is_nan(a) || is_nan(b)) : !is_nan(a);
   if (!valid) return;
-  seen = 1;
+  seen = min(2, seen + 1);
   if (KIND == kECount) {
     s[0] += 1.0;
     s[1] += alpha;
@@ -154,20 +158,20 @@ __device__ __forceinline__ double exp_out(const double* s, int seen, double mw) 
     const double m = s[1] / s[2];
     const double var_biased = s[0] / s[2] - m * m;
     const double bias = 1.0 - s[3] / (s[2] * s[2]);
-    if (!(s[4] >= mw && bias > 0)) return nan;
+    if (!(s[4] >= mw && bias > 0 && seen >= 2)) return nan;
     const double v = var_biased / bias;
     return KIND == kEStd ? sqrt(v) : v;
   }
   const double bias = 1.0 - s[4] / (s[3] * s[3]);
   if (KIND == kECov) {
     const double cov_biased = (s[2] - s[0] * s[1] / s[3]) / s[3];
-    return (s[5] >= mw && bias > 0) ? cov_biased / bias : nan;
+    return (s[5] >= mw && bias > 0 && seen >= 2) ? cov_biased / bias : nan;
   }
   // kECorr
   const double cov = s[2] - s[0] * s[1] / s[3];
   const double v1 = s[6] - s[0] * s[0] / s[3];
   const double v2 = s[7] - s[1] * s[1] / s[3];
-  if (!(s[5] >= mw && bias > 0)) return nan;
+  if (!(s[5] >= mw && bias > 0 && seen >= 2)) return nan;
   const double den = sqrt(v1 * v2);
   return den > 0 ? cov / den : nan;
 }
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
           const int seen_j = (int)ld_agent_f64(pj + kMaxNS + 3 + NS);
 #pragma unroll
           for (int c = 0; c < NS; ++c) sin[c] = (c == Sp::SW2 ? acc.p2 : acc.p) * sj[c] + acc.b[c];
-          sin_seen = seen_j | acc.seen;
+          sin_seen = min(2, seen_j + acc.seen);
           break;
         }
         if (f == 0u) break;  // expired wait (counted above): continue from zero state
@@ -359,7 +363,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
 #pragma unroll
     for (int c = 0; c < NS; ++c)
       st_agent_f64(pl + kMaxNS + 3 + c, (c == Sp::SW2 ? agg.p2 : agg.p) * sin[c] + agg.b[c]);
-    st_agent_f64(pl + kMaxNS + 3 + NS, (double)(sin_seen | agg.seen));
+    st_agent_f64(pl + kMaxNS + 3 + NS, (double)min(2, sin_seen + agg.seen));
     st_release(&frow[tile], 2u);
 #pragma unroll
     for (int c = 0; c < NS; ++c) s_in[c] = sin[c];
@@ -371,7 +375,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   double s[NS];
 #pragma unroll
   for (int c = 0; c < NS; ++c) s[c] = (c == Sp::SW2 ? excl.p2 : excl.p) * s_in[c] + excl.b[c];
-  int seen = s_in_seen | excl.seen;
+  int seen = min(2, s_in_seen + excl.seen);
   TA res[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) {

@@ -805,7 +805,11 @@ void orc_exact_move(int kind, const double* x, const double* y, double* out, int
 /* Exponentially weighted statistics by the reference's recurrences carried
  * in long double. kind: 0 nancount, 1 nanmean, 2 nansum, 3 nanvar, 4 nanstd,
  * 5 nancov, 6 nancorr (numbagg/moving_exp.py:12-335). alpha holds the loop
- * dtype's values widened to float64; ld_alpha 0 shares one alpha row. */
+ * dtype's values widened to float64; ld_alpha 0 shares one alpha row.
+ * With a single valid observation the bias correction 1 - sw2/sw^2 is exactly
+ * zero, so var/std/cov/corr are NaN until two observations have been seen (the
+ * reference evaluates the expression in floating point, where it can land a
+ * few ULPs either side of zero when alpha varies by position). */
 void orc_exact_move_exp(int kind, const double* x, const double* y, const double* alpha,
                         int64_t ld_alpha, double* out, int64_t rows, int64_t n,
                         double min_weight) {
@@ -816,7 +820,7 @@ void orc_exact_move_exp(int kind, const double* x, const double* y, const double
     const double* al = alpha + r * ld_alpha;
     double* o = out + r * n;
     long double s1 = 0, s2 = 0, s11 = 0, s22 = 0, s12 = 0, sw = 0, sw2 = 0, wt = 0;
-    int seen = 0;
+    int seen = 0, nobs = 0;
     for (int64_t i = 0; i < n; ++i) {
       long double d = 1.0L - (long double)al[i];
       s1 *= d; s2 *= d; s11 *= d; s22 *= d; s12 *= d; sw *= d; sw2 *= d * d; wt *= d;
@@ -826,9 +830,11 @@ void orc_exact_move_exp(int kind, const double* x, const double* y, const double
         s1 += xa; s2 += xb; s11 += xa * xa; s22 += xb * xb; s12 += xa * xb;
         sw += 1; sw2 += 1; wt += al[i];
         seen = 1;
+        nobs += nobs < 2;
       }
       int ok = wt >= (long double)min_weight;
       long double bias = 1 - sw2 / (sw * sw);
+      if (kind >= 3 && nobs < 2) bias = 0;
       double res = NAN;
       switch (kind) {
         case 0: res = ok ? (double)sw : NAN; break;
