@@ -147,6 +147,7 @@ def build_workloads(torch, nb, lib, want_all):
 
     wl.append(Workload("ffill", {"shape": [1024, 1_000_000], "dtype": "f32", "limit": None, "nan": "30% geometric runs"},
                        x.numel() * 8, run_ff, cpu_ff, kernel="fill_kernel"))
+    wl[-1].copy_src = x
     return wl
 
 
@@ -178,6 +179,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--all", action="store_true", help="also time every hot-path function")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-copy", action="store_true", help="skip the streaming-copy reference")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -233,6 +235,26 @@ def main(argv=None):
                        "bytes": w.bytes, "config": w.config}
         print(f"# {w.name}: {ms:.3f} ms  {gbs:.0f} GB/s  {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak", file=sys.stderr, flush=True)
 
+    # Streaming-copy reference on the ffill input: the practical HBM ceiling
+    # (read + write of 8.19 GB), and the kernel used to scale PMC counters.
+    copy_gbs = None
+    if not args.no_copy:
+        src = wl[-1].copy_src
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(3):
+            dst.copy_(src)
+        b.record(stream)
+        torch.cuda.synchronize()
+        copy_ms = a.elapsed_time(b) / 3
+        copy_gbs = 2 * src.numel() * src.element_size() / (copy_ms * 1e-3) / 1e9
+        del dst
+        print(f"# copy reference: {copy_ms:.3f} ms  {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+
     ms_step = wall * 1e3 / args.steps
     total_bytes = sum(w.bytes for w in wl)
     value = total_bytes / (ms_step * 1e-3) / 1e9
@@ -282,6 +304,7 @@ def main(argv=None):
             "functions": {w.name: w.config for w in wl},
             "hbm_peak_GBs": HBM_PEAK_GBS,
             "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+            "copy_reference_GBs": round(copy_gbs, 1) if copy_gbs else None,
         },
         "per_function": per,
         "roofline": {
