@@ -134,6 +134,7 @@ __device__ __forceinline__ void spin_pause() { __builtin_amdgcn_s_sleep(1); }
 // LDS index with one padding slot per 16 elements: threads that each own 16
 // consecutive elements then hit distinct banks on every strided access.
 __host__ __device__ __forceinline__ int64_t pad16(int64_t e) { return e + (e >> 4); }
+__host__ __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 
 // Zero-fill kernel used to reset look-back status words and ticket counters.
 // (A kernel rather than a memset node so the call stays graph-capture safe.)

@@ -26,6 +26,9 @@
 // TwoSum pairs; the moment kernels (var/std/cov/corr) work on data shifted by a
 // per-tile constant with float64 first and second moments; float64 data uses
 // double-double throughout (TwoProd for squares and cross products).
+#include <cstdlib>
+#include <cstring>
+
 #include "common.hpp"
 
 namespace nbg {
@@ -72,10 +75,32 @@ __device__ __forceinline__ void sub_prod(Comp<double>& acc, double a, double b) 
 template <typename T, int KIND>
 struct Traits;
 
-// First-moment-only kinds (sum, mean).
-template <typename T>
+// Exact difference head - tail as an accumulator value: in float64 the
+// difference of two float32 values is exact; TwoSum pairs carry it exactly.
+template <typename A, typename T>
+__device__ __forceinline__ A exact_diff(T h, T t);
+template <>
+__device__ __forceinline__ D exact_diff<D, float>(float h, float t) {
+  return D{(double)h - (double)t};
+}
+template <>
+__device__ __forceinline__ Comp<float> exact_diff<Comp<float>, float>(float h, float t) {
+  Comp<float> r;
+  two_sum(h, -t, r.hi, r.lo);
+  return r;
+}
+template <>
+__device__ __forceinline__ Comp<double> exact_diff<Comp<double>, double>(double h, double t) {
+  Comp<double> r;
+  two_sum(h, -t, r.hi, r.lo);
+  return r;
+}
+
+// First-moment-only kinds (sum, mean). A: float64 (float32 data, default),
+// FP32 TwoSum pair (float32 data, selectable), double-double (float64 data).
+template <typename T, typename A>
 struct SumStats {
-  using A = Comp<T>;  // FP32 TwoSum pair for float32, double-double for float64
+  using Acc = A;
   int c;
   A s;
   __device__ __forceinline__ static SumStats zero() { return SumStats{0, A::zero()}; }
@@ -170,40 +195,56 @@ struct MomStats {
   }
 };
 
-template <typename T>
-__device__ __forceinline__ T mean_of(const Comp<T>& s, int c);
-template <>
-__device__ __forceinline__ float mean_of<float>(const Comp<float>& s, int c) {
-  return (float)(s.to_double() / (double)c);
+// 1/c for a positive count: hardware reciprocal plus one Newton step (about
+// 1 ULP of float64, far below the float32 rounding of the result).
+__device__ __forceinline__ double recip(int c) {
+  const double dc = (double)c;
+  double r = __builtin_amdgcn_rcp(dc);
+  const double e = fma(-dc, r, 1.0);
+  return fma(r, e, r);
 }
-template <>
-__device__ __forceinline__ double mean_of<double>(const Comp<double>& s, int c) {
+
+__device__ __forceinline__ float mean_of(const D& s, int c) { return (float)(s.v * recip(c)); }
+__device__ __forceinline__ float mean_of(const Comp<float>& s, int c) {
+  return (float)(s.to_double() * recip(c));
+}
+__device__ __forceinline__ double mean_of(const Comp<double>& s, int c) {
   const double dc = (double)c;
   const double q = s.hi / dc;
   const double r = fma(-q, dc, s.hi) + s.lo;  // exact remainder of hi, plus lo
   return q + r / dc;
 }
 
-template <typename T>
-struct Traits<T, kSum> {
-  using S = SumStats<T>;
+// ACC32: float32 data accumulates in FP32 TwoSum pairs instead of float64.
+template <typename T, bool ACC32>
+using SumAccT = typename std::conditional<
+    sizeof(T) == 8, Comp<double>, typename std::conditional<ACC32, Comp<float>, D>::type>::type;
+
+template <typename T, int KIND, bool ACC32 = false>
+struct SumTraits {
+  using S = SumStats<T, SumAccT<T, ACC32>>;
   static constexpr int NARGS = 1;
   static constexpr bool SHIFT = false;
   __device__ static T out(const S& s, int64_t mc) {
-    if (s.c < mc) return canonical_nan<T>();
-    return (T)s.s.to_double();
-  }
-};
-template <typename T>
-struct Traits<T, kMean> {
-  using S = SumStats<T>;
-  static constexpr int NARGS = 1;
-  static constexpr bool SHIFT = false;
-  __device__ static T out(const S& s, int64_t mc) {
+    if (KIND == kSum) {
+      if (s.c < mc) return canonical_nan<T>();
+      return (T)s.s.to_double();
+    }
     if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
-    return mean_of<T>(s.s, s.c);
+    return (T)mean_of(s.s, s.c);
   }
 };
+template <typename T>
+struct Traits<T, kSum> : SumTraits<T, kSum> {};
+template <typename T>
+struct Traits<T, kMean> : SumTraits<T, kMean> {};
+// Variants with FP32 TwoSum accumulators for float32 data.
+constexpr int kSum32 = 16 + kSum;
+constexpr int kMean32 = 16 + kMean;
+template <typename T>
+struct Traits<T, kSum32> : SumTraits<T, kSum, true> {};
+template <typename T>
+struct Traits<T, kMean32> : SumTraits<T, kMean, true> {};
 template <typename T, int KIND>
 struct VarTraits {
   using S = MomStats<T, 1>;
@@ -330,9 +371,11 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
   T* ro = out + row * ld_out;
   const int tid = threadIdx.x;
 
-  // Segment staged in LDS: [s0a, t0 + TILE) (TAIL_LDS) or [t0, t0 + TILE).
-  const int64_t seg_begin = TAIL_LDS ? (t0 - window > 0 ? t0 - window : 0) : t0;
-  const int64_t s0a = seg_begin & ~(int64_t)(VEC - 1);
+  // Segment staged in LDS: [s0a, t0 + TILE) with s0a <= t0 - window (TAIL_LDS;
+  // positions before the row start are staged as NaN, so every tail index is
+  // simply head index - window), or [t0, t0 + TILE).
+  const int64_t seg_begin = TAIL_LDS ? t0 - window : t0;
+  const int64_t s0a = seg_begin & ~(int64_t)(VEC - 1);  // floor, also for negatives
   const int64_t seg_len = t0 + TILE - s0a;
   const int64_t lds_elems = pad16(seg_len) + 16;
   T* la = reinterpret_cast<T*>(smem_raw);
@@ -341,7 +384,7 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
   // 1. stage the segment (wave-striped vector loads, NaN past the row end).
   for (int64_t e = (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
     const int64_t p = s0a + e;
-    if (VEC > 1 && p + VEC <= n) {
+    if (VEC > 1 && p >= 0 && p + VEC <= n) {
       Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(ra + p);
 #pragma unroll
       for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
@@ -354,26 +397,27 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
         const int64_t q = p + v;
-        la[pad16(e + v)] = (q < n) ? ra[q] : canonical_nan<T>();
-        if (Tr::NARGS == 2) lb[pad16(e + v)] = (q < n) ? rb[q] : canonical_nan<T>();
+        const bool in = q >= 0 && q < n;
+        la[pad16(e + v)] = in ? ra[q] : canonical_nan<T>();
+        if (Tr::NARGS == 2) lb[pad16(e + v)] = in ? rb[q] : canonical_nan<T>();
       }
     }
   }
   if (Tr::SHIFT && tid < 2) s_shift[tid] = T(0);
   __syncthreads();
 
-  // Per-tile shifts for the moment kinds: the first valid value among the first
-  // 256 staged inputs (any constant works; one near the data keeps the float64
+  // Per-tile shifts for the moment kinds: the first valid value among the tile's
+  // first 256 inputs (any constant works; one near the data keeps the float64
   // moments small).
   T ka = T(0), kb = T(0);
   if (Tr::SHIFT) {
     __shared__ int s_first[2];
     if (tid < 2) s_first[tid] = 0x7fffffff;
     __syncthreads();
-    if (tid < seg_len) {
-      if (!is_nan(la[pad16(tid)])) atomicMin(&s_first[0], tid);
-      if (Tr::NARGS == 2 && !is_nan(lb[pad16(tid)])) atomicMin(&s_first[1], tid);
-    }
+    // first 256 inputs of the tile itself (always staged; NaN past the row end)
+    const int e = (int)(t0 - s0a) + tid;
+    if (!is_nan(la[pad16(e)])) atomicMin(&s_first[0], e);
+    if (Tr::NARGS == 2 && !is_nan(lb[pad16(e)])) atomicMin(&s_first[1], e);
     __syncthreads();
     ka = s_first[0] != 0x7fffffff ? la[pad16(s_first[0])] : T(0);
     if (Tr::NARGS == 2) kb = s_first[1] != 0x7fffffff ? lb[pad16(s_first[1])] : T(0);
@@ -385,17 +429,18 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
 
   // 2. halo statistics W0 = window ending at t0-1.
   S w0 = S::zero();
-  {
+  if (TAIL_LDS) {
+    const int e_end = (int)(t0 - s0a);
+    for (int e = (int)(t0 - window - s0a) + tid; e < e_end; e += kMoveThreads) {
+      const T a = la[pad16(e)];
+      const T b = (Tr::NARGS == 2) ? lb[pad16(e)] : T(0);
+      if (valid(a, b)) w0.add_obs(a, b, ka, kb);
+    }
+  } else {
     const int64_t h_begin = t0 - window > 0 ? t0 - window : 0;
     for (int64_t p = h_begin + tid; p < t0; p += kMoveThreads) {
-      T a, b = T(0);
-      if (TAIL_LDS) {
-        a = la[pad16(p - s0a)];
-        if (Tr::NARGS == 2) b = lb[pad16(p - s0a)];
-      } else {
-        a = ra[p];
-        if (Tr::NARGS == 2) b = rb[p];
-      }
+      const T a = ra[p];
+      const T b = (Tr::NARGS == 2) ? rb[p] : T(0);
       if (valid(a, b)) w0.add_obs(a, b, ka, kb);
     }
   }
@@ -404,47 +449,77 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
   // 3. head/tail chunks and the per-thread delta. Chunks are re-read from LDS
   //    (or global memory for long windows) rather than held in registers.
   const int64_t c0 = t0 + (int64_t)tid * K;  // first output position of this thread
-  const int64_t eh = TAIL_LDS ? c0 - s0a : (int64_t)tid * K;  // its LDS element index
+  // LDS element index of the thread's first head / tail input (32-bit).
+  const int eh = TAIL_LDS ? (int)(c0 - s0a) : tid * K;
+  const int et = eh - (int)(TAIL_LDS ? window : 0);
   auto head = [&](int j, T& a, T& b) {
     a = la[pad16(eh + j)];
     b = (Tr::NARGS == 2) ? lb[pad16(eh + j)] : T(0);
   };
   auto tail = [&](int j, T& a, T& b) {
-    const int64_t pt = c0 + j - window;
-    if (pt < 0 || pt >= n) {
-      a = canonical_nan<T>();
-      b = canonical_nan<T>();
-    } else if (TAIL_LDS) {
-      a = la[pad16(pt - s0a)];
-      b = (Tr::NARGS == 2) ? lb[pad16(pt - s0a)] : T(0);
+    if (TAIL_LDS) {
+      a = la[pad16(et + j)];
+      b = (Tr::NARGS == 2) ? lb[pad16(et + j)] : T(0);
     } else {
-      a = ra[pt];
-      b = (Tr::NARGS == 2) ? rb[pt] : T(0);
+      const int64_t pt = c0 + j - window;
+      const bool in = pt >= 0 && pt < n;
+      a = in ? ra[pt] : canonical_nan<T>();
+      b = (Tr::NARGS == 2 && in) ? rb[pt] : canonical_nan<T>();
     }
   };
-  S delta = S::zero();
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    T a, b;
-    head(j, a, b);
-    if (valid(a, b)) delta.add_obs(a, b, ka, kb);
-    tail(j, a, b);
-    if (valid(a, b)) delta.sub_obs(a, b, ka, kb);
-  }
-  S total;
-  S st = block_excl_scan(delta, total, sh);
-  st.merge(w0);  // window ending at c0 - 1
-
-  // 4. sequential add/remove over the thread's chunk.
   T res[K];
+  if constexpr (!Tr::SHIFT) {
+    // Sum/mean: the exact per-position differences head - tail stay in
+    // registers between the scan and the output pass.
+    using Acc = typename S::Acc;
+    Acc dj[K];
+    unsigned int hbits = 0u, tbits = 0u;
+    S delta = S::zero();
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    T a, b;
-    head(j, a, b);
-    if (valid(a, b)) st.add_obs(a, b, ka, kb);
-    tail(j, a, b);
-    if (valid(a, b)) st.sub_obs(a, b, ka, kb);
-    res[j] = Tr::out(st, min_count);
+    for (int j = 0; j < K; ++j) {
+      T h, t, u;
+      head(j, h, u);
+      tail(j, t, u);
+      const bool hv = !is_nan(h), tv = !is_nan(t);
+      dj[j] = exact_diff<Acc, T>(hv ? h : T(0), tv ? t : T(0));
+      hbits |= (unsigned)hv << j;
+      tbits |= (unsigned)tv << j;
+      delta.s.add(dj[j]);
+      delta.c += (int)hv - (int)tv;
+    }
+    S total;
+    S st = block_excl_scan(delta, total, sh);
+    st.merge(w0);  // window ending at c0 - 1
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      st.c += (int)((hbits >> j) & 1u) - (int)((tbits >> j) & 1u);
+      st.s.add(dj[j]);
+      res[j] = Tr::out(st, min_count);
+    }
+  } else {
+    S delta = S::zero();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      T a, b;
+      head(j, a, b);
+      if (valid(a, b)) delta.add_obs(a, b, ka, kb);
+      tail(j, a, b);
+      if (valid(a, b)) delta.sub_obs(a, b, ka, kb);
+    }
+    S total;
+    S st = block_excl_scan(delta, total, sh);
+    st.merge(w0);  // window ending at c0 - 1
+
+    // 4. sequential add/remove over the thread's chunk.
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      T a, b;
+      head(j, a, b);
+      if (valid(a, b)) st.add_obs(a, b, ka, kb);
+      tail(j, a, b);
+      if (valid(a, b)) st.sub_obs(a, b, ka, kb);
+      res[j] = Tr::out(st, min_count);
+    }
   }
   if (c0 + K <= n && VEC > 1) {
 #pragma unroll
@@ -508,6 +583,14 @@ int move_launch(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64_t
   return check_hip(hipGetLastError(), "move_kernel launch");
 }
 
+// NBG_F32_ACCUMULATOR=twosum selects FP32 TwoSum accumulators for float32
+// sum/mean instead of float64 (the float64 default is both more precise and
+// cheaper on CDNA4, where a float64 add issues at the float32 rate).
+bool f32_twosum_requested() {
+  const char* v = getenv("NBG_F32_ACCUMULATOR");
+  return v != nullptr && strcmp(v, "twosum") == 0;
+}
+
 template <int KIND>
 int move_entry(const char* name, int dtype, const void* a, const void* b, void* out, int64_t rows,
                int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window,
@@ -525,6 +608,11 @@ int move_entry(const char* name, int dtype, const void* a, const void* b, void* 
   hipStream_t s = static_cast<hipStream_t>(stream);
   switch (dtype) {
     case NBG_F32:
+      if constexpr (KIND == kSum || KIND == kMean) {
+        if (f32_twosum_requested())
+          return move_launch<float, KIND + 16>((const float*)a, (const float*)b, (float*)out, rows,
+                                               n, ld_a, ld_b, ld_out, window, min_count, s);
+      }
       return move_launch<float, KIND>((const float*)a, (const float*)b, (float*)out, rows, n, ld_a,
                                       ld_b, ld_out, window, min_count, s);
     case NBG_F64:
