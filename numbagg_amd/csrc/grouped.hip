@@ -98,6 +98,7 @@ struct GArgs {
   void* out;
   int64_t rows, n, ld_v, num_labels, ddof;
   int64_t chunk, chunks_per_row;
+  int64_t chunk_r4;  // chunk length when four rows share a workgroup
   // global accumulators (rows x num_labels each)
   void* acc0;
   void* acc1;
@@ -111,27 +112,31 @@ __device__ __forceinline__ void g_atomic_add(unsigned long long* p, unsigned lon
   atomicAdd(p, v);
 }
 
-// One accumulation pass over a (row, chunk). LDS: per-label accumulators.
-template <typename TV, typename TL, int PASS, bool LDS>
+// One accumulation pass over R rows x one chunk of positions. The R rows
+// share each label load. LDS (privatised strategy): per-(row, label)
+// accumulators, flushed once with global atomics.
+template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT>
 __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   using SA = typename SumAcc<TV>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int64_t L = A.num_labels;
-  const int64_t row = (int64_t)(blockIdx.x / (uint64_t)A.chunks_per_row);
+  const int64_t L1 = A.num_labels;        // labels per row
+  const int64_t L = L1 * R;               // LDS slots (row-major: r * L1 + label)
+  const int64_t row0 = (int64_t)(blockIdx.x / (uint64_t)A.chunks_per_row) * R;
   const int64_t ck = (int64_t)(blockIdx.x % (uint64_t)A.chunks_per_row);
   const int64_t c0 = ck * A.chunk;
   const int64_t c1 = (c0 + A.chunk < A.n) ? c0 + A.chunk : A.n;
-  const TV* vr = static_cast<const TV*>(A.values) + row * A.ld_v;
+  const int nr = (int)((A.rows - row0) < R ? (A.rows - row0) : R);  // rows present
+  const TV* vbase = static_cast<const TV*>(A.values) + row0 * A.ld_v;
   const TL* lab = static_cast<const TL*>(A.labels);
   const int tid = threadIdx.x;
 
-  // Global accumulators of this row.
-  SA* g_s = static_cast<SA*>(A.acc0) + row * L;
-  unsigned long long* g_u0 = static_cast<unsigned long long*>(A.acc0) + row * L;
-  unsigned long long* g_u1 = static_cast<unsigned long long*>(A.acc1) + row * L;
-  double* g_d0 = static_cast<double*>(A.acc0) + row * L;
-  double* g_d1 = static_cast<double*>(A.acc1) + row * L;
-  const double* g_mean = A.means ? A.means + row * L : nullptr;
+  // Global accumulators of these rows (slot r * L1 + label, rows contiguous).
+  SA* g_s = static_cast<SA*>(A.acc0) + row0 * L1;
+  unsigned long long* g_u0 = static_cast<unsigned long long*>(A.acc0) + row0 * L1;
+  unsigned long long* g_u1 = static_cast<unsigned long long*>(A.acc1) + row0 * L1;
+  double* g_d0 = static_cast<double*>(A.acc0) + row0 * L1;
+  double* g_d1 = static_cast<double*>(A.acc1) + row0 * L1;
+  const double* g_mean = A.means ? A.means + row0 * L1 : nullptr;
 
   // LDS accumulators.
   SA* l_s = reinterpret_cast<SA*>(smem_raw);
@@ -142,11 +147,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   double* l_d1 = l_d0 + L;
   double* l_mean = l_d0 + 2 * L;
   unsigned long long* l_key = l_u + L;  // kPKeyIdx: staged per-label maxima
+  const int64_t Lp = (int64_t)nr * L1;  // slots actually used
   if (LDS) {
-    for (int64_t l = tid; l < L; l += kGroupThreads) {
+    for (int64_t l = tid; l < Lp; l += kGroupThreads) {
       if constexpr (PASS == kPSum) {
         l_s[l] = SA(0);
-        l_c[l] = 0u;
+        if (COUNT) l_c[l] = 0u;
       } else if constexpr (PASS == kPCount) {
         l_u32[l] = 0u;
       } else if constexpr (PASS == kPPacked || PASS == kPKey || PASS == kPLast) {
@@ -165,17 +171,18 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
     __syncthreads();
   }
 
-  auto update = [&](TV v, TL lbl, int64_t i) {
-    if (lbl < 0 || (int64_t)lbl >= L) return;
+  // Update slot r * L1 + label with value v at position i.
+  auto update = [&](TV v, TL lbl, int64_t i, int r) {
+    if (lbl < 0 || (int64_t)lbl >= L1) return;
     if (!gvalid(v)) return;
-    const int64_t l = (int64_t)lbl;
+    const int64_t l = (int64_t)r * L1 + (int64_t)lbl;
     if constexpr (PASS == kPSum) {
       if (LDS) {
         atomicAdd(&l_s[l], (SA)v);  // ds_add_f64 / ds_add_u64
-        atomicAdd(&l_c[l], 1u);
+        if (COUNT) atomicAdd(&l_c[l], 1u);
       } else {
         g_atomic_add(&g_s[l], (SA)v);
-        atomicAdd(&g_u1[l], 1ull);
+        if (COUNT) atomicAdd(&g_u1[l], 1ull);
       }
     } else if constexpr (PASS == kPCount) {
       if (LDS) atomicAdd(&l_u32[l], 1u);
@@ -223,36 +230,48 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   constexpr int VE = VV < VL ? VV : VL;  // elements per vector step
   constexpr int64_t STEP = (int64_t)kGroupThreads * VE;
   constexpr int64_t SUPER = STEP * kGroupUnroll;
-  const bool aligned = (((uintptr_t)(vr + c0)) % 16 == 0) && (((uintptr_t)(lab + c0)) % 16 == 0);
+  const bool aligned = (((uintptr_t)(vbase + c0)) % 16 == 0) && (((uintptr_t)(lab + c0)) % 16 == 0) &&
+                       (nr == 1 || (A.ld_v % VE) == 0);
   const int64_t nfull = aligned ? (c1 - c0) / SUPER : 0;
   {
     struct alignas(16) V { TV v[VE]; };
     struct alignas(VE * sizeof(TL)) W { TL v[VE]; };
     for (int64_t k = 0; k < nfull; ++k) {
       const int64_t base = c0 + k * SUPER + (int64_t)tid * VE;
-      V vv[kGroupUnroll];
       W ll[kGroupUnroll];
 #pragma unroll
-      for (int u = 0; u < kGroupUnroll; ++u) {
-        vv[u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
-        ll[u] = *reinterpret_cast<const W*>(lab + base + u * STEP);
+      for (int u = 0; u < kGroupUnroll; ++u) ll[u] = *reinterpret_cast<const W*>(lab + base + u * STEP);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r >= nr) break;
+        const TV* vr = vbase + (int64_t)r * A.ld_v;
+        V vv[kGroupUnroll];
+#pragma unroll
+        for (int u = 0; u < kGroupUnroll; ++u) vv[u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+#pragma unroll
+        for (int u = 0; u < kGroupUnroll; ++u)
+#pragma unroll
+          for (int e = 0; e < VE; ++e) update(vv[u].v[e], ll[u].v[e], base + u * STEP + e, r);
       }
-#pragma unroll
-      for (int u = 0; u < kGroupUnroll; ++u)
-#pragma unroll
-        for (int e = 0; e < VE; ++e) update(vv[u].v[e], ll[u].v[e], base + u * STEP + e);
     }
   }
-  for (int64_t j = c0 + nfull * SUPER + tid; j < c1; j += kGroupThreads) update(vr[j], lab[j], j);
+  for (int64_t j = c0 + nfull * SUPER + tid; j < c1; j += kGroupThreads) {
+    const TL lj = lab[j];
+    for (int r = 0; r < nr; ++r) update(vbase[(int64_t)r * A.ld_v + j], lj, j, r);
+  }
 
   if (LDS) {
     __syncthreads();
-    for (int64_t l = tid; l < L; l += kGroupThreads) {
+    for (int64_t l = tid; l < Lp; l += kGroupThreads) {
       if constexpr (PASS == kPSum) {
-        const unsigned int c = l_c[l];
-        if (c) {
+        if (COUNT) {
+          const unsigned int c = l_c[l];
+          if (c) {
+            g_atomic_add(&g_s[l], l_s[l]);
+            atomicAdd(&g_u1[l], (unsigned long long)c);
+          }
+        } else if (l_s[l] != SA(0)) {
           g_atomic_add(&g_s[l], l_s[l]);
-          atomicAdd(&g_u1[l], (unsigned long long)c);
         }
       } else if constexpr (PASS == kPCount) {
         if (l_u32[l]) atomicAdd(&g_u0[l], (unsigned long long)l_u32[l]);
@@ -347,27 +366,46 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
   }
 }
 
-template <typename TV, typename TL, int PASS, bool LDS>
+// LDS budget for privatised accumulators (bytes per workgroup).
+constexpr int64_t kGroupLdsBudget = 48 * 1024;
+
+// LDS bytes per (row, label) slot of a pass.
+int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
+  switch (pass) {
+    case kPSum: return sa_bytes + (count ? 4 : 0);
+    case kPCount: return 4;
+    case kPDev: return 24;
+    case kPKeyIdx: return 16;
+    default: return 8;
+  }
+}
+
+template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT>
 int launch_pass(GArgs a, size_t lds, hipStream_t s) {
-  const int64_t blocks = a.rows * a.chunks_per_row;
-  hipLaunchKernelGGL((group_pass_kernel<TV, TL, PASS, LDS>), dim3((unsigned)blocks),
+  a.chunks_per_row = (a.n + a.chunk - 1) / a.chunk;
+  const int64_t groups = (a.rows + R - 1) / R;
+  const int64_t blocks = groups * a.chunks_per_row;
+  if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "group: grid too large");
+  hipLaunchKernelGGL((group_pass_kernel<TV, TL, PASS, LDS, R, COUNT>), dim3((unsigned)blocks),
                      dim3(kGroupThreads), LDS ? lds : 0, s, a);
   return check_hip(hipGetLastError(), "group_pass_kernel launch");
 }
 
-template <typename TV, typename TL, int PASS>
+// Rows per workgroup for the LDS strategy: 4 when four rows' accumulators fit
+// the LDS budget (the rows then share each label load), else 1.
+constexpr int kGroupRowsPerBlock = 4;
+
+template <typename TV, typename TL, int PASS, bool COUNT = true>
 int run_pass(GArgs a, bool lds, hipStream_t s) {
-  size_t bytes = 0;
   const int64_t L = a.num_labels;
-  switch (PASS) {
-    case kPSum: bytes = L * (sizeof(typename SumAcc<TV>::type) + 4); break;
-    case kPCount: bytes = L * 4; break;
-    case kPDev: bytes = L * 24; break;
-    case kPKeyIdx: bytes = L * 16; break;
-    default: bytes = L * 8; break;
+  const int64_t slot = pass_slot_bytes(PASS, COUNT, sizeof(typename SumAcc<TV>::type));
+  if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
+  if (a.rows >= kGroupRowsPerBlock && L * slot * kGroupRowsPerBlock <= kGroupLdsBudget) {
+    a.chunk = a.chunk_r4;
+    return launch_pass<TV, TL, PASS, true, kGroupRowsPerBlock, COUNT>(
+        a, (size_t)(L * slot * kGroupRowsPerBlock), s);
   }
-  if (lds) return launch_pass<TV, TL, PASS, true>(a, bytes, s);
-  return launch_pass<TV, TL, PASS, false>(a, 0, s);
+  return launch_pass<TV, TL, PASS, true, 1, COUNT>(a, (size_t)(L * slot), s);
 }
 
 int launch_init(void* acc0, unsigned long long init0, void* acc1, unsigned long long init1,
@@ -392,9 +430,6 @@ int launch_finalize(GArgs a, double* means_out, hipStream_t s) {
 }
 
 constexpr bool float_only_kind(int kind) { return kind == kGMean || kind == kGVar || kind == kGStd; }
-
-// LDS budget for privatised accumulators (bytes per workgroup).
-constexpr int64_t kGroupLdsBudget = 48 * 1024;
 
 int64_t lds_bytes_per_label(int kind, int vdtype) {
   switch (kind) {
@@ -430,21 +465,28 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
     return set_error(NBG_ERR_INVALID, "group: %lld labels do not fit the LDS strategy", (long long)L);
   if (strategy == NBG_GROUP_GLOBAL) lds = false;
 
-  // Chunking: about 8 workgroups per CU over all rows, 4096-aligned chunks.
-  const int64_t target_blocks = 2048;
-  int64_t chunks = (target_blocks + a.rows - 1) / a.rows;
-  int64_t chunk = (a.n + chunks - 1) / chunks;
-  if (chunk < 8192) chunk = 8192;
-  chunk = (chunk + 4095) / 4096 * 4096;
-  a.chunk = chunk;
-  a.chunks_per_row = (a.n + chunk - 1) / chunk;
-  if (a.rows * a.chunks_per_row > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "group: grid too large");
+  // Chunking: about 8 workgroups per CU over all row groups, 4096-aligned.
+  auto pick_chunk = [&](int64_t row_groups) {
+    const int64_t target_blocks = 2048;
+    int64_t chunks = (target_blocks + row_groups - 1) / row_groups;
+    int64_t chunk = (a.n + chunks - 1) / chunks;
+    if (chunk < 8192) chunk = 8192;
+    return (chunk + 4095) / 4096 * 4096;
+  };
+  a.chunk = pick_chunk(a.rows);
+  a.chunk_r4 = pick_chunk((a.rows + kGroupRowsPerBlock - 1) / kGroupRowsPerBlock);
+  a.chunks_per_row = (a.n + a.chunk - 1) / a.chunk;
   a.means = nullptr;
   a.argmin = (KIND == kGArgMin);
   int rc = 0;
   const bool packed = a.n < 0xffffffffLL;  // indices fit the low 32 bits
   switch (KIND) {
     case kGSum:
+      if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = nullptr;
+      if ((rc = run_pass<TV, TL, kPSum, false>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
     case kGMean:
       if ((rc = launch_init(acc0, 0ull, acc1, 0ull, per, s))) return rc;
       a.acc0 = acc0;
