@@ -199,6 +199,27 @@ int nbg_group_nanstd(int vdtype, const void* values, int ldtype,
                      int strategy, void* ws, size_t ws_bytes,
                      void* stream); /* grouped.py:179-208 */
 
+/* ---- ragged batched serving path (no reference counterpart: its reference
+ *      semantics are per-series plain calls of move_mean, moving.py:12-55,
+ *      and group_nanmean, grouped.py:7-27) ---------------------------------- */
+/* nseries float32 series concatenated in `values` (series s occupies
+ * [offsets[s], offsets[s+1])), with int32 labels laid out alike; series s has
+ * label_offsets[s+1] - label_offsets[s] labels and its group means go to
+ * group_out[label_offsets[s] : label_offsets[s+1]]. Work items are 4096-long
+ * tiles of each series: tile_offsets[s] is the first item of series s and
+ * tile_series[t] the series of item t (total_tiles items). strategy[s]: 0 =
+ * LDS-privatised label accumulators (up to 1024 labels), 1 = global atomics.
+ * One persistent launch with an atomic work queue; move_out is bitwise equal
+ * to per-series move_mean calls. All arrays are device arrays. */
+size_t nbg_ragged_workspace_bytes(int64_t nseries, int64_t total_labels);
+int nbg_ragged_move_mean_group_nanmean(
+    const float* values, const int32_t* labels, const int64_t* offsets,
+    int64_t nseries, const int64_t* label_offsets, int64_t total_labels,
+    const int64_t* tile_offsets, const int32_t* tile_series,
+    int64_t total_tiles, const uint8_t* strategy, int64_t window,
+    int64_t min_count, float* move_out, float* group_out, void* ws,
+    size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,12 @@ for _name in ("nansum", "nanmean", "nancount", "nanargmax", "nanargmin", "nanfir
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP
 for _name in ("nanvar", "nanstd"):
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP_DDOF
+_SIGNATURES["nbg_ragged_workspace_bytes"] = (_c_size, [_c_i64, _c_i64])
+_SIGNATURES["nbg_ragged_move_mean_group_nanmean"] = (
+    _c_int,
+    [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_i64, _c_i64,
+     _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr],
+)
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None

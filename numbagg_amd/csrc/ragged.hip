@@ -1,0 +1,219 @@
+// Ragged batched serving path: many float32 series of uneven lengths, each
+// with move_mean (window w) and group_nanmean over its own int32 labels, in one
+// persistent launch. No reference counterpart: its reference semantics are
+// per-series plain calls of move_mean (numbagg/moving.py:12-55) and
+// group_nanmean (numbagg/grouped.py:7-27).
+//
+// Design:
+//   * work items are (series, tile) pairs, tile = 4096 positions from the
+//     series start, exactly the tiles a plain move_mean launch on that series
+//     uses, so the move_mean outputs are bitwise identical to plain launches;
+//   * a persistent grid (one resident wave of workgroups) pulls items from an
+//     atomic work-queue counter, a few items per grab; every workgroup leaves
+//     the loop once the counter passes the last item;
+//   * each item evaluates its rolling tile (halo staged in LDS) and then folds
+//     the tile's values into the series' label accumulators, either in
+//     LDS-privatised slots flushed once per tile (few labels, or skewed labels)
+//     or straight into global float64/uint32 accumulators (many labels) — the
+//     strategy is chosen per series by the host (heuristic, autotuner or cost
+//     model);
+//   * a finalize kernel writes group means (float64 sum / count, rounded once).
+#include "rolling.hpp"
+
+namespace nbg {
+namespace {
+
+constexpr int kRaggedGrab = 4;       // work items per queue grab
+constexpr int kRaggedLdsLabels = 1024;  // max labels of the LDS strategy
+
+struct RaggedArgs {
+  const float* values;
+  const int32_t* labels;
+  const int64_t* offsets;        // nseries + 1
+  const int64_t* label_offsets;  // nseries + 1
+  const int64_t* tile_offsets;   // nseries + 1
+  const int32_t* tile_series;    // total_tiles
+  const uint8_t* strategy;       // nseries: 0 = LDS-privatised, 1 = global atomics
+  float* move_out;
+  double* gsum;
+  unsigned int* gcount;
+  unsigned int* queue;
+  int64_t total_tiles, window, min_count;
+  size_t stage_bytes;  // dynamic LDS used by the rolling tile
+};
+
+__global__ void ragged_init_kernel(double* gsum, unsigned int* gcount, int64_t total_labels,
+                                   unsigned int* queue) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total_labels; i += stride) {
+    gsum[i] = 0.0;
+    gcount[i] = 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
+}
+
+template <int VEC>
+__device__ __forceinline__ void ragged_rolling(const RaggedArgs& A, int64_t off, int64_t n,
+                                               int64_t tile, unsigned char* smem,
+                                               RollingShared<Traits<float, kMean>::S>& rs) {
+  rolling_tile<float, kMean, true, VEC>(A.values + off, nullptr, A.move_out + off, n, A.window,
+                                        A.min_count, tile, smem, rs);
+}
+
+__global__ __launch_bounds__(kMoveThreads) void ragged_kernel(RaggedArgs A) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ RollingShared<Traits<float, kMean>::S> rs;
+  __shared__ long long s_base;
+  __shared__ int s_series;
+  constexpr int TILE = kMoveThreads * 16;
+  const int tid = threadIdx.x;
+  double* l_sum = reinterpret_cast<double*>(smem_raw + A.stage_bytes);
+  unsigned int* l_cnt = reinterpret_cast<unsigned int*>(l_sum + kRaggedLdsLabels);
+
+  while (true) {
+    if (tid == 0) s_base = (long long)atomicAdd(A.queue, (unsigned)kRaggedGrab);
+    __syncthreads();
+    const int64_t base = s_base;
+    __syncthreads();
+    if (base >= A.total_tiles) break;  // uniform: every workgroup exits here
+    const int64_t stop = base + kRaggedGrab < A.total_tiles ? base + kRaggedGrab : A.total_tiles;
+    for (int64_t t = base; t < stop; ++t) {
+      if (tid == 0) s_series = A.tile_series[t];
+      __syncthreads();
+      const int s = s_series;
+      const int64_t off = A.offsets[s];
+      const int64_t n = A.offsets[s + 1] - off;
+      const int64_t tile = t - A.tile_offsets[s];
+      // 1. move_mean of this tile (identical arithmetic to the plain kernel).
+      if ((((uintptr_t)(A.values + off)) % 16 == 0) && (((uintptr_t)(A.move_out + off)) % 16 == 0))
+        ragged_rolling<4>(A, off, n, tile, smem_raw, rs);
+      else
+        ragged_rolling<1>(A, off, n, tile, smem_raw, rs);
+      // 2. group_nanmean accumulation of this tile's values.
+      const int64_t lo = A.label_offsets[s];
+      const int64_t K = A.label_offsets[s + 1] - lo;
+      const int64_t p0 = tile * TILE;
+      const int64_t p1 = p0 + TILE < n ? p0 + TILE : n;
+      const float* v = A.values + off;
+      const int32_t* lab = A.labels + off;
+      if (A.strategy[s] == 0 && K <= kRaggedLdsLabels) {
+        for (int64_t l = tid; l < K; l += kMoveThreads) {
+          l_sum[l] = 0.0;
+          l_cnt[l] = 0u;
+        }
+        __syncthreads();
+        for (int64_t p = p0 + tid; p < p1; p += kMoveThreads) {
+          const int32_t l = lab[p];
+          const float x = v[p];
+          if (l >= 0 && l < K && !is_nan(x)) {
+            atomicAdd(&l_sum[l], (double)x);
+            atomicAdd(&l_cnt[l], 1u);
+          }
+        }
+        __syncthreads();
+        for (int64_t l = tid; l < K; l += kMoveThreads) {
+          const unsigned int c = l_cnt[l];
+          if (c) {
+            unsafeAtomicAdd(&A.gsum[lo + l], l_sum[l]);
+            atomicAdd(&A.gcount[lo + l], c);
+          }
+        }
+        __syncthreads();
+      } else {
+        for (int64_t p = p0 + tid; p < p1; p += kMoveThreads) {
+          const int32_t l = lab[p];
+          const float x = v[p];
+          if (l >= 0 && l < K && !is_nan(x)) {
+            unsafeAtomicAdd(&A.gsum[lo + l], (double)x);
+            atomicAdd(&A.gcount[lo + l], 1u);
+          }
+        }
+      }
+    }
+  }
+}
+
+__global__ void ragged_finalize_kernel(const double* gsum, const unsigned int* gcount, float* out,
+                                       int64_t total_labels) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total_labels; i += stride) {
+    const unsigned int c = gcount[i];
+    out[i] = c ? (float)(gsum[i] / (double)c) : canonical_nan<float>();
+  }
+}
+
+size_t ragged_ws_bytes(int64_t total_labels) {
+  return (size_t)total_labels * 8 + (size_t)total_labels * 4 + 256;
+}
+
+}  // namespace
+}  // namespace nbg
+
+extern "C" {
+
+size_t nbg_ragged_workspace_bytes(int64_t nseries, int64_t total_labels) {
+  (void)nseries;
+  return nbg::ragged_ws_bytes(total_labels);
+}
+
+int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* labels,
+                                       const int64_t* offsets, int64_t nseries,
+                                       const int64_t* label_offsets, int64_t total_labels,
+                                       const int64_t* tile_offsets, const int32_t* tile_series,
+                                       int64_t total_tiles, const uint8_t* strategy,
+                                       int64_t window, int64_t min_count, float* move_out,
+                                       float* group_out, void* ws, size_t ws_bytes, void* stream) {
+  using namespace nbg;
+  if (int rc = injected_error("nbg_ragged_move_mean_group_nanmean")) return rc;
+  if (nseries < 0 || total_labels < 0 || total_tiles < 0)
+    return set_error(NBG_ERR_INVALID, "ragged: negative size");
+  if (window <= 0) return set_error(NBG_ERR_INVALID, "ragged: window not in valid range: %lld", (long long)window);
+  if (min_count < 0) return set_error(NBG_ERR_INVALID, "ragged: min_count must be positive: %lld", (long long)min_count);
+  if (window > lds_window_limit<float, 1>())
+    return set_error(NBG_ERR_INVALID, "ragged: window %lld exceeds the staged-halo limit %lld",
+                     (long long)window, (long long)lds_window_limit<float, 1>());
+  if (nseries == 0) return NBG_OK;
+  if (!values || !labels || !offsets || !label_offsets || !tile_offsets || !tile_series || !strategy ||
+      !move_out || (!group_out && total_labels))
+    return set_error(NBG_ERR_INVALID, "ragged: null pointer");
+  const size_t need = ragged_ws_bytes(total_labels);
+  if (ws == nullptr || ws_bytes < need)
+    return set_error(NBG_ERR_INVALID, "ragged: workspace too small (%zu < %zu)", ws_bytes, need);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  RaggedArgs a{};
+  a.values = values;
+  a.labels = labels;
+  a.offsets = offsets;
+  a.label_offsets = label_offsets;
+  a.tile_offsets = tile_offsets;
+  a.tile_series = tile_series;
+  a.strategy = strategy;
+  a.move_out = move_out;
+  a.gsum = static_cast<double*>(ws);
+  a.gcount = reinterpret_cast<unsigned int*>(a.gsum + total_labels);
+  a.queue = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + need - 256);
+  a.total_tiles = total_tiles;
+  a.window = window;
+  a.min_count = min_count;
+  constexpr int TILE = kMoveThreads * 16;
+  a.stage_bytes = ((size_t)(pad16((int64_t)TILE + window + 4) + 16) * sizeof(float) + 15) / 16 * 16;
+  const size_t lds = a.stage_bytes + kRaggedLdsLabels * (8 + 4);
+  int64_t init_blocks = (total_labels + 255) / 256;
+  if (init_blocks > 4096) init_blocks = 4096;
+  if (init_blocks < 1) init_blocks = 1;
+  hipLaunchKernelGGL(ragged_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, a.gsum,
+                     a.gcount, total_labels, a.queue);
+  NBG_HIP_TRY(hipGetLastError());
+  int cap = device_capacity_blocks((const void*)ragged_kernel, kMoveThreads, lds);
+  if (cap <= 0) return set_error(NBG_ERR_HIP, "ragged: cannot query occupancy");
+  int64_t grid = total_tiles < cap ? total_tiles : cap;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(ragged_kernel, dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
+  NBG_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(ragged_finalize_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, a.gsum,
+                     a.gcount, group_out, total_labels);
+  NBG_HIP_TRY(hipGetLastError());
+  return NBG_OK;
+}
+
+}  // extern "C"

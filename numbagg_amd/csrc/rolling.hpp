@@ -1,0 +1,542 @@
+// Rolling-window tile machinery shared by the plain (moving.hip) and the
+// ragged batched (ragged.hip) kernels: window statistics accumulators and the
+// per-tile evaluation `rolling_tile`.
+#pragma once
+
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace nbg {
+namespace {
+
+constexpr int kMoveThreads = 256;
+
+enum MoveKind { kSum = 0, kMean = 1, kVar = 2, kStd = 3, kCov = 4, kCorr = 5 };
+
+// ---- scalar accumulators ---------------------------------------------------
+// Plain float64 accumulator with the same interface as Comp<F>.
+struct D {
+  double v;
+  __device__ __forceinline__ static D zero() { return D{0.0}; }
+  __device__ __forceinline__ void add(double x) { v += x; }
+  __device__ __forceinline__ void add(const D& o) { v += o.v; }
+  __device__ __forceinline__ void sub(const D& o) { v -= o.v; }
+  __device__ __forceinline__ double to_double() const { return v; }
+};
+
+__device__ __forceinline__ D shfl_up_acc(const D& a, int d) { return D{shfl_up(a.v, d)}; }
+template <typename F>
+__device__ __forceinline__ Comp<F> shfl_up_acc(const Comp<F>& a, int d) {
+  return Comp<F>{shfl_up(a.hi, d), shfl_up(a.lo, d)};
+}
+
+// a*b added to an accumulator: exact product via fma for double-double.
+__device__ __forceinline__ void add_prod(D& acc, double a, double b) { acc.v += a * b; }
+__device__ __forceinline__ void add_prod(Comp<double>& acc, double a, double b) {
+  double p = a * b;
+  double e = fma(a, b, -p);
+  acc.add(p);
+  acc.add(e);
+}
+__device__ __forceinline__ void sub_prod(D& acc, double a, double b) { acc.v -= a * b; }
+__device__ __forceinline__ void sub_prod(Comp<double>& acc, double a, double b) {
+  double p = a * b;
+  double e = fma(a, b, -p);
+  acc.add(-p);
+  acc.add(-e);
+}
+
+// ---- per-kind window statistics ---------------------------------------------
+template <typename T, int KIND>
+struct Traits;
+
+// Exact difference head - tail as an accumulator value: in float64 the
+// difference of two float32 values is exact; TwoSum pairs carry it exactly.
+template <typename A, typename T>
+__device__ __forceinline__ A exact_diff(T h, T t);
+template <>
+__device__ __forceinline__ D exact_diff<D, float>(float h, float t) {
+  return D{(double)h - (double)t};
+}
+template <>
+__device__ __forceinline__ Comp<float> exact_diff<Comp<float>, float>(float h, float t) {
+  Comp<float> r;
+  two_sum(h, -t, r.hi, r.lo);
+  return r;
+}
+template <>
+__device__ __forceinline__ Comp<double> exact_diff<Comp<double>, double>(double h, double t) {
+  Comp<double> r;
+  two_sum(h, -t, r.hi, r.lo);
+  return r;
+}
+
+// First-moment-only kinds (sum, mean). A: float64 (float32 data, default),
+// FP32 TwoSum pair (float32 data, selectable), double-double (float64 data).
+template <typename T, typename A>
+struct SumStats {
+  using Acc = A;
+  int c;
+  A s;
+  __device__ __forceinline__ static SumStats zero() { return SumStats{0, A::zero()}; }
+  __device__ __forceinline__ void add_obs(T a, T, T, T) {
+    c += 1;
+    s.add(a);
+  }
+  __device__ __forceinline__ void sub_obs(T a, T, T, T) {
+    c -= 1;
+    s.add(-a);
+  }
+  __device__ __forceinline__ void merge(const SumStats& o) {
+    c += o.c;
+    s.add(o.s);
+  }
+  __device__ __forceinline__ void unmerge(const SumStats& o) {
+    c -= o.c;
+    s.sub(o.s);
+  }
+  __device__ __forceinline__ SumStats up(int d) const {
+    return SumStats{shfl_up(c, d), shfl_up_acc(s, d)};
+  }
+};
+
+// Moment kinds on shifted data: y = x - k.
+template <typename T, int NM>  // NM: 1 = (s1, s11), 2 = (s1, s2, s12[, s11, s22])
+struct MomStats {
+  using A = typename std::conditional<sizeof(T) == 4, D, Comp<double>>::type;
+  int c;
+  A s1, s2, s11, s22, s12;
+  __device__ __forceinline__ static MomStats zero() {
+    return MomStats{0, A::zero(), A::zero(), A::zero(), A::zero(), A::zero()};
+  }
+  // a, b: observations; ka, kb: shifts.
+  __device__ __forceinline__ void add_obs(T a, T b, T ka, T kb) {
+    const double ya = (double)a - (double)ka;
+    c += 1;
+    s1.add(ya);
+    add_prod(s11, ya, ya);
+    if (NM == 2) {
+      const double yb = (double)b - (double)kb;
+      s2.add(yb);
+      add_prod(s12, ya, yb);
+      add_prod(s22, yb, yb);
+    }
+  }
+  __device__ __forceinline__ void sub_obs(T a, T b, T ka, T kb) {
+    const double ya = (double)a - (double)ka;
+    c -= 1;
+    s1.add(-ya);
+    sub_prod(s11, ya, ya);
+    if (NM == 2) {
+      const double yb = (double)b - (double)kb;
+      s2.add(-yb);
+      sub_prod(s12, ya, yb);
+      sub_prod(s22, yb, yb);
+    }
+  }
+  __device__ __forceinline__ void merge(const MomStats& o) {
+    c += o.c;
+    s1.add(o.s1);
+    s11.add(o.s11);
+    if (NM == 2) {
+      s2.add(o.s2);
+      s12.add(o.s12);
+      s22.add(o.s22);
+    }
+  }
+  __device__ __forceinline__ void unmerge(const MomStats& o) {
+    c -= o.c;
+    s1.sub(o.s1);
+    s11.sub(o.s11);
+    if (NM == 2) {
+      s2.sub(o.s2);
+      s12.sub(o.s12);
+      s22.sub(o.s22);
+    }
+  }
+  __device__ __forceinline__ MomStats up(int d) const {
+    MomStats r;
+    r.c = shfl_up(c, d);
+    r.s1 = shfl_up_acc(s1, d);
+    r.s11 = shfl_up_acc(s11, d);
+    if (NM == 2) {
+      r.s2 = shfl_up_acc(s2, d);
+      r.s12 = shfl_up_acc(s12, d);
+      r.s22 = shfl_up_acc(s22, d);
+    } else {
+      r.s2 = r.s12 = r.s22 = A::zero();
+    }
+    return r;
+  }
+};
+
+// 1/c for a positive count: hardware reciprocal plus one Newton step (about
+// 1 ULP of float64, far below the float32 rounding of the result).
+__device__ __forceinline__ double recip(int c) {
+  const double dc = (double)c;
+  double r = __builtin_amdgcn_rcp(dc);
+  const double e = fma(-dc, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ float mean_of(const D& s, int c) { return (float)(s.v * recip(c)); }
+__device__ __forceinline__ float mean_of(const Comp<float>& s, int c) {
+  return (float)(s.to_double() * recip(c));
+}
+__device__ __forceinline__ double mean_of(const Comp<double>& s, int c) {
+  const double dc = (double)c;
+  const double q = s.hi / dc;
+  const double r = fma(-q, dc, s.hi) + s.lo;  // exact remainder of hi, plus lo
+  return q + r / dc;
+}
+
+// ACC32: float32 data accumulates in FP32 TwoSum pairs instead of float64.
+template <typename T, bool ACC32>
+using SumAccT = typename std::conditional<
+    sizeof(T) == 8, Comp<double>, typename std::conditional<ACC32, Comp<float>, D>::type>::type;
+
+template <typename T, int KIND, bool ACC32 = false>
+struct SumTraits {
+  using S = SumStats<T, SumAccT<T, ACC32>>;
+  static constexpr int NARGS = 1;
+  static constexpr bool SHIFT = false;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (KIND == kSum) {
+      if (s.c < mc) return canonical_nan<T>();
+      return (T)s.s.to_double();
+    }
+    if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
+    return (T)mean_of(s.s, s.c);
+  }
+};
+template <typename T>
+struct Traits<T, kSum> : SumTraits<T, kSum> {};
+template <typename T>
+struct Traits<T, kMean> : SumTraits<T, kMean> {};
+// Variants with FP32 TwoSum accumulators for float32 data.
+constexpr int kSum32 = 16 + kSum;
+constexpr int kMean32 = 16 + kMean;
+template <typename T>
+struct Traits<T, kSum32> : SumTraits<T, kSum, true> {};
+template <typename T>
+struct Traits<T, kMean32> : SumTraits<T, kMean, true> {};
+template <typename T, int KIND>
+struct VarTraits {
+  using S = MomStats<T, 1>;
+  static constexpr int NARGS = 1;
+  static constexpr bool SHIFT = true;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
+    const double c = (double)s.c;
+    const double m1 = s.s1.to_double();
+    const double var = (s.s11.to_double() - m1 * m1 / c) / (c - 1.0);
+    return (T)(KIND == kStd ? sqrt(var) : var);
+  }
+};
+template <typename T>
+struct Traits<T, kVar> : VarTraits<T, kVar> {};
+template <typename T>
+struct Traits<T, kStd> : VarTraits<T, kStd> {};
+template <typename T>
+struct Traits<T, kCov> {
+  using S = MomStats<T, 2>;
+  static constexpr int NARGS = 2;
+  static constexpr bool SHIFT = true;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
+    const double c = (double)s.c;
+    return (T)((s.s12.to_double() - s.s1.to_double() * s.s2.to_double() / c) / (c - 1.0));
+  }
+};
+template <typename T>
+struct Traits<T, kCorr> {
+  using S = MomStats<T, 2>;
+  static constexpr int NARGS = 2;
+  static constexpr bool SHIFT = true;
+  __device__ static T out(const S& s, int64_t mc) {
+    if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
+    const double r = 1.0 / (double)s.c;
+    const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
+    const double va = s.s11.to_double() * r - ma * ma;
+    const double vb = s.s22.to_double() * r - mb * mb;
+    const double cv = s.s12.to_double() * r - ma * mb;
+    const double vv = va * vb;
+    return vv > 0 ? (T)(cv / sqrt(vv)) : canonical_nan<T>();
+  }
+};
+
+// ---- block-level helpers ------------------------------------------------------
+constexpr int kMoveWaves = kMoveThreads / kWave;
+
+template <typename S>
+struct BlockShared {
+  S wave_tot[kMoveWaves];
+};
+
+// Inclusive wave scan of S (sum-like merge).
+template <typename S>
+__device__ __forceinline__ S wave_incl_scan(S v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    S o = v.up(d);
+    if (lane >= d) {
+      o.merge(v);
+      v = o;
+    }
+  }
+  return v;
+}
+
+// Exclusive block scan; also returns the block total.
+template <typename S>
+__device__ __forceinline__ S block_excl_scan(S v, S& total, BlockShared<S>& sh) {
+  S incl = wave_incl_scan(v);
+  if (lane_id() == kWave - 1) sh.wave_tot[wave_id()] = incl;
+  __syncthreads();
+  S prefix = S::zero();
+  total = S::zero();
+#pragma unroll
+  for (int w = 0; w < kMoveWaves; ++w) {
+    if (w < wave_id()) prefix.merge(sh.wave_tot[w]);
+    total.merge(sh.wave_tot[w]);
+  }
+  __syncthreads();
+  // exclusive = prefix + (incl - own)
+  S excl = incl;
+  excl.unmerge(v);
+  prefix.merge(excl);
+  return prefix;
+}
+
+template <typename S>
+__device__ __forceinline__ S block_reduce(S v, BlockShared<S>& sh) {
+  S total;
+  (void)block_excl_scan(v, total, sh);
+  return total;
+}
+
+
+template <typename T>
+struct alignas(16) Vec16 {
+  T v[16 / sizeof(T)];
+};
+
+// ---- the kernel -------------------------------------------------------------------
+// Shared-memory scratch of one rolling tile besides the staged segment.
+template <typename S>
+struct RollingShared {
+  BlockShared<S> sh;
+  int first[2];
+};
+
+// One output tile [tile*TILE, tile*TILE + TILE) of one row (ra/rb/ro point at
+// the row). `smem_raw` holds the staged segment. Ends with a barrier so the
+// caller may reuse the shared memory for its next tile.
+template <typename T, int KIND, bool TAIL_LDS, int VEC>
+__device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* __restrict__ rb,
+                                             T* __restrict__ ro, int64_t n, int64_t window,
+                                             int64_t min_count, int64_t tile,
+                                             unsigned char* smem_raw,
+                                             RollingShared<typename Traits<T, KIND>::S>& rs) {
+  using Tr = Traits<T, KIND>;
+  using S = typename Tr::S;
+  constexpr int K = 64 / (int)sizeof(T);  // 16 float32 / 8 float64 outputs per thread
+  constexpr int TILE = kMoveThreads * K;
+  BlockShared<S>& sh = rs.sh;
+  const int64_t t0 = tile * TILE;
+  const int tid = threadIdx.x;
+
+  // Segment staged in LDS: [s0a, t0 + TILE) with s0a <= t0 - window (TAIL_LDS;
+  // positions before the row start are staged as NaN, so every tail index is
+  // simply head index - window), or [t0, t0 + TILE).
+  const int64_t seg_begin = TAIL_LDS ? t0 - window : t0;
+  const int64_t s0a = seg_begin & ~(int64_t)(VEC - 1);  // floor, also for negatives
+  const int64_t seg_len = t0 + TILE - s0a;
+  const int64_t lds_elems = pad16(seg_len) + 16;
+  T* la = reinterpret_cast<T*>(smem_raw);
+  T* lb = la + lds_elems;
+
+  // 1. stage the segment (wave-striped vector loads, NaN past the row end).
+  for (int64_t e = (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
+    const int64_t p = s0a + e;
+    if (VEC > 1 && p >= 0 && p + VEC <= n) {
+      Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(ra + p);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
+      if (Tr::NARGS == 2) {
+        Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rb + p);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const int64_t q = p + v;
+        const bool in = q >= 0 && q < n;
+        la[pad16(e + v)] = in ? ra[q] : canonical_nan<T>();
+        if (Tr::NARGS == 2) lb[pad16(e + v)] = in ? rb[q] : canonical_nan<T>();
+      }
+    }
+  }
+  __syncthreads();
+
+  // Per-tile shifts for the moment kinds: the first valid value among the tile's
+  // first 256 inputs (any constant works; one near the data keeps the float64
+  // moments small).
+  T ka = T(0), kb = T(0);
+  if (Tr::SHIFT) {
+    int* s_first = rs.first;
+    if (tid < 2) s_first[tid] = 0x7fffffff;
+    __syncthreads();
+    // first 256 inputs of the tile itself (always staged; NaN past the row end)
+    const int e = (int)(t0 - s0a) + tid;
+    if (!is_nan(la[pad16(e)])) atomicMin(&s_first[0], e);
+    if (Tr::NARGS == 2 && !is_nan(lb[pad16(e)])) atomicMin(&s_first[1], e);
+    __syncthreads();
+    ka = s_first[0] != 0x7fffffff ? la[pad16(s_first[0])] : T(0);
+    if (Tr::NARGS == 2) kb = s_first[1] != 0x7fffffff ? lb[pad16(s_first[1])] : T(0);
+  }
+
+  auto valid = [&](T a, T b) -> bool {
+    return (Tr::NARGS == 2) ? !(is_nan(a) || is_nan(b)) : !is_nan(a);
+  };
+
+  // 2. halo statistics W0 = window ending at t0-1.
+  S w0 = S::zero();
+  if (TAIL_LDS) {
+    const int e_end = (int)(t0 - s0a);
+    for (int e = (int)(t0 - window - s0a) + tid; e < e_end; e += kMoveThreads) {
+      const T a = la[pad16(e)];
+      const T b = (Tr::NARGS == 2) ? lb[pad16(e)] : T(0);
+      if (valid(a, b)) w0.add_obs(a, b, ka, kb);
+    }
+  } else {
+    const int64_t h_begin = t0 - window > 0 ? t0 - window : 0;
+    for (int64_t p = h_begin + tid; p < t0; p += kMoveThreads) {
+      const T a = ra[p];
+      const T b = (Tr::NARGS == 2) ? rb[p] : T(0);
+      if (valid(a, b)) w0.add_obs(a, b, ka, kb);
+    }
+  }
+  w0 = block_reduce(w0, sh);
+
+  // 3. head/tail chunks and the per-thread delta. Chunks are re-read from LDS
+  //    (or global memory for long windows) rather than held in registers.
+  const int64_t c0 = t0 + (int64_t)tid * K;  // first output position of this thread
+  // LDS element index of the thread's first head / tail input (32-bit).
+  const int eh = TAIL_LDS ? (int)(c0 - s0a) : tid * K;
+  const int et = eh - (int)(TAIL_LDS ? window : 0);
+  auto head = [&](int j, T& a, T& b) {
+    a = la[pad16(eh + j)];
+    b = (Tr::NARGS == 2) ? lb[pad16(eh + j)] : T(0);
+  };
+  auto tail = [&](int j, T& a, T& b) {
+    if (TAIL_LDS) {
+      a = la[pad16(et + j)];
+      b = (Tr::NARGS == 2) ? lb[pad16(et + j)] : T(0);
+    } else {
+      const int64_t pt = c0 + j - window;
+      const bool in = pt >= 0 && pt < n;
+      a = in ? ra[pt] : canonical_nan<T>();
+      b = (Tr::NARGS == 2 && in) ? rb[pt] : canonical_nan<T>();
+    }
+  };
+  T res[K];
+  if constexpr (!Tr::SHIFT) {
+    // Sum/mean: the exact per-position differences head - tail stay in
+    // registers between the scan and the output pass.
+    using Acc = typename S::Acc;
+    Acc dj[K];
+    unsigned int hbits = 0u, tbits = 0u;
+    S delta = S::zero();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      T h, t, u;
+      head(j, h, u);
+      tail(j, t, u);
+      const bool hv = !is_nan(h), tv = !is_nan(t);
+      dj[j] = exact_diff<Acc, T>(hv ? h : T(0), tv ? t : T(0));
+      hbits |= (unsigned)hv << j;
+      tbits |= (unsigned)tv << j;
+      delta.s.add(dj[j]);
+      delta.c += (int)hv - (int)tv;
+    }
+    S total;
+    S st = block_excl_scan(delta, total, sh);
+    st.merge(w0);  // window ending at c0 - 1
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      st.c += (int)((hbits >> j) & 1u) - (int)((tbits >> j) & 1u);
+      st.s.add(dj[j]);
+      res[j] = Tr::out(st, min_count);
+    }
+  } else {
+    S delta = S::zero();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      T a, b;
+      head(j, a, b);
+      if (valid(a, b)) delta.add_obs(a, b, ka, kb);
+      tail(j, a, b);
+      if (valid(a, b)) delta.sub_obs(a, b, ka, kb);
+    }
+    S total;
+    S st = block_excl_scan(delta, total, sh);
+    st.merge(w0);  // window ending at c0 - 1
+
+    // 4. sequential add/remove over the thread's chunk.
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      T a, b;
+      head(j, a, b);
+      if (valid(a, b)) st.add_obs(a, b, ka, kb);
+      tail(j, a, b);
+      if (valid(a, b)) st.sub_obs(a, b, ka, kb);
+      res[j] = Tr::out(st, min_count);
+    }
+  }
+  if (c0 + K <= n && VEC > 1) {
+#pragma unroll
+    for (int j = 0; j < K; j += VEC) {
+      Vec16<T> vo;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) vo.v[v] = res[j + v];
+      *reinterpret_cast<Vec16<T>*>(ro + c0 + j) = vo;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (c0 + j < n) ro[c0 + j] = res[j];
+  }
+  __syncthreads();
+}
+
+template <typename T, int KIND, bool TAIL_LDS, int VEC>
+__global__ __launch_bounds__(kMoveThreads) void move_kernel(
+    const T* __restrict__ xa, const T* __restrict__ xb, T* __restrict__ out, int64_t rows,
+    int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
+    int64_t tiles_per_row) {
+  using Tr = Traits<T, KIND>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ RollingShared<typename Tr::S> rs;
+  // Consecutive blocks walk consecutive tiles of a row (neighbouring tiles
+  // share their halo through L2).
+  const int64_t row = (int64_t)(blockIdx.x / (uint64_t)tiles_per_row);
+  const int64_t tile = (int64_t)(blockIdx.x % (uint64_t)tiles_per_row);
+  rolling_tile<T, KIND, TAIL_LDS, VEC>(xa + row * ld_a, (Tr::NARGS == 2) ? xb + row * ld_b : nullptr,
+                                       out + row * ld_out, n, window, min_count, tile, smem_raw, rs);
+}
+
+// Largest window whose halo is staged in LDS: the padded segment of
+// TILE + window elements per input must fit a 60 KB budget.
+template <typename T, int NARGS>
+constexpr int64_t lds_window_limit() {
+  constexpr int K = 64 / (int)sizeof(T);
+  constexpr int64_t budget = 60 * 1024;
+  return budget * 16 / (17 * (int64_t)sizeof(T) * NARGS) - kMoveThreads * K - 64;
+}
+
+}  // namespace
+}  // namespace nbg

@@ -1,0 +1,108 @@
+"""Ragged batched path: one persistent launch vs per-series plain launches.
+
+move_mean outputs are bitwise equal to per-series plain move_mean calls (same
+tiles, same arithmetic); group means match the per-series plain
+group_nanmean calls (float64 sums rounded once; the summation order differs,
+so equality is checked bitwise with at most 1-ULP slack) and the float64 oracle.
+Graph replays are checked against the eager result, with eager work and
+synchronisations between replays.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(nseries, lo, hi, klo, khi, seed, device="cuda"):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    lengths = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), nseries))).astype(np.int64)
+    ks = np.round(np.exp(rng.uniform(np.log(klo), np.log(khi), nseries))).astype(np.int64)
+    total = int(lengths.sum())
+    vals = rng.standard_normal(total).astype(np.float32)
+    vals[rng.random(total) < 0.05] = np.nan
+    labs = np.empty(total, dtype=np.int32)
+    off = 0
+    for L, K in zip(lengths, ks):
+        u = rng.random(L)
+        # Zipf(1.1)-like skew over K labels (continuous inverse CDF)
+        x = (1.0 - u * (1.0 - float(K) ** -0.1)) ** -10.0
+        labs[off : off + L] = np.clip(np.floor(x).astype(np.int64) - 1, 0, K - 1)
+        off += L
+    return lengths, ks, torch.from_numpy(vals).to(device), torch.from_numpy(labs).to(device)
+
+
+def _bits(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("strategy", ["heuristic", "lds", "global"])
+def test_ragged_matches_plain_launches(gpu_lib, strategy):
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    lengths, ks, v, lab = make_batch(120, 100, 60_000, 10, 3000, seed=1)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50, strategy=strategy)
+    mm, gm = rb(v, lab)
+    pm, pg = rb.plain(v, lab)
+    np.testing.assert_array_equal(_bits(mm), _bits(pm))
+    a, b = _bits(gm).astype(np.int64), _bits(pg).astype(np.int64)
+    assert np.all(np.abs(a - b) <= 1)
+    assert np.mean(a == b) > 0.999
+
+
+def test_ragged_against_oracle(gpu_lib):
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+    from oracle import exact
+
+    lengths, ks, v, lab = make_batch(40, 100, 20_000, 10, 2000, seed=2)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=40)
+    mm, gm = rb(v, lab)
+    vn, ln, mmn, gmn = v.cpu().numpy(), lab.cpu().numpy(), mm.cpu().numpy(), gm.cpu().numpy()
+    for s in range(len(lengths)):
+        o, n = rb.offsets[s], lengths[s]
+        want = exact.move("move_mean", vn[o : o + n], 50, 40)[0]
+        got = mmn[o : o + n]
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        ok = ~np.isnan(want)
+        assert np.allclose(got[ok], want[ok], rtol=2e-7, atol=1e-7)
+        lo, k = rb.label_offsets[s], ks[s]
+        gwant = exact.group("group_nanmean", vn[o : o + n], ln[o : o + n], int(k))[0]
+        ggot = gmn[lo : lo + k]
+        assert np.array_equal(np.isnan(ggot), np.isnan(gwant))
+        ok = ~np.isnan(gwant)
+        assert np.allclose(ggot[ok], gwant[ok], rtol=1e-6, atol=1e-7)
+
+
+def test_ragged_graph_replays(gpu_lib):
+    import torch
+
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    lengths, ks, v, lab = make_batch(60, 100, 30_000, 10, 2000, seed=3)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50)
+    em, eg = rb(v, lab)
+    em, eg = _bits(em), _bits(eg).astype(np.int64)
+    replay, mo, go = rb.graph(v, lab)
+    for i in range(4):
+        mo.fill_(0)
+        go.fill_(0)
+        replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(_bits(mo), em)
+        assert np.all(np.abs(_bits(go).astype(np.int64) - eg) <= 1)
+        # eager work between replays
+        _ = rb(v, lab)
+        torch.cuda.synchronize()
+
+
+def test_ragged_errors(gpu_lib):
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    with pytest.raises(ValueError, match="window not in valid range"):
+        RaggedMoveMeanGroupMean([100, 50], [3, 3], window=60)
+    with pytest.raises(ValueError, match="min_count must be positive"):
+        RaggedMoveMeanGroupMean([100, 50], [3, 3], window=10, min_count=-1)
