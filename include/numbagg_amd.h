@@ -220,6 +220,26 @@ int nbg_ragged_move_mean_group_nanmean(
     int64_t min_count, float* move_out, float* group_out, void* ws,
     size_t ws_bytes, void* stream);
 
+/* ---- strategy cost model (no reference counterpart): fp32 MLP kernels ---- */
+/* Row-major x (batch, in_dim), w (out_dim, in_dim), b (out_dim), y (batch,
+ * out_dim); act: 0 identity, 1 ReLU. Backward takes the forward output y and
+ * dy, writes dz (batch, out_dim) scratch, dw, db and (if dx != NULL) dx.
+ * Weight gradients are fixed-order reductions (deterministic). */
+int nbg_dense_forward(const float* x, const float* w, const float* b, float* y,
+                      int64_t batch, int64_t in_dim, int64_t out_dim, int act,
+                      void* stream);
+int nbg_dense_backward(const float* x, const float* w, const float* y,
+                       const float* dy, float* dx, float* dw, float* db,
+                       float* dz, int64_t batch, int64_t in_dim,
+                       int64_t out_dim, int act, void* stream);
+/* dy = 2 (y - t) / n; *loss += mean squared error (loss may be NULL). */
+int nbg_mse_grad(const float* y, const float* t, float* dy, float* loss,
+                 int64_t n, void* stream);
+/* One Adam update of n parameters (step counts from 1). */
+int nbg_adam_step(float* p, const float* g, float* m, float* v, int64_t n,
+                  float lr, float beta1, float beta2, float eps, int64_t step,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,17 @@ _SIGNATURES["nbg_ragged_move_mean_group_nanmean"] = (
     [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_i64, _c_i64,
      _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr],
 )
+_c_flt = ctypes.c_float
+_SIGNATURES["nbg_dense_forward"] = (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_int, _c_ptr])
+_SIGNATURES["nbg_dense_backward"] = (
+    _c_int,
+    [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_int, _c_ptr],
+)
+_SIGNATURES["nbg_mse_grad"] = (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr])
+_SIGNATURES["nbg_adam_step"] = (
+    _c_int,
+    [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_flt, _c_flt, _c_flt, _c_flt, _c_i64, _c_ptr],
+)
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None

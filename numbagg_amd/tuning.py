@@ -1,0 +1,362 @@
+"""Strategy selection for the ragged batched path: autotuner and cost model.
+
+For each series of a ragged batch the kernel can fold values into its label
+accumulators through LDS-privatised slots (flushed per tile) or through global
+atomics. Which is faster depends on the series' length, label count, label
+skew, NaN fraction (and window, through the tile's rolling work).
+
+* ``Autotuner`` measures both strategies exhaustively: it times the ragged
+  kernel on homogeneous synthetic batches (one feature point each) with HIP
+  events and records ns/element in a JSON timing DB, versioned by GPU
+  architecture and a hash of the kernel source.
+* ``CostModel`` is a 5 -> 64 -> 64 -> 2 fp32 MLP predicting log ns/element of
+  both strategies from (log2 length, log2 labels, skew, NaN fraction, log2
+  window). It is trained on the GPU by this library's own kernels
+  (nbg_dense_forward / nbg_dense_backward / nbg_mse_grad / nbg_adam_step) on
+  the autotuner's timings; inference uses nbg_dense_forward.
+* ``choose_strategies`` picks per-series strategies with the cost model, the
+  exhaustive tuner (the fallback, and the judge of the model's regret) or a
+  label-count heuristic.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import json
+import math
+import pathlib
+
+import numpy as np
+import torch
+
+from . import _device as dev
+from . import _lib
+from .ragged import LDS_MAX_LABELS, STRATEGY_GLOBAL, STRATEGY_LDS, RaggedMoveMeanGroupMean
+
+DATA_DIR = pathlib.Path(__file__).resolve().parent / "tuning_data"
+DB_PATH = DATA_DIR / "ragged_timing_db.json"
+MODEL_PATH = DATA_DIR / "ragged_cost_model.npz"
+N_FEATURES = 5
+STRATEGIES = (STRATEGY_LDS, STRATEGY_GLOBAL)
+
+
+def kernel_version() -> str:
+    """Hash of the sources whose timings the DB and the model describe."""
+    h = hashlib.sha256()
+    csrc = pathlib.Path(__file__).resolve().parent / "csrc"
+    for name in ("ragged.hip", "rolling.hpp", "common.hpp"):
+        h.update((csrc / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def device_arch() -> str:
+    try:
+        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    except Exception:  # pragma: no cover - no device
+        return "unknown"
+
+
+def features(lengths, num_labels, skew, nan_fraction, window) -> np.ndarray:
+    """(S, 5) float32 feature matrix."""
+    lengths = np.asarray(lengths, dtype=np.float64)
+    num_labels = np.asarray(num_labels, dtype=np.float64)
+    s = np.broadcast_to(np.asarray(skew, dtype=np.float64), lengths.shape)
+    f = np.broadcast_to(np.asarray(nan_fraction, dtype=np.float64), lengths.shape)
+    w = np.broadcast_to(np.asarray(window, dtype=np.float64), lengths.shape)
+    return np.stack(
+        [np.log2(lengths), np.log2(np.maximum(num_labels, 1)), s, f, np.log2(np.maximum(w, 1))], axis=1
+    ).astype(np.float32)
+
+
+def batch_statistics(rb: RaggedMoveMeanGroupMean, values: torch.Tensor, labels: torch.Tensor):
+    """Per-series skew (share of the most frequent label) and NaN fraction."""
+    dev_ = values.device
+    seg = torch.repeat_interleave(torch.arange(rb.nseries, device=dev_), torch.from_numpy(rb.lengths).to(dev_))
+    lab_off = torch.from_numpy(rb.label_offsets[:-1]).to(dev_)
+    lab = labels.long()
+    ok = (lab >= 0) & (lab < torch.from_numpy(rb.num_labels).to(dev_)[seg])
+    glab = (lab_off[seg] + lab.clamp(min=0))[ok]
+    counts = torch.bincount(glab, minlength=rb.total_labels).float()
+    nl = torch.from_numpy(rb.num_labels).to(dev_)
+    top = torch.segment_reduce(counts, "max", lengths=nl, unsafe=True) if rb.total_labels else counts
+    top = torch.nan_to_num(top, nan=0.0, neginf=0.0)
+    L = torch.from_numpy(rb.lengths).to(dev_).float()
+    skew = (top / L).cpu().numpy()
+    nan_cnt = torch.zeros(rb.nseries, device=dev_).index_add_(0, seg, torch.isnan(values).float())
+    return skew, (nan_cnt / L).cpu().numpy()
+
+
+def synthetic_labels(length: int, k: int, zipf: float, gen: torch.Generator, device) -> torch.Tensor:
+    """int32 labels over k classes: uniform (zipf == 0) or Zipf(zipf)-like via a
+    continuous inverse CDF."""
+    u = torch.rand(length, generator=gen, device=device, dtype=torch.float64)
+    if zipf == 0:
+        return torch.clamp((u * k).long(), max=k - 1).to(torch.int32)
+    a = zipf - 1.0
+    x = (1.0 - u * (1.0 - float(k) ** -a)) ** (-1.0 / a)
+    return torch.clamp(torch.floor(x).long() - 1, 0, k - 1).to(torch.int32)
+
+
+class Autotuner:
+    """Exhaustive strategy timing of the ragged kernel, cached in a JSON DB."""
+
+    def __init__(self, db_path: pathlib.Path | str = DB_PATH, target_elements: int = 1 << 22, reps: int = 3):
+        self.db_path = pathlib.Path(db_path)
+        self.target_elements = target_elements
+        self.reps = reps
+        self.meta = {"arch": device_arch(), "kernel": kernel_version()}
+        self.entries: list[dict] = []
+        if self.db_path.exists():
+            try:
+                db = json.loads(self.db_path.read_text())
+                if db.get("meta") == self.meta:
+                    self.entries = db.get("entries", [])
+            except (OSError, ValueError):
+                self.entries = []
+
+    def save(self) -> None:
+        self.db_path.parent.mkdir(parents=True, exist_ok=True)
+        self.db_path.write_text(json.dumps({"meta": self.meta, "entries": self.entries}, indent=1))
+
+    def _time(self, rb, values, labels, strategy) -> float:
+        rb.set_strategy(np.full(rb.nseries, strategy, dtype=np.uint8))
+        mo = torch.empty_like(values)
+        go = torch.empty(rb.total_labels, dtype=torch.float32, device=values.device)
+        rb(values, labels, mo, go)
+        stream = torch.cuda.current_stream()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(self.reps):
+            rb(values, labels, mo, go)
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) * 1e6 / self.reps / values.numel()  # ns per element
+
+    def measure(self, length: int, k: int, zipf: float, nan: float, window: int, seed: int = 0) -> dict:
+        """Time both strategies on a homogeneous batch for one feature point."""
+        dev.require_gpu()
+        device = torch.device("cuda")
+        gen = torch.Generator(device=device).manual_seed(seed)
+        s = max(1, self.target_elements // length)
+        lengths = np.full(s, length, dtype=np.int64)
+        ks = np.full(s, k, dtype=np.int64)
+        total = int(lengths.sum())
+        values = torch.randn(total, generator=gen, device=device, dtype=torch.float32)
+        values[torch.rand(total, generator=gen, device=device) < nan] = float("nan")
+        labels = synthetic_labels(total, k, zipf, gen, device)
+        rb = RaggedMoveMeanGroupMean(lengths, ks, window=min(window, length), min_count=1)
+        skew, nanf = batch_statistics(rb, values, labels)
+        times = {}
+        for st in STRATEGIES:
+            if st == STRATEGY_LDS and k > LDS_MAX_LABELS:
+                times[st] = math.inf
+                continue
+            times[st] = self._time(rb, values, labels, st)
+        entry = {
+            "length": length, "num_labels": k, "zipf": zipf, "nan": nan, "window": window,
+            "skew": float(np.mean(skew)), "nan_fraction": float(np.mean(nanf)),
+            "ns_lds": times[STRATEGY_LDS], "ns_global": times[STRATEGY_GLOBAL],
+        }
+        self.entries.append(entry)
+        return entry
+
+    def run_grid(self, lengths=None, ks=None, zipfs=(0.0, 1.1, 2.0), nans=(0.05, 0.4), windows=(20, 200), seed=0):
+        lengths = lengths or [2**e for e in range(7, 21, 2)]
+        ks = ks or [8, 32, 128, 512, 1024]
+        i = 0
+        for L in lengths:
+            for k in ks:
+                for z in zipfs:
+                    for f in nans:
+                        for w in windows:
+                            self.measure(L, k, z, f, w, seed=seed + i)
+                            i += 1
+        return self.entries
+
+    def training_set(self):
+        """Features (n, 5) and targets (n, 2) = log ns/element per strategy."""
+        rows = [e for e in self.entries if math.isfinite(e["ns_lds"]) and math.isfinite(e["ns_global"])]
+        X = features([e["length"] for e in rows], [e["num_labels"] for e in rows], [e["skew"] for e in rows],
+                     [e["nan_fraction"] for e in rows], [e["window"] for e in rows])
+        T = np.log(np.array([[e["ns_lds"], e["ns_global"]] for e in rows], dtype=np.float64)).astype(np.float32)
+        return X, T
+
+
+class CostModel:
+    """5 -> 64 -> 64 -> 2 fp32 MLP on this library's HIP kernels."""
+
+    DIMS = (N_FEATURES, 64, 64, 2)
+
+    def __init__(self, seed: int = 0, device=None):
+        dev.require_gpu()
+        self.device = torch.device(device or "cuda")
+        rng = np.random.default_rng(seed)
+        self.params: list[torch.Tensor] = []
+        for fan_in, fan_out in zip(self.DIMS[:-1], self.DIMS[1:]):
+            w = rng.standard_normal((fan_out, fan_in)) * math.sqrt(2.0 / fan_in)
+            self.params += [torch.tensor(w, dtype=torch.float32, device=self.device),
+                            torch.zeros(fan_out, dtype=torch.float32, device=self.device)]
+        self.x_mean = np.zeros(N_FEATURES, np.float32)
+        self.x_std = np.ones(N_FEATURES, np.float32)
+        self.t_mean = np.zeros(2, np.float32)
+        self.t_std = np.ones(2, np.float32)
+        self.meta: dict = {}
+        self._adam = None
+        self.step = 0
+
+    # ---- kernels -----------------------------------------------------------
+    def _forward(self, x: torch.Tensor):
+        acts = [x]
+        h = x
+        nl = len(self.DIMS) - 1
+        for li in range(nl):
+            w, b = self.params[2 * li], self.params[2 * li + 1]
+            y = torch.empty((h.shape[0], w.shape[0]), dtype=torch.float32, device=self.device)
+            act = 1 if li < nl - 1 else 0
+            _lib.call("nbg_dense_forward", dev.ptr(h), dev.ptr(w), dev.ptr(b), dev.ptr(y), h.shape[0],
+                      w.shape[1], w.shape[0], act, dev.stream_handle())
+            acts.append(y)
+            h = y
+        return acts
+
+    def _backward(self, acts, dy: torch.Tensor):
+        grads = [None] * len(self.params)
+        nl = len(self.DIMS) - 1
+        for li in reversed(range(nl)):
+            w = self.params[2 * li]
+            x, y = acts[li], acts[li + 1]
+            dw = torch.empty_like(w)
+            db = torch.empty_like(self.params[2 * li + 1])
+            dz = torch.empty_like(y)
+            dx = torch.empty_like(x) if li > 0 else None
+            act = 1 if li < nl - 1 else 0
+            _lib.call("nbg_dense_backward", dev.ptr(x), dev.ptr(w), dev.ptr(y), dev.ptr(dy), dev.ptr(dx),
+                      dev.ptr(dw), dev.ptr(db), dev.ptr(dz), x.shape[0], w.shape[1], w.shape[0], act,
+                      dev.stream_handle())
+            grads[2 * li], grads[2 * li + 1] = dw, db
+            dy = dx
+        return grads
+
+    def loss_and_grads(self, x: torch.Tensor, t: torch.Tensor):
+        acts = self._forward(x)
+        y = acts[-1]
+        dy = torch.empty_like(y)
+        loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+        _lib.call("nbg_mse_grad", dev.ptr(y), dev.ptr(t), dev.ptr(dy), dev.ptr(loss), y.numel(), dev.stream_handle())
+        return loss, self._backward(acts, dy)
+
+    def _adam_step(self, grads, lr: float):
+        if self._adam is None:
+            self._adam = [(torch.zeros_like(p), torch.zeros_like(p)) for p in self.params]
+        self.step += 1
+        for p, g, (m, v) in zip(self.params, grads, self._adam):
+            _lib.call("nbg_adam_step", dev.ptr(p), dev.ptr(g), dev.ptr(m), dev.ptr(v), p.numel(),
+                      ctypes.c_float(lr), ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8),
+                      self.step, dev.stream_handle())
+
+    # ---- training / inference -----------------------------------------------
+    def fit(self, X: np.ndarray, T: np.ndarray, epochs: int = 2000, lr: float = 3e-3, seed: int = 0) -> list[float]:
+        X = np.asarray(X, np.float32)
+        T = np.asarray(T, np.float32)
+        self.x_mean, self.x_std = X.mean(0), X.std(0) + 1e-6
+        self.t_mean, self.t_std = T.mean(0), T.std(0) + 1e-6
+        x = torch.from_numpy((X - self.x_mean) / self.x_std).to(self.device)
+        t = torch.from_numpy((T - self.t_mean) / self.t_std).to(self.device)
+        history = []
+        for ep in range(epochs):
+            loss, grads = self.loss_and_grads(x, t)
+            self._adam_step(grads, lr)
+            if ep % 100 == 0 or ep == epochs - 1:
+                history.append(float(loss.item()))
+        return history
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        """Predicted log ns/element for (LDS, global), shape (n, 2)."""
+        X = np.asarray(X, np.float32)
+        x = torch.from_numpy((X - self.x_mean) / self.x_std).to(self.device)
+        y = self._forward(x)[-1].cpu().numpy()
+        return y * self.t_std + self.t_mean
+
+    def choose(self, X: np.ndarray, num_labels: np.ndarray) -> np.ndarray:
+        pred = self.predict(X)
+        st = np.where(pred[:, 0] <= pred[:, 1], STRATEGY_LDS, STRATEGY_GLOBAL)
+        return np.where(np.asarray(num_labels) > LDS_MAX_LABELS, STRATEGY_GLOBAL, st).astype(np.uint8)
+
+    def save(self, path: pathlib.Path | str = MODEL_PATH) -> None:
+        path = pathlib.Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        arrays = {f"p{i}": p.cpu().numpy() for i, p in enumerate(self.params)}
+        np.savez(path, x_mean=self.x_mean, x_std=self.x_std, t_mean=self.t_mean, t_std=self.t_std,
+                 meta=np.array(json.dumps(self.meta)), **arrays)
+
+    @classmethod
+    def load(cls, path: pathlib.Path | str = MODEL_PATH, device=None) -> "CostModel | None":
+        """The saved model, or None when missing, unreadable or built for another
+        architecture / kernel version (callers then fall back to the tuner)."""
+        path = pathlib.Path(path)
+        if not path.exists():
+            return None
+        try:
+            with np.load(path, allow_pickle=False) as z:
+                meta = json.loads(str(z["meta"]))
+                m = cls(device=device)
+                m.params = [torch.from_numpy(z[f"p{i}"].copy()).to(m.device) for i in range(len(m.params))]
+                m.x_mean, m.x_std, m.t_mean, m.t_std = z["x_mean"], z["x_std"], z["t_mean"], z["t_std"]
+                m.meta = meta
+        except (OSError, ValueError, KeyError):
+            return None
+        if meta.get("arch") != device_arch() or meta.get("kernel") != kernel_version():
+            return None
+        return m
+
+
+def exhaustive_strategies(rb: RaggedMoveMeanGroupMean, values, labels, skew, nanf, reps: int = 2) -> np.ndarray:
+    """Per-bucket exhaustive choice: series are bucketed by (log2 length,
+    log2 labels, skew, NaN fraction); each bucket's own sub-batch is timed with
+    both strategies and the faster one is kept."""
+    key = np.stack([
+        np.round(np.log2(rb.lengths)), np.round(np.log2(np.maximum(rb.num_labels, 1))),
+        np.round(skew * 10), np.round(nanf * 10)], axis=1)
+    _, bucket = np.unique(key, axis=0, return_inverse=True)
+    bucket = bucket.ravel()
+    out = np.where(rb.num_labels > LDS_MAX_LABELS, STRATEGY_GLOBAL, STRATEGY_LDS).astype(np.uint8)
+    tuner = Autotuner(db_path=pathlib.Path("/nonexistent/db.json"), reps=reps)
+    for b in np.unique(bucket):
+        idx = np.nonzero(bucket == b)[0]
+        if np.all(rb.num_labels[idx] > LDS_MAX_LABELS):
+            continue
+        # time a sample of the bucket: up to 64 series / about 8M elements
+        budget, take = 1 << 23, []
+        for i in idx[:64]:
+            take.append(i)
+            budget -= int(rb.lengths[i])
+            if budget <= 0:
+                break
+        sample = np.array(take)
+        sel = np.concatenate([np.arange(rb.offsets[i], rb.offsets[i + 1]) for i in sample])
+        sel_t = torch.from_numpy(sel).to(values.device)
+        sub = RaggedMoveMeanGroupMean(rb.lengths[sample], rb.num_labels[sample], window=rb.window, min_count=rb.min_count)
+        sv, sl = values[sel_t].contiguous(), labels[sel_t].contiguous()
+        t_lds = tuner._time(sub, sv, sl, STRATEGY_LDS)
+        t_glb = tuner._time(sub, sv, sl, STRATEGY_GLOBAL)
+        out[idx] = STRATEGY_LDS if t_lds <= t_glb else STRATEGY_GLOBAL
+    return np.where(rb.num_labels > LDS_MAX_LABELS, STRATEGY_GLOBAL, out).astype(np.uint8)
+
+
+def choose_strategies(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "model", model: CostModel | None = None):
+    """Per-series strategies for `rb` by "model" (falls back to "exhaustive"
+    when no valid model is available), "exhaustive" or "heuristic"."""
+    from .ragged import heuristic_strategy
+
+    if method == "heuristic":
+        return heuristic_strategy(rb.lengths, rb.num_labels)
+    skew, nanf = batch_statistics(rb, values, labels)
+    if method == "model":
+        model = model or CostModel.load()
+        if model is not None:
+            return model.choose(features(rb.lengths, rb.num_labels, skew, nanf, rb.window), rb.num_labels)
+        method = "exhaustive"
+    if method == "exhaustive":
+        return exhaustive_strategies(rb, values, labels, skew, nanf)
+    raise ValueError(f"unknown method {method!r}")
