@@ -50,6 +50,10 @@ const char* nbg_last_error(void);
 int nbg_spin_timeouts(unsigned int* count);
 int nbg_reset_spin_timeouts(void);
 
+/* Streaming copy of `bytes` (16-byte aligned, multiple of 16). No reference
+ * counterpart: a calibration kernel for bandwidth and traffic measurements. */
+int nbg_stream_copy(const void* src, void* dst, size_t bytes, void* stream);
+
 /* ---- fill (numbagg/funcs.py:294-326 bfill/ffill; ndfill.__call__ at
  *      numbagg/decorators.py:465-487) ------------------------------------- */
 /* limit: the reference's `lives_remaining` budget (>= 0; None -> n). */
@@ -205,12 +209,14 @@ int nbg_group_nanstd(int vdtype, const void* values, int ldtype,
 /* nseries float32 series concatenated in `values` (series s occupies
  * [offsets[s], offsets[s+1])), with int32 labels laid out alike; series s has
  * label_offsets[s+1] - label_offsets[s] labels and its group means go to
- * group_out[label_offsets[s] : label_offsets[s+1]]. Work items are 4096-long
- * tiles of each series: tile_offsets[s] is the first item of series s and
+ * group_out[label_offsets[s] : label_offsets[s+1]]. Work items are tiles of
+ * nbg_ragged_tile_size() positions of each series: tile_offsets[s] is the first item of series s and
  * tile_series[t] the series of item t (total_tiles items). strategy[s]: 0 =
  * LDS-privatised label accumulators (up to 1024 labels), 1 = global atomics.
  * One persistent launch with an atomic work queue; move_out is bitwise equal
  * to per-series move_mean calls. All arrays are device arrays. */
+/* Work-item (tile) length; the same tiles a plain float32 move_mean uses. */
+int64_t nbg_ragged_tile_size(void);
 size_t nbg_ragged_workspace_bytes(int64_t nseries, int64_t total_labels);
 int nbg_ragged_move_mean_group_nanmean(
     const float* values, const int32_t* labels, const int64_t* offsets,

@@ -29,6 +29,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "nbg_last_error": (ctypes.c_char_p, []),
     "nbg_spin_timeouts": (_c_int, [ctypes.POINTER(ctypes.c_uint)]),
     "nbg_reset_spin_timeouts": (_c_int, []),
+    "nbg_stream_copy": (_c_int, [_c_ptr, _c_ptr, _c_size, _c_ptr]),
     "nbg_fill_workspace_bytes": (_c_size, [_c_i64, _c_i64, _c_int]),
     "nbg_ffill": (
         _c_int,
@@ -78,6 +79,7 @@ for _name in ("nansum", "nanmean", "nancount", "nanargmax", "nanargmin", "nanfir
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP
 for _name in ("nanvar", "nanstd"):
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP_DDOF
+_SIGNATURES["nbg_ragged_tile_size"] = (_c_i64, [])
 _SIGNATURES["nbg_ragged_workspace_bytes"] = (_c_size, [_c_i64, _c_i64])
 _SIGNATURES["nbg_ragged_move_mean_group_nanmean"] = (
     _c_int,

@@ -79,13 +79,45 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   __shared__ int s_first_valid;
 
   const int64_t row = (int64_t)(blockIdx.x % (uint64_t)rows);
-  int64_t ticket = 0;
+  const T* xr = x + row * ld_x;
+  T* orow = out + row * ld_out;
+  const int w = wave_id();
+  const int l = lane_id();
+  T vals[U][VEC];
+  auto load_tile = [&](int64_t tk) {
+    const int64_t tl = REV ? (tiles_per_row - 1 - tk) : tk;
+    const int64_t q0 = tl * TILE;
+    const bool q_full = (q0 + TILE <= n);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e0 = ((w * U + u) * kWave + l) * VEC;
+      if (q_full && VEC > 1) {
+        VecT<T> t = *reinterpret_cast<const VecT<T>*>(xr + q0 + e0);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) vals[u][v] = t.v[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const int64_t p = q0 + e0 + v;
+          vals[u][v] = (p < n) ? xr[p] : canonical_nan<T>();
+        }
+      }
+    }
+  };
+
+  // Blocks of a row are dispatched in order, so a block almost always receives
+  // ticket blockIdx / rows: its tile is loaded while the ticket atomic is in
+  // flight, and reloaded only when the ticket differs.
+  const int64_t guess = (int64_t)(blockIdx.x / (uint64_t)rows);
+  int64_t ticket = guess;
   if (LOOKBACK) {
     if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&counters[row], 1u);
+    load_tile(guess);
     __syncthreads();
     ticket = s_ticket;
+    if (ticket != guess) load_tile(ticket);
   } else {
-    ticket = (int64_t)(blockIdx.x / (uint64_t)rows);
+    load_tile(guess);
   }
   // Tiles are aligned to the row start; bfill walks them from the last one.
   const int64_t tile = REV ? (tiles_per_row - 1 - ticket) : ticket;
@@ -93,29 +125,7 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   // Scan-order coordinate of the tile's first element (may be < 0 for the
   // partial last tile in bfill; only real elements are ever addressed).
   const int64_t s_base = REV ? (n - p0 - TILE) : p0;
-
-  const T* xr = x + row * ld_x;
-  T* orow = out + row * ld_out;
-  const int w = wave_id();
-  const int l = lane_id();
   const bool full = (p0 + TILE <= n);
-
-  T vals[U][VEC];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int e0 = ((w * U + u) * kWave + l) * VEC;
-    if (full && VEC > 1) {
-      VecT<T> t = *reinterpret_cast<const VecT<T>*>(xr + p0 + e0);
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) vals[u][v] = t.v[v];
-    } else {
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        const int64_t p = p0 + e0 + v;
-        vals[u][v] = (p < n) ? xr[p] : canonical_nan<T>();
-      }
-    }
-  }
 
   // Scan-order offset inside the tile of element (u, v) of this lane.
   auto so_of = [&](int u, int v) -> int {

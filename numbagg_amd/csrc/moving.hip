@@ -34,16 +34,15 @@
 namespace nbg {
 namespace {
 
-template <typename T, int KIND>
-int move_launch(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64_t ld_a,
-                int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count, hipStream_t s) {
+template <typename T, int KIND, int KP>
+int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64_t ld_a,
+                  int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count, hipStream_t s) {
   using Tr = Traits<T, KIND>;
-  constexpr int K = 64 / (int)sizeof(T);
-  constexpr int TILE = kMoveThreads * K;
+  constexpr int TILE = kMoveThreads * KP;
   const int64_t tiles = (n + TILE - 1) / TILE;
   const int64_t blocks = rows * tiles;
   if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move: too many tiles");
-  const bool tail_lds = window <= lds_window_limit<T, Tr::NARGS>();
+  const bool tail_lds = window <= lds_window_limit<T, Tr::NARGS>() - (KP - default_items<T>()) * kMoveThreads;
   int vec = pick_vec<T>(a, ld_a, rows);
   if (Tr::NARGS == 2) {
     int vb = pick_vec<T>(b, ld_b, rows);
@@ -55,21 +54,45 @@ int move_launch(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64_t
   }
   constexpr int MAXV = 16 / (int)sizeof(T);
   const int v_use = (vec == MAXV) ? MAXV : 1;
-  const int64_t seg_len = tail_lds ? (TILE + window + MAXV) : TILE;
-  const size_t lds = (size_t)(pad16(seg_len) + 16) * sizeof(T) * Tr::NARGS;
+  constexpr bool LUT_OK = (KIND == kMean && sizeof(T) == 4);
+  const bool use_lut = LUT_OK && mean_uses_lut(window);
+  size_t lds = (size_t)rolling_stage_bytes<T>(KP, window, tail_lds, Tr::NARGS);
+  if (use_lut) lds += (size_t)(window + 1) * sizeof(double);
   dim3 grid((unsigned)blocks), block(kMoveThreads);
-#define NBG_MOVE_LAUNCH(TL, V)                                                                   \
-  hipLaunchKernelGGL((move_kernel<T, KIND, TL, V>), grid, block, lds, s, a, b, out, rows, n,     \
+#define NBG_MOVE_LAUNCH(TL, V, LU)                                                                   \
+  hipLaunchKernelGGL((move_kernel<T, KIND, TL, V, KP, LU>), grid, block, lds, s, a, b, out, rows, n, \
                      ld_a, ld_b, ld_out, window, min_count, tiles)
-  if (tail_lds) {
-    if (v_use == MAXV) NBG_MOVE_LAUNCH(true, MAXV);
-    else NBG_MOVE_LAUNCH(true, 1);
+  if (use_lut) {
+    if constexpr (LUT_OK) {
+      if (tail_lds) {
+        if (v_use == MAXV) NBG_MOVE_LAUNCH(true, MAXV, true);
+        else NBG_MOVE_LAUNCH(true, 1, true);
+      } else {
+        if (v_use == MAXV) NBG_MOVE_LAUNCH(false, MAXV, true);
+        else NBG_MOVE_LAUNCH(false, 1, true);
+      }
+    }
+  } else if (tail_lds) {
+    if (v_use == MAXV) NBG_MOVE_LAUNCH(true, MAXV, false);
+    else NBG_MOVE_LAUNCH(true, 1, false);
   } else {
-    if (v_use == MAXV) NBG_MOVE_LAUNCH(false, MAXV);
-    else NBG_MOVE_LAUNCH(false, 1);
+    if (v_use == MAXV) NBG_MOVE_LAUNCH(false, MAXV, false);
+    else NBG_MOVE_LAUNCH(false, 1, false);
   }
 #undef NBG_MOVE_LAUNCH
   return check_hip(hipGetLastError(), "move_kernel launch");
+}
+
+
+template <typename T, int KIND>
+int move_launch(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64_t ld_a,
+                int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count, hipStream_t s) {
+  if constexpr (sizeof(T) == 4 && (KIND == kSum || KIND == kMean)) {
+    if (f32_sum_items() == 32)
+      return move_launch_k<T, KIND, 32>(a, b, out, rows, n, ld_a, ld_b, ld_out, window, min_count, s);
+  }
+  return move_launch_k<T, KIND, default_items<T>()>(a, b, out, rows, n, ld_a, ld_b, ld_out, window,
+                                                     min_count, s);
 }
 
 // NBG_F32_ACCUMULATOR=twosum selects FP32 TwoSum accumulators for float32

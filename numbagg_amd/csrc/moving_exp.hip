@@ -211,44 +211,56 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   TA* out = static_cast<TA*>(A.out);
 
   const int64_t row = (int64_t)(blockIdx.x % (uint64_t)A.rows);
-  if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&A.counters[row], 1u);
-  __syncthreads();
-  const int64_t tile = s_ticket;
-  const int64_t t0 = tile * TILE;
   const TX* ra = xa + row * A.ld_a;
   const TX* rb = (Sp::NARGS == 2) ? xb + row * A.ld_b : nullptr;
   const TA* rl = al ? al + row * A.ld_alpha : nullptr;
   TA* ro = out + row * A.ld_out;
   const int tid = threadIdx.x;
   const int64_t n = A.n;
-
-  // Stage the tile (coalesced across the block; NaN past the row end).
   TX* la = reinterpret_cast<TX*>(smem_raw);
   TX* lb = la + pad16(TILE) + 16;
-  const bool full = t0 + TILE <= n;
   constexpr int VEC = 16 / (int)sizeof(TX);
-  const bool vec_ok = full && (((uintptr_t)(ra + t0)) % 16 == 0) &&
-                      (Sp::NARGS == 1 || ((uintptr_t)(rb + t0)) % 16 == 0);
-  if (vec_ok) {
-    struct alignas(16) V { TX v[VEC]; };
-    for (int e = tid * VEC; e < TILE; e += kExpThreads * VEC) {
-      V va = *reinterpret_cast<const V*>(ra + t0 + e);
+
+  // Stage tile tk in LDS (coalesced across the block; NaN past the row end).
+  auto stage = [&](int64_t tk) {
+    const int64_t q0 = tk * TILE;
+    const bool q_vec = (q0 + TILE <= n) && (((uintptr_t)(ra + q0)) % 16 == 0) &&
+                       (Sp::NARGS == 1 || ((uintptr_t)(rb + q0)) % 16 == 0);
+    if (q_vec) {
+      struct alignas(16) V { TX v[VEC]; };
+      for (int e = tid * VEC; e < TILE; e += kExpThreads * VEC) {
+        V va = *reinterpret_cast<const V*>(ra + q0 + e);
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
-      if (Sp::NARGS == 2) {
-        V vb = *reinterpret_cast<const V*>(rb + t0 + e);
+        for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
+        if (Sp::NARGS == 2) {
+          V vb = *reinterpret_cast<const V*>(rb + q0 + e);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
+          for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
+        }
+      }
+    } else {
+      for (int e = tid; e < TILE; e += kExpThreads) {
+        const int64_t p = q0 + e;
+        la[pad16(e)] = p < n ? ra[p] : canonical_nan<TX>();
+        if (Sp::NARGS == 2) lb[pad16(e)] = p < n ? rb[p] : canonical_nan<TX>();
       }
     }
-  } else {
-    for (int e = tid; e < TILE; e += kExpThreads) {
-      const int64_t p = t0 + e;
-      la[pad16(e)] = p < n ? ra[p] : canonical_nan<TX>();
-      if (Sp::NARGS == 2) lb[pad16(e)] = p < n ? rb[p] : canonical_nan<TX>();
-    }
-  }
+  };
+  // Blocks of a row are dispatched in order, so the ticket is almost always
+  // blockIdx / rows: that tile is staged while the ticket atomic is in flight
+  // and restaged only when the ticket differs.
+  const int64_t guess = (int64_t)(blockIdx.x / (uint64_t)A.rows);
+  if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&A.counters[row], 1u);
+  stage(guess);
   __syncthreads();
+  const int64_t tile = s_ticket;
+  if (tile != guess) {
+    __syncthreads();
+    stage(tile);
+    __syncthreads();
+  }
+  const int64_t t0 = tile * TILE;
+  const bool full = t0 + TILE <= n;
 
   const int64_t e0 = (int64_t)tid * K;
   auto alpha_at = [&](int64_t p) -> double {

@@ -5,9 +5,10 @@
 // group_nanmean (numbagg/grouped.py:7-27).
 //
 // Design:
-//   * work items are (series, tile) pairs, tile = 4096 positions from the
-//     series start, exactly the tiles a plain move_mean launch on that series
-//     uses, so the move_mean outputs are bitwise identical to plain launches;
+//   * work items are (series, tile) pairs, tiles counted from the series start
+//     with exactly the length and arithmetic a plain move_mean launch on that
+//     series uses, so the move_mean outputs are bitwise identical to plain
+//     launches;
 //   * a persistent grid (one resident wave of workgroups) pulls items from an
 //     atomic work-queue counter, a few items per grab; every workgroup leaves
 //     the loop once the counter passes the last item;
@@ -26,6 +27,9 @@ namespace {
 constexpr int kRaggedGrab = 4;       // work items per queue grab
 constexpr int kRaggedLdsLabels = 1024;  // max labels of the LDS strategy
 
+// The ragged kernel evaluates move_mean tiles with exactly the variant a plain
+// float32 move_mean launch uses (items per thread, 1/c table), so both produce
+// bitwise-identical outputs.
 struct RaggedArgs {
   const float* values;
   const int32_t* labels;
@@ -40,6 +44,7 @@ struct RaggedArgs {
   unsigned int* queue;
   int64_t total_tiles, window, min_count;
   size_t stage_bytes;  // dynamic LDS used by the rolling tile
+  size_t lut_bytes;    // 1/c table after the staged segment (0 when unused)
 };
 
 __global__ void ragged_init_kernel(double* gsum, unsigned int* gcount, int64_t total_labels,
@@ -52,22 +57,26 @@ __global__ void ragged_init_kernel(double* gsum, unsigned int* gcount, int64_t t
   if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
 }
 
-template <int VEC>
+template <int VEC, int KP, bool LUT>
 __device__ __forceinline__ void ragged_rolling(const RaggedArgs& A, int64_t off, int64_t n,
                                                int64_t tile, unsigned char* smem,
-                                               RollingShared<Traits<float, kMean>::S>& rs) {
-  rolling_tile<float, kMean, true, VEC>(A.values + off, nullptr, A.move_out + off, n, A.window,
-                                        A.min_count, tile, smem, rs);
+                                               RollingShared<Traits<float, kMean>::S>& rs,
+                                               const double* lut) {
+  rolling_tile<float, kMean, true, VEC, KP, LUT>(A.values + off, nullptr, A.move_out + off, n,
+                                                 A.window, A.min_count, tile, smem, rs, lut);
 }
 
+template <int KP, bool LUT>
 __global__ __launch_bounds__(kMoveThreads) void ragged_kernel(RaggedArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ RollingShared<Traits<float, kMean>::S> rs;
   __shared__ long long s_base;
   __shared__ int s_series;
-  constexpr int TILE = kMoveThreads * 16;
+  constexpr int TILE = kMoveThreads * KP;
   const int tid = threadIdx.x;
-  double* l_sum = reinterpret_cast<double*>(smem_raw + A.stage_bytes);
+  double* lut = reinterpret_cast<double*>(smem_raw + A.stage_bytes);
+  if constexpr (LUT) fill_recip_lut(lut, A.window);  // ordered by the loop's first barrier
+  double* l_sum = reinterpret_cast<double*>(smem_raw + A.stage_bytes + A.lut_bytes);
   unsigned int* l_cnt = reinterpret_cast<unsigned int*>(l_sum + kRaggedLdsLabels);
 
   while (true) {
@@ -86,9 +95,9 @@ __global__ __launch_bounds__(kMoveThreads) void ragged_kernel(RaggedArgs A) {
       const int64_t tile = t - A.tile_offsets[s];
       // 1. move_mean of this tile (identical arithmetic to the plain kernel).
       if ((((uintptr_t)(A.values + off)) % 16 == 0) && (((uintptr_t)(A.move_out + off)) % 16 == 0))
-        ragged_rolling<4>(A, off, n, tile, smem_raw, rs);
+        ragged_rolling<4, KP, LUT>(A, off, n, tile, smem_raw, rs, lut);
       else
-        ragged_rolling<1>(A, off, n, tile, smem_raw, rs);
+        ragged_rolling<1, KP, LUT>(A, off, n, tile, smem_raw, rs, lut);
       // 2. group_nanmean accumulation of this tile's values.
       const int64_t lo = A.label_offsets[s];
       const int64_t K = A.label_offsets[s + 1] - lo;
@@ -151,6 +160,8 @@ size_t ragged_ws_bytes(int64_t total_labels) {
 
 extern "C" {
 
+int64_t nbg_ragged_tile_size(void) { return (int64_t)nbg::kMoveThreads * nbg::f32_sum_items(); }
+
 size_t nbg_ragged_workspace_bytes(int64_t nseries, int64_t total_labels) {
   (void)nseries;
   return nbg::ragged_ws_bytes(total_labels);
@@ -195,20 +206,30 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
   a.total_tiles = total_tiles;
   a.window = window;
   a.min_count = min_count;
-  constexpr int TILE = kMoveThreads * 16;
-  a.stage_bytes = ((size_t)(pad16((int64_t)TILE + window + 4) + 16) * sizeof(float) + 15) / 16 * 16;
-  const size_t lds = a.stage_bytes + kRaggedLdsLabels * (8 + 4);
+  const int kp = f32_sum_items();
+  const bool lut = mean_uses_lut(window);
+  a.stage_bytes = (size_t)rolling_stage_bytes<float>(kp, window, true, 1);
+  a.lut_bytes = lut ? ((size_t)(window + 1) * sizeof(double) + 15) / 16 * 16 : 0;
+  const size_t lds = a.stage_bytes + a.lut_bytes + kRaggedLdsLabels * (8 + 4);
   int64_t init_blocks = (total_labels + 255) / 256;
   if (init_blocks > 4096) init_blocks = 4096;
   if (init_blocks < 1) init_blocks = 1;
   hipLaunchKernelGGL(ragged_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, a.gsum,
                      a.gcount, total_labels, a.queue);
   NBG_HIP_TRY(hipGetLastError());
-  int cap = device_capacity_blocks((const void*)ragged_kernel, kMoveThreads, lds);
+  const void* kfn = kp == 32 ? (lut ? (const void*)ragged_kernel<32, true> : (const void*)ragged_kernel<32, false>)
+                             : (lut ? (const void*)ragged_kernel<16, true> : (const void*)ragged_kernel<16, false>);
+  int cap = device_capacity_blocks(kfn, kMoveThreads, lds);
   if (cap <= 0) return set_error(NBG_ERR_HIP, "ragged: cannot query occupancy");
   int64_t grid = total_tiles < cap ? total_tiles : cap;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(ragged_kernel, dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
+  if (kp == 32) {
+    if (lut) hipLaunchKernelGGL((ragged_kernel<32, true>), dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
+    else hipLaunchKernelGGL((ragged_kernel<32, false>), dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
+  } else {
+    if (lut) hipLaunchKernelGGL((ragged_kernel<16, true>), dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
+    else hipLaunchKernelGGL((ragged_kernel<16, false>), dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
+  }
   NBG_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(ragged_finalize_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, a.gsum,
                      a.gcount, group_out, total_labels);

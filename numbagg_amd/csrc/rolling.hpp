@@ -3,6 +3,7 @@
 // per-tile evaluation `rolling_tile`.
 #pragma once
 
+#include <cstdlib>
 #include <type_traits>
 
 #include "common.hpp"
@@ -105,6 +106,7 @@ struct SumStats {
 template <typename T, int NM>  // NM: 1 = (s1, s11), 2 = (s1, s2, s12[, s11, s22])
 struct MomStats {
   using A = typename std::conditional<sizeof(T) == 4, D, Comp<double>>::type;
+  using Acc = void;  // (no first-moment fast path)
   int c;
   A s1, s2, s11, s22, s12;
   __device__ __forceinline__ static MomStats zero() {
@@ -201,12 +203,15 @@ struct SumTraits {
   using S = SumStats<T, SumAccT<T, ACC32>>;
   static constexpr int NARGS = 1;
   static constexpr bool SHIFT = false;
-  __device__ static T out(const S& s, int64_t mc) {
+  __device__ static T out(const S& s, int64_t mc, const double* lut = nullptr) {
     if (KIND == kSum) {
       if (s.c < mc) return canonical_nan<T>();
       return (T)s.s.to_double();
     }
     if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
+    if constexpr (std::is_same<typename S::Acc, D>::value) {
+      if (lut != nullptr) return (T)(s.s.v * lut[s.c]);  // correctly rounded 1/c from LDS
+    }
     return (T)mean_of(s.s, s.c);
   }
 };
@@ -226,7 +231,7 @@ struct VarTraits {
   using S = MomStats<T, 1>;
   static constexpr int NARGS = 1;
   static constexpr bool SHIFT = true;
-  __device__ static T out(const S& s, int64_t mc) {
+  __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
     if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
     const double c = (double)s.c;
     const double m1 = s.s1.to_double();
@@ -243,7 +248,7 @@ struct Traits<T, kCov> {
   using S = MomStats<T, 2>;
   static constexpr int NARGS = 2;
   static constexpr bool SHIFT = true;
-  __device__ static T out(const S& s, int64_t mc) {
+  __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
     if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
     const double c = (double)s.c;
     return (T)((s.s12.to_double() - s.s1.to_double() * s.s2.to_double() / c) / (c - 1.0));
@@ -254,7 +259,7 @@ struct Traits<T, kCorr> {
   using S = MomStats<T, 2>;
   static constexpr int NARGS = 2;
   static constexpr bool SHIFT = true;
-  __device__ static T out(const S& s, int64_t mc) {
+  __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
     if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
     const double r = 1.0 / (double)s.c;
     const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
@@ -334,15 +339,34 @@ struct RollingShared {
 // One output tile [tile*TILE, tile*TILE + TILE) of one row (ra/rb/ro point at
 // the row). `smem_raw` holds the staged segment. Ends with a barrier so the
 // caller may reuse the shared memory for its next tile.
-template <typename T, int KIND, bool TAIL_LDS, int VEC>
+// Default outputs per thread: 16 float32 / 8 float64 (64 bytes of input).
+template <typename T>
+constexpr int default_items() {
+  return 64 / (int)sizeof(T);
+}
+
+// Staged-segment LDS bytes of one rolling tile (both inputs), 16-aligned; a
+// 1/c table for mean follows when `lut` is set.
+template <typename T>
+__host__ __device__ inline int64_t rolling_stage_bytes(int items, int64_t window, bool tail_lds, int nargs) {
+  const int64_t seg = (int64_t)kMoveThreads * items + (tail_lds ? window + 16 : 0);
+  return ((pad16(seg) + 16) * (int64_t)sizeof(T) * nargs + 15) / 16 * 16;
+}
+
+// One output tile [tile*TILE, tile*TILE + TILE) of one row (ra/rb/ro point at
+// the row), TILE = 256 threads x KP items. `smem_raw` holds the staged segment;
+// `lut` (mean only, may be null) holds 1/c for c = 0..window in LDS. Ends with a
+// barrier so the caller may reuse the shared memory for its next tile.
+template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false>
 __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* __restrict__ rb,
                                              T* __restrict__ ro, int64_t n, int64_t window,
                                              int64_t min_count, int64_t tile,
                                              unsigned char* smem_raw,
-                                             RollingShared<typename Traits<T, KIND>::S>& rs) {
+                                             RollingShared<typename Traits<T, KIND>::S>& rs,
+                                             const double* lut = nullptr) {
   using Tr = Traits<T, KIND>;
   using S = typename Tr::S;
-  constexpr int K = 64 / (int)sizeof(T);  // 16 float32 / 8 float64 outputs per thread
+  constexpr int K = KP;
   constexpr int TILE = kMoveThreads * K;
   BlockShared<S>& sh = rs.sh;
   const int64_t t0 = tile * TILE;
@@ -350,9 +374,10 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
 
   // Segment staged in LDS: [s0a, t0 + TILE) with s0a <= t0 - window (TAIL_LDS;
   // positions before the row start are staged as NaN, so every tail index is
-  // simply head index - window), or [t0, t0 + TILE).
+  // simply head index - window), or [t0, t0 + TILE). s0a is a multiple of 16,
+  // so each thread's head chunk starts a padded 16-slot group.
   const int64_t seg_begin = TAIL_LDS ? t0 - window : t0;
-  const int64_t s0a = seg_begin & ~(int64_t)(VEC - 1);  // floor, also for negatives
+  const int64_t s0a = seg_begin & ~(int64_t)15;  // floor, also for negatives
   const int64_t seg_len = t0 + TILE - s0a;
   const int64_t lds_elems = pad16(seg_len) + 16;
   T* la = reinterpret_cast<T*>(smem_raw);
@@ -428,9 +453,10 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   // LDS element index of the thread's first head / tail input (32-bit).
   const int eh = TAIL_LDS ? (int)(c0 - s0a) : tid * K;
   const int et = eh - (int)(TAIL_LDS ? window : 0);
+  const int ehp = pad16(eh);  // eh is a multiple of 16: pad16(eh + j) = ehp + j + j / 16
   auto head = [&](int j, T& a, T& b) {
-    a = la[pad16(eh + j)];
-    b = (Tr::NARGS == 2) ? lb[pad16(eh + j)] : T(0);
+    a = la[ehp + j + (j >> 4)];
+    b = (Tr::NARGS == 2) ? lb[ehp + j + (j >> 4)] : T(0);
   };
   auto tail = [&](int j, T& a, T& b) {
     if (TAIL_LDS) {
@@ -444,12 +470,55 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     }
   };
   T res[K];
-  if constexpr (!Tr::SHIFT) {
+  if constexpr (!Tr::SHIFT && std::is_same<typename S::Acc, D>::value) {
+    // float32 sum/mean with float64 accumulation, branch-free: exact
+    // differences and count deltas are formed first, then one running float64
+    // sum and int32 count produce the outputs (1/c from the LDS table).
+    double dj[K];
+    int dc[K];
+    double dsum = 0.0;
+    int dcnt = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      T h, t, u;
+      head(j, h, u);
+      tail(j, t, u);
+      const bool hv = !is_nan(h), tv = !is_nan(t);
+      dj[j] = (hv ? (double)h : 0.0) - (tv ? (double)t : 0.0);
+      dc[j] = (int)hv - (int)tv;
+      dsum += dj[j];
+      dcnt += dc[j];
+    }
+    S delta = S::zero();
+    delta.c = dcnt;
+    delta.s.v = dsum;
+    S total;
+    S st = block_excl_scan(delta, total, sh);
+    st.merge(w0);  // window ending at c0 - 1
+    const int64_t mce = KIND == kSum ? min_count : (min_count > 1 ? min_count : 1);
+    const int mc32 = mce > 0x7fffffffLL ? 0x7fffffff : (int)mce;
+    int c = st.c;
+    double acc = st.s.v;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      c += dc[j];
+      acc += dj[j];
+      double v;
+      if constexpr (KIND == kSum) {
+        v = acc;
+      } else if constexpr (LUT) {
+        v = acc * lut[c];  // c in [0, window]; correctly rounded 1/c
+      } else {
+        v = acc * recip(c > 0 ? c : 1);
+      }
+      res[j] = c >= mc32 ? (T)v : canonical_nan<T>();
+    }
+  } else if constexpr (!Tr::SHIFT) {
     // Sum/mean: the exact per-position differences head - tail stay in
     // registers between the scan and the output pass.
     using Acc = typename S::Acc;
     Acc dj[K];
-    unsigned int hbits = 0u, tbits = 0u;
+    unsigned long long hbits = 0ull, tbits = 0ull;
     S delta = S::zero();
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -458,8 +527,8 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       tail(j, t, u);
       const bool hv = !is_nan(h), tv = !is_nan(t);
       dj[j] = exact_diff<Acc, T>(hv ? h : T(0), tv ? t : T(0));
-      hbits |= (unsigned)hv << j;
-      tbits |= (unsigned)tv << j;
+      hbits |= (unsigned long long)hv << j;
+      tbits |= (unsigned long long)tv << j;
       delta.s.add(dj[j]);
       delta.c += (int)hv - (int)tv;
     }
@@ -468,9 +537,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     st.merge(w0);  // window ending at c0 - 1
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      st.c += (int)((hbits >> j) & 1u) - (int)((tbits >> j) & 1u);
+      st.c += (int)((hbits >> j) & 1ull) - (int)((tbits >> j) & 1ull);
       st.s.add(dj[j]);
-      res[j] = Tr::out(st, min_count);
+      res[j] = Tr::out(st, min_count, lut);
     }
   } else {
     S delta = S::zero();
@@ -494,7 +563,7 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       if (valid(a, b)) st.add_obs(a, b, ka, kb);
       tail(j, a, b);
       if (valid(a, b)) st.sub_obs(a, b, ka, kb);
-      res[j] = Tr::out(st, min_count);
+      res[j] = Tr::out(st, min_count, nullptr);
     }
   }
   if (c0 + K <= n && VEC > 1) {
@@ -513,7 +582,29 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   __syncthreads();
 }
 
-template <typename T, int KIND, bool TAIL_LDS, int VEC>
+// Largest window for which mean keeps a 1/c table in LDS.
+constexpr int64_t kMeanLutWindow = 4096;
+
+// lut[c] = 1/c (correctly rounded), c = 0..window; lut[0] is never used.
+__device__ __forceinline__ void fill_recip_lut(double* lut, int64_t window) {
+  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads) lut[c] = c ? 1.0 / (double)c : 0.0;
+}
+
+// Items per thread for float32 sum/mean: 16 (default) or 32, from
+// NBG_MOVE_ITEMS; a tuning knob (more items amortise the per-tile scan and
+// halo over more outputs, fewer keep more waves resident). Plain and ragged
+// launches read the same setting, so their tiles stay identical.
+inline int f32_sum_items() {
+  const char* v = getenv("NBG_MOVE_ITEMS");
+  if (v != nullptr && v[0] == '3' && v[1] == '2' && v[2] == '\0') return 32;
+  return 16;
+}
+
+// Whether a float32 mean of this window uses the table (plain and ragged
+// launches must agree, so both ask here).
+__host__ __device__ inline bool mean_uses_lut(int64_t window) { return window <= kMeanLutWindow; }
+
+template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false>
 __global__ __launch_bounds__(kMoveThreads) void move_kernel(
     const T* __restrict__ xa, const T* __restrict__ xb, T* __restrict__ out, int64_t rows,
     int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
@@ -525,15 +616,21 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
   // share their halo through L2).
   const int64_t row = (int64_t)(blockIdx.x / (uint64_t)tiles_per_row);
   const int64_t tile = (int64_t)(blockIdx.x % (uint64_t)tiles_per_row);
-  rolling_tile<T, KIND, TAIL_LDS, VEC>(xa + row * ld_a, (Tr::NARGS == 2) ? xb + row * ld_b : nullptr,
-                                       out + row * ld_out, n, window, min_count, tile, smem_raw, rs);
+  double* lut = nullptr;
+  if constexpr (LUT) {
+    lut = reinterpret_cast<double*>(smem_raw + rolling_stage_bytes<T>(KP, window, TAIL_LDS, Tr::NARGS));
+    fill_recip_lut(lut, window);  // the tile's first barrier orders these writes before any read
+  }
+  rolling_tile<T, KIND, TAIL_LDS, VEC, KP, LUT>(
+      xa + row * ld_a, (Tr::NARGS == 2) ? xb + row * ld_b : nullptr, out + row * ld_out, n, window,
+      min_count, tile, smem_raw, rs, lut);
 }
 
 // Largest window whose halo is staged in LDS: the padded segment of
 // TILE + window elements per input must fit a 60 KB budget.
 template <typename T, int NARGS>
 constexpr int64_t lds_window_limit() {
-  constexpr int K = 64 / (int)sizeof(T);
+  constexpr int K = default_items<T>();
   constexpr int64_t budget = 60 * 1024;
   return budget * 16 / (17 * (int64_t)sizeof(T) * NARGS) - kMoveThreads * K - 64;
 }

@@ -90,9 +90,39 @@ int launch_zero(void* p, size_t bytes, hipStream_t s) {
   return check_hip(hipGetLastError(), "zero_words_kernel launch");
 }
 
+// Streaming copy used to calibrate bandwidth measurements: the practical HBM
+// ceiling of a read+write stream, and the scale of the FETCH_SIZE/WRITE_SIZE
+// counters for a kernel of known traffic.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src,
+                                                          uint4* __restrict__ dst, int64_t n16) {
+  constexpr int U = 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n16; i += U * stride) {
+    uint4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace nbg
 
 extern "C" {
+
+int nbg_stream_copy(const void* src, void* dst, size_t bytes, void* stream) {
+  if (src == nullptr || dst == nullptr || bytes % 16 != 0 || ((uintptr_t)src % 16) || ((uintptr_t)dst % 16))
+    return nbg::set_error(NBG_ERR_INVALID, "nbg_stream_copy: 16-byte aligned pointers and size required");
+  const int64_t n16 = (int64_t)(bytes / 16);
+  if (n16 == 0) return NBG_OK;
+  int64_t blocks = (n16 + 255) / 256;
+  if (blocks > 256 * 32) blocks = 256 * 32;
+  hipLaunchKernelGGL(nbg::stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const uint4*)src, (uint4*)dst, n16);
+  return nbg::check_hip(hipGetLastError(), "stream_copy_kernel launch");
+}
 
 const char* nbg_version(void) { return "numbagg_amd 0.1.0 (gfx950)"; }
 

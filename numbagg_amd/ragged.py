@@ -25,7 +25,6 @@ import torch
 from . import _device as dev
 from . import _lib
 
-TILE = 4096
 LDS_MAX_LABELS = 1024
 STRATEGY_LDS = 0
 STRATEGY_GLOBAL = 1
@@ -69,7 +68,8 @@ class RaggedMoveMeanGroupMean:
         self.nseries = int(lengths.size)
         self.offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
         self.label_offsets = np.concatenate([[0], np.cumsum(num_labels)]).astype(np.int64)
-        tiles = (lengths + TILE - 1) // TILE
+        self.tile = int(_lib.query("nbg_ragged_tile_size"))
+        tiles = (lengths + self.tile - 1) // self.tile
         self.tile_offsets = np.concatenate([[0], np.cumsum(tiles)]).astype(np.int64)
         self.total = int(self.offsets[-1])
         self.total_labels = int(self.label_offsets[-1])
