@@ -291,10 +291,12 @@ class CostModel:
                  meta=np.array(json.dumps(self.meta)), **arrays)
 
     @classmethod
-    def load(cls, path: pathlib.Path | str = MODEL_PATH, device=None) -> "CostModel | None":
+    def load(cls, path: pathlib.Path | str | None = None, device=None) -> "CostModel | None":
         """The saved model, or None when missing, unreadable or built for another
         architecture / kernel version (callers then fall back to the tuner)."""
-        path = pathlib.Path(path)
+        import os
+
+        path = pathlib.Path(path or os.environ.get("NBG_COST_MODEL", MODEL_PATH))
         if not path.exists():
             return None
         try:
