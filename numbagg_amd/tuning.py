@@ -69,22 +69,52 @@ def features(lengths, num_labels, skew, nan_fraction, window) -> np.ndarray:
     ).astype(np.float32)
 
 
-def batch_statistics(rb: RaggedMoveMeanGroupMean, values: torch.Tensor, labels: torch.Tensor):
-    """Per-series skew (share of the most frequent label) and NaN fraction."""
+def series_chunks(lengths: np.ndarray, max_elements: int = 1 << 27):
+    """Consecutive series ranges [s0, s1) holding at most `max_elements`
+    elements each (a single longer series gets a range of its own), so that
+    per-element temporaries stay far below 2**31 elements."""
+    s0, acc = 0, 0
+    for i, n in enumerate(np.asarray(lengths, dtype=np.int64)):
+        if acc and acc + int(n) > max_elements:
+            yield s0, i
+            s0, acc = i, 0
+        acc += int(n)
+    if s0 < len(lengths):
+        yield s0, len(lengths)
+
+
+def batch_statistics(rb: RaggedMoveMeanGroupMean, values: torch.Tensor, labels: torch.Tensor,
+                     max_elements: int = 1 << 27):
+    """Per-series skew (share of the most frequent label) and NaN fraction,
+    computed over chunks of whole series."""
     dev_ = values.device
-    seg = torch.repeat_interleave(torch.arange(rb.nseries, device=dev_), torch.from_numpy(rb.lengths).to(dev_))
-    lab_off = torch.from_numpy(rb.label_offsets[:-1]).to(dev_)
-    lab = labels.long()
-    ok = (lab >= 0) & (lab < torch.from_numpy(rb.num_labels).to(dev_)[seg])
-    glab = (lab_off[seg] + lab.clamp(min=0))[ok]
-    counts = torch.bincount(glab, minlength=rb.total_labels).float()
-    nl = torch.from_numpy(rb.num_labels).to(dev_)
-    top = torch.segment_reduce(counts, "max", lengths=nl, unsafe=True) if rb.total_labels else counts
-    top = torch.nan_to_num(top, nan=0.0, neginf=0.0)
-    L = torch.from_numpy(rb.lengths).to(dev_).float()
-    skew = (top / L).cpu().numpy()
-    nan_cnt = torch.zeros(rb.nseries, device=dev_).index_add_(0, seg, torch.isnan(values).float())
-    return skew, (nan_cnt / L).cpu().numpy()
+    skew = np.zeros(rb.nseries)
+    nanf = np.zeros(rb.nseries)
+    for s0, s1 in series_chunks(rb.lengths, max_elements):
+        o0, o1 = int(rb.offsets[s0]), int(rb.offsets[s1])
+        if o1 == o0:
+            continue
+        lens = torch.from_numpy(rb.lengths[s0:s1]).to(dev_)
+        seg = torch.repeat_interleave(torch.arange(s1 - s0, device=dev_), lens)
+        k = torch.from_numpy(rb.num_labels[s0:s1]).to(dev_)
+        loff = torch.from_numpy(rb.label_offsets[s0:s1] - rb.label_offsets[s0]).to(dev_)
+        nlab = int(rb.label_offsets[s1] - rb.label_offsets[s0])
+        lab = labels[o0:o1].long()
+        v = values[o0:o1]
+        ok = (lab >= 0) & (lab < k[seg])
+        glab = (loff[seg] + lab.clamp(min=0))[ok]
+        counts = torch.bincount(glab, minlength=nlab).float()
+        if nlab:
+            top = torch.segment_reduce(counts, "max", lengths=k, unsafe=True)
+            top = torch.nan_to_num(top, nan=0.0, neginf=0.0)
+        else:
+            top = torch.zeros(s1 - s0, device=dev_)
+        L = lens.float()
+        skew[s0:s1] = (top / L).cpu().numpy()
+        nan_cnt = torch.zeros(s1 - s0, device=dev_).index_add_(0, seg, torch.isnan(v).float())
+        nanf[s0:s1] = (nan_cnt / L).cpu().numpy()
+        del seg, lab, v, ok, glab, counts
+    return skew, nanf
 
 
 def synthetic_labels(length: int, k: int, zipf: float, gen: torch.Generator, device) -> torch.Tensor:

@@ -1,0 +1,139 @@
+"""Per-output error report against the exact oracle on the benchmark inputs'
+distributions (scaled-down rows): random walk + 1e4 offset with 2% NaN for the
+rolling and exponential scans (window 1000 / min_count 500, alpha 0.01),
+N(0, 1) with 5% NaN and uniform or Zipf(1.1) labels (1000) for the grouped
+reductions, geometric NaN runs for ffill/bfill.
+
+For every output it records the maximum relative error |gpu - exact| / |exact|
+and, for float32 outputs, the maximum error in float32 ULPs, beside the same
+figures for the C restatement of the reference. The bars asserted here are the
+per-function bars of the parity tests; setting NBG_PRECISION_REPORT=<path>
+also writes the table as JSON.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ._data import runs_array, walk_array
+
+pytestmark = pytest.mark.gpu
+
+REPORT: dict[str, dict] = {}
+
+
+def _rel(got, want):
+    ok = ~np.isnan(want)
+    if not ok.any():
+        return 0.0
+    g = np.asarray(got, dtype=np.float64)[ok]
+    w = want[ok]
+    return float(np.max(np.abs(g - w) / np.maximum(np.abs(w), np.finfo(np.float64).tiny)))
+
+
+def _ulp(got, want):
+    ok = ~np.isnan(want)
+    if not ok.any() or got.dtype != np.float32:
+        return None
+    w32 = want[ok].astype(np.float32)
+    sp = np.spacing(np.abs(w32)).astype(np.float64)
+    return float(np.max(np.abs(got[ok].astype(np.float64) - want[ok]) / sp))
+
+
+def _record(name, got, ref, want, rel_bar=None, ulp_bar=None):
+    assert np.array_equal(np.isnan(got), np.isnan(want)), f"{name}: NaN mask differs from the exact oracle"
+    rec = {
+        "gpu_max_rel": _rel(got, want),
+        "gpu_max_ulp": _ulp(got, want),
+        "ref_max_rel": _rel(ref, want) if ref is not None else None,
+        "ref_max_ulp": _ulp(ref, want) if ref is not None else None,
+        "outputs": int(np.count_nonzero(~np.isnan(want))),
+    }
+    REPORT[name] = rec
+    print(f"{name}: {rec}")
+    if rel_bar is not None:
+        assert rec["gpu_max_rel"] <= rel_bar, (name, rec)
+    if ulp_bar is not None:
+        assert rec["gpu_max_ulp"] <= ulp_bar, (name, rec)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _write_report():
+    yield
+    path = os.environ.get("NBG_PRECISION_REPORT")
+    if path and REPORT:
+        with open(path, "w") as f:
+            json.dump(REPORT, f, indent=1, sort_keys=True)
+
+
+@pytest.mark.parametrize("func", ["move_sum", "move_mean", "move_std", "move_var", "move_cov", "move_corr"])
+def test_report_rolling(gpu_lib, func):
+    import numbagg_amd as nb
+    from oracle import exact, restated
+
+    x = walk_array(4, 100_000, seed=31)
+    y = walk_array(4, 100_000, seed=32) if func in ("move_cov", "move_corr") else None
+    args = (x,) if y is None else (x, y)
+    got = getattr(nb, func)(*args, window=1000, min_count=500)
+    want = exact.move(func, x, 1000, 500, y=y)
+    ref = restated.move(func, x, 1000, 500, y=y)
+    bars = {"move_cov": None, "move_corr": None}
+    _record(f"{func} f32 w1000", got, ref, want, ulp_bar=bars.get(func, 1.0))
+
+
+@pytest.mark.parametrize(
+    "func",
+    ["move_exp_nancount", "move_exp_nanmean", "move_exp_nansum", "move_exp_nanvar", "move_exp_nanstd",
+     "move_exp_nancov", "move_exp_nancorr"],
+)
+def test_report_exp(gpu_lib, func):
+    import numbagg_amd as nb
+    from oracle import exact, restated
+
+    x = walk_array(4, 200_000, seed=33)
+    y = walk_array(4, 200_000, seed=34) if func in ("move_exp_nancov", "move_exp_nancorr") else None
+    args = (x,) if y is None else (x, y)
+    got = getattr(nb, func)(*args, alpha=0.01)
+    want = exact.move_exp(func, x, 0.01, y=y, loop_dtype=np.float64)
+    ref = restated.move_exp(func, x.astype(np.float64), 0.01, y=None if y is None else y.astype(np.float64))
+    moments = func in ("move_exp_nanvar", "move_exp_nanstd", "move_exp_nancov", "move_exp_nancorr")
+    _record(f"{func} alpha0.01 (float64 loop)", got, ref, want, rel_bar=None if moments else 1e-11)
+
+
+@pytest.mark.parametrize("labels", ["uniform", "zipf"])
+@pytest.mark.parametrize("func", ["group_nansum", "group_nanmean", "group_nanvar", "group_nanstd"])
+def test_report_grouped(gpu_lib, func, labels):
+    import numbagg_amd as nb
+    from oracle import exact, restated
+
+    rng = np.random.default_rng(35)
+    n, L = 4_000_000, 1000
+    v = rng.standard_normal((2, n), dtype=np.float32)
+    v[rng.random(v.shape) < 0.05] = np.nan
+    if labels == "uniform":
+        lab = rng.integers(0, L, n).astype(np.int32)
+    else:
+        p = np.arange(1, L + 1, dtype=np.float64) ** -1.1
+        lab = rng.choice(L, size=n, p=p / p.sum()).astype(np.int32)
+    got = getattr(nb, func)(v, lab, num_labels=L, axis=-1)
+    want = exact.group(func, v, lab, L)
+    ref = restated.group(func, v, lab.astype(np.int64), L)
+    moments = func in ("group_nanvar", "group_nanstd")
+    _record(f"{func} f32 {labels}", got, ref, want, rel_bar=2e-7 if moments else None)
+
+
+@pytest.mark.parametrize("func", ["ffill", "bfill"])
+@pytest.mark.parametrize("limit", [None, 16])
+def test_report_fill(gpu_lib, func, limit):
+    import numbagg_amd as nb
+    from oracle import exact
+
+    x = runs_array(4, 1_000_000, seed=36)
+    got = getattr(nb, func)(x, limit=limit)
+    want = exact.fill(func, x, limit)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    REPORT[f"{func} limit={limit}"] = {"bit_exact": True, "outputs": int(got.size)}
