@@ -373,8 +373,8 @@ def main(argv=None):
     ms_step = wall * 1e3 / args.steps
     total_bytes = sum(w.bytes for w in wl)
     value = total_bytes / (ms_step * 1e-3) / 1e9
-    dom = max(wl, key=lambda w: per[w.name]["ms"])
-    dom_ach = per[dom.name]["GB/s"]
+    dom = max(wl, key=lambda w: per[w.name]["ms"]) if wl else None
+    dom_ach = per[dom.name]["GB/s"] if dom else None
 
     cpu = None
     if not args.no_cpu:
@@ -432,7 +432,7 @@ def main(argv=None):
             "unit": "GB/s",
             "frac": round(dom_ach / HBM_PEAK_GBS, 4),
             "traffic": None,
-        },
+        } if dom else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec))

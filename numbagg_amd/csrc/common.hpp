@@ -133,6 +133,20 @@ __device__ __forceinline__ void spin_pause() { __builtin_amdgcn_s_sleep(1); }
 
 // LDS index with one padding slot per 16 elements: threads that each own 16
 // consecutive elements then hit distinct banks on every strided access.
+// Workgroups of one launch are handed to the 8 XCDs round-robin (workgroup b
+// runs on XCD b % 8), and each XCD has its own L2. xcd_order(b, n) renumbers
+// the grid so that consecutive logical ids run on the same XCD one after the
+// other: logical id v = (b % 8) * (n / 8) + b / 8 for the first 8 * (n / 8)
+// workgroups (a bijection onto [0, 8 * (n / 8))), the rest keep their id.
+// Kernels whose neighbouring work items share inputs (a rolling tile and the
+// halo of the next one, the row groups of one label chunk) then find those
+// inputs in their XCD's L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n) {
+  const uint32_t per = n / kXcds;
+  return b < per * kXcds ? (b % kXcds) * per + b / kXcds : b;
+}
+
 __host__ __device__ __forceinline__ int64_t pad16(int64_t e) { return e + (e >> 4); }
 __host__ __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 

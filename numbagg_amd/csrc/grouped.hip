@@ -121,8 +121,13 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t L1 = A.num_labels;        // labels per row
   const int64_t L = L1 * R;               // LDS slots (row-major: r * L1 + label)
-  const int64_t row0 = (int64_t)(blockIdx.x / (uint64_t)A.chunks_per_row) * R;
-  const int64_t ck = (int64_t)(blockIdx.x % (uint64_t)A.chunks_per_row);
+  // Chunk-major order on XCD-consecutive logical ids: the row groups of one
+  // chunk run on the same XCD back to back and read its labels from that
+  // XCD's L2 after the first of them.
+  const uint64_t groups = (uint64_t)((A.rows + R - 1) / R);
+  const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
+  const int64_t row0 = (int64_t)(v % groups) * R;
+  const int64_t ck = (int64_t)(v / groups);
   const int64_t c0 = ck * A.chunk;
   const int64_t c1 = (c0 + A.chunk < A.n) ? c0 + A.chunk : A.n;
   const int nr = (int)((A.rows - row0) < R ? (A.rows - row0) : R);  // rows present

@@ -24,8 +24,8 @@
 namespace nbg {
 namespace {
 
-constexpr int kRaggedGrab = 4;       // work items per queue grab
-constexpr int kRaggedLdsLabels = 1024;  // max labels of the LDS strategy
+constexpr int kRaggedGrab = 8;       // work items (tiles) per queue grab
+constexpr int kRaggedLdsLabels = 1024;  // labels [0, 1024) of a series privatised in LDS
 
 // The ragged kernel evaluates move_mean tiles with exactly the variant a plain
 // float32 move_mean launch uses (items per thread, 1/c table), so both produce
@@ -66,8 +66,22 @@ __device__ __forceinline__ void ragged_rolling(const RaggedArgs& A, int64_t off,
                                                  A.window, A.min_count, tile, smem, rs, lut);
 }
 
+// Flushes the LDS label slots [0, nl) of series `s` into its global
+// accumulators (only slots that saw a value), then waits for the workgroup.
+__device__ __forceinline__ void ragged_flush(const RaggedArgs& A, int64_t lo, int64_t nl,
+                                             const double* l_sum, const unsigned int* l_cnt) {
+  for (int64_t l = threadIdx.x; l < nl; l += kMoveThreads) {
+    const unsigned int c = l_cnt[l];
+    if (c) {
+      unsafeAtomicAdd(&A.gsum[lo + l], l_sum[l]);
+      atomicAdd(&A.gcount[lo + l], c);
+    }
+  }
+  __syncthreads();
+}
+
 template <int KP, bool LUT>
-__global__ __launch_bounds__(kMoveThreads) void ragged_kernel(RaggedArgs A) {
+__global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4))) void ragged_kernel(RaggedArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ RollingShared<Traits<float, kMean>::S> rs;
   __shared__ long long s_base;
@@ -78,6 +92,7 @@ __global__ __launch_bounds__(kMoveThreads) void ragged_kernel(RaggedArgs A) {
   if constexpr (LUT) fill_recip_lut(lut, A.window);  // ordered by the loop's first barrier
   double* l_sum = reinterpret_cast<double*>(smem_raw + A.stage_bytes + A.lut_bytes);
   unsigned int* l_cnt = reinterpret_cast<unsigned int*>(l_sum + kRaggedLdsLabels);
+  const float* la = reinterpret_cast<const float*>(smem_raw);  // the staged segment
 
   while (true) {
     if (tid == 0) s_base = (long long)atomicAdd(A.queue, (unsigned)kRaggedGrab);
@@ -86,6 +101,10 @@ __global__ __launch_bounds__(kMoveThreads) void ragged_kernel(RaggedArgs A) {
     __syncthreads();
     if (base >= A.total_tiles) break;  // uniform: every workgroup exits here
     const int64_t stop = base + kRaggedGrab < A.total_tiles ? base + kRaggedGrab : A.total_tiles;
+    // LDS label slots stay live across consecutive tiles of one series within
+    // the grab and are flushed when the series changes or the grab ends.
+    int cur = -1;
+    int64_t cur_lo = 0, cur_nl = 0;
     for (int64_t t = base; t < stop; ++t) {
       if (tid == 0) s_series = A.tile_series[t];
       __syncthreads();
@@ -93,52 +112,66 @@ __global__ __launch_bounds__(kMoveThreads) void ragged_kernel(RaggedArgs A) {
       const int64_t off = A.offsets[s];
       const int64_t n = A.offsets[s + 1] - off;
       const int64_t tile = t - A.tile_offsets[s];
+      const int64_t lo = A.label_offsets[s];
+      const int64_t K = A.label_offsets[s + 1] - lo;
+      // Strategy 0 keeps labels below kRaggedLdsLabels in LDS (all of them when
+      // K fits) and sends the rest to global atomics; strategy 1 is global only.
+      const int64_t nl = A.strategy[s] == 0 ? (K < kRaggedLdsLabels ? K : kRaggedLdsLabels) : 0;
+      if (s != cur) {
+        if (cur_nl > 0) ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
+        for (int64_t l = tid; l < nl; l += kMoveThreads) {
+          l_sum[l] = 0.0;
+          l_cnt[l] = 0u;
+        }
+        cur = s;
+        cur_lo = lo;
+        cur_nl = nl;  // the zeroing is ordered before use by rolling_tile's barriers
+      }
       // 1. move_mean of this tile (identical arithmetic to the plain kernel).
       if ((((uintptr_t)(A.values + off)) % 16 == 0) && (((uintptr_t)(A.move_out + off)) % 16 == 0))
         ragged_rolling<4, KP, LUT>(A, off, n, tile, smem_raw, rs, lut);
       else
         ragged_rolling<1, KP, LUT>(A, off, n, tile, smem_raw, rs, lut);
-      // 2. group_nanmean accumulation of this tile's values.
-      const int64_t lo = A.label_offsets[s];
-      const int64_t K = A.label_offsets[s + 1] - lo;
-      const int64_t p0 = tile * TILE;
-      const int64_t p1 = p0 + TILE < n ? p0 + TILE : n;
-      const float* v = A.values + off;
+      // Labels of this thread's KP positions (vector loads when aligned).
+      const int64_t t0 = tile * TILE;
+      const int64_t c0 = t0 + (int64_t)tid * KP;
       const int32_t* lab = A.labels + off;
-      if (A.strategy[s] == 0 && K <= kRaggedLdsLabels) {
-        for (int64_t l = tid; l < K; l += kMoveThreads) {
-          l_sum[l] = 0.0;
-          l_cnt[l] = 0u;
+      int32_t lj[KP];
+      if (c0 + KP <= n && ((uintptr_t)(lab + c0)) % 16 == 0) {
+#pragma unroll
+        for (int j = 0; j < KP; j += 4) {
+          const int4 q = *reinterpret_cast<const int4*>(lab + c0 + j);
+          lj[j] = q.x;
+          lj[j + 1] = q.y;
+          lj[j + 2] = q.z;
+          lj[j + 3] = q.w;
         }
-        __syncthreads();
-        for (int64_t p = p0 + tid; p < p1; p += kMoveThreads) {
-          const int32_t l = lab[p];
-          const float x = v[p];
-          if (l >= 0 && l < K && !is_nan(x)) {
+      } else {
+#pragma unroll
+        for (int j = 0; j < KP; ++j) lj[j] = c0 + j < n ? lab[c0 + j] : -1;
+      }
+      // 2. group_nanmean accumulation of the thread's KP values, read from the
+      //    staged segment (staged from s0a = floor16(t0 - window); positions
+      //    past the series end hold NaN and carry label -1 here).
+      const int64_t s0a = (t0 - A.window) & ~(int64_t)15;
+      const int ehp = pad16((int)(c0 - s0a));
+#pragma unroll
+      for (int j = 0; j < KP; ++j) {
+        const float x = la[ehp + j + (j >> 4)];
+        const int32_t l = lj[j];
+        if (l >= 0 && l < K && !is_nan(x)) {
+          if (l < nl) {
             atomicAdd(&l_sum[l], (double)x);
             atomicAdd(&l_cnt[l], 1u);
-          }
-        }
-        __syncthreads();
-        for (int64_t l = tid; l < K; l += kMoveThreads) {
-          const unsigned int c = l_cnt[l];
-          if (c) {
-            unsafeAtomicAdd(&A.gsum[lo + l], l_sum[l]);
-            atomicAdd(&A.gcount[lo + l], c);
-          }
-        }
-        __syncthreads();
-      } else {
-        for (int64_t p = p0 + tid; p < p1; p += kMoveThreads) {
-          const int32_t l = lab[p];
-          const float x = v[p];
-          if (l >= 0 && l < K && !is_nan(x)) {
+          } else {
             unsafeAtomicAdd(&A.gsum[lo + l], (double)x);
             atomicAdd(&A.gcount[lo + l], 1u);
           }
         }
       }
+      __syncthreads();  // staged segment reads done before the next tile stages
     }
+    if (cur_nl > 0) ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
   }
 }
 

@@ -497,21 +497,32 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     st.merge(w0);  // window ending at c0 - 1
     const int64_t mce = KIND == kSum ? min_count : (min_count > 1 ? min_count : 1);
     const int mc32 = mce > 0x7fffffffLL ? 0x7fffffff : (int)mce;
+    int cc[K];
     int c = st.c;
-    double acc = st.s.v;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       c += dc[j];
-      acc += dj[j];
-      double v;
-      if constexpr (KIND == kSum) {
-        v = acc;
-      } else if constexpr (LUT) {
-        v = acc * lut[c];  // c in [0, window]; correctly rounded 1/c
-      } else {
-        v = acc * recip(c > 0 ? c : 1);
+      cc[j] = c;
+    }
+    // 1/c for every output first (all table loads in flight together).
+    double rc[K];
+    if constexpr (KIND != kSum) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        if constexpr (LUT) rc[j] = lut[cc[j]];  // c in [0, window]
+        else rc[j] = recip(cc[j] > 0 ? cc[j] : 1);
       }
-      res[j] = c >= mc32 ? (T)v : canonical_nan<T>();
+    }
+    double acc = st.s.v;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      acc += dj[j];
+      const double v = (KIND == kSum) ? acc : acc * rc[j];
+      // Adding +0 (enough observations) or NaN (too few) instead of
+      // selecting keeps the compiler from branching around each output; the
+      // +0 also maps a -0 result to +0, as the reference's running sum
+      // (started at +0) never yields -0.
+      res[j] = (T)v + (cc[j] >= mc32 ? T(0) : canonical_nan<T>());
     }
   } else if constexpr (!Tr::SHIFT) {
     // Sum/mean: the exact per-position differences head - tail stay in
@@ -585,9 +596,10 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
 // Largest window for which mean keeps a 1/c table in LDS.
 constexpr int64_t kMeanLutWindow = 4096;
 
-// lut[c] = 1/c (correctly rounded), c = 0..window; lut[0] is never used.
+// lut[c] = 1/c (hardware reciprocal + one Newton step), c = 0..window; lut[0]
+// is never used.
 __device__ __forceinline__ void fill_recip_lut(double* lut, int64_t window) {
-  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads) lut[c] = c ? 1.0 / (double)c : 0.0;
+  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads) lut[c] = c ? recip((int)c) : 0.0;
 }
 
 // Items per thread for float32 sum/mean: 16 (default) or 32, from
@@ -612,10 +624,11 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
   using Tr = Traits<T, KIND>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ RollingShared<typename Tr::S> rs;
-  // Consecutive blocks walk consecutive tiles of a row (neighbouring tiles
-  // share their halo through L2).
-  const int64_t row = (int64_t)(blockIdx.x / (uint64_t)tiles_per_row);
-  const int64_t tile = (int64_t)(blockIdx.x % (uint64_t)tiles_per_row);
+  // Consecutive logical ids walk consecutive tiles of a row on one XCD, so a
+  // tile's halo (the end of the previous tile) is read from that XCD's L2.
+  const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
+  const int64_t row = (int64_t)(v / (uint64_t)tiles_per_row);
+  const int64_t tile = (int64_t)(v % (uint64_t)tiles_per_row);
   double* lut = nullptr;
   if constexpr (LUT) {
     lut = reinterpret_cast<double*>(smem_raw + rolling_stage_bytes<T>(KP, window, TAIL_LDS, Tr::NARGS));

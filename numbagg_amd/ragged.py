@@ -8,10 +8,12 @@ The reference semantics are the per-series plain calls
 ``group_nanmean(series, labels, num_labels=K)`` (numbagg/grouped.py:7-27);
 move_mean outputs are bitwise equal to them.
 
-The per-series group-by strategy (LDS-privatised accumulators or global
-atomics) comes from ``strategy``: "heuristic" (a label-count threshold),
-"lds", "global", or an explicit per-series array (what the autotuner or the
-cost model in ``numbagg_amd.tuning`` produce). The launch sequence can be
+The per-series group-by strategy comes from ``strategy``: STRATEGY_LDS keeps
+the accumulators of labels below LDS_MAX_LABELS in LDS (all labels when the
+series has no more; the rest go to global atomics), STRATEGY_GLOBAL sends
+every label to global atomics. "heuristic" (LDS when the labels fit, global
+otherwise), "lds", "global", or an explicit per-series array (what the
+autotuner or the cost model in ``numbagg_amd.tuning`` produce) select them. The launch sequence can be
 captured once in a HIP graph and replayed (``graph``).
 """
 
@@ -31,10 +33,10 @@ STRATEGY_GLOBAL = 1
 
 
 def heuristic_strategy(lengths: np.ndarray, num_labels: np.ndarray) -> np.ndarray:
-    """LDS-privatised accumulators when a tile's flush (num_labels slots) is
-    small next to the tile itself, global atomics otherwise."""
+    """The label-count rule: LDS-privatised accumulators when all of a
+    series' labels fit the LDS slots, global atomics otherwise."""
     k = np.asarray(num_labels)
-    return np.where(k <= 256, STRATEGY_LDS, STRATEGY_GLOBAL).astype(np.uint8)
+    return np.where(k <= LDS_MAX_LABELS, STRATEGY_LDS, STRATEGY_GLOBAL).astype(np.uint8)
 
 
 class RaggedMoveMeanGroupMean:
@@ -100,8 +102,8 @@ class RaggedMoveMeanGroupMean:
             arr = np.asarray(strategy, dtype=np.uint8)
             if arr.shape != (self.nseries,):
                 raise ValueError("strategy array must have one entry per series")
-        # series with more labels than the LDS slots always use global atomics
-        arr = np.where(self.num_labels > LDS_MAX_LABELS, STRATEGY_GLOBAL, arr).astype(np.uint8)
+        if np.any(arr > STRATEGY_GLOBAL):
+            raise ValueError("strategy entries must be STRATEGY_LDS (0) or STRATEGY_GLOBAL (1)")
         self.strategy = arr
         self._d_strategy = torch.from_numpy(arr.copy()).to(self.device if hasattr(self, "device") else "cuda")
 

@@ -1,9 +1,12 @@
 """Strategy selection for the ragged batched path: autotuner and cost model.
 
 For each series of a ragged batch the kernel can fold values into its label
-accumulators through LDS-privatised slots (flushed per tile) or through global
-atomics. Which is faster depends on the series' length, label count, label
-skew, NaN fraction (and window, through the tile's rolling work).
+accumulators through LDS-privatised slots for the labels below LDS_MAX_LABELS
+(flushed when the workgroup moves to another series; larger labels still go
+to global atomics) or through global atomics only. Which is faster depends on
+the series' length, label count, label skew (how much of the mass the low
+labels carry, and how often lanes collide on one address), NaN fraction (and
+window, through the tile's rolling work).
 
 * ``Autotuner`` measures both strategies exhaustively: it times the ragged
   kernel on homogeneous synthetic batches (one feature point each) with HIP
@@ -179,9 +182,6 @@ class Autotuner:
         skew, nanf = batch_statistics(rb, values, labels)
         times = {}
         for st in STRATEGIES:
-            if st == STRATEGY_LDS and k > LDS_MAX_LABELS:
-                times[st] = math.inf
-                continue
             times[st] = self._time(rb, values, labels, st)
         entry = {
             "length": length, "num_labels": k, "zipf": zipf, "nan": nan, "window": window,
@@ -193,7 +193,7 @@ class Autotuner:
 
     def run_grid(self, lengths=None, ks=None, zipfs=(0.0, 1.1, 2.0), nans=(0.05, 0.4), windows=(20, 200), seed=0):
         lengths = lengths or [2**e for e in range(7, 21, 2)]
-        ks = ks or [8, 32, 128, 512, 1024]
+        ks = ks or [8, 32, 128, 512, 1024, 4096, 16384, 65536]
         i = 0
         for L in lengths:
             for k in ks:
@@ -310,8 +310,7 @@ class CostModel:
 
     def choose(self, X: np.ndarray, num_labels: np.ndarray) -> np.ndarray:
         pred = self.predict(X)
-        st = np.where(pred[:, 0] <= pred[:, 1], STRATEGY_LDS, STRATEGY_GLOBAL)
-        return np.where(np.asarray(num_labels) > LDS_MAX_LABELS, STRATEGY_GLOBAL, st).astype(np.uint8)
+        return np.where(pred[:, 0] <= pred[:, 1], STRATEGY_LDS, STRATEGY_GLOBAL).astype(np.uint8)
 
     def save(self, path: pathlib.Path | str = MODEL_PATH) -> None:
         path = pathlib.Path(path)
@@ -352,12 +351,10 @@ def exhaustive_strategies(rb: RaggedMoveMeanGroupMean, values, labels, skew, nan
         np.round(skew * 10), np.round(nanf * 10)], axis=1)
     _, bucket = np.unique(key, axis=0, return_inverse=True)
     bucket = bucket.ravel()
-    out = np.where(rb.num_labels > LDS_MAX_LABELS, STRATEGY_GLOBAL, STRATEGY_LDS).astype(np.uint8)
+    out = np.full(rb.nseries, STRATEGY_LDS, dtype=np.uint8)
     tuner = Autotuner(db_path=pathlib.Path("/nonexistent/db.json"), reps=reps)
     for b in np.unique(bucket):
         idx = np.nonzero(bucket == b)[0]
-        if np.all(rb.num_labels[idx] > LDS_MAX_LABELS):
-            continue
         # time a sample of the bucket: up to 64 series / about 8M elements
         budget, take = 1 << 23, []
         for i in idx[:64]:
@@ -373,7 +370,7 @@ def exhaustive_strategies(rb: RaggedMoveMeanGroupMean, values, labels, skew, nan
         t_lds = tuner._time(sub, sv, sl, STRATEGY_LDS)
         t_glb = tuner._time(sub, sv, sl, STRATEGY_GLOBAL)
         out[idx] = STRATEGY_LDS if t_lds <= t_glb else STRATEGY_GLOBAL
-    return np.where(rb.num_labels > LDS_MAX_LABELS, STRATEGY_GLOBAL, out).astype(np.uint8)
+    return out
 
 
 def choose_strategies(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "model", model: CostModel | None = None):

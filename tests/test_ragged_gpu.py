@@ -54,12 +54,15 @@ def test_ragged_matches_plain_launches(gpu_lib, strategy):
     assert np.mean(a == b) > 0.999
 
 
-def test_ragged_against_oracle(gpu_lib):
+@pytest.mark.parametrize("strategy,khi", [("heuristic", 2000), ("lds", 50_000), ("global", 50_000)])
+def test_ragged_against_oracle(gpu_lib, strategy, khi):
+    """Includes series with more labels than the LDS slots: with "lds" their
+    low labels are privatised and the rest go to global atomics."""
     from numbagg_amd.ragged import RaggedMoveMeanGroupMean
     from oracle import exact
 
-    lengths, ks, v, lab = make_batch(40, 100, 20_000, 10, 2000, seed=2)
-    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=40)
+    lengths, ks, v, lab = make_batch(40, 100, 20_000, 10, khi, seed=2)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=40, strategy=strategy)
     mm, gm = rb(v, lab)
     vn, ln, mmn, gmn = v.cpu().numpy(), lab.cpu().numpy(), mm.cpu().numpy(), gm.cpu().numpy()
     for s in range(len(lengths)):
