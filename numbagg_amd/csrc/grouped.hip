@@ -97,8 +97,7 @@ struct GArgs {
   const void* labels;
   void* out;
   int64_t rows, n, ld_v, num_labels, ddof;
-  int64_t chunk, chunks_per_row;
-  int64_t chunk_r4;  // chunk length when four rows share a workgroup
+  int64_t chunk, chunks_per_row;  // set per launch (launch_pass)
   // global accumulators (rows x num_labels each)
   void* acc0;
   void* acc1;
@@ -177,10 +176,18 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   }
 
   // Update slot r * L1 + label with value v at position i.
+  // One unsigned compare rejects both negative and too-large labels (32-bit
+  // for int32 labels: every valid label is below 2**31).
+  const uint32_t lim32 = L1 < 0x80000000LL ? (uint32_t)L1 : 0x80000000u;
   auto update = [&](TV v, TL lbl, int64_t i, int r) {
-    if (lbl < 0 || (int64_t)lbl >= L1) return;
+    if constexpr (sizeof(TL) == 4) {
+      if ((uint32_t)lbl >= lim32) return;
+    } else {
+      if ((uint64_t)lbl >= (uint64_t)L1) return;
+    }
     if (!gvalid(v)) return;
-    const int64_t l = (int64_t)r * L1 + (int64_t)lbl;
+    // LDS slots fit 32-bit indexing (the LDS budget bounds R * L1)
+    const int64_t l = LDS ? (int64_t)(r * (int)L1 + (int)lbl) : (int64_t)r * L1 + (int64_t)lbl;
     if constexpr (PASS == kPSum) {
       if (LDS) {
         atomicAdd(&l_s[l], (SA)v);  // ds_add_f64 / ds_add_u64
@@ -243,20 +250,27 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
     struct alignas(VE * sizeof(TL)) W { TL v[VE]; };
     for (int64_t k = 0; k < nfull; ++k) {
       const int64_t base = c0 + k * SUPER + (int64_t)tid * VE;
+      // All loads of the super-step (labels once, then every row's values)
+      // are issued before any update, so one wait covers them.
       W ll[kGroupUnroll];
+      V vv[R][kGroupUnroll];
 #pragma unroll
       for (int u = 0; u < kGroupUnroll; ++u) ll[u] = *reinterpret_cast<const W*>(lab + base + u * STEP);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if (r >= nr) break;
-        const TV* vr = vbase + (int64_t)r * A.ld_v;
-        V vv[kGroupUnroll];
+        if (r < nr) {
+          const TV* vr = vbase + (int64_t)r * A.ld_v;
 #pragma unroll
-        for (int u = 0; u < kGroupUnroll; ++u) vv[u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+          for (int u = 0; u < kGroupUnroll; ++u) vv[r][u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r >= nr) break;
 #pragma unroll
         for (int u = 0; u < kGroupUnroll; ++u)
 #pragma unroll
-          for (int e = 0; e < VE; ++e) update(vv[u].v[e], ll[u].v[e], base + u * STEP + e, r);
+          for (int e = 0; e < VE; ++e) update(vv[r][u].v[e], ll[u].v[e], base + u * STEP + e, r);
       }
     }
   }
@@ -385,10 +399,22 @@ int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
   }
 }
 
+// Chunking: the row groups' positions are split so that the whole grid is
+// one wave of resident workgroups (the device's capacity for this kernel and
+// LDS size): no partly filled second wave, and each workgroup flushes its
+// accumulators once. Chunks are multiples of 4096 positions (whole vector
+// super-steps) and at least 4096.
 template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT>
 int launch_pass(GArgs a, size_t lds, hipStream_t s) {
-  a.chunks_per_row = (a.n + a.chunk - 1) / a.chunk;
   const int64_t groups = (a.rows + R - 1) / R;
+  int cap = device_capacity_blocks((const void*)group_pass_kernel<TV, TL, PASS, LDS, R, COUNT>,
+                                   kGroupThreads, LDS ? lds : 0);
+  if (cap <= 0) cap = 2048;
+  const int64_t per_group = cap / groups > 1 ? cap / groups : 1;
+  int64_t chunk = (a.n + per_group - 1) / per_group;
+  chunk = (chunk + 4095) / 4096 * 4096;
+  a.chunk = chunk;
+  a.chunks_per_row = (a.n + a.chunk - 1) / a.chunk;
   const int64_t blocks = groups * a.chunks_per_row;
   if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "group: grid too large");
   hipLaunchKernelGGL((group_pass_kernel<TV, TL, PASS, LDS, R, COUNT>), dim3((unsigned)blocks),
@@ -406,7 +432,6 @@ int run_pass(GArgs a, bool lds, hipStream_t s) {
   const int64_t slot = pass_slot_bytes(PASS, COUNT, sizeof(typename SumAcc<TV>::type));
   if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
   if (a.rows >= kGroupRowsPerBlock && L * slot * kGroupRowsPerBlock <= kGroupLdsBudget) {
-    a.chunk = a.chunk_r4;
     return launch_pass<TV, TL, PASS, true, kGroupRowsPerBlock, COUNT>(
         a, (size_t)(L * slot * kGroupRowsPerBlock), s);
   }
@@ -470,17 +495,6 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
     return set_error(NBG_ERR_INVALID, "group: %lld labels do not fit the LDS strategy", (long long)L);
   if (strategy == NBG_GROUP_GLOBAL) lds = false;
 
-  // Chunking: about 8 workgroups per CU over all row groups, 4096-aligned.
-  auto pick_chunk = [&](int64_t row_groups) {
-    const int64_t target_blocks = 2048;
-    int64_t chunks = (target_blocks + row_groups - 1) / row_groups;
-    int64_t chunk = (a.n + chunks - 1) / chunks;
-    if (chunk < 8192) chunk = 8192;
-    return (chunk + 4095) / 4096 * 4096;
-  };
-  a.chunk = pick_chunk(a.rows);
-  a.chunk_r4 = pick_chunk((a.rows + kGroupRowsPerBlock - 1) / kGroupRowsPerBlock);
-  a.chunks_per_row = (a.n + a.chunk - 1) / a.chunk;
   a.means = nullptr;
   a.argmin = (KIND == kGArgMin);
   int rc = 0;

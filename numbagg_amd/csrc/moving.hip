@@ -40,7 +40,16 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   using Tr = Traits<T, KIND>;
   constexpr int TILE = kMoveThreads * KP;
   const int64_t tiles = (n + TILE - 1) / TILE;
-  const int64_t blocks = rows * tiles;
+  // Spans of consecutive tiles per workgroup: as long as possible (up to
+  // kMaxSpan) while the grid keeps about 16 workgroups per CU.
+  // NBG_MOVE_SPAN=<k> forces k (results do not depend on it; tests use it to
+  // run carried halos on small inputs).
+  int64_t span = rows * tiles / (256 * 16);
+  if (const char* e = getenv("NBG_MOVE_SPAN")) span = atoll(e);
+  span = span < 1 ? 1 : (span > kMaxSpan ? kMaxSpan : span);
+  if (span > tiles) span = tiles;
+  const int64_t spans = (tiles + span - 1) / span;
+  const int64_t blocks = rows * spans;
   if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move: too many tiles");
   const bool tail_lds = window <= lds_window_limit<T, Tr::NARGS>() - (KP - default_items<T>()) * kMoveThreads;
   int vec = pick_vec<T>(a, ld_a, rows);
@@ -61,7 +70,7 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   dim3 grid((unsigned)blocks), block(kMoveThreads);
 #define NBG_MOVE_LAUNCH(TL, V, LU)                                                                   \
   hipLaunchKernelGGL((move_kernel<T, KIND, TL, V, KP, LU>), grid, block, lds, s, a, b, out, rows, n, \
-                     ld_a, ld_b, ld_out, window, min_count, tiles)
+                     ld_a, ld_b, ld_out, window, min_count, tiles, span)
   if (use_lut) {
     if constexpr (LUT_OK) {
       if (tail_lds) {

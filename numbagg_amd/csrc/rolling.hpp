@@ -355,15 +355,17 @@ __host__ __device__ inline int64_t rolling_stage_bytes(int items, int64_t window
 
 // One output tile [tile*TILE, tile*TILE + TILE) of one row (ra/rb/ro point at
 // the row), TILE = 256 threads x KP items. `smem_raw` holds the staged segment;
-// `lut` (mean only, may be null) holds 1/c for c = 0..window in LDS. Ends with a
-// barrier so the caller may reuse the shared memory for its next tile.
+// `lut` (mean only, may be null) holds 1/c for c = 0..window in LDS; `staged`
+// leading segment elements are already in LDS (span kernels carry the halo).
+// Ends with a barrier so the caller may reuse the shared memory for its next
+// tile.
 template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false>
 __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* __restrict__ rb,
                                              T* __restrict__ ro, int64_t n, int64_t window,
                                              int64_t min_count, int64_t tile,
                                              unsigned char* smem_raw,
                                              RollingShared<typename Traits<T, KIND>::S>& rs,
-                                             const double* lut = nullptr) {
+                                             const double* lut = nullptr, int staged = 0) {
   using Tr = Traits<T, KIND>;
   using S = typename Tr::S;
   constexpr int K = KP;
@@ -384,7 +386,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   T* lb = la + lds_elems;
 
   // 1. stage the segment (wave-striped vector loads, NaN past the row end).
-  for (int64_t e = (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
+  //    The first `staged` elements (a multiple of 16) may already be in LDS:
+  //    the halo carried over from the previous tile of a span.
+  for (int64_t e = (int64_t)staged + (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
     const int64_t p = s0a + e;
     if (VEC > 1 && p >= 0 && p + VEC <= n) {
       Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(ra + p);
@@ -616,27 +620,63 @@ inline int f32_sum_items() {
 // launches must agree, so both ask here).
 __host__ __device__ inline bool mean_uses_lut(int64_t window) { return window <= kMeanLutWindow; }
 
+// Tiles per span: a workgroup evaluates up to this many consecutive tiles of a
+// row, carrying each tile's halo (the last ceil16(window) staged inputs) to the
+// front of LDS for the next tile instead of staging it again from global
+// memory, and filling the 1/c table once.
+constexpr int64_t kMaxSpan = 16;
+
+// The staged-halo length of a tile, ceil16(window); carrying it needs the
+// source [TILE, TILE + H) and destination [0, H) to be disjoint.
+__host__ __device__ inline int64_t carried_halo(int64_t window) { return (window + 15) & ~(int64_t)15; }
+
 template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false>
 __global__ __launch_bounds__(kMoveThreads) void move_kernel(
     const T* __restrict__ xa, const T* __restrict__ xb, T* __restrict__ out, int64_t rows,
     int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
-    int64_t tiles_per_row) {
+    int64_t tiles_per_row, int64_t span) {
   using Tr = Traits<T, KIND>;
+  constexpr int TILE = kMoveThreads * KP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ RollingShared<typename Tr::S> rs;
-  // Consecutive logical ids walk consecutive tiles of a row on one XCD, so a
-  // tile's halo (the end of the previous tile) is read from that XCD's L2.
+  // Consecutive logical ids walk consecutive spans of a row on one XCD, so a
+  // span's first halo (the end of the previous span) is read from that XCD's L2.
+  const int64_t spans_per_row = (tiles_per_row + span - 1) / span;
   const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
-  const int64_t row = (int64_t)(v / (uint64_t)tiles_per_row);
-  const int64_t tile = (int64_t)(v % (uint64_t)tiles_per_row);
+  const int64_t row = (int64_t)(v / (uint64_t)spans_per_row);
+  const int64_t t_begin = (int64_t)(v % (uint64_t)spans_per_row) * span;
+  const int64_t t_end = t_begin + span < tiles_per_row ? t_begin + span : tiles_per_row;
   double* lut = nullptr;
   if constexpr (LUT) {
     lut = reinterpret_cast<double*>(smem_raw + rolling_stage_bytes<T>(KP, window, TAIL_LDS, Tr::NARGS));
     fill_recip_lut(lut, window);  // the tile's first barrier orders these writes before any read
   }
-  rolling_tile<T, KIND, TAIL_LDS, VEC, KP, LUT>(
-      xa + row * ld_a, (Tr::NARGS == 2) ? xb + row * ld_b : nullptr, out + row * ld_out, n, window,
-      min_count, tile, smem_raw, rs, lut);
+  const T* ra = xa + row * ld_a;
+  const T* rb = (Tr::NARGS == 2) ? xb + row * ld_b : nullptr;
+  T* ro = out + row * ld_out;
+  const int64_t H = carried_halo(window);
+  const bool carry = TAIL_LDS && H <= TILE;
+  // Padded LDS element count of one input's staged segment (the second input
+  // follows it), as rolling_tile lays it out.
+  const int64_t lds_elems = pad16((int64_t)TILE + H) + 16;
+  int staged = 0;
+  for (int64_t tile = t_begin; tile < t_end; ++tile) {
+    rolling_tile<T, KIND, TAIL_LDS, VEC, KP, LUT>(ra, rb, ro, n, window, min_count, tile, smem_raw,
+                                                  rs, lut, staged);
+    if (carry && tile + 1 < t_end) {
+      // The next tile's segment starts TILE elements later: its first H
+      // elements are this segment's last H. Source and destination are
+      // disjoint (H <= TILE); the barrier orders the copy before the next
+      // tile stages over the source.
+      T* la = reinterpret_cast<T*>(smem_raw);
+      for (int e = threadIdx.x; e < (int)H; e += kMoveThreads) {
+        la[pad16(e)] = la[pad16(e + TILE)];
+        if (Tr::NARGS == 2) la[lds_elems + pad16(e)] = la[lds_elems + pad16(e + TILE)];
+      }
+      __syncthreads();
+      staged = (int)H;
+    }
+  }
 }
 
 // Largest window whose halo is staged in LDS: the padded segment of
