@@ -190,9 +190,10 @@ def bench_ragged(torch, nb, steps, judge=False):
     model = tuning.CostModel.load()
     source = "cost model" if model is not None else "exhaustive autotuner (no valid cost model)"
     t0 = time.perf_counter()
-    strat = tuning.choose_strategies(rb, values, labels, method="model", model=model)
+    cap, strat = tuning.choose_launch(rb, values, labels, method="model", model=model)
     choose_s = time.perf_counter() - t0
     rb.set_strategy(strat)
+    rb.set_lds_labels(cap)
     replay, mo, go = rb.graph(values, labels)
     stream = torch.cuda.current_stream()
     replay()
@@ -224,14 +225,19 @@ def bench_ragged(torch, nb, steps, judge=False):
         "move_mean_bitwise_equal": move_equal,
         "group_mean_mismatches": int((gdiff > 0).sum().item()), "group_mean_max_ulp": int(gdiff.max().item()),
         "strategy_source": source, "strategy_choice_s": round(choose_s, 2),
-        "lds_series": int((strat == 0).sum()),
+        "lds_labels": int(cap), "lds_series": int((strat == 0).sum()),
     }
     if judge:
-        ex = tuning.choose_strategies(rb, values, labels, method="exhaustive")
+        ex_cap, ex = tuning.choose_launch(rb, values, labels, method="exhaustive")
         times = {}
-        for name, st in (("chosen", strat), ("exhaustive", ex), ("heuristic", ragged.heuristic_strategy(lengths, ks)),
-                         ("all_global", np.ones(rb.nseries, np.uint8))):
+        n = rb.nseries
+        for name, c, st in (("chosen", cap, strat), ("exhaustive", ex_cap, ex),
+                            ("heuristic", ragged.LDS_MAX_LABELS, ragged.heuristic_strategy(lengths, ks)),
+                            ("all_global", ragged.LDS_MAX_LABELS, np.ones(n, np.uint8)),
+                            ("all_lds_1024", ragged.LDS_MAX_LABELS, np.zeros(n, np.uint8)),
+                            ("all_lds_4096", 4096, np.zeros(n, np.uint8))):
             rb.set_strategy(st)
+            rb.set_lds_labels(c)
             rb(values, labels, mo, go)
             torch.cuda.synchronize()
             a.record(stream)
@@ -241,8 +247,11 @@ def bench_ragged(torch, nb, steps, judge=False):
             torch.cuda.synchronize()
             times[name] = a.elapsed_time(b) / steps
         rec["strategy_ms"] = {k: round(v, 3) for k, v in times.items()}
+        rec["exhaustive_lds_labels"] = int(ex_cap)
         rec["regret_vs_exhaustive"] = round(times["chosen"] / times["exhaustive"] - 1, 4)
         rec["gain_vs_heuristic"] = round(times["heuristic"] / times["chosen"] - 1, 4)
+        rec["gain_vs_best_fixed"] = round(min(times["all_lds_1024"], times["all_lds_4096"], times["all_global"])
+                                          / times["chosen"] - 1, 4)
     del values, labels, mo, go, pm, pg
     return rec
 

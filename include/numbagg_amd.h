@@ -212,19 +212,24 @@ int nbg_group_nanstd(int vdtype, const void* values, int ldtype,
  * group_out[label_offsets[s] : label_offsets[s+1]]. Work items are tiles of
  * nbg_ragged_tile_size() positions of each series: tile_offsets[s] is the first item of series s and
  * tile_series[t] the series of item t (total_tiles items). strategy[s]: 0 =
- * LDS-privatised label accumulators (up to 1024 labels), 1 = global atomics.
- * One persistent launch with an atomic work queue; move_out is bitwise equal
- * to per-series move_mean calls. All arrays are device arrays. */
+ * labels below lds_labels accumulate in LDS-privatised slots (the rest, if
+ * any, in global float64/uint32 accumulators), 1 = global atomics only.
+ * lds_labels (1..nbg_ragged_max_lds_labels(), 0 = 1024) sets the LDS slots of
+ * the launch: more slots keep more labels out of global atomics but leave
+ * fewer workgroups resident per CU. One persistent launch with an atomic
+ * work queue; move_out is bitwise equal to per-series move_mean calls. All
+ * arrays are device arrays. */
 /* Work-item (tile) length; the same tiles a plain float32 move_mean uses. */
 int64_t nbg_ragged_tile_size(void);
+int64_t nbg_ragged_max_lds_labels(void);
 size_t nbg_ragged_workspace_bytes(int64_t nseries, int64_t total_labels);
 int nbg_ragged_move_mean_group_nanmean(
     const float* values, const int32_t* labels, const int64_t* offsets,
     int64_t nseries, const int64_t* label_offsets, int64_t total_labels,
     const int64_t* tile_offsets, const int32_t* tile_series,
-    int64_t total_tiles, const uint8_t* strategy, int64_t window,
-    int64_t min_count, float* move_out, float* group_out, void* ws,
-    size_t ws_bytes, void* stream);
+    int64_t total_tiles, const uint8_t* strategy, int64_t lds_labels,
+    int64_t window, int64_t min_count, float* move_out, float* group_out,
+    void* ws, size_t ws_bytes, void* stream);
 
 /* ---- strategy cost model (no reference counterpart): fp32 MLP kernels ---- */
 /* Row-major x (batch, in_dim), w (out_dim, in_dim), b (out_dim), y (batch,

@@ -23,6 +23,9 @@
 //   bits 63..62  flag (1 = aggregate: tile holds no valid value;
 //                      2 = inclusive prefix)
 //   bits 61..0   1 + scan-order index of the last valid value (0 = none)
+#include <cstdlib>
+#include <cstring>
+
 #include "common.hpp"
 
 namespace nbg {
@@ -275,12 +278,23 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   }
 }
 
-template <typename T>
-int fill_impl(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int64_t ld_x,
-              int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, hipStream_t s) {
+// Input bytes per thread of a tile: 64 by default (16 float32 / 8 float64
+// elements); NBG_FILL_BYTES=16 or 32 selects smaller tiles (one or two 16-byte
+// vectors per thread: more workgroups, more look-back tiles). The workspace
+// query reads the same setting.
+int fill_bytes_per_thread() {
+  const char* v = getenv("NBG_FILL_BYTES");
+  if (v != nullptr && strcmp(v, "16") == 0) return 16;
+  if (v != nullptr && strcmp(v, "32") == 0) return 32;
+  return 64;
+}
+
+template <typename T, int BYTES>
+int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int64_t ld_x,
+                int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, hipStream_t s) {
   const T* x = static_cast<const T*>(xv);
   T* out = static_cast<T*>(ov);
-  constexpr int TILE = kFillThreads * 64 / (int)sizeof(T);  // 16 elements x 4 B or 8 x 8 B per thread
+  constexpr int TILE = kFillThreads * BYTES / (int)sizeof(T);
   const int64_t tiles = (n + TILE - 1) / TILE;
   const int64_t blocks = rows * tiles;
   if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "fill: too many tiles (%lld)", (long long)blocks);
@@ -301,7 +315,7 @@ int fill_impl(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int64
   const int vx = pick_vec<T>(x, ld_x, rows), vo = pick_vec<T>(out, ld_out, rows);
   const int vec = vx < vo ? vx : vo;
   constexpr int MAXV = 16 / (int)sizeof(T);
-  constexpr int UV = 64 / (int)sizeof(T) / MAXV;  // slots per thread at full vector width
+  constexpr int UV = BYTES / (int)sizeof(T) / MAXV;  // slots per thread at full vector width
   dim3 grid((unsigned)blocks), block(kFillThreads);
 #define NBG_FILL_LAUNCH(V, U)                                                                       \
   do {                                                                                            \
@@ -329,6 +343,16 @@ int fill_impl(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int64
   return check_hip(hipGetLastError(), "fill_kernel launch");
 }
 
+template <typename T>
+int fill_impl(bool rev, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
+              int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, hipStream_t s) {
+  switch (fill_bytes_per_thread()) {
+    case 16: return fill_impl_b<T, 16>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
+    case 32: return fill_impl_b<T, 32>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
+    default: return fill_impl_b<T, 64>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
+  }
+}
+
 int fill_entry(const char* name, bool rev, int dtype, const void* x, void* out, int64_t rows,
                int64_t n, int64_t ld_x, int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes,
                void* stream) {
@@ -352,7 +376,7 @@ int fill_entry(const char* name, bool rev, int dtype, const void* x, void* out, 
 extern "C" {
 
 size_t nbg_fill_workspace_bytes(int64_t rows, int64_t n, int dtype) {
-  const int64_t tile = (dtype == NBG_F64) ? nbg::kFillThreads * 8 : nbg::kFillThreads * 16;
+  const int64_t tile = (int64_t)nbg::kFillThreads * nbg::fill_bytes_per_thread() / (dtype == NBG_F64 ? 8 : 4);
   const int64_t tiles = (n + tile - 1) / tile;
   if (rows <= 0 || tiles <= 1) return 0;
   return (size_t)(rows * tiles) * 8 + (size_t)((rows * 4 + 7) / 8) * 8;

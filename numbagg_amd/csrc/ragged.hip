@@ -25,7 +25,8 @@ namespace nbg {
 namespace {
 
 constexpr int kRaggedGrab = 8;       // work items (tiles) per queue grab
-constexpr int kRaggedLdsLabels = 1024;  // labels [0, 1024) of a series privatised in LDS
+constexpr int64_t kRaggedLdsLabels = 1024;     // default LDS label slots per launch
+constexpr int64_t kRaggedMaxLdsLabels = 8192;  // 96 KB of slots
 
 // The ragged kernel evaluates move_mean tiles with exactly the variant a plain
 // float32 move_mean launch uses (items per thread, 1/c table), so both produce
@@ -45,6 +46,7 @@ struct RaggedArgs {
   int64_t total_tiles, window, min_count;
   size_t stage_bytes;  // dynamic LDS used by the rolling tile
   size_t lut_bytes;    // 1/c table after the staged segment (0 when unused)
+  int64_t lds_labels;  // LDS label slots (labels [0, lds_labels) of a series)
 };
 
 __global__ void ragged_init_kernel(double* gsum, unsigned int* gcount, int64_t total_labels,
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4)
   double* lut = reinterpret_cast<double*>(smem_raw + A.stage_bytes);
   if constexpr (LUT) fill_recip_lut(lut, A.window);  // ordered by the loop's first barrier
   double* l_sum = reinterpret_cast<double*>(smem_raw + A.stage_bytes + A.lut_bytes);
-  unsigned int* l_cnt = reinterpret_cast<unsigned int*>(l_sum + kRaggedLdsLabels);
+  unsigned int* l_cnt = reinterpret_cast<unsigned int*>(l_sum + A.lds_labels);
   const float* la = reinterpret_cast<const float*>(smem_raw);  // the staged segment
 
   while (true) {
@@ -114,9 +116,9 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4)
       const int64_t tile = t - A.tile_offsets[s];
       const int64_t lo = A.label_offsets[s];
       const int64_t K = A.label_offsets[s + 1] - lo;
-      // Strategy 0 keeps labels below kRaggedLdsLabels in LDS (all of them when
-      // K fits) and sends the rest to global atomics; strategy 1 is global only.
-      const int64_t nl = A.strategy[s] == 0 ? (K < kRaggedLdsLabels ? K : kRaggedLdsLabels) : 0;
+      // Strategy 0 keeps labels below lds_labels in LDS (all of them when K
+      // fits) and sends the rest to global atomics; strategy 1 is global only.
+      const int64_t nl = A.strategy[s] == 0 ? (K < A.lds_labels ? K : A.lds_labels) : 0;
       if (s != cur) {
         if (cur_nl > 0) ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
         for (int64_t l = tid; l < nl; l += kMoveThreads) {
@@ -195,6 +197,8 @@ extern "C" {
 
 int64_t nbg_ragged_tile_size(void) { return (int64_t)nbg::kMoveThreads * nbg::f32_sum_items(); }
 
+int64_t nbg_ragged_max_lds_labels(void) { return nbg::kRaggedMaxLdsLabels; }
+
 size_t nbg_ragged_workspace_bytes(int64_t nseries, int64_t total_labels) {
   (void)nseries;
   return nbg::ragged_ws_bytes(total_labels);
@@ -205,7 +209,7 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
                                        const int64_t* label_offsets, int64_t total_labels,
                                        const int64_t* tile_offsets, const int32_t* tile_series,
                                        int64_t total_tiles, const uint8_t* strategy,
-                                       int64_t window, int64_t min_count, float* move_out,
+                                       int64_t lds_labels, int64_t window, int64_t min_count, float* move_out,
                                        float* group_out, void* ws, size_t ws_bytes, void* stream) {
   using namespace nbg;
   if (int rc = injected_error("nbg_ragged_move_mean_group_nanmean")) return rc;
@@ -213,6 +217,10 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
     return set_error(NBG_ERR_INVALID, "ragged: negative size");
   if (window <= 0) return set_error(NBG_ERR_INVALID, "ragged: window not in valid range: %lld", (long long)window);
   if (min_count < 0) return set_error(NBG_ERR_INVALID, "ragged: min_count must be positive: %lld", (long long)min_count);
+  if (lds_labels == 0) lds_labels = kRaggedLdsLabels;
+  if (lds_labels < 1 || lds_labels > kRaggedMaxLdsLabels)
+    return set_error(NBG_ERR_INVALID, "ragged: lds_labels %lld not in [1, %lld]", (long long)lds_labels,
+                     (long long)kRaggedMaxLdsLabels);
   if (window > lds_window_limit<float, 1>())
     return set_error(NBG_ERR_INVALID, "ragged: window %lld exceeds the staged-halo limit %lld",
                      (long long)window, (long long)lds_window_limit<float, 1>());
@@ -243,7 +251,8 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
   const bool lut = mean_uses_lut(window);
   a.stage_bytes = (size_t)rolling_stage_bytes<float>(kp, window, true, 1);
   a.lut_bytes = lut ? ((size_t)(window + 1) * sizeof(double) + 15) / 16 * 16 : 0;
-  const size_t lds = a.stage_bytes + a.lut_bytes + kRaggedLdsLabels * (8 + 4);
+  a.lds_labels = lds_labels;
+  const size_t lds = a.stage_bytes + a.lut_bytes + (size_t)lds_labels * (8 + 4);
   int64_t init_blocks = (total_labels + 255) / 256;
   if (init_blocks > 4096) init_blocks = 4096;
   if (init_blocks < 1) init_blocks = 1;
@@ -253,7 +262,9 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
   const void* kfn = kp == 32 ? (lut ? (const void*)ragged_kernel<32, true> : (const void*)ragged_kernel<32, false>)
                              : (lut ? (const void*)ragged_kernel<16, true> : (const void*)ragged_kernel<16, false>);
   int cap = device_capacity_blocks(kfn, kMoveThreads, lds);
-  if (cap <= 0) return set_error(NBG_ERR_HIP, "ragged: cannot query occupancy");
+  if (cap <= 0)
+    return set_error(NBG_ERR_INVALID, "ragged: no workgroup fits (%zu bytes of LDS for window %lld and %lld label slots)",
+                     lds, (long long)window, (long long)lds_labels);
   int64_t grid = total_tiles < cap ? total_tiles : cap;
   if (grid < 1) grid = 1;
   if (kp == 32) {

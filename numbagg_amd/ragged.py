@@ -9,9 +9,10 @@ The reference semantics are the per-series plain calls
 move_mean outputs are bitwise equal to them.
 
 The per-series group-by strategy comes from ``strategy``: STRATEGY_LDS keeps
-the accumulators of labels below LDS_MAX_LABELS in LDS (all labels when the
-series has no more; the rest go to global atomics), STRATEGY_GLOBAL sends
-every label to global atomics. "heuristic" (LDS when the labels fit, global
+the accumulators of labels below the launch's ``lds_labels`` slots in LDS
+(all labels when the series has no more; the rest go to global atomics),
+STRATEGY_GLOBAL sends every label to global atomics. ``lds_labels``
+(default LDS_MAX_LABELS = 1024) trades LDS slots against resident workgroups. "heuristic" (LDS when the labels fit, global
 otherwise), "lds", "global", or an explicit per-series array (what the
 autotuner or the cost model in ``numbagg_amd.tuning`` produce) select them. The launch sequence can be
 captured once in a HIP graph and replayed (``graph``).
@@ -34,7 +35,7 @@ STRATEGY_GLOBAL = 1
 
 def heuristic_strategy(lengths: np.ndarray, num_labels: np.ndarray) -> np.ndarray:
     """The label-count rule: LDS-privatised accumulators when all of a
-    series' labels fit the LDS slots, global atomics otherwise."""
+    series' labels fit the default LDS slots, global atomics otherwise."""
     k = np.asarray(num_labels)
     return np.where(k <= LDS_MAX_LABELS, STRATEGY_LDS, STRATEGY_GLOBAL).astype(np.uint8)
 
@@ -48,7 +49,8 @@ class RaggedMoveMeanGroupMean:
     sum(num_labels) entries, series s at label_offsets[s]:label_offsets[s+1].
     """
 
-    def __init__(self, lengths, num_labels, *, window: int, min_count: int | None = None, strategy="heuristic", device=None):
+    def __init__(self, lengths, num_labels, *, window: int, min_count: int | None = None, strategy="heuristic",
+                 lds_labels: int = LDS_MAX_LABELS, device=None):
         dev.require_gpu()
         lengths = np.asarray(lengths, dtype=np.int64)
         num_labels = np.asarray(num_labels, dtype=np.int64)
@@ -80,6 +82,7 @@ class RaggedMoveMeanGroupMean:
             raise ValueError("too many tiles for one ragged launch")
         tile_series = np.repeat(np.arange(self.nseries, dtype=np.int32), tiles)
         self.set_strategy(strategy)
+        self.set_lds_labels(lds_labels)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.device)  # noqa: E731
         self._d_offsets = t(self.offsets)
         self._d_label_offsets = t(self.label_offsets)
@@ -87,6 +90,14 @@ class RaggedMoveMeanGroupMean:
         self._d_tile_series = t(tile_series)
         ws = _lib.query("nbg_ragged_workspace_bytes", self.nseries, self.total_labels)
         self._ws = torch.empty(int(ws), dtype=torch.uint8, device=self.device)
+
+    def set_lds_labels(self, lds_labels: int) -> None:
+        """LDS label slots of the launch (labels below it are privatised for
+        STRATEGY_LDS series)."""
+        cap = int(_lib.query("nbg_ragged_max_lds_labels"))
+        if not 1 <= int(lds_labels) <= cap:
+            raise ValueError(f"lds_labels must be in [1, {cap}]: {lds_labels}")
+        self.lds_labels = int(lds_labels)
 
     def set_strategy(self, strategy) -> None:
         if isinstance(strategy, str):
@@ -122,7 +133,7 @@ class RaggedMoveMeanGroupMean:
             "nbg_ragged_move_mean_group_nanmean",
             dev.ptr(values), dev.ptr(labels), dev.ptr(self._d_offsets), self.nseries,
             dev.ptr(self._d_label_offsets), self.total_labels, dev.ptr(self._d_tile_offsets),
-            dev.ptr(self._d_tile_series), self.total_tiles, dev.ptr(self._d_strategy),
+            dev.ptr(self._d_tile_series), self.total_tiles, dev.ptr(self._d_strategy), self.lds_labels,
             self.window, self.min_count, dev.ptr(move_out), dev.ptr(group_out),
             dev.ptr(self._ws), self._ws.numel(), dev.stream_handle(),
         )

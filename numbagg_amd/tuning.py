@@ -1,25 +1,27 @@
 """Strategy selection for the ragged batched path: autotuner and cost model.
 
 For each series of a ragged batch the kernel can fold values into its label
-accumulators through LDS-privatised slots for the labels below LDS_MAX_LABELS
-(flushed when the workgroup moves to another series; larger labels still go
-to global atomics) or through global atomics only. Which is faster depends on
+accumulators through LDS-privatised slots for the labels below the launch's
+LDS slot count (flushed when the workgroup moves to another series; larger
+labels still go to global atomics) or through global atomics only. Which is faster depends on
 the series' length, label count, label skew (how much of the mass the low
 labels carry, and how often lanes collide on one address), NaN fraction (and
 window, through the tile's rolling work).
 
-* ``Autotuner`` measures both strategies exhaustively: it times the ragged
-  kernel on homogeneous synthetic batches (one feature point each) with HIP
-  events and records ns/element in a JSON timing DB, versioned by GPU
-  architecture and a hash of the kernel source.
-* ``CostModel`` is a 5 -> 64 -> 64 -> 2 fp32 MLP predicting log ns/element of
-  both strategies from (log2 length, log2 labels, skew, NaN fraction, log2
+* ``Autotuner`` measures every launch configuration exhaustively: (LDS label
+  slots in CAPACITIES) x (strategy), on homogeneous synthetic batches (one
+  feature point each), timed with HIP events; ns/element go to a JSON timing
+  DB versioned by GPU architecture and a hash of the kernel source. More
+  slots keep more labels out of global atomics but leave fewer workgroups
+  resident, which slows the rolling part: neither wins everywhere.
+* ``CostModel`` is a 5 -> 64 -> 64 -> 4 fp32 MLP predicting log ns/element of
+  each configuration from (log2 length, log2 labels, skew, NaN fraction, log2
   window). It is trained on the GPU by this library's own kernels
   (nbg_dense_forward / nbg_dense_backward / nbg_mse_grad / nbg_adam_step) on
   the autotuner's timings; inference uses nbg_dense_forward.
-* ``choose_strategies`` picks per-series strategies with the cost model, the
-  exhaustive tuner (the fallback, and the judge of the model's regret) or a
-  label-count heuristic.
+* ``choose_launch`` picks the launch's LDS label slots and per-series
+  strategies with the cost model, the exhaustive tuner (the fallback, and the
+  judge of the model's regret) or a label-count heuristic.
 """
 
 from __future__ import annotations
@@ -42,6 +44,10 @@ DB_PATH = DATA_DIR / "ragged_timing_db.json"
 MODEL_PATH = DATA_DIR / "ragged_cost_model.npz"
 N_FEATURES = 5
 STRATEGIES = (STRATEGY_LDS, STRATEGY_GLOBAL)
+# LDS label slots per launch the tuner considers, and the launch
+# configurations it times: (slots, strategy), in the cost model's output order.
+CAPACITIES = (LDS_MAX_LABELS, 4096)
+CONFIGS = tuple((c, st) for c in CAPACITIES for st in STRATEGIES)
 
 
 def kernel_version() -> str:
@@ -134,11 +140,15 @@ def synthetic_labels(length: int, k: int, zipf: float, gen: torch.Generator, dev
 class Autotuner:
     """Exhaustive strategy timing of the ragged kernel, cached in a JSON DB."""
 
-    def __init__(self, db_path: pathlib.Path | str = DB_PATH, target_elements: int = 1 << 22, reps: int = 3):
+    def __init__(self, db_path: pathlib.Path | str = DB_PATH, target_elements: int = 1 << 25, reps: int = 3,
+                 max_labels: int = 1 << 26):
         self.db_path = pathlib.Path(db_path)
+        # Batches of about target_elements (long enough for steady-state
+        # timings) with at most max_labels labels in total.
         self.target_elements = target_elements
+        self.max_labels = max_labels
         self.reps = reps
-        self.meta = {"arch": device_arch(), "kernel": kernel_version()}
+        self.meta = {"arch": device_arch(), "kernel": kernel_version(), "configs": [list(c) for c in CONFIGS]}
         self.entries: list[dict] = []
         if self.db_path.exists():
             try:
@@ -152,8 +162,11 @@ class Autotuner:
         self.db_path.parent.mkdir(parents=True, exist_ok=True)
         self.db_path.write_text(json.dumps({"meta": self.meta, "entries": self.entries}, indent=1))
 
-    def _time(self, rb, values, labels, strategy) -> float:
-        rb.set_strategy(np.full(rb.nseries, strategy, dtype=np.uint8))
+    def _time(self, rb, values, labels, strategy, lds_labels: int = LDS_MAX_LABELS) -> float:
+        """ns per element of one launch; `strategy` is one value or a per-series array."""
+        st = np.broadcast_to(np.asarray(strategy, dtype=np.uint8), (rb.nseries,))
+        rb.set_strategy(np.ascontiguousarray(st))
+        rb.set_lds_labels(lds_labels)
         mo = torch.empty_like(values)
         go = torch.empty(rb.total_labels, dtype=torch.float32, device=values.device)
         rb(values, labels, mo, go)
@@ -167,11 +180,12 @@ class Autotuner:
         return a.elapsed_time(b) * 1e6 / self.reps / values.numel()  # ns per element
 
     def measure(self, length: int, k: int, zipf: float, nan: float, window: int, seed: int = 0) -> dict:
-        """Time both strategies on a homogeneous batch for one feature point."""
+        """Time every launch configuration on a homogeneous batch for one
+        feature point."""
         dev.require_gpu()
         device = torch.device("cuda")
         gen = torch.Generator(device=device).manual_seed(seed)
-        s = max(1, self.target_elements // length)
+        s = max(1, min(self.target_elements // length, self.max_labels // max(k, 1)))
         lengths = np.full(s, length, dtype=np.int64)
         ks = np.full(s, k, dtype=np.int64)
         total = int(lengths.sum())
@@ -180,13 +194,10 @@ class Autotuner:
         labels = synthetic_labels(total, k, zipf, gen, device)
         rb = RaggedMoveMeanGroupMean(lengths, ks, window=min(window, length), min_count=1)
         skew, nanf = batch_statistics(rb, values, labels)
-        times = {}
-        for st in STRATEGIES:
-            times[st] = self._time(rb, values, labels, st)
+        ns = [self._time(rb, values, labels, st, cap) for cap, st in CONFIGS]
         entry = {
             "length": length, "num_labels": k, "zipf": zipf, "nan": nan, "window": window,
-            "skew": float(np.mean(skew)), "nan_fraction": float(np.mean(nanf)),
-            "ns_lds": times[STRATEGY_LDS], "ns_global": times[STRATEGY_GLOBAL],
+            "skew": float(np.mean(skew)), "nan_fraction": float(np.mean(nanf)), "ns": ns,
         }
         self.entries.append(entry)
         return entry
@@ -205,21 +216,22 @@ class Autotuner:
         return self.entries
 
     def training_set(self):
-        """Features (n, 5) and targets (n, 2) = log ns/element per strategy."""
-        rows = [e for e in self.entries if math.isfinite(e["ns_lds"]) and math.isfinite(e["ns_global"])]
+        """Features (n, 5), targets (n, len(CONFIGS)) = log ns/element per
+        configuration, and the points' label counts."""
+        rows = [e for e in self.entries if all(math.isfinite(v) for v in e["ns"])]
         X = features([e["length"] for e in rows], [e["num_labels"] for e in rows], [e["skew"] for e in rows],
                      [e["nan_fraction"] for e in rows], [e["window"] for e in rows])
-        T = np.log(np.array([[e["ns_lds"], e["ns_global"]] for e in rows], dtype=np.float64)).astype(np.float32)
-        return X, T
+        T = np.log(np.array([e["ns"] for e in rows], dtype=np.float64)).astype(np.float32)
+        return X, T, np.array([e["num_labels"] for e in rows])
 
 
 class CostModel:
-    """5 -> 64 -> 64 -> 2 fp32 MLP on this library's HIP kernels."""
+    """5 -> 64 -> 64 -> n_out fp32 MLP on this library's HIP kernels; by
+    default one output (log ns/element) per launch configuration in CONFIGS."""
 
-    DIMS = (N_FEATURES, 64, 64, 2)
-
-    def __init__(self, seed: int = 0, device=None):
+    def __init__(self, seed: int = 0, device=None, n_out: int = len(CONFIGS)):
         dev.require_gpu()
+        self.DIMS = (N_FEATURES, 64, 64, int(n_out))
         self.device = torch.device(device or "cuda")
         rng = np.random.default_rng(seed)
         self.params: list[torch.Tensor] = []
@@ -229,8 +241,8 @@ class CostModel:
                             torch.zeros(fan_out, dtype=torch.float32, device=self.device)]
         self.x_mean = np.zeros(N_FEATURES, np.float32)
         self.x_std = np.ones(N_FEATURES, np.float32)
-        self.t_mean = np.zeros(2, np.float32)
-        self.t_std = np.ones(2, np.float32)
+        self.t_mean = np.zeros(self.DIMS[-1], np.float32)
+        self.t_std = np.ones(self.DIMS[-1], np.float32)
         self.meta: dict = {}
         self._adam = None
         self.step = 0
@@ -302,15 +314,32 @@ class CostModel:
         return history
 
     def predict(self, X: np.ndarray) -> np.ndarray:
-        """Predicted log ns/element for (LDS, global), shape (n, 2)."""
+        """Predicted targets (log ns/element per output), shape (n, n_out)."""
         X = np.asarray(X, np.float32)
         x = torch.from_numpy((X - self.x_mean) / self.x_std).to(self.device)
         y = self._forward(x)[-1].cpu().numpy()
         return y * self.t_std + self.t_mean
 
-    def choose(self, X: np.ndarray, num_labels: np.ndarray) -> np.ndarray:
-        pred = self.predict(X)
-        return np.where(pred[:, 0] <= pred[:, 1], STRATEGY_LDS, STRATEGY_GLOBAL).astype(np.uint8)
+    def choose(self, X: np.ndarray, num_labels: np.ndarray | None = None) -> np.ndarray:
+        """Index of the output predicted fastest, per row (uint8)."""
+        return np.argmin(self.predict(X), axis=1).astype(np.uint8)
+
+    def choose_launch(self, X: np.ndarray, lengths: np.ndarray) -> tuple[int, np.ndarray]:
+        """(LDS label slots, per-series strategies) minimising the predicted
+        time of one launch over a batch: for each slot count, every series
+        takes its faster predicted strategy and the batch costs the sum of
+        length x ns/element; the cheapest slot count wins."""
+        pred = np.exp(self.predict(X).astype(np.float64))
+        lengths = np.asarray(lengths, dtype=np.float64)
+        best = None
+        for ci, cap in enumerate(CAPACITIES):
+            cols = [CONFIGS.index((cap, st)) for st in STRATEGIES]
+            t = pred[:, cols]
+            pick = np.argmin(t, axis=1)
+            cost = float((lengths * t[np.arange(len(t)), pick]).sum())
+            if best is None or cost < best[0]:
+                best = (cost, cap, np.asarray(STRATEGIES, dtype=np.uint8)[pick])
+        return int(best[1]), best[2]
 
     def save(self, path: pathlib.Path | str = MODEL_PATH) -> None:
         path = pathlib.Path(path)
@@ -331,7 +360,7 @@ class CostModel:
         try:
             with np.load(path, allow_pickle=False) as z:
                 meta = json.loads(str(z["meta"]))
-                m = cls(device=device)
+                m = cls(device=device, n_out=int(z["p5"].shape[0]))
                 m.params = [torch.from_numpy(z[f"p{i}"].copy()).to(m.device) for i in range(len(m.params))]
                 m.x_mean, m.x_std, m.t_mean, m.t_std = z["x_mean"], z["x_std"], z["t_mean"], z["t_std"]
                 m.meta = meta
@@ -339,20 +368,24 @@ class CostModel:
             return None
         if meta.get("arch") != device_arch() or meta.get("kernel") != kernel_version():
             return None
+        if meta.get("configs") != [list(c) for c in CONFIGS]:
+            return None
         return m
 
 
-def exhaustive_strategies(rb: RaggedMoveMeanGroupMean, values, labels, skew, nanf, reps: int = 2) -> np.ndarray:
-    """Per-bucket exhaustive choice: series are bucketed by (log2 length,
-    log2 labels, skew, NaN fraction); each bucket's own sub-batch is timed with
-    both strategies and the faster one is kept."""
+def exhaustive_launch(rb: RaggedMoveMeanGroupMean, values, labels, skew, nanf, reps: int = 2) -> tuple[int, np.ndarray]:
+    """Exhaustive choice of (LDS label slots, per-series strategies). Series are
+    bucketed by (log2 length, log2 labels, skew, NaN fraction); for each slot
+    count, each bucket's own sub-batch is timed with both strategies and the
+    faster one is kept; the whole batch is then timed once per slot count and
+    the faster launch wins."""
     key = np.stack([
         np.round(np.log2(rb.lengths)), np.round(np.log2(np.maximum(rb.num_labels, 1))),
         np.round(skew * 10), np.round(nanf * 10)], axis=1)
     _, bucket = np.unique(key, axis=0, return_inverse=True)
     bucket = bucket.ravel()
-    out = np.full(rb.nseries, STRATEGY_LDS, dtype=np.uint8)
     tuner = Autotuner(db_path=pathlib.Path("/nonexistent/db.json"), reps=reps)
+    samples = []
     for b in np.unique(bucket):
         idx = np.nonzero(bucket == b)[0]
         # time a sample of the bucket: up to 64 series / about 8M elements
@@ -366,26 +399,33 @@ def exhaustive_strategies(rb: RaggedMoveMeanGroupMean, values, labels, skew, nan
         sel = np.concatenate([np.arange(rb.offsets[i], rb.offsets[i + 1]) for i in sample])
         sel_t = torch.from_numpy(sel).to(values.device)
         sub = RaggedMoveMeanGroupMean(rb.lengths[sample], rb.num_labels[sample], window=rb.window, min_count=rb.min_count)
-        sv, sl = values[sel_t].contiguous(), labels[sel_t].contiguous()
-        t_lds = tuner._time(sub, sv, sl, STRATEGY_LDS)
-        t_glb = tuner._time(sub, sv, sl, STRATEGY_GLOBAL)
-        out[idx] = STRATEGY_LDS if t_lds <= t_glb else STRATEGY_GLOBAL
-    return out
+        samples.append((idx, sub, values[sel_t].contiguous(), labels[sel_t].contiguous()))
+    best = None
+    for cap in CAPACITIES:
+        out = np.full(rb.nseries, STRATEGY_LDS, dtype=np.uint8)
+        for idx, sub, sv, sl in samples:
+            t = [tuner._time(sub, sv, sl, st, cap) for st in STRATEGIES]
+            out[idx] = STRATEGIES[int(np.argmin(t))]
+        t_all = tuner._time(rb, values, labels, out, cap)
+        if best is None or t_all < best[0]:
+            best = (t_all, cap, out)
+    return int(best[1]), best[2]
 
 
-def choose_strategies(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "model", model: CostModel | None = None):
-    """Per-series strategies for `rb` by "model" (falls back to "exhaustive"
-    when no valid model is available), "exhaustive" or "heuristic"."""
+def choose_launch(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "model", model: CostModel | None = None):
+    """(LDS label slots, per-series strategies) for `rb` by "model" (falls back
+    to "exhaustive" when no valid model is available), "exhaustive" or
+    "heuristic" (default slots, the label-count rule)."""
     from .ragged import heuristic_strategy
 
     if method == "heuristic":
-        return heuristic_strategy(rb.lengths, rb.num_labels)
+        return LDS_MAX_LABELS, heuristic_strategy(rb.lengths, rb.num_labels)
     skew, nanf = batch_statistics(rb, values, labels)
     if method == "model":
         model = model or CostModel.load()
         if model is not None:
-            return model.choose(features(rb.lengths, rb.num_labels, skew, nanf, rb.window), rb.num_labels)
+            return model.choose_launch(features(rb.lengths, rb.num_labels, skew, nanf, rb.window), rb.lengths)
         method = "exhaustive"
     if method == "exhaustive":
-        return exhaustive_strategies(rb, values, labels, skew, nanf)
+        return exhaustive_launch(rb, values, labels, skew, nanf)
     raise ValueError(f"unknown method {method!r}")

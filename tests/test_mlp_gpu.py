@@ -40,7 +40,7 @@ def test_forward_backward_match_numpy(gpu_lib):
 
     from numbagg_amd.tuning import CostModel
 
-    m = CostModel(seed=3)
+    m = CostModel(seed=3, n_out=2)
     rng = np.random.default_rng(0)
     x = rng.standard_normal((257, 5)).astype(np.float32)
     t = rng.standard_normal((257, 2)).astype(np.float32)
@@ -63,7 +63,7 @@ def test_finite_differences(gpu_lib):
 
     from numbagg_amd.tuning import CostModel
 
-    m = CostModel(seed=1)
+    m = CostModel(seed=1, n_out=2)
     rng = np.random.default_rng(1)
     x = torch.from_numpy(rng.standard_normal((64, 5)).astype(np.float32)).cuda()
     t = torch.from_numpy(rng.standard_normal((64, 2)).astype(np.float32)).cuda()
@@ -124,7 +124,7 @@ def test_model_learns_a_strategy_boundary(gpu_lib):
     t_glb = 0.08 + 0.3 * skew
     T = np.log(np.stack([t_lds, t_glb], 1))
     tr, te = slice(0, 450), slice(450, n)
-    m = CostModel(seed=0)
+    m = CostModel(seed=0, n_out=2)
     hist = m.fit(X[tr], T[tr], epochs=1500, lr=3e-3)
     assert hist[-1] < 0.05 * hist[0]
     choice = m.choose(X[te], K[te])

@@ -40,12 +40,13 @@ def _bits(t):
     return t.cpu().numpy().view(np.uint32)
 
 
-@pytest.mark.parametrize("strategy", ["heuristic", "lds", "global"])
-def test_ragged_matches_plain_launches(gpu_lib, strategy):
+@pytest.mark.parametrize("strategy,lds_labels", [("heuristic", 1024), ("lds", 1024), ("global", 1024),
+                                                 ("lds", 4096), ("lds", 100), ("lds", 8192)])
+def test_ragged_matches_plain_launches(gpu_lib, strategy, lds_labels):
     from numbagg_amd.ragged import RaggedMoveMeanGroupMean
 
     lengths, ks, v, lab = make_batch(120, 100, 60_000, 10, 3000, seed=1)
-    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50, strategy=strategy)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50, strategy=strategy, lds_labels=lds_labels)
     mm, gm = rb(v, lab)
     pm, pg = rb.plain(v, lab)
     np.testing.assert_array_equal(_bits(mm), _bits(pm))
@@ -109,3 +110,12 @@ def test_ragged_errors(gpu_lib):
         RaggedMoveMeanGroupMean([100, 50], [3, 3], window=60)
     with pytest.raises(ValueError, match="min_count must be positive"):
         RaggedMoveMeanGroupMean([100, 50], [3, 3], window=10, min_count=-1)
+
+
+def test_ragged_lds_labels_range(gpu_lib):
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    with pytest.raises(ValueError, match="lds_labels"):
+        RaggedMoveMeanGroupMean([100, 200], [5, 6], window=10, lds_labels=0)
+    with pytest.raises(ValueError, match="lds_labels"):
+        RaggedMoveMeanGroupMean([100, 200], [5, 6], window=10, lds_labels=100_000)
