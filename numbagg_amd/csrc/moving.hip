@@ -68,9 +68,19 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   size_t lds = (size_t)rolling_stage_bytes<T>(KP, window, tail_lds, Tr::NARGS);
   if (use_lut) lds += (size_t)(window + 1) * sizeof(double);
   dim3 grid((unsigned)blocks), block(kMoveThreads);
-#define NBG_MOVE_LAUNCH(TL, V, LU)                                                                   \
-  hipLaunchKernelGGL((move_kernel<T, KIND, TL, V, KP, LU>), grid, block, lds, s, a, b, out, rows, n, \
-                     ld_a, ld_b, ld_out, window, min_count, tiles, span)
+  // NBG_MOVE_PREFETCH=1 prefetches each next tile of a span into registers
+  // while the current one is evaluated (more VGPRs, fewer resident waves).
+  const char* pfe = getenv("NBG_MOVE_PREFETCH");
+  const bool pf = pfe != nullptr && pfe[0] == '1' && pfe[1] == '\0';
+#define NBG_MOVE_LAUNCH(TL, V, LU)                                                                        \
+  do {                                                                                                    \
+    if (pf && TL)                                                                                         \
+      hipLaunchKernelGGL((move_kernel<T, KIND, TL, V, KP, LU, TL>), grid, block, lds, s, a, b, out, rows, \
+                         n, ld_a, ld_b, ld_out, window, min_count, tiles, span);                          \
+    else                                                                                                  \
+      hipLaunchKernelGGL((move_kernel<T, KIND, TL, V, KP, LU, false>), grid, block, lds, s, a, b, out,    \
+                         rows, n, ld_a, ld_b, ld_out, window, min_count, tiles, span);                    \
+  } while (0)
   if (use_lut) {
     if constexpr (LUT_OK) {
       if (tail_lds) {

@@ -630,8 +630,9 @@ constexpr int64_t kMaxSpan = 16;
 // source [TILE, TILE + H) and destination [0, H) to be disjoint.
 __host__ __device__ inline int64_t carried_halo(int64_t window) { return (window + 15) & ~(int64_t)15; }
 
-template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false>
-__global__ __launch_bounds__(kMoveThreads) void move_kernel(
+template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false,
+          bool PF = false>
+__global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(PF ? 3 : 4))) void move_kernel(
     const T* __restrict__ xa, const T* __restrict__ xb, T* __restrict__ out, int64_t rows,
     int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
     int64_t tiles_per_row, int64_t span) {
@@ -659,22 +660,69 @@ __global__ __launch_bounds__(kMoveThreads) void move_kernel(
   // Padded LDS element count of one input's staged segment (the second input
   // follows it), as rolling_tile lays it out.
   const int64_t lds_elems = pad16((int64_t)TILE + H) + 16;
+  const int seg_len = (int)(TILE + H);
+  T* la = reinterpret_cast<T*>(smem_raw);
+  // PF: software pipeline over the span: the next tile's new inputs (positions
+  // [t0 + TILE, t0 + 2 TILE), segment offsets [H, H + TILE)) are loaded into
+  // registers before the current tile is evaluated, and written to LDS after
+  // it, behind the carried halo. Thread tid holds the elements it would stage
+  // itself: offsets H + tid*VEC + m*256*VEC + v.
+  constexpr int NV = KP / VEC;  // vectors of VEC elements per thread and input
+  T pa[KP], pb[(Tr::NARGS == 2) ? KP : 1];
+  auto prefetch = [&](int64_t next_tile) {
+    const int64_t t0n = next_tile * TILE;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      const int64_t p = t0n + ((int64_t)threadIdx.x + (int64_t)m * kMoveThreads) * VEC;
+      if (VEC > 1 && p + VEC <= n) {
+        const Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(ra + p);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) pa[m * VEC + v] = va.v[v];
+        if constexpr (Tr::NARGS == 2) {
+          const Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rb + p);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) pb[m * VEC + v] = vb.v[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const bool in = p + v < n;
+          pa[m * VEC + v] = in ? ra[p + v] : canonical_nan<T>();
+          if constexpr (Tr::NARGS == 2) pb[m * VEC + v] = in ? rb[p + v] : canonical_nan<T>();
+        }
+      }
+    }
+  };
   int staged = 0;
   for (int64_t tile = t_begin; tile < t_end; ++tile) {
+    const bool more = carry && tile + 1 < t_end;
+    if (PF && more) prefetch(tile + 1);
     rolling_tile<T, KIND, TAIL_LDS, VEC, KP, LUT>(ra, rb, ro, n, window, min_count, tile, smem_raw,
                                                   rs, lut, staged);
-    if (carry && tile + 1 < t_end) {
+    if (more) {
       // The next tile's segment starts TILE elements later: its first H
-      // elements are this segment's last H. Source and destination are
-      // disjoint (H <= TILE); the barrier orders the copy before the next
-      // tile stages over the source.
-      T* la = reinterpret_cast<T*>(smem_raw);
+      // elements are this segment's last H (source and destination disjoint,
+      // H <= TILE), the rest comes from the prefetched registers. The barrier
+      // orders both before the next tile reads the segment.
       for (int e = threadIdx.x; e < (int)H; e += kMoveThreads) {
         la[pad16(e)] = la[pad16(e + TILE)];
         if (Tr::NARGS == 2) la[lds_elems + pad16(e)] = la[lds_elems + pad16(e + TILE)];
       }
       __syncthreads();
-      staged = (int)H;
+      if constexpr (PF) {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+          const int e = (int)H + ((int)threadIdx.x + m * kMoveThreads) * VEC;
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            la[pad16(e + v)] = pa[m * VEC + v];
+            if constexpr (Tr::NARGS == 2) la[lds_elems + pad16(e + v)] = pb[m * VEC + v];
+          }
+        }
+        staged = seg_len;  // fully staged; rolling_tile's first barrier publishes these writes
+      } else {
+        staged = (int)H;  // rolling_tile stages the rest
+      }
     }
   }
 }
