@@ -197,17 +197,19 @@ def test_move_axes_and_layouts(gpu_lib):
 @pytest.mark.parametrize("func", FUNCS1 + FUNCS2)
 @pytest.mark.parametrize("window", [1, 37, 1000, 2047, 2048, 4081, 4096, 6000])
 def test_move_span_carried_halo_is_bitwise_neutral(gpu_lib, monkeypatch, func, dtype, window):
-    """A workgroup walking a span of tiles carries each halo forward in LDS;
-    outputs must be bitwise identical to one tile per workgroup (every tile
-    evaluates its window from the same staged inputs)."""
+    """A workgroup walking a span of tiles carries each halo forward in LDS
+    (and optionally prefetches the next tile into registers); outputs must be
+    bitwise identical to one tile per workgroup (every tile evaluates its
+    window from the same staged inputs)."""
     import numbagg_amd as nb
 
     x = walk_array(3, 5 * 4096 + 123, seed=window, dtype=dtype)
     y = walk_array(3, 5 * 4096 + 123, seed=window + 1, dtype=dtype) if func in FUNCS2 else None
     args = (x,) if y is None else (x, y)
     outs = []
-    for span in ("1", "2", "4", "16"):
+    for span, prefetch in (("1", "0"), ("2", "0"), ("4", "1"), ("16", "0"), ("16", "1")):
         monkeypatch.setenv("NBG_MOVE_SPAN", span)
+        monkeypatch.setenv("NBG_MOVE_PREFETCH", prefetch)
         outs.append(getattr(nb, func)(*args, window=window, min_count=1))
     for o in outs[1:]:
         np.testing.assert_array_equal(o.view(np.uint8), outs[0].view(np.uint8))
