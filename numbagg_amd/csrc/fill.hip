@@ -22,7 +22,10 @@
 // tiles through one 64-bit status word per tile:
 //   bits 63..62  flag (1 = aggregate: tile holds no valid value;
 //                      2 = inclusive prefix)
-//   bits 61..0   1 + scan-order index of the last valid value (0 = none)
+//   bits 61..0   1 + scan-order index of the last valid value (0 = none).
+// For float32 rows shorter than 2**30 the word also carries the value
+// (PACK): bits 61..32 = 1 + index, bits 31..0 = the value's bits, so the tile
+// that finds its carry has its value without a second, dependent load.
 #include <cstdlib>
 #include <cstring>
 
@@ -37,6 +40,11 @@ constexpr int kFillThreads = kFillWaves * kWave;
 constexpr unsigned long long kFlagA = 1ull << 62;
 constexpr unsigned long long kFlagP = 2ull << 62;
 constexpr unsigned long long kIdxMask = (1ull << 62) - 1;
+constexpr int64_t kPackMaxN = (1ll << 30) - 2;  // PACK: 1 + index fits 30 bits
+
+__device__ __forceinline__ unsigned long long pack_carry(long long idx, float v) {
+  return ((unsigned long long)(idx + 1) << 32) | (unsigned long long)__float_as_uint(v);
+}
 
 template <typename T>
 struct alignas(16) VecT;
@@ -68,7 +76,7 @@ __device__ __forceinline__ Last<T> wave_incl_last(Last<T> a) {
   return a;
 }
 
-template <typename T, int VEC, int U, bool REV, bool LOOKBACK>
+template <typename T, int VEC, int U, bool REV, bool LOOKBACK, bool PACK = false>
 __global__ __launch_bounds__(kFillThreads) void fill_kernel(
     const T* __restrict__ x, T* __restrict__ out, int64_t rows, int64_t n, int64_t ld_x,
     int64_t ld_out, int64_t limit, int64_t tiles_per_row, unsigned int* __restrict__ counters,
@@ -193,11 +201,15 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   // Cross-tile carry (decoupled look-back), by thread 0.
   if (threadIdx.x == 0) {
     long long carry = -1;
+    T cval = T(0);
+    bool have_val = false;
     if (LOOKBACK) {
       unsigned long long* srow = status + row * tiles_per_row;
       // Status is indexed by scan-order ticket.
       if (tile_agg.so >= 0) {
-        st_agent(&srow[ticket], kFlagP | (unsigned long long)(s_base + tile_agg.so + 1));
+        const long long idx = s_base + tile_agg.so;
+        if constexpr (PACK) st_agent(&srow[ticket], kFlagP | pack_carry(idx, tile_agg.val));
+        else st_agent(&srow[ticket], kFlagP | (unsigned long long)(idx + 1));
       } else if (ticket > 0) {
         st_agent(&srow[ticket], kFlagA);
       } else {
@@ -206,6 +218,7 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
       // The carry matters unless the tile's first element in scan order is valid.
       if (ticket > 0 && !s_first_valid) {
         int64_t j = ticket - 1;
+        unsigned long long found = 0ull;  // payload of the first prefix found
         while (true) {
           unsigned long long st;
           unsigned int spins = 0;
@@ -218,19 +231,32 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
             }
           }
           if ((st >> 62) == 2) {
-            carry = (long long)(st & kIdxMask) - 1;
+            found = st & kIdxMask;
             break;
           }
           if (j == 0) break;
           --j;
         }
-        if (tile_agg.so < 0) st_agent(&srow[ticket], kFlagP | (unsigned long long)(carry + 1));
+        if constexpr (PACK) {
+          carry = (long long)(found >> 32) - 1;
+          if (carry >= 0) {
+            cval = __uint_as_float((unsigned int)(found & 0xffffffffull));
+            have_val = true;
+          }
+        } else {
+          carry = (long long)found - 1;
+        }
+        if (tile_agg.so < 0) st_agent(&srow[ticket], kFlagP | found);
       }
     }
     s_carry = carry;
     if (carry >= 0) {
-      const int64_t pc = REV ? (n - 1 - carry) : carry;
-      s_carry_val = xr[pc];
+      if (have_val) {
+        s_carry_val = cval;
+      } else {
+        const int64_t pc = REV ? (n - 1 - carry) : carry;
+        s_carry_val = xr[pc];
+      }
     }
   }
   __syncthreads();
@@ -317,29 +343,33 @@ int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int
   constexpr int MAXV = 16 / (int)sizeof(T);
   constexpr int UV = BYTES / (int)sizeof(T) / MAXV;  // slots per thread at full vector width
   dim3 grid((unsigned)blocks), block(kFillThreads);
-#define NBG_FILL_LAUNCH(V, U)                                                                       \
-  do {                                                                                            \
-    if (rev) {                                                                                    \
-      if (lookback)                                                                               \
-        hipLaunchKernelGGL((fill_kernel<T, V, U, true, true>), grid, block, 0, s, x, out, rows, n, \
-                           ld_x, ld_out, limit, tiles, counters, status, to);                     \
-      else                                                                                        \
-        hipLaunchKernelGGL((fill_kernel<T, V, U, true, false>), grid, block, 0, s, x, out, rows,  \
-                           n, ld_x, ld_out, limit, tiles, counters, status, to);                  \
-    } else {                                                                                      \
-      if (lookback)                                                                               \
-        hipLaunchKernelGGL((fill_kernel<T, V, U, false, true>), grid, block, 0, s, x, out, rows,  \
-                           n, ld_x, ld_out, limit, tiles, counters, status, to);                  \
-      else                                                                                        \
-        hipLaunchKernelGGL((fill_kernel<T, V, U, false, false>), grid, block, 0, s, x, out, rows, \
-                           n, ld_x, ld_out, limit, tiles, counters, status, to);                  \
-    }                                                                                             \
+  // PACK: the status word carries the float32 carry value (rows < 2**30;
+  // NBG_FILL_PACK=0 turns it off, so tests reach the other path on short rows).
+  constexpr bool PACK_OK = sizeof(T) == 4;
+  const char* pk = getenv("NBG_FILL_PACK");
+  const bool pack = PACK_OK && n <= kPackMaxN && !(pk != nullptr && strcmp(pk, "0") == 0);
+#define NBG_FILL_LAUNCH_P(V, U, R, LB, P)                                                          \
+  hipLaunchKernelGGL((fill_kernel<T, V, U, R, LB, P>), grid, block, 0, s, x, out, rows, n, ld_x, \
+                     ld_out, limit, tiles, counters, status, to)
+#define NBG_FILL_LAUNCH(V, U)                         \
+  do {                                                \
+    if (!lookback) {                                  \
+      if (rev) NBG_FILL_LAUNCH_P(V, U, true, false, false); \
+      else NBG_FILL_LAUNCH_P(V, U, false, false, false);    \
+    } else if (pack) {                                \
+      if (rev) NBG_FILL_LAUNCH_P(V, U, true, true, PACK_OK); \
+      else NBG_FILL_LAUNCH_P(V, U, false, true, PACK_OK);    \
+    } else {                                          \
+      if (rev) NBG_FILL_LAUNCH_P(V, U, true, true, false);  \
+      else NBG_FILL_LAUNCH_P(V, U, false, true, false);     \
+    }                                                 \
   } while (0)
   if (vec == MAXV)
     NBG_FILL_LAUNCH(MAXV, UV);
   else
     NBG_FILL_LAUNCH(1, UV * MAXV);
 #undef NBG_FILL_LAUNCH
+#undef NBG_FILL_LAUNCH_P
   return check_hip(hipGetLastError(), "fill_kernel launch");
 }
 

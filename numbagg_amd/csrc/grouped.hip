@@ -26,7 +26,7 @@ namespace nbg {
 namespace {
 
 constexpr int kGroupThreads = 256;
-constexpr int kGroupUnroll = 4;
+constexpr int kGroupUnroll = 4;  // default vectors per thread and super-step
 
 enum GKind {
   kGSum = 0,
@@ -114,7 +114,8 @@ __device__ __forceinline__ void g_atomic_add(unsigned long long* p, unsigned lon
 // One accumulation pass over R rows x one chunk of positions. The R rows
 // share each label load. LDS (privatised strategy): per-(row, label)
 // accumulators, flushed once with global atomics.
-template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT>
+template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT, int U = kGroupUnroll,
+          bool PIPE = false>
 __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   using SA = typename SumAcc<TV>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -235,43 +236,83 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
     }
   };
 
-  // Stream the chunk: every thread keeps kGroupUnroll vectors of values and
-  // labels in flight; full super-steps first, then a coalesced remainder.
+  // Stream the chunk: every thread keeps U vectors of values and labels in
+  // flight; full super-steps first, then a coalesced remainder.
   constexpr int VV = 16 / (int)sizeof(TV);
   constexpr int VL = 16 / (int)sizeof(TL);
   constexpr int VE = VV < VL ? VV : VL;  // elements per vector step
   constexpr int64_t STEP = (int64_t)kGroupThreads * VE;
-  constexpr int64_t SUPER = STEP * kGroupUnroll;
+  constexpr int64_t SUPER = STEP * U;
   const bool aligned = (((uintptr_t)(vbase + c0)) % 16 == 0) && (((uintptr_t)(lab + c0)) % 16 == 0) &&
                        (nr == 1 || (A.ld_v % VE) == 0);
   const int64_t nfull = aligned ? (c1 - c0) / SUPER : 0;
-  {
-    struct alignas(16) V { TV v[VE]; };
-    struct alignas(VE * sizeof(TL)) W { TL v[VE]; };
+  struct alignas(16) V { TV v[VE]; };
+  struct alignas(VE * sizeof(TL)) W { TL v[VE]; };
+  if constexpr (!PIPE) {
     for (int64_t k = 0; k < nfull; ++k) {
       const int64_t base = c0 + k * SUPER + (int64_t)tid * VE;
       // All loads of the super-step (labels once, then every row's values)
       // are issued before any update, so one wait covers them.
-      W ll[kGroupUnroll];
-      V vv[R][kGroupUnroll];
+      W ll[U];
+      V vv[R][U];
 #pragma unroll
-      for (int u = 0; u < kGroupUnroll; ++u) ll[u] = *reinterpret_cast<const W*>(lab + base + u * STEP);
+      for (int u = 0; u < U; ++u) ll[u] = *reinterpret_cast<const W*>(lab + base + u * STEP);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (r < nr) {
           const TV* vr = vbase + (int64_t)r * A.ld_v;
 #pragma unroll
-          for (int u = 0; u < kGroupUnroll; ++u) vv[r][u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+          for (int u = 0; u < U; ++u) vv[r][u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
         }
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (r >= nr) break;
 #pragma unroll
-        for (int u = 0; u < kGroupUnroll; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
           for (int e = 0; e < VE; ++e) update(vv[r][u].v[e], ll[u].v[e], base + u * STEP + e, r);
       }
+    }
+  } else if (nfull > 0) {
+    // Pipelined: the values of the next row (or the next super-step's labels
+    // and first row) are loaded while the current row is folded in.
+    W ll[U], lln[U];
+    V vc[U], vn[U];
+    {
+      const int64_t b0 = c0 + (int64_t)tid * VE;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ll[u] = *reinterpret_cast<const W*>(lab + b0 + u * STEP);
+        vc[u] = *reinterpret_cast<const V*>(vbase + b0 + u * STEP);
+      }
+    }
+    for (int64_t k = 0; k < nfull; ++k) {
+      const int64_t base = c0 + k * SUPER + (int64_t)tid * VE;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r < nr) {
+          if (r + 1 < nr) {
+            const TV* vr = vbase + (int64_t)(r + 1) * A.ld_v;
+#pragma unroll
+            for (int u = 0; u < U; ++u) vn[u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+          } else if (k + 1 < nfull) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              lln[u] = *reinterpret_cast<const W*>(lab + base + SUPER + u * STEP);
+              vn[u] = *reinterpret_cast<const V*>(vbase + base + SUPER + u * STEP);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int e = 0; e < VE; ++e) update(vc[u].v[e], ll[u].v[e], base + u * STEP + e, r);
+#pragma unroll
+          for (int u = 0; u < U; ++u) vc[u] = vn[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) ll[u] = lln[u];
     }
   }
   for (int64_t j = c0 + nfull * SUPER + tid; j < c1; j += kGroupThreads) {
@@ -404,10 +445,11 @@ int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
 // LDS size): no partly filled second wave, and each workgroup flushes its
 // accumulators once. Chunks are multiples of 4096 positions (whole vector
 // super-steps) and at least 4096.
-template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT>
+template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT, int U = kGroupUnroll,
+          bool PIPE = false>
 int launch_pass(GArgs a, size_t lds, hipStream_t s) {
   const int64_t groups = (a.rows + R - 1) / R;
-  int cap = device_capacity_blocks((const void*)group_pass_kernel<TV, TL, PASS, LDS, R, COUNT>,
+  int cap = device_capacity_blocks((const void*)group_pass_kernel<TV, TL, PASS, LDS, R, COUNT, U, PIPE>,
                                    kGroupThreads, LDS ? lds : 0);
   if (cap <= 0) cap = 2048;
   const int64_t per_group = cap / groups > 1 ? cap / groups : 1;
@@ -417,25 +459,46 @@ int launch_pass(GArgs a, size_t lds, hipStream_t s) {
   a.chunks_per_row = (a.n + a.chunk - 1) / a.chunk;
   const int64_t blocks = groups * a.chunks_per_row;
   if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "group: grid too large");
-  hipLaunchKernelGGL((group_pass_kernel<TV, TL, PASS, LDS, R, COUNT>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((group_pass_kernel<TV, TL, PASS, LDS, R, COUNT, U, PIPE>), dim3((unsigned)blocks),
                      dim3(kGroupThreads), LDS ? lds : 0, s, a);
   return check_hip(hipGetLastError(), "group_pass_kernel launch");
 }
 
 // Rows per workgroup for the LDS strategy: 4 when four rows' accumulators fit
-// the LDS budget (the rows then share each label load), else 1.
+// the LDS budget (the rows then share each label load), else 2, else 1.
 constexpr int kGroupRowsPerBlock = 4;
+
+// Tuning knobs of the sum pass (sum, mean, and the first pass of var/std):
+// NBG_GROUP_ROWS (1, 2, 4: rows per workgroup, at most), NBG_GROUP_UNROLL
+// (2 or 4 vectors per thread and super-step), NBG_GROUP_PIPE=1 (load the next
+// row while folding the current one). Results do not depend on them.
+int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return (v != nullptr && v[0] != '\0') ? atoi(v) : dflt;
+}
+
+template <typename TV, typename TL, int PASS, bool COUNT, int R>
+int launch_rows(GArgs a, size_t lds, hipStream_t s) {
+  if constexpr (PASS == kPSum) {
+    const int u = env_int("NBG_GROUP_UNROLL", kGroupUnroll);
+    const bool pipe = env_int("NBG_GROUP_PIPE", 0) == 1;
+    if (pipe) return launch_pass<TV, TL, PASS, true, R, COUNT, kGroupUnroll, true>(a, lds, s);
+    if (u == 2) return launch_pass<TV, TL, PASS, true, R, COUNT, 2, false>(a, lds, s);
+  }
+  return launch_pass<TV, TL, PASS, true, R, COUNT>(a, lds, s);
+}
 
 template <typename TV, typename TL, int PASS, bool COUNT = true>
 int run_pass(GArgs a, bool lds, hipStream_t s) {
   const int64_t L = a.num_labels;
   const int64_t slot = pass_slot_bytes(PASS, COUNT, sizeof(typename SumAcc<TV>::type));
   if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
-  if (a.rows >= kGroupRowsPerBlock && L * slot * kGroupRowsPerBlock <= kGroupLdsBudget) {
-    return launch_pass<TV, TL, PASS, true, kGroupRowsPerBlock, COUNT>(
-        a, (size_t)(L * slot * kGroupRowsPerBlock), s);
-  }
-  return launch_pass<TV, TL, PASS, true, 1, COUNT>(a, (size_t)(L * slot), s);
+  const int rmax = PASS == kPSum ? env_int("NBG_GROUP_ROWS", kGroupRowsPerBlock) : kGroupRowsPerBlock;
+  if (rmax >= 4 && a.rows >= 4 && L * slot * 4 <= kGroupLdsBudget)
+    return launch_rows<TV, TL, PASS, COUNT, 4>(a, (size_t)(L * slot * 4), s);
+  if (rmax >= 2 && a.rows >= 2 && L * slot * 2 <= kGroupLdsBudget)
+    return launch_rows<TV, TL, PASS, COUNT, 2>(a, (size_t)(L * slot * 2), s);
+  return launch_rows<TV, TL, PASS, COUNT, 1>(a, (size_t)(L * slot), s);
 }
 
 int launch_init(void* acc0, unsigned long long init0, void* acc1, unsigned long long init1,

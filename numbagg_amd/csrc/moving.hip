@@ -68,10 +68,11 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   size_t lds = (size_t)rolling_stage_bytes<T>(KP, window, tail_lds, Tr::NARGS);
   if (use_lut) lds += (size_t)(window + 1) * sizeof(double);
   dim3 grid((unsigned)blocks), block(kMoveThreads);
-  // NBG_MOVE_PREFETCH=1 prefetches each next tile of a span into registers
-  // while the current one is evaluated (more VGPRs, fewer resident waves).
+  // Each next tile of a span is prefetched into registers while the current
+  // one is evaluated (more VGPRs, fewer resident waves; measured faster on
+  // MI355X); NBG_MOVE_PREFETCH=0 turns it off.
   const char* pfe = getenv("NBG_MOVE_PREFETCH");
-  const bool pf = pfe != nullptr && pfe[0] == '1' && pfe[1] == '\0';
+  const bool pf = !(pfe != nullptr && pfe[0] == '0' && pfe[1] == '\0');
 #define NBG_MOVE_LAUNCH(TL, V, LU)                                                                        \
   do {                                                                                                    \
     if (pf && TL)                                                                                         \

@@ -462,10 +462,17 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     a = la[ehp + j + (j >> 4)];
     b = (Tr::NARGS == 2) ? lb[ehp + j + (j >> 4)] : T(0);
   };
+  // With K a multiple of 16, tail inputs sit at a block-uniform misalignment
+  // mis = (-window) & 15 behind the thread's 16-aligned head chunk:
+  // pad16(et + j) = tbase + j + ((mis + j) >> 4), a per-thread base plus a
+  // uniform offset per j.
+  const int mis = (int)((-window) & 15);
+  const int tbase = pad16(eh - (int)((window + 15) & ~(int64_t)15)) + mis;
   auto tail = [&](int j, T& a, T& b) {
     if (TAIL_LDS) {
-      a = la[pad16(et + j)];
-      b = (Tr::NARGS == 2) ? lb[pad16(et + j)] : T(0);
+      const int o = (K % 16 == 0) ? tbase + j + ((mis + j) >> 4) : pad16(et + j);
+      a = la[o];
+      b = (Tr::NARGS == 2) ? lb[o] : T(0);
     } else {
       const int64_t pt = c0 + j - window;
       const bool in = pt >= 0 && pt < n;
@@ -488,7 +495,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       head(j, h, u);
       tail(j, t, u);
       const bool hv = !is_nan(h), tv = !is_nan(t);
-      dj[j] = (hv ? (double)h : 0.0) - (tv ? (double)t : 0.0);
+      // NaN -> 0 selected in float32 before widening (one select each)
+      const T h0 = hv ? h : T(0), t0v = tv ? t : T(0);
+      dj[j] = (double)h0 - (double)t0v;
       dc[j] = (int)hv - (int)tv;
       dsum += dj[j];
       dcnt += dc[j];

@@ -27,16 +27,19 @@ def test_fill_bit_exact(gpu_lib, dtype, func, limit, shape):
     np.testing.assert_array_equal(_bits(got), _bits(restated.fill(func, x, limit)))
 
 
+@pytest.mark.parametrize("pack", ["1", "0"])
 @pytest.mark.parametrize("fill_bytes", ["16", "32", "64"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("func", ["ffill", "bfill"])
 @pytest.mark.parametrize("limit", [None, 3, 700])
-def test_fill_tile_sizes_bit_exact(gpu_lib, monkeypatch, fill_bytes, dtype, func, limit):
-    """Every tile size (bytes per thread) gives the same bits."""
+def test_fill_tile_sizes_bit_exact(gpu_lib, monkeypatch, pack, fill_bytes, dtype, func, limit):
+    """Every tile size (bytes per thread), with the carry value packed in the
+    look-back status word or loaded from the input, gives the same bits."""
     import numbagg_amd as nb
     from oracle import exact
 
     monkeypatch.setenv("NBG_FILL_BYTES", fill_bytes)
+    monkeypatch.setenv("NBG_FILL_PACK", pack)
     for shape in [(5, 1023), (3, 40_961), (64, 3000)]:
         x = runs_array(*shape, seed=shape[1] + 7, dtype=dtype)
         got = getattr(nb, func)(x, limit=limit)
