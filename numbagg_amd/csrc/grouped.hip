@@ -468,10 +468,15 @@ int launch_pass(GArgs a, size_t lds, hipStream_t s) {
 // the LDS budget (the rows then share each label load), else 2, else 1.
 constexpr int kGroupRowsPerBlock = 4;
 
-// Tuning knobs of the sum pass (sum, mean, and the first pass of var/std):
-// NBG_GROUP_ROWS (1, 2, 4: rows per workgroup, at most), NBG_GROUP_UNROLL
-// (2 or 4 vectors per thread and super-step), NBG_GROUP_PIPE=1 (load the next
-// row while folding the current one). Results do not depend on them.
+// Shape of the sum pass (sum, mean, and the first pass of var/std): at most
+// 2 rows per workgroup and 2 vectors per thread and super-step (measured
+// fastest on MI355X for (64, 4M) x 1000 labels: 16 KB of LDS slots and ~50
+// VGPRs let 8 workgroups per CU stay resident). Knobs: NBG_GROUP_ROWS (1, 2,
+// 4), NBG_GROUP_UNROLL (2, 4), NBG_GROUP_PIPE=1 (load the next row while
+// folding the current one). Results do not depend on them.
+constexpr int kSumRows = 2;
+constexpr int kSumUnroll = 2;
+
 int env_int(const char* name, int dflt) {
   const char* v = getenv(name);
   return (v != nullptr && v[0] != '\0') ? atoi(v) : dflt;
@@ -480,9 +485,12 @@ int env_int(const char* name, int dflt) {
 template <typename TV, typename TL, int PASS, bool COUNT, int R>
 int launch_rows(GArgs a, size_t lds, hipStream_t s) {
   if constexpr (PASS == kPSum) {
-    const int u = env_int("NBG_GROUP_UNROLL", kGroupUnroll);
+    const int u = env_int("NBG_GROUP_UNROLL", kSumUnroll);
     const bool pipe = env_int("NBG_GROUP_PIPE", 0) == 1;
-    if (pipe) return launch_pass<TV, TL, PASS, true, R, COUNT, kGroupUnroll, true>(a, lds, s);
+    if (pipe) {
+      if (u == 2) return launch_pass<TV, TL, PASS, true, R, COUNT, 2, true>(a, lds, s);
+      return launch_pass<TV, TL, PASS, true, R, COUNT, 4, true>(a, lds, s);
+    }
     if (u == 2) return launch_pass<TV, TL, PASS, true, R, COUNT, 2, false>(a, lds, s);
   }
   return launch_pass<TV, TL, PASS, true, R, COUNT>(a, lds, s);
@@ -493,7 +501,7 @@ int run_pass(GArgs a, bool lds, hipStream_t s) {
   const int64_t L = a.num_labels;
   const int64_t slot = pass_slot_bytes(PASS, COUNT, sizeof(typename SumAcc<TV>::type));
   if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
-  const int rmax = PASS == kPSum ? env_int("NBG_GROUP_ROWS", kGroupRowsPerBlock) : kGroupRowsPerBlock;
+  const int rmax = PASS == kPSum ? env_int("NBG_GROUP_ROWS", kSumRows) : kGroupRowsPerBlock;
   if (rmax >= 4 && a.rows >= 4 && L * slot * 4 <= kGroupLdsBudget)
     return launch_rows<TV, TL, PASS, COUNT, 4>(a, (size_t)(L * slot * 4), s);
   if (rmax >= 2 && a.rows >= 2 && L * slot * 2 <= kGroupLdsBudget)

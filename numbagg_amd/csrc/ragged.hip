@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4)
   constexpr int TILE = kMoveThreads * KP;
   const int tid = threadIdx.x;
   double* lut = reinterpret_cast<double*>(smem_raw + A.stage_bytes);
-  if constexpr (LUT) fill_recip_lut(lut, A.window);  // ordered by the loop's first barrier
+  if constexpr (LUT) fill_recip_lut(lut, A.window, A.min_count > 1 ? A.min_count : 1);  // ordered by the loop's first barrier
   double* l_sum = reinterpret_cast<double*>(smem_raw + A.stage_bytes + A.lut_bytes);
   unsigned int* l_cnt = reinterpret_cast<unsigned int*>(l_sum + A.lds_labels);
   const float* la = reinterpret_cast<const float*>(smem_raw);  // the staged segment

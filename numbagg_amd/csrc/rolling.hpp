@@ -530,12 +530,15 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       acc += dj[j];
-      const double v = (KIND == kSum) ? acc : acc * rc[j];
-      // Adding +0 (enough observations) or NaN (too few) instead of
-      // selecting keeps the compiler from branching around each output; the
-      // +0 also maps a -0 result to +0, as the reference's running sum
-      // (started at +0) never yields -0.
-      res[j] = (T)v + (cc[j] >= mc32 ? T(0) : canonical_nan<T>());
+      if constexpr (KIND != kSum && LUT) {
+        // the table holds NaN below the effective min_count
+        res[j] = (T)(acc * rc[j]);
+      } else {
+        const double v = (KIND == kSum) ? acc : acc * rc[j];
+        // Adding +0 (enough observations) or NaN (too few) instead of
+        // selecting keeps the compiler from branching around each output.
+        res[j] = (T)v + (cc[j] >= mc32 ? T(0) : canonical_nan<T>());
+      }
     }
   } else if constexpr (!Tr::SHIFT) {
     // Sum/mean: the exact per-position differences head - tail stay in
@@ -609,10 +612,12 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
 // Largest window for which mean keeps a 1/c table in LDS.
 constexpr int64_t kMeanLutWindow = 4096;
 
-// lut[c] = 1/c (hardware reciprocal + one Newton step), c = 0..window; lut[0]
-// is never used.
-__device__ __forceinline__ void fill_recip_lut(double* lut, int64_t window) {
-  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads) lut[c] = c ? recip((int)c) : 0.0;
+// lut[c] = 1/c (hardware reciprocal + one Newton step) for c >= mce, NaN for
+// c < mce (too few observations: the product sum * lut[c] is then NaN, the
+// masked output), c = 0..window. mce = max(min_count, 1).
+__device__ __forceinline__ void fill_recip_lut(double* lut, int64_t window, int64_t mce) {
+  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads)
+    lut[c] = c >= mce ? recip((int)c) : canonical_nan<double>();
 }
 
 // Items per thread for float32 sum/mean: 16 (default) or 32, from
@@ -659,7 +664,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(PF
   double* lut = nullptr;
   if constexpr (LUT) {
     lut = reinterpret_cast<double*>(smem_raw + rolling_stage_bytes<T>(KP, window, TAIL_LDS, Tr::NARGS));
-    fill_recip_lut(lut, window);  // the tile's first barrier orders these writes before any read
+    fill_recip_lut(lut, window, min_count > 1 ? min_count : 1);  // ordered by the tile's first barrier
   }
   const T* ra = xa + row * ld_a;
   const T* rb = (Tr::NARGS == 2) ? xb + row * ld_b : nullptr;

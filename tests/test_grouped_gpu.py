@@ -83,6 +83,12 @@ def test_group_float_outputs(gpu_lib, strategy, func, dtype, shape, L, zipf):
 
     v, lab = _data(*shape, L, dtype, seed=shape[1] + 3 * L, zipf=zipf)
     got = _call(func, v, lab, num_labels=L, axis=-1)
+    _check_float(func, dtype, v, lab, L, got)
+
+
+def _check_float(func, dtype, v, lab, L, got):
+    from oracle import exact
+
     want = exact.group(func, v, lab, L)
     assert np.array_equal(np.isnan(got), np.isnan(want))
     ok = ~np.isnan(want)
@@ -104,6 +110,23 @@ def test_group_float_outputs(gpu_lib, strategy, func, dtype, shape, L, zipf):
     else:
         rtol = 2e-7 if dtype == np.float32 else 1e-12
         assert_allclose(got[ok], want[ok], rtol=rtol)
+
+
+@pytest.mark.parametrize("rows,unroll,pipe", [("4", "4", "0"), ("1", "2", "0"), ("2", "2", "0"), ("2", "4", "1"),
+                                              ("4", "2", "1"), ("1", "4", "0")])
+@pytest.mark.parametrize("func", FLOAT_FUNCS)
+def test_group_sum_pass_shapes(gpu_lib, monkeypatch, rows, unroll, pipe, func):
+    """Every shape of the sum pass (rows per workgroup, vectors per thread,
+    pipelined loads) meets the same bars; 9 rows leave partial row groups."""
+    import numbagg_amd as nb
+
+    monkeypatch.setenv("NBG_GROUP_ROWS", rows)
+    monkeypatch.setenv("NBG_GROUP_UNROLL", unroll)
+    monkeypatch.setenv("NBG_GROUP_PIPE", pipe)
+    for dtype in (np.float32, np.float64):
+        v, lab = _data(9, 300_000, 1000, dtype, seed=int(rows) * 10 + int(unroll), zipf=True)
+        got = getattr(nb, func)(v, lab, num_labels=1000, axis=-1)
+        _check_float(func, dtype, v, lab, 1000, got)
 
 
 def test_group_ddof(gpu_lib):

@@ -213,3 +213,18 @@ def test_move_span_carried_halo_is_bitwise_neutral(gpu_lib, monkeypatch, func, d
         outs.append(getattr(nb, func)(*args, window=window, min_count=1))
     for o in outs[1:]:
         np.testing.assert_array_equal(o.view(np.uint8), outs[0].view(np.uint8))
+
+
+@pytest.mark.parametrize("func", ["move_sum", "move_mean"])
+@pytest.mark.parametrize("shape,window,min_count", [((3, 500), 10, None), ((4, 9000), 1000, 500), ((2, 20_000), 20, 20),
+                                                    ((1, 30_000), 12_000, 100)])
+def test_move_f32_twosum_accumulator(gpu_lib, monkeypatch, func, shape, window, min_count):
+    """The FP32 TwoSum accumulator (NBG_F32_ACCUMULATOR=twosum) meets the same
+    bar as the float64 default: <= 1 float32 ULP of the exact value."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    monkeypatch.setenv("NBG_F32_ACCUMULATOR", "twosum")
+    x = walk_array(*shape, seed=window + shape[1] + 5)
+    got = getattr(nb, func)(x, window=window, min_count=min_count)
+    check_f32(func, got, exact.move(func, x, window, min_count))
