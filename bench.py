@@ -125,6 +125,7 @@ def build_workloads(torch, nb, lib, want_all):
 
         wl.append(Workload("move_mean", {"shape": [256, 2_000_000], "dtype": "f32", "window": 1000, "min_count": 500},
                            x.numel() * 8, run_mm, cpu_mm, kernel="move_kernel"))
+        wl[-1].inputs = {"x": x}
     if has_entry(lib, "nbg_group_nansum"):
         v, lab = gen_group(torch, 64, 4_000_000, 1000, seed=12)
 
@@ -136,7 +137,8 @@ def build_workloads(torch, nb, lib, want_all):
             return ("group_nansum", vs, dict(labels=lab.cpu().numpy(), num_labels=1000)), vs.nbytes + lab.numel() * 4
 
         wl.append(Workload("group_nansum", {"shape": [64, 4_000_000], "dtype": "f32", "num_labels": 1000, "labels": "uniform int32"},
-                           v.numel() * 4 + lab.numel() * 4 + 64 * 1000 * 4, run_gs, cpu_gs, kernel="group_kernel"))
+                           v.numel() * 4 + lab.numel() * 4 + 64 * 1000 * 4, run_gs, cpu_gs, kernel="group_pass_kernel"))
+        wl[-1].inputs = {"v": v, "lab": lab}
     x = gen_runs(torch, 1024, 1_000_000, seed=13)
 
     def run_ff(x=x):
@@ -150,6 +152,64 @@ def build_workloads(torch, nb, lib, want_all):
                        x.numel() * 8, run_ff, cpu_ff, kernel="fill_kernel"))
     wl[-1].copy_src = x
     return wl
+
+
+def build_extra(torch, nb, wl):
+    """Every other hot-path function on the same synthetic inputs (timed with
+    --all, reported beside the headline step, not part of `value`)."""
+    ins = {w.name: getattr(w, "inputs", {}) for w in wl}
+    extra = []
+    if "move_mean" in ins:
+        x = ins["move_mean"]["x"]
+        y = gen_walk(torch, 256, 2_000_000, seed=21)
+        n = x.numel()
+        cfg = {"shape": [256, 2_000_000], "dtype": "f32", "window": 1000, "min_count": 500}
+        for f in ("move_sum", "move_std", "move_var"):
+            extra.append(Workload(f, cfg, n * 8, (lambda f=f: getattr(nb, f)(x, window=1000, min_count=500))))
+        for f in ("move_cov", "move_corr"):
+            extra.append(Workload(f, cfg, n * 12, (lambda f=f: getattr(nb, f)(x, y, window=1000, min_count=500))))
+        ecfg = {"shape": [256, 2_000_000], "dtype": "f32 in, f64 out (Python-float alpha)", "alpha": 0.01}
+        for f in ("move_exp_nanmean", "move_exp_nanvar", "move_exp_nanstd", "move_exp_nansum", "move_exp_nancount"):
+            extra.append(Workload(f, ecfg, n * 12, (lambda f=f: getattr(nb, f)(x, alpha=0.01))))
+        for f in ("move_exp_nancorr", "move_exp_nancov"):
+            extra.append(Workload(f, ecfg, n * 16, (lambda f=f: getattr(nb, f)(x, y, alpha=0.01))))
+    if "group_nansum" in ins:
+        v, lab = ins["group_nansum"]["v"], ins["group_nansum"]["lab"]
+        _, labz = gen_group(torch, 1, 4_000_000, 1000, seed=22, zipf=1.1)
+        nbytes = v.numel() * 4 + lab.numel() * 4 + 64 * 1000 * 4
+        for zname, lb in (("uniform", lab), ("zipf1.1", labz)):
+            cfg = {"shape": [64, 4_000_000], "dtype": "f32", "num_labels": 1000, "labels": zname}
+            for f in ("group_nansum", "group_nanmean", "group_nanvar", "group_nanstd", "group_nancount",
+                      "group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast"):
+                if zname == "uniform" and f == "group_nansum":
+                    continue
+                extra.append(Workload(f"{f}[{zname}]", cfg, nbytes,
+                                      (lambda f=f, lb=lb: getattr(nb, f)(v, lb, num_labels=1000, axis=-1))))
+    src = [w.copy_src for w in wl if hasattr(w, "copy_src")]
+    if src:
+        x = src[0]
+        cfg = {"shape": [1024, 1_000_000], "dtype": "f32", "nan": "30% geometric runs"}
+        extra.append(Workload("bfill", dict(cfg, limit=None), x.numel() * 8, lambda: nb.bfill(x)))
+        extra.append(Workload("ffill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda: nb.ffill(x, limit=16)))
+        extra.append(Workload("bfill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda: nb.bfill(x, limit=16)))
+    return extra
+
+
+def time_workload(torch, w, steps, stream):
+    """Mean HIP-event time (ms) of `steps` calls after one warm-up call."""
+    w.run()
+    torch.cuda.synchronize()
+    evs = []
+    for _ in range(steps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        out = w.run()
+        b.record(stream)
+        evs.append((a, b))
+        del out
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in evs]))
 
 
 def gen_ragged(torch, nseries, seed, lo=100, hi=1_000_000, klo=10, khi=100_000, p_nan=0.05, zipf=1.1):
@@ -256,6 +316,21 @@ def bench_ragged(torch, nb, steps, judge=False):
     return rec
 
 
+def measured_traffic(w):
+    """HBM bytes per launch of workload `w`'s kernel (read + write), from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json
+    (scaled by the streaming-copy calibration), or None when not measured."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(w.name)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("kernel") != w.kernel:
+        return None
+    return int(rec["read_bytes"] + rec["write_bytes"])
+
+
 def time_cpu(spec, budget_s=4.0):
     """Median wall time of the oracle restatement on the sample (>=3 runs)."""
     from oracle import restated
@@ -282,7 +357,8 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--all", action="store_true", help="also time every hot-path function")
+    ap.add_argument("--all", action="store_true",
+                    help="also time every other hot-path function on the same inputs (reported, not in value)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-copy", action="store_true", help="skip the streaming-copy reference")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batched comparison")
@@ -373,6 +449,18 @@ def main(argv=None):
         del dst
         print(f"# copy reference: {copy_ms:.3f} ms  {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
+    all_rec = None
+    if args.all:
+        all_rec = {}
+        for w in build_extra(torch, nb, wl):
+            ms = time_workload(torch, w, max(3, args.steps), stream)
+            gbs = w.bytes / (ms * 1e-3) / 1e9
+            all_rec[w.name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                               "bytes": w.bytes, "config": w.config}
+            print(f"# all {w.name}: {ms:.3f} ms  {gbs:.0f} GB/s  {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak",
+                  file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     ragged_rec = None
     if not args.no_ragged:
         ragged_rec = bench_ragged(torch, nb, max(3, args.steps), judge=args.judge)
@@ -431,6 +519,7 @@ def main(argv=None):
             "copy_reference_GBs": round(copy_gbs, 1) if copy_gbs else None,
         },
         "per_function": per,
+        "other_functions": all_rec,
         "ragged": ragged_rec,
         "roofline": {
             "bound": "hbm",
@@ -440,7 +529,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom_ach / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dom),
         } if dom else None,
         "cpu_baseline": cpu,
     }

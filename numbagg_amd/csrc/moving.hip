@@ -75,7 +75,7 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   const bool pf = !(pfe != nullptr && pfe[0] == '0' && pfe[1] == '\0');
 #define NBG_MOVE_LAUNCH(TL, V, LU)                                                                        \
   do {                                                                                                    \
-    if (pf && TL)                                                                                         \
+    if (pf && TL && Tr::NARGS == 1)                                                                       \
       hipLaunchKernelGGL((move_kernel<T, KIND, TL, V, KP, LU, TL>), grid, block, lds, s, a, b, out, rows, \
                          n, ld_a, ld_b, ld_out, window, min_count, tiles, span);                          \
     else                                                                                                  \

@@ -517,27 +517,33 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       c += dc[j];
       cc[j] = c;
     }
-    // 1/c for every output first (all table loads in flight together).
-    double rc[K];
-    if constexpr (KIND != kSum) {
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        if constexpr (LUT) rc[j] = lut[cc[j]];  // c in [0, window]
-        else rc[j] = recip(cc[j] > 0 ? cc[j] : 1);
-      }
-    }
+    // Outputs in groups of 8: the group's 1/c values are loaded first (table
+    // loads in flight together, with half the registers of all K at once).
     double acc = st.s.v;
+    constexpr int G = K < 8 ? K : 8;
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      acc += dj[j];
-      if constexpr (KIND != kSum && LUT) {
-        // the table holds NaN below the effective min_count
-        res[j] = (T)(acc * rc[j]);
-      } else {
-        const double v = (KIND == kSum) ? acc : acc * rc[j];
-        // Adding +0 (enough observations) or NaN (too few) instead of
-        // selecting keeps the compiler from branching around each output.
-        res[j] = (T)v + (cc[j] >= mc32 ? T(0) : canonical_nan<T>());
+    for (int g = 0; g < K; g += G) {
+      double rc[G];
+      if constexpr (KIND != kSum) {
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+          if constexpr (LUT) rc[i] = lut[cc[g + i]];  // c in [0, window]
+          else rc[i] = recip(cc[g + i] > 0 ? cc[g + i] : 1);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int j = g + i;
+        acc += dj[j];
+        if constexpr (KIND != kSum && LUT) {
+          // the table holds NaN below the effective min_count
+          res[j] = (T)(acc * rc[i]);
+        } else {
+          const double v = (KIND == kSum) ? acc : acc * rc[i];
+          // Adding +0 (enough observations) or NaN (too few) instead of
+          // selecting keeps the compiler from branching around each output.
+          res[j] = (T)v + (cc[j] >= mc32 ? T(0) : canonical_nan<T>());
+        }
       }
     }
   } else if constexpr (!Tr::SHIFT) {
@@ -644,9 +650,16 @@ constexpr int64_t kMaxSpan = 16;
 // source [TILE, TILE + H) and destination [0, H) to be disjoint.
 __host__ __device__ inline int64_t carried_halo(int64_t window) { return (window + 15) & ~(int64_t)15; }
 
+// The float32 single-input prefetch variants are held to 128 VGPRs (4 waves
+// per SIMD); every other variant keeps the compiler's register choice.
+template <typename T, int KIND, bool PF, int KP>
+constexpr int move_min_waves() {
+  return (PF && sizeof(T) == 4 && KP == 16 && Traits<T, KIND>::NARGS == 1) ? 4 : 1;
+}
+
 template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false,
           bool PF = false>
-__global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(PF ? 3 : 4))) void move_kernel(
+__global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(move_min_waves<T, KIND, PF, KP>()))) void move_kernel(
     const T* __restrict__ xa, const T* __restrict__ xb, T* __restrict__ out, int64_t rows,
     int64_t n, int64_t ld_a, int64_t ld_b, int64_t ld_out, int64_t window, int64_t min_count,
     int64_t tiles_per_row, int64_t span) {
