@@ -304,6 +304,274 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   }
 }
 
+// ---- span kernel -------------------------------------------------------------
+// One workgroup walks `span` consecutive tiles of a row (one ticket per span).
+// The next tile is prefetched into registers while the current one is scanned;
+// the running carry stays block-uniform across the span's tiles, and each tile
+// with a valid value publishes its prefix at once. Only the span's leading run
+// of NaNs (scan positions before its first valid value) depends on earlier
+// spans: those outputs are written after the span, once one look-back has
+// found the carry, so a workgroup never waits in the middle of its span.
+template <typename T, int VEC, int U, bool REV, bool PACK, bool PF = false>
+__global__ __launch_bounds__(kFillThreads) void fill_span_kernel(
+    const T* __restrict__ x, T* __restrict__ out, int64_t rows, int64_t n, int64_t ld_x,
+    int64_t ld_out, int64_t limit, int64_t tiles_per_row, int64_t span,
+    unsigned int* __restrict__ counters, unsigned long long* __restrict__ status,
+    unsigned int* __restrict__ timeouts) {
+  constexpr int TILE = kFillThreads * U * VEC;
+  __shared__ int s_ticket;
+  __shared__ int s_wso[kFillWaves];
+  __shared__ T s_wval[kFillWaves];
+  __shared__ int s_wfirst[kFillWaves];
+  __shared__ long long s_carry;
+  __shared__ T s_carry_val;
+
+  const int64_t row = (int64_t)(blockIdx.x % (uint64_t)rows);
+  const T* xr = x + row * ld_x;
+  T* orow = out + row * ld_out;
+  unsigned long long* srow = status + row * tiles_per_row;
+  const int w = wave_id();
+  const int l = lane_id();
+  if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&counters[row], 1u);
+  __syncthreads();
+  const int64_t tk0 = (int64_t)s_ticket * span;
+  const int64_t tk1 = tk0 + span < tiles_per_row ? tk0 + span : tiles_per_row;
+
+  auto s_base_of = [&](int64_t tk) -> int64_t {
+    const int64_t tile = REV ? (tiles_per_row - 1 - tk) : tk;
+    return REV ? (n - tile * TILE - TILE) : tile * TILE;
+  };
+  auto so_of = [&](int u, int v) -> int {
+    const int e = ((w * U + u) * kWave + l) * VEC + v;
+    return REV ? (TILE - 1 - e) : e;
+  };
+  auto load_tile = [&](int64_t tk, T (&vals)[U][VEC]) {
+    const int64_t tile = REV ? (tiles_per_row - 1 - tk) : tk;
+    const int64_t q0 = tile * TILE;
+    const bool q_full = (q0 + TILE <= n);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e0 = ((w * U + u) * kWave + l) * VEC;
+      if (q_full && VEC > 1) {
+        VecT<T> t = *reinterpret_cast<const VecT<T>*>(xr + q0 + e0);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) vals[u][v] = t.v[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const int64_t p = q0 + e0 + v;
+          vals[u][v] = (p < n) ? xr[p] : canonical_nan<T>();
+        }
+      }
+    }
+  };
+
+  // Block-uniform state across the span's tiles.
+  bool known = (tk0 == 0);  // carry into the current tile known (none before the row)
+  long long run_idx = -1;   // scan-order index of the last valid value so far (-1 none)
+  T run_val = T(0);
+  long long lead_end = -1;  // scan position of the span's first valid value, once seen
+
+  // PF: the next tile is loaded into registers while this one is scanned.
+  T cur[U][VEC];
+  if constexpr (PF) load_tile(tk0, cur);
+  for (int64_t tk = tk0; tk < tk1; ++tk) {
+    T nxt[U][VEC];
+    if constexpr (PF) {
+      if (tk + 1 < tk1) load_tile(tk + 1, nxt);
+    } else {
+      load_tile(tk, cur);
+    }
+    const int64_t s_base = s_base_of(tk);
+    const int64_t tile = REV ? (tiles_per_row - 1 - tk) : tk;
+    const int64_t p0 = tile * TILE;
+    const bool full = (p0 + TILE <= n);
+
+    // In-tile scan: per-slot exclusive carries, wave aggregates, first valid.
+    Last<T> cin[U];
+    Last<T> wcarry{-1, T(0)};
+    int wfirst = 0x7fffffff;
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      const int u = REV ? (U - 1 - i) : i;
+      Last<T> agg{-1, T(0)};
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        if (!is_nan(cur[u][v])) {
+          const int so = so_of(u, v);
+          if (so > agg.so) {
+            agg.so = so;
+            agg.val = cur[u][v];
+          }
+          wfirst = so < wfirst ? so : wfirst;
+        }
+      }
+      Last<T> incl = wave_incl_last<REV>(agg);
+      Last<T> excl;
+      excl.so = REV ? shfl_down(incl.so, 1) : shfl_up(incl.so, 1);
+      excl.val = REV ? shfl_down(incl.val, 1) : shfl_up(incl.val, 1);
+      const bool first_lane = REV ? (l == kWave - 1) : (l == 0);
+      if (first_lane || excl.so < 0) excl = wcarry;
+      cin[u] = excl;
+      Last<T> tail;
+      tail.so = shfl(incl.so, REV ? 0 : kWave - 1);
+      tail.val = shfl(incl.val, REV ? 0 : kWave - 1);
+      if (tail.so >= 0) wcarry = tail;
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      const int o = shfl_down(wfirst, d);
+      wfirst = o < wfirst ? o : wfirst;
+    }
+    if (l == 0) {
+      s_wso[w] = wcarry.so;
+      s_wval[w] = wcarry.val;
+      s_wfirst[w] = wfirst;
+    }
+    __syncthreads();
+    Last<T> wave_in{-1, T(0)};
+    Last<T> tile_agg{-1, T(0)};
+    int tile_first = 0x7fffffff;
+#pragma unroll
+    for (int i = 0; i < kFillWaves; ++i) {
+      const int ww = REV ? (kFillWaves - 1 - i) : i;
+      if (s_wso[ww] >= 0) {
+        if ((REV && ww > w) || (!REV && ww < w)) {
+          wave_in.so = s_wso[ww];
+          wave_in.val = s_wval[ww];
+        }
+        tile_agg.so = s_wso[ww];
+        tile_agg.val = s_wval[ww];
+      }
+      tile_first = s_wfirst[ww] < tile_first ? s_wfirst[ww] : tile_first;
+    }
+
+    // Publish this tile's status (never waits).
+    if (threadIdx.x == 0) {
+      if (tile_agg.so >= 0) {
+        const long long idx = s_base + tile_agg.so;
+        if constexpr (PACK) st_agent(&srow[tk], kFlagP | pack_carry(idx, tile_agg.val));
+        else st_agent(&srow[tk], kFlagP | (unsigned long long)(idx + 1));
+      } else if (known) {
+        if constexpr (PACK) st_agent(&srow[tk], kFlagP | (run_idx >= 0 ? pack_carry(run_idx, run_val) : 0ull));
+        else st_agent(&srow[tk], kFlagP | (unsigned long long)(run_idx + 1));
+      } else {
+        st_agent(&srow[tk], kFlagA);
+      }
+    }
+
+    // Outputs; positions with no valid value before them in the span while
+    // the carry is pending are left for the end of the span.
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      long long r_abs = known ? run_idx : -2;  // -2: carry not known yet
+      T r_val = run_val;
+      if (wave_in.so >= 0) {
+        r_abs = s_base + wave_in.so;
+        r_val = wave_in.val;
+      }
+      if (cin[u].so >= 0) {
+        r_abs = s_base + cin[u].so;
+        r_val = cin[u].val;
+      }
+      T res[VEC];
+      bool keep[VEC];
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) {
+        const int v = REV ? (VEC - 1 - i) : i;
+        const int64_t s_abs = s_base + so_of(u, v);
+        if (!is_nan(cur[u][v])) {
+          r_abs = s_abs;
+          r_val = cur[u][v];
+        }
+        keep[v] = r_abs != -2;
+        res[v] = (r_abs >= 0 && (s_abs - r_abs) <= limit) ? r_val : canonical_nan<T>();
+      }
+      const int e0 = ((w * U + u) * kWave + l) * VEC;
+      bool all_keep = true;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) all_keep = all_keep && keep[v];
+      if (full && VEC > 1 && all_keep) {
+        VecT<T> t;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) t.v[v] = res[v];
+        *reinterpret_cast<VecT<T>*>(orow + p0 + e0) = t;
+      } else {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const int64_t p = p0 + e0 + v;
+          if (p < n && keep[v]) orow[p] = res[v];
+        }
+      }
+    }
+
+    // Block-uniform updates for the next tile.
+    if (tile_agg.so >= 0) {
+      if (!known) lead_end = s_base + tile_first;
+      run_idx = s_base + tile_agg.so;
+      run_val = tile_agg.val;
+      known = true;
+    }
+    if constexpr (PF) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) cur[u][v] = nxt[u][v];
+    }
+    __syncthreads();  // LDS aggregates reused by the next tile
+  }
+
+  // The span's leading run [span start, first valid value) in scan order:
+  // one look-back for the carry into the span, then its outputs.
+  if (tk0 > 0) {
+    const int64_t sb0 = s_base_of(tk0);
+    const int64_t span_s0 = sb0 > 0 ? sb0 : 0;
+    const int64_t sb1 = s_base_of(tk1 - 1) + TILE;
+    const int64_t span_s1 = sb1 < n ? sb1 : n;
+    const int64_t lead = lead_end >= 0 ? lead_end : span_s1;
+    if (lead > span_s0) {
+      if (threadIdx.x == 0) {
+        unsigned long long found = 0ull;
+        for (int64_t j = tk0 - 1;; --j) {
+          unsigned long long st;
+          unsigned int spins = 0;
+          while (((st = ld_agent(&srow[j])) >> 62) == 0) {
+            spin_pause();
+            if (++spins > kSpinLimit) {
+              atomicAdd(timeouts, 1u);
+              st = kFlagP;
+              break;
+            }
+          }
+          if ((st >> 62) == 2) {
+            found = st & kIdxMask;
+            break;
+          }
+          if (j == 0) break;
+        }
+        long long carry;
+        if constexpr (PACK) {
+          carry = (long long)(found >> 32) - 1;
+          if (carry >= 0) s_carry_val = __uint_as_float((unsigned int)(found & 0xffffffffull));
+        } else {
+          carry = (long long)found - 1;
+          if (carry >= 0) s_carry_val = xr[REV ? (n - 1 - carry) : carry];
+        }
+        s_carry = carry;
+        // a span without a valid value now knows its prefix
+        if (!known) st_agent(&srow[tk1 - 1], kFlagP | found);
+      }
+      __syncthreads();
+      const long long carry = s_carry;
+      const T cval = carry >= 0 ? s_carry_val : T(0);
+      for (int64_t s = span_s0 + threadIdx.x; s < lead; s += kFillThreads) {
+        const int64_t p = REV ? (n - 1 - s) : s;
+        orow[p] = (carry >= 0 && (s - carry) <= limit) ? cval : canonical_nan<T>();
+      }
+    }
+  }
+}
+
 // Input bytes per thread of a tile: 64 by default (16 float32 / 8 float64
 // elements); NBG_FILL_BYTES=16 or 32 selects smaller tiles (one or two 16-byte
 // vectors per thread: more workgroups, more look-back tiles). The workspace
@@ -351,11 +619,37 @@ int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int
 #define NBG_FILL_LAUNCH_P(V, U, R, LB, P)                                                          \
   hipLaunchKernelGGL((fill_kernel<T, V, U, R, LB, P>), grid, block, 0, s, x, out, rows, n, ld_x, \
                      ld_out, limit, tiles, counters, status, to)
+  // Spans: NBG_FILL_SPAN=<k> tiles per workgroup (1 = one tile each).
+  int64_t span = 1;
+  if (const char* e = getenv("NBG_FILL_SPAN")) span = atoll(e);
+  span = span < 1 ? 1 : (span > 16 ? 16 : span);
+  if (span > tiles) span = tiles;
+  const int64_t spans = (tiles + span - 1) / span;
+  dim3 sgrid((unsigned)(rows * spans));
+  const char* fpf = getenv("NBG_FILL_PREFETCH");
+  const bool fill_pf = fpf != nullptr && strcmp(fpf, "1") == 0;
+#define NBG_FILL_SPAN_P(V, U, R, P)                                                                       \
+  do {                                                                                                    \
+    if (fill_pf)                                                                                          \
+      hipLaunchKernelGGL((fill_span_kernel<T, V, U, R, P, true>), sgrid, block, 0, s, x, out, rows, n,   \
+                         ld_x, ld_out, limit, tiles, span, counters, status, to);                         \
+    else                                                                                                  \
+      hipLaunchKernelGGL((fill_span_kernel<T, V, U, R, P, false>), sgrid, block, 0, s, x, out, rows, n,  \
+                         ld_x, ld_out, limit, tiles, span, counters, status, to);                         \
+  } while (0)
 #define NBG_FILL_LAUNCH(V, U)                         \
   do {                                                \
     if (!lookback) {                                  \
       if (rev) NBG_FILL_LAUNCH_P(V, U, true, false, false); \
       else NBG_FILL_LAUNCH_P(V, U, false, false, false);    \
+    } else if (span > 1) {                            \
+      if (pack) {                                     \
+        if (rev) NBG_FILL_SPAN_P(V, U, true, PACK_OK); \
+        else NBG_FILL_SPAN_P(V, U, false, PACK_OK);    \
+      } else {                                        \
+        if (rev) NBG_FILL_SPAN_P(V, U, true, false);  \
+        else NBG_FILL_SPAN_P(V, U, false, false);     \
+      }                                               \
     } else if (pack) {                                \
       if (rev) NBG_FILL_LAUNCH_P(V, U, true, true, PACK_OK); \
       else NBG_FILL_LAUNCH_P(V, U, false, true, PACK_OK);    \
@@ -370,6 +664,7 @@ int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int
     NBG_FILL_LAUNCH(1, UV * MAXV);
 #undef NBG_FILL_LAUNCH
 #undef NBG_FILL_LAUNCH_P
+#undef NBG_FILL_SPAN_P
   return check_hip(hipGetLastError(), "fill_kernel launch");
 }
 

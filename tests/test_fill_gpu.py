@@ -46,6 +46,30 @@ def test_fill_tile_sizes_bit_exact(gpu_lib, monkeypatch, pack, fill_bytes, dtype
         np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
 
 
+@pytest.mark.parametrize("span,prefetch", [("2", "0"), ("3", "1"), ("8", "0"), ("16", "1")])
+@pytest.mark.parametrize("pack", ["1", "0"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("func", ["ffill", "bfill"])
+@pytest.mark.parametrize("limit", [None, 5, 9000])
+def test_fill_spans_bit_exact(gpu_lib, monkeypatch, span, prefetch, pack, dtype, func, limit):
+    """Span workgroups (several tiles per ticket, the leading NaN run written
+    after one look-back) give the same bits, including rows whose spans hold
+    no valid value at all."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    monkeypatch.setenv("NBG_FILL_SPAN", span)
+    monkeypatch.setenv("NBG_FILL_PREFETCH", prefetch)
+    monkeypatch.setenv("NBG_FILL_PACK", pack)
+    for shape in [(5, 1023), (3, 40_961), (16, 70_000)]:
+        x = runs_array(*shape, seed=shape[1] + int(span), dtype=dtype)
+        x[1, 100:30_000] = np.nan  # NaN stretches longer than whole spans
+        x[2, :] = np.nan
+        x[2, shape[1] // 2] = 1.0
+        got = getattr(nb, func)(x, limit=limit)
+        np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
+
+
 @pytest.mark.parametrize("fill_bytes", ["16", "64"])
 @pytest.mark.parametrize("func", ["ffill", "bfill"])
 def test_fill_long_nan_runs_cross_many_tiles(gpu_lib, monkeypatch, fill_bytes, func):
