@@ -165,14 +165,14 @@ def build_extra(torch, nb, wl):
         n = x.numel()
         cfg = {"shape": [256, 2_000_000], "dtype": "f32", "window": 1000, "min_count": 500}
         for f in ("move_sum", "move_std", "move_var"):
-            extra.append(Workload(f, cfg, n * 8, (lambda f=f: getattr(nb, f)(x, window=1000, min_count=500))))
+            extra.append(Workload(f, cfg, n * 8, (lambda f=f, x=x: getattr(nb, f)(x, window=1000, min_count=500))))
         for f in ("move_cov", "move_corr"):
-            extra.append(Workload(f, cfg, n * 12, (lambda f=f: getattr(nb, f)(x, y, window=1000, min_count=500))))
+            extra.append(Workload(f, cfg, n * 12, (lambda f=f, x=x, y=y: getattr(nb, f)(x, y, window=1000, min_count=500))))
         ecfg = {"shape": [256, 2_000_000], "dtype": "f32 in, f64 out (Python-float alpha)", "alpha": 0.01}
         for f in ("move_exp_nanmean", "move_exp_nanvar", "move_exp_nanstd", "move_exp_nansum", "move_exp_nancount"):
-            extra.append(Workload(f, ecfg, n * 12, (lambda f=f: getattr(nb, f)(x, alpha=0.01))))
+            extra.append(Workload(f, ecfg, n * 12, (lambda f=f, x=x: getattr(nb, f)(x, alpha=0.01))))
         for f in ("move_exp_nancorr", "move_exp_nancov"):
-            extra.append(Workload(f, ecfg, n * 16, (lambda f=f: getattr(nb, f)(x, y, alpha=0.01))))
+            extra.append(Workload(f, ecfg, n * 16, (lambda f=f, x=x, y=y: getattr(nb, f)(x, y, alpha=0.01))))
     if "group_nansum" in ins:
         v, lab = ins["group_nansum"]["v"], ins["group_nansum"]["lab"]
         _, labz = gen_group(torch, 1, 4_000_000, 1000, seed=22, zipf=1.1)
@@ -184,14 +184,14 @@ def build_extra(torch, nb, wl):
                 if zname == "uniform" and f == "group_nansum":
                     continue
                 extra.append(Workload(f"{f}[{zname}]", cfg, nbytes,
-                                      (lambda f=f, lb=lb: getattr(nb, f)(v, lb, num_labels=1000, axis=-1))))
+                                      (lambda f=f, v=v, lb=lb: getattr(nb, f)(v, lb, num_labels=1000, axis=-1))))
     src = [w.copy_src for w in wl if hasattr(w, "copy_src")]
     if src:
         x = src[0]
         cfg = {"shape": [1024, 1_000_000], "dtype": "f32", "nan": "30% geometric runs"}
-        extra.append(Workload("bfill", dict(cfg, limit=None), x.numel() * 8, lambda: nb.bfill(x)))
-        extra.append(Workload("ffill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda: nb.ffill(x, limit=16)))
-        extra.append(Workload("bfill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda: nb.bfill(x, limit=16)))
+        extra.append(Workload("bfill", dict(cfg, limit=None), x.numel() * 8, lambda x=x: nb.bfill(x)))
+        extra.append(Workload("ffill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda x=x: nb.ffill(x, limit=16)))
+        extra.append(Workload("bfill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda x=x: nb.bfill(x, limit=16)))
     return extra
 
 
