@@ -111,11 +111,25 @@ __device__ __forceinline__ unsigned long long ld_agent(const unsigned long long*
 __device__ __forceinline__ void st_agent(unsigned long long* p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ unsigned int ld_acquire(const unsigned int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Multi-word publication (flag + payload) where every payload word is itself
+// written and read with the relaxed agent-scope atomics below. Agent-scope
+// release/acquire would write back / invalidate the whole per-XCD L2 on every
+// publish and every poll; since no payload word lives in a non-coherent cache,
+// ordering is all that is needed: the writer waits until its payload stores
+// have completed (s_waitcnt with all counters at zero) before storing the
+// flag, and the reader issues its payload loads only after the flag load has
+// returned (the spin loop branches on its value). The signal fences keep the
+// compiler from moving the atomics across these points.
+__device__ __forceinline__ unsigned int ld_flag(const unsigned int* p) {
+  const unsigned int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return v;
 }
-__device__ __forceinline__ void st_release(unsigned int* p, unsigned int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_flag(unsigned int* p, unsigned int v) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double ld_agent_f64(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -328,14 +328,14 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
 #pragma unroll
       for (int c = 0; c < NS; ++c) st_agent_f64(pl + 2 + c, agg.b[c]);
       st_agent_f64(pl + 2 + NS, (double)agg.seen);
-      st_release(&frow[tile], 1u);
+      st_flag(&frow[tile], 1u);
       // walk back, composing aggregates until an inclusive state is found
       Map<NS> acc = Map<NS>::identity();  // maps of tiles (j, tile) composed
       int64_t j = tile - 1;
       while (true) {
         unsigned int f;
         unsigned int spins = 0;
-        while ((f = ld_acquire(&frow[j])) == 0u) {
+        while ((f = ld_flag(&frow[j])) == 0u) {
           spin_pause();
           if (++spins > kSpinLimit) {
             atomicAdd(A.timeouts, 1u);
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
     for (int c = 0; c < NS; ++c)
       st_agent_f64(pl + kMaxNS + 3 + c, (c == Sp::SW2 ? agg.p2 : agg.p) * sin[c] + agg.b[c]);
     st_agent_f64(pl + kMaxNS + 3 + NS, (double)min(2, sin_seen + agg.seen));
-    st_release(&frow[tile], 2u);
+    st_flag(&frow[tile], 2u);
 #pragma unroll
     for (int c = 0; c < NS; ++c) s_in[c] = sin[c];
     s_in_seen = sin_seen;
