@@ -573,13 +573,14 @@ __global__ __launch_bounds__(kFillThreads) void fill_span_kernel(
 }
 
 // Input bytes per thread of a tile: 64 by default (16 float32 / 8 float64
-// elements); NBG_FILL_BYTES=16 or 32 selects smaller tiles (one or two 16-byte
-// vectors per thread: more workgroups, more look-back tiles). The workspace
-// query reads the same setting.
+// elements); NBG_FILL_BYTES=16, 32 or 128 selects other tiles (1, 2 or 8
+// 16-byte vectors per thread: more or fewer workgroups and look-back tiles).
+// The workspace query reads the same setting.
 int fill_bytes_per_thread() {
   const char* v = getenv("NBG_FILL_BYTES");
   if (v != nullptr && strcmp(v, "16") == 0) return 16;
   if (v != nullptr && strcmp(v, "32") == 0) return 32;
+  if (v != nullptr && strcmp(v, "128") == 0) return 128;
   return 64;
 }
 
@@ -674,6 +675,7 @@ int fill_impl(bool rev, const void* x, void* out, int64_t rows, int64_t n, int64
   switch (fill_bytes_per_thread()) {
     case 16: return fill_impl_b<T, 16>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
     case 32: return fill_impl_b<T, 32>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
+    case 128: return fill_impl_b<T, 128>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
     default: return fill_impl_b<T, 64>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
   }
 }

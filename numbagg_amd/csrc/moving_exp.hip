@@ -107,43 +107,50 @@ __device__ __forceinline__ Map<NS> shfl_up_map(const Map<NS>& m, int d) {
   return r;
 }
 
-// One step of the reference recurrence on state s (components per ExpSpec).
+// One step of the reference recurrence on state s (components per ExpSpec):
+// every component decays (by d, sum_weight_2 by d*d) and, when the
+// observation is valid, gains its contribution. Written branch-free as one
+// fused multiply-add per component, s = decay * s + contribution, with
+// invalid observations contributing 0: one rounding per step where the
+// reference's `s *= d; s += x` has two, so the state stays at least as close
+// to the exact recurrence.
 template <int KIND>
 __device__ __forceinline__ void exp_step(double* s, int& seen, double a, double b, double alpha) {
   using Sp = ExpSpec<KIND>;
   const double d = 1.0 - alpha;
   const double d2 = d * d;
-#pragma unroll
-  for (int c = 0; c < Sp::NS; ++c) s[c] *= (c == Sp::SW2 ? d2 : d);
   const bool valid = (Sp::NARGS == 2) ? !(is_nan(a) || is_nan(b)) : !is_nan(a);
-  if (!valid) return;
-  seen = min(2, seen + 1);
+  const double v = valid ? 1.0 : 0.0;
+  const double a0 = valid ? a : 0.0;
+  const double b0 = (Sp::NARGS == 2 && valid) ? b : 0.0;
+  const double w = v * alpha;
+  seen = min(2, seen + (int)valid);
   if (KIND == kECount) {
-    s[0] += 1.0;
-    s[1] += alpha;
+    s[0] = fma(d, s[0], v);
+    s[1] = fma(d, s[1], w);
   } else if (KIND == kEMean) {
-    s[0] += a;
-    s[1] += 1.0;
-    s[2] += alpha;
+    s[0] = fma(d, s[0], a0);
+    s[1] = fma(d, s[1], v);
+    s[2] = fma(d, s[2], w);
   } else if (KIND == kESum) {
-    s[0] += a;
-    s[1] += alpha;
+    s[0] = fma(d, s[0], a0);
+    s[1] = fma(d, s[1], w);
   } else if (KIND == kEVar || KIND == kEStd) {
-    s[0] += a * a;
-    s[1] += a;
-    s[2] += 1.0;
-    s[3] += 1.0;
-    s[4] += alpha;
+    s[0] = fma(d, s[0], a0 * a0);
+    s[1] = fma(d, s[1], a0);
+    s[2] = fma(d, s[2], v);
+    s[3] = fma(d2, s[3], v);
+    s[4] = fma(d, s[4], w);
   } else {
-    s[0] += a;
-    s[1] += b;
-    s[2] += a * b;
-    s[3] += 1.0;
-    s[4] += 1.0;
-    s[5] += alpha;
+    s[0] = fma(d, s[0], a0);
+    s[1] = fma(d, s[1], b0);
+    s[2] = fma(d, s[2], a0 * b0);
+    s[3] = fma(d, s[3], v);
+    s[4] = fma(d2, s[4], v);
+    s[5] = fma(d, s[5], w);
     if (KIND == kECorr) {
-      s[6] += a * a;
-      s[7] += b * b;
+      s[6] = fma(d, s[6], a0 * a0);
+      s[7] = fma(d, s[7], b0 * b0);
     }
   }
 }
@@ -193,7 +200,7 @@ struct ExpArgs {
   unsigned int* timeouts;
 };
 
-template <typename TX, typename TA, int KIND>
+template <typename TX, typename TA, int KIND, bool SCALAR>
 __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   using Sp = ExpSpec<KIND>;
   constexpr int NS = Sp::NS;
@@ -263,8 +270,11 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   const bool full = t0 + TILE <= n;
 
   const int64_t e0 = (int64_t)tid * K;
+  // SCALAR: one alpha for every position (kernel-uniform decay factors).
+  const double alpha_s = (double)(TA)A.alpha_scalar;
   auto alpha_at = [&](int64_t p) -> double {
-    return rl ? (double)rl[p] : (double)(TA)A.alpha_scalar;
+    if constexpr (SCALAR) return alpha_s;
+    else return (double)rl[p];
   };
 
   // 1. this thread's map.
@@ -275,10 +285,8 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
     for (int c = 0; c < NS; ++c) s[c] = 0.0;
     int seen = 0;
     double p = 1.0, p2 = 1.0;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
+    auto step = [&](int j) {
       const int64_t pos = t0 + e0 + j;
-      if (pos >= n) break;
       const double alpha = alpha_at(pos);
       const double d = 1.0 - alpha;
       p *= d;
@@ -286,6 +294,12 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
       const double a = (double)la[pad16(e0 + j)];
       const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
       exp_step<KIND>(s, seen, a, b, alpha);
+    };
+    if (full) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) step(j);  // no per-position bounds checks
+    } else {
+      for (int j = 0; j < K && t0 + e0 + j < n; ++j) step(j);
     }
     m.p = p;
     m.p2 = p2;
@@ -389,16 +403,19 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   for (int c = 0; c < NS; ++c) s[c] = (c == Sp::SW2 ? excl.p2 : excl.p) * s_in[c] + excl.b[c];
   int seen = min(2, s_in_seen + excl.seen);
   TA res[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
+  auto rerun = [&](int j) {
     const int64_t pos = t0 + e0 + j;
-    if (pos < n) {
-      const double alpha = alpha_at(pos);
-      const double a = (double)la[pad16(e0 + j)];
-      const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
-      exp_step<KIND>(s, seen, a, b, alpha);
-      res[j] = (TA)exp_out<KIND>(s, seen, A.min_weight);
-    }
+    const double alpha = alpha_at(pos);
+    const double a = (double)la[pad16(e0 + j)];
+    const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
+    exp_step<KIND>(s, seen, a, b, alpha);
+    res[j] = (TA)exp_out<KIND>(s, seen, A.min_weight);
+  };
+  if (full) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) rerun(j);
+  } else {
+    for (int j = 0; j < K && t0 + e0 + j < n; ++j) rerun(j);
   }
   constexpr int OV = 16 / (int)sizeof(TA);  // outputs per 16-byte store
   if (full && ((uintptr_t)(ro + t0 + e0)) % 16 == 0) {
@@ -445,7 +462,10 @@ int exp_launch(ExpArgs a, void* ws, size_t ws_bytes, hipStream_t s) {
   if (int rc = launch_zero(ws, zero_bytes, s)) return rc;
   constexpr int NARGS = ExpSpec<KIND>::NARGS;
   const size_t lds = (size_t)(pad16(exp_tile<TX>()) + 16) * sizeof(TX) * NARGS;
-  hipLaunchKernelGGL((exp_kernel<TX, TA, KIND>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
+  if (a.alpha == nullptr)
+    hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, true>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
+  else
+    hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, false>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
   return check_hip(hipGetLastError(), "exp_kernel launch");
 }
 

@@ -28,7 +28,7 @@ def test_fill_bit_exact(gpu_lib, dtype, func, limit, shape):
 
 
 @pytest.mark.parametrize("pack", ["1", "0"])
-@pytest.mark.parametrize("fill_bytes", ["16", "32", "64"])
+@pytest.mark.parametrize("fill_bytes", ["16", "32", "64", "128"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("func", ["ffill", "bfill"])
 @pytest.mark.parametrize("limit", [None, 3, 700])
