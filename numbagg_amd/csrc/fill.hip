@@ -13,8 +13,10 @@
 // workgroup takes a ticket from its row's counter, so tiles of a row start in
 // scan order and a tile only ever waits on tiles that are already running
 // (decoupled look-back, deadlock-free whatever the residency). Blocks are
-// interleaved over rows (blockIdx % rows), so a tile's predecessor was dispatched
-// `rows` blocks earlier and has normally published its prefix already.
+// interleaved over rows (by default tile t of every row before tile t + 1 of
+// any), so a tile's predecessor was dispatched `rows` blocks earlier and has
+// normally published its prefix already; its status word is read while the
+// tile's own loads are in flight.
 //
 // Inside a tile, loads are wave-striped vectors (every load instruction of a wave
 // covers 64*VEC contiguous elements); the carry "last valid (offset, value)" is
@@ -76,11 +78,33 @@ __device__ __forceinline__ Last<T> wave_incl_last(Last<T> a) {
   return a;
 }
 
+// Dispatch order of a row's tiles: runs of `run` consecutive tiles of one row,
+// rows taking turns (run = 1: row-interleaved, tile t of every row before tile
+// t + 1 of any; run = tiles: row-major). Block b gets its row and the tile it
+// will most likely draw as its ticket; the mapping is a bijection onto
+// rows x tiles (the last, shorter runs are interleaved the same way).
+__device__ __forceinline__ void fill_block_coords(uint32_t b, int64_t rows, int64_t tiles, int64_t run,
+                                                  int64_t& row, int64_t& tile) {
+  const uint32_t r = (uint32_t)rows, g = (uint32_t)run;
+  const uint32_t full = (uint32_t)tiles / g * g;
+  const uint32_t head = r * full;  // rows * tiles < 2**31
+  if (b < head) {
+    const uint32_t per = r * g;
+    const uint32_t sup = b / per, rem = b % per;
+    row = rem / g;
+    tile = (int64_t)sup * g + rem % g;
+  } else {
+    const uint32_t tl = (uint32_t)tiles - full, b2 = b - head;
+    row = b2 / tl;
+    tile = full + b2 % tl;
+  }
+}
+
 template <typename T, int VEC, int U, bool REV, bool LOOKBACK, bool PACK = false>
 __global__ __launch_bounds__(kFillThreads) void fill_kernel(
     const T* __restrict__ x, T* __restrict__ out, int64_t rows, int64_t n, int64_t ld_x,
     int64_t ld_out, int64_t limit, int64_t tiles_per_row, unsigned int* __restrict__ counters,
-    unsigned long long* __restrict__ status, unsigned int* __restrict__ timeouts) {
+    unsigned long long* __restrict__ status, unsigned int* __restrict__ timeouts, int64_t run) {
   constexpr int TILE = kFillThreads * U * VEC;
   __shared__ int s_ticket;
   __shared__ int s_wso[kFillWaves];
@@ -89,7 +113,8 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   __shared__ T s_carry_val;
   __shared__ int s_first_valid;
 
-  const int64_t row = (int64_t)(blockIdx.x % (uint64_t)rows);
+  int64_t row, guess;
+  fill_block_coords(blockIdx.x, rows, tiles_per_row, run, row, guess);
   const T* xr = x + row * ld_x;
   T* orow = out + row * ld_out;
   const int w = wave_id();
@@ -117,12 +142,18 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   };
 
   // Blocks of a row are dispatched in order, so a block almost always receives
-  // ticket blockIdx / rows: its tile is loaded while the ticket atomic is in
-  // flight, and reloaded only when the ticket differs.
-  const int64_t guess = (int64_t)(blockIdx.x / (uint64_t)rows);
+  // the ticket of its dispatch position in the row: its tile is loaded while
+  // the ticket atomic is in flight, and reloaded only when the ticket differs.
   int64_t ticket = guess;
+  // Thread 0 also reads the status word of the likely predecessor (tile
+  // guess - 1) now: when it already holds an inclusive prefix, the look-back
+  // after the scan needs no further round trip.
+  unsigned long long early = 0ull;
   if (LOOKBACK) {
-    if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&counters[row], 1u);
+    if (threadIdx.x == 0) {
+      s_ticket = (int)atomicAdd(&counters[row], 1u);
+      if (guess > 0) early = ld_agent(&status[row * tiles_per_row + guess - 1]);
+    }
     load_tile(guess);
     __syncthreads();
     ticket = s_ticket;
@@ -219,7 +250,12 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
       if (ticket > 0 && !s_first_valid) {
         int64_t j = ticket - 1;
         unsigned long long found = 0ull;  // payload of the first prefix found
-        while (true) {
+        // An inclusive prefix is final once published, so the early read is
+        // used as is; anything else (not yet published, or a tile without a
+        // valid value) goes through the look-back loop.
+        const bool early_hit = ticket == guess && (early >> 62) == 2;
+        if (early_hit) found = early & kIdxMask;
+        while (!early_hit) {
           unsigned long long st;
           unsigned int spins = 0;
           while (((st = ld_agent(&srow[j])) >> 62) == 0) {
@@ -617,9 +653,13 @@ int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int
   constexpr bool PACK_OK = sizeof(T) == 4;
   const char* pk = getenv("NBG_FILL_PACK");
   const bool pack = PACK_OK && n <= kPackMaxN && !(pk != nullptr && strcmp(pk, "0") == 0);
+  // Tiles per dispatch run of one row: NBG_FILL_RUN (default 1, row-interleaved).
+  int64_t run = 1;
+  if (const char* e = getenv("NBG_FILL_RUN")) run = atoll(e);
+  run = run < 1 ? 1 : (run > tiles ? tiles : run);
 #define NBG_FILL_LAUNCH_P(V, U, R, LB, P)                                                          \
   hipLaunchKernelGGL((fill_kernel<T, V, U, R, LB, P>), grid, block, 0, s, x, out, rows, n, ld_x, \
-                     ld_out, limit, tiles, counters, status, to)
+                     ld_out, limit, tiles, counters, status, to, run)
   // Spans: NBG_FILL_SPAN=<k> tiles per workgroup (1 = one tile each).
   int64_t span = 1;
   if (const char* e = getenv("NBG_FILL_SPAN")) span = atoll(e);

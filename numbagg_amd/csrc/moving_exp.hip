@@ -84,23 +84,24 @@ struct Map {
   }
 };
 
-// Apply `later` after `earlier`.
+// Apply `later` after `earlier`. Kinds without a d*d component (SW2 < 0)
+// leave p2 at 1.
 template <int NS, int SW2>
 __device__ __forceinline__ Map<NS> compose(const Map<NS>& earlier, const Map<NS>& later) {
   Map<NS> r;
   r.p = later.p * earlier.p;
-  r.p2 = later.p2 * earlier.p2;
+  r.p2 = SW2 >= 0 ? later.p2 * earlier.p2 : 1.0;
 #pragma unroll
   for (int c = 0; c < NS; ++c) r.b[c] = (c == SW2 ? later.p2 : later.p) * earlier.b[c] + later.b[c];
   r.seen = min(2, earlier.seen + later.seen);
   return r;
 }
 
-template <int NS>
+template <int NS, int SW2>
 __device__ __forceinline__ Map<NS> shfl_up_map(const Map<NS>& m, int d) {
   Map<NS> r;
   r.p = shfl_up(m.p, d);
-  r.p2 = shfl_up(m.p2, d);
+  r.p2 = SW2 >= 0 ? shfl_up(m.p2, d) : 1.0;
 #pragma unroll
   for (int c = 0; c < NS; ++c) r.b[c] = shfl_up(m.b[c], d);
   r.seen = shfl_up(m.seen, d);
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
       const double alpha = alpha_at(pos);
       const double d = 1.0 - alpha;
       p *= d;
-      p2 *= d * d;
+      if (Sp::SW2 >= 0) p2 *= d * d;
       const double a = (double)la[pad16(e0 + j)];
       const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
       exp_step<KIND>(s, seen, a, b, alpha);
@@ -313,10 +314,10 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   Map<NS> incl = m;
 #pragma unroll
   for (int dd = 1; dd < kWave; dd <<= 1) {
-    Map<NS> o = shfl_up_map(incl, dd);
+    Map<NS> o = shfl_up_map<NS, Sp::SW2>(incl, dd);
     if (lane >= dd) incl = compose<NS, Sp::SW2>(o, incl);
   }
-  Map<NS> excl = shfl_up_map(incl, 1);
+  Map<NS> excl = shfl_up_map<NS, Sp::SW2>(incl, 1);
   if (lane == 0) excl = Map<NS>::identity();
   if (lane == kWave - 1) s_wave[wave] = incl;
   __syncthreads();

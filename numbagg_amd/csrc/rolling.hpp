@@ -102,11 +102,16 @@ struct SumStats {
   }
 };
 
-// Moment kinds on shifted data: y = x - k.
-template <typename T, int NM>  // NM: 1 = (s1, s11), 2 = (s1, s2, s12[, s11, s22])
+// Moment kinds on shifted data: y = x - k. NM selects the sums kept:
+// 1 = (s1, s11) for var/std; 2 = (s1, s2, s12, s11, s22) for corr;
+// 3 = (s1, s2, s12) for cov.
+template <typename T, int NM>
 struct MomStats {
   using A = typename std::conditional<sizeof(T) == 4, D, Comp<double>>::type;
   using Acc = void;  // (no first-moment fast path)
+  static constexpr bool B = NM >= 2;     // second input
+  static constexpr bool SQA = NM <= 2;   // sum of squares of the first
+  static constexpr bool SQB = NM == 2;   // sum of squares of the second
   int c;
   A s1, s2, s11, s22, s12;
   __device__ __forceinline__ static MomStats zero() {
@@ -117,57 +122,55 @@ struct MomStats {
     const double ya = (double)a - (double)ka;
     c += 1;
     s1.add(ya);
-    add_prod(s11, ya, ya);
-    if (NM == 2) {
+    if (SQA) add_prod(s11, ya, ya);
+    if (B) {
       const double yb = (double)b - (double)kb;
       s2.add(yb);
       add_prod(s12, ya, yb);
-      add_prod(s22, yb, yb);
+      if (SQB) add_prod(s22, yb, yb);
     }
   }
   __device__ __forceinline__ void sub_obs(T a, T b, T ka, T kb) {
     const double ya = (double)a - (double)ka;
     c -= 1;
     s1.add(-ya);
-    sub_prod(s11, ya, ya);
-    if (NM == 2) {
+    if (SQA) sub_prod(s11, ya, ya);
+    if (B) {
       const double yb = (double)b - (double)kb;
       s2.add(-yb);
       sub_prod(s12, ya, yb);
-      sub_prod(s22, yb, yb);
+      if (SQB) sub_prod(s22, yb, yb);
     }
   }
   __device__ __forceinline__ void merge(const MomStats& o) {
     c += o.c;
     s1.add(o.s1);
-    s11.add(o.s11);
-    if (NM == 2) {
+    if (SQA) s11.add(o.s11);
+    if (B) {
       s2.add(o.s2);
       s12.add(o.s12);
-      s22.add(o.s22);
+      if (SQB) s22.add(o.s22);
     }
   }
   __device__ __forceinline__ void unmerge(const MomStats& o) {
     c -= o.c;
     s1.sub(o.s1);
-    s11.sub(o.s11);
-    if (NM == 2) {
+    if (SQA) s11.sub(o.s11);
+    if (B) {
       s2.sub(o.s2);
       s12.sub(o.s12);
-      s22.sub(o.s22);
+      if (SQB) s22.sub(o.s22);
     }
   }
   __device__ __forceinline__ MomStats up(int d) const {
-    MomStats r;
+    MomStats r = zero();
     r.c = shfl_up(c, d);
     r.s1 = shfl_up_acc(s1, d);
-    r.s11 = shfl_up_acc(s11, d);
-    if (NM == 2) {
+    if (SQA) r.s11 = shfl_up_acc(s11, d);
+    if (B) {
       r.s2 = shfl_up_acc(s2, d);
       r.s12 = shfl_up_acc(s12, d);
-      r.s22 = shfl_up_acc(s22, d);
-    } else {
-      r.s2 = r.s12 = r.s22 = A::zero();
+      if (SQB) r.s22 = shfl_up_acc(s22, d);
     }
     return r;
   }
@@ -226,17 +229,61 @@ template <typename T>
 struct Traits<T, kSum32> : SumTraits<T, kSum, true> {};
 template <typename T>
 struct Traits<T, kMean32> : SumTraits<T, kMean, true> {};
+// Reciprocals 1/c and 1/(c - 1) of a window count for the float32 moment
+// outputs (c >= 0; the count-1 guards keep both finite for c < 2, where the
+// caller masks the output anyway).
+struct CountRecips {
+  double r, r1;
+};
+__device__ __forceinline__ CountRecips count_recips(int c) {
+  return CountRecips{recip(c > 0 ? c : 1), recip(c > 1 ? c - 1 : 1)};
+}
+
+// 1/sqrt(v) for v in [2**-1000, 2**1000]: the hardware estimate y refined by
+// one Newton step y * (1.5 - v y^2 / 2), which squares its relative error.
+__device__ __forceinline__ double rsqrt_refined(double v) {
+  const double y = __builtin_amdgcn_rsq(v);
+  return y * fma(-(0.5 * v) * y, y, 1.5);
+}
+
+// sqrt(v) for a result rounded to float32, without the float64 square-root
+// expansion: s = v * rsqrt(v) plus one Newton correction (relative error
+// about the square of the estimate's). Above 2**300 the float32 result is
+// infinite and below 2**-300 it is zero either way, so v is clamped there;
+// NaN and negative v give NaN, as sqrt does.
+__device__ __forceinline__ double sqrt_for_f32(double v) {
+  const double vc = v > 0x1p300 ? 0x1p300 : v;
+  const bool zero = vc >= 0.0 && vc < 0x1p-300;
+  const double vz = zero ? 1.0 : vc;
+  const double y = __builtin_amdgcn_rsq(vz);
+  const double sq = vz * y;
+  const double res = fma(fma(-sq, sq, vz), 0.5 * y, sq);
+  return zero ? 0.0 : res;
+}
+
+// float32 moment outputs are formed in float64 with 1/c and 1/(c - 1) from
+// refined reciprocals (about 1 ULP of float64 each, far below the float32
+// rounding of the result) and masked by a select, so they compile without a
+// branch or a float64 division; float64 outputs keep the reference's divisions.
 template <typename T, int KIND>
 struct VarTraits {
   using S = MomStats<T, 1>;
   static constexpr int NARGS = 1;
   static constexpr bool SHIFT = true;
   __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
-    if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
-    const double c = (double)s.c;
+    const int64_t mce = mc > 2 ? mc : 2;
     const double m1 = s.s1.to_double();
-    const double var = (s.s11.to_double() - m1 * m1 / c) / (c - 1.0);
-    return (T)(KIND == kStd ? sqrt(var) : var);
+    if constexpr (sizeof(T) == 4) {
+      const CountRecips q = count_recips(s.c);
+      double var = (s.s11.to_double() - m1 * m1 * q.r) * q.r1;
+      var = s.c >= mce ? var : canonical_nan<double>();
+      return (T)(KIND == kStd ? sqrt_for_f32(var) : var);
+    } else {
+      if (s.c < mce) return canonical_nan<T>();
+      const double c = (double)s.c;
+      const double var = (s.s11.to_double() - m1 * m1 / c) / (c - 1.0);
+      return (T)(KIND == kStd ? sqrt(var) : var);
+    }
   }
 };
 template <typename T>
@@ -245,13 +292,20 @@ template <typename T>
 struct Traits<T, kStd> : VarTraits<T, kStd> {};
 template <typename T>
 struct Traits<T, kCov> {
-  using S = MomStats<T, 2>;
+  using S = MomStats<T, 3>;
   static constexpr int NARGS = 2;
   static constexpr bool SHIFT = true;
   __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
-    if (s.c < (mc > 2 ? mc : 2)) return canonical_nan<T>();
-    const double c = (double)s.c;
-    return (T)((s.s12.to_double() - s.s1.to_double() * s.s2.to_double() / c) / (c - 1.0));
+    const int64_t mce = mc > 2 ? mc : 2;
+    if constexpr (sizeof(T) == 4) {
+      const CountRecips q = count_recips(s.c);
+      const double cov = (s.s12.to_double() - s.s1.to_double() * s.s2.to_double() * q.r) * q.r1;
+      return (T)(s.c >= mce ? cov : canonical_nan<double>());
+    } else {
+      if (s.c < mce) return canonical_nan<T>();
+      const double c = (double)s.c;
+      return (T)((s.s12.to_double() - s.s1.to_double() * s.s2.to_double() / c) / (c - 1.0));
+    }
   }
 };
 template <typename T>
@@ -260,14 +314,28 @@ struct Traits<T, kCorr> {
   static constexpr int NARGS = 2;
   static constexpr bool SHIFT = true;
   __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
-    if (s.c < (mc > 1 ? mc : 1)) return canonical_nan<T>();
-    const double r = 1.0 / (double)s.c;
-    const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
-    const double va = s.s11.to_double() * r - ma * ma;
-    const double vb = s.s22.to_double() * r - mb * mb;
-    const double cv = s.s12.to_double() * r - ma * mb;
-    const double vv = va * vb;
-    return vv > 0 ? (T)(cv / sqrt(vv)) : canonical_nan<T>();
+    const int64_t mce = mc > 1 ? mc : 1;
+    if constexpr (sizeof(T) == 4) {
+      const double r = recip(s.c > 0 ? s.c : 1);
+      const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
+      const double va = s.s11.to_double() * r - ma * ma;
+      const double vb = s.s22.to_double() * r - mb * mb;
+      const double cv = s.s12.to_double() * r - ma * mb;
+      const double vv = va * vb;
+      const bool in_range = vv > 0x1p-1000 && vv < 0x1p1000;
+      double res = cv * rsqrt_refined(in_range ? vv : 1.0);
+      if (!in_range && vv > 0) res = cv / sqrt(vv);  // extreme scales only
+      return (T)((s.c >= mce && vv > 0) ? res : canonical_nan<double>());
+    } else {
+      if (s.c < mce) return canonical_nan<T>();
+      const double r = 1.0 / (double)s.c;
+      const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
+      const double va = s.s11.to_double() * r - ma * ma;
+      const double vb = s.s22.to_double() * r - mb * mb;
+      const double cv = s.s12.to_double() * r - ma * mb;
+      const double vv = va * vb;
+      return vv > 0 ? (T)(cv / sqrt(vv)) : canonical_nan<T>();
+    }
   }
 };
 

@@ -70,6 +70,25 @@ def test_fill_spans_bit_exact(gpu_lib, monkeypatch, span, prefetch, pack, dtype,
         np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
 
 
+@pytest.mark.parametrize("run", ["2", "5", "16", "100000"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("func", ["ffill", "bfill"])
+@pytest.mark.parametrize("limit", [None, 7])
+def test_fill_dispatch_runs_bit_exact(gpu_lib, monkeypatch, run, dtype, func, limit):
+    """Dispatching runs of consecutive tiles of one row (up to row-major order)
+    gives the same bits, with tile counts that are and are not multiples of
+    the run."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    monkeypatch.setenv("NBG_FILL_RUN", run)
+    for shape in [(5, 1023), (3, 40_961), (16, 70_000), (2, 65_536)]:
+        x = runs_array(*shape, seed=shape[1] + len(run), dtype=dtype)
+        x[1, 100:30_000] = np.nan
+        got = getattr(nb, func)(x, limit=limit)
+        np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
+
+
 @pytest.mark.parametrize("fill_bytes", ["16", "64"])
 @pytest.mark.parametrize("func", ["ffill", "bfill"])
 def test_fill_long_nan_runs_cross_many_tiles(gpu_lib, monkeypatch, fill_bytes, func):
