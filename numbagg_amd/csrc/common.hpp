@@ -95,6 +95,29 @@ __device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
 
 template <typename T>
 __device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, d, kWave); }
+
+// DPP lane moves: cross-lane reads inside the VALU, without the LDS permute
+// path that __shfl takes. CTRL: row_shr:n = 0x110 + n (n = 1..15),
+// wave_shr:1 (whole wave, by one lane) = 0x138,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143; ROW_MASK selects the 16-lane
+// rows written. Lanes that are not written (source outside the row, or row
+// masked off) return `old`. The whole wave must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_move(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float old, float v) {
+  return __int_as_float(dpp_move<CTRL, ROW_MASK>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double old, double v) {
+  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL, ROW_MASK>((int)o, (int)x);
+  const int hi = dpp_move<CTRL, ROW_MASK>((int)(o >> 32), (int)(x >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
+}
+
 template <typename T>
 __device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, d, kWave); }
 template <typename T>

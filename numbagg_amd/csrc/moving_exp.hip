@@ -97,15 +97,32 @@ __device__ __forceinline__ Map<NS> compose(const Map<NS>& earlier, const Map<NS>
   return r;
 }
 
-template <int NS, int SW2>
-__device__ __forceinline__ Map<NS> shfl_up_map(const Map<NS>& m, int d) {
+// A map moved across lanes by DPP (CTRL / RM as for dpp_move); lanes the
+// move does not reach get the identity map.
+template <int CTRL, int RM, int NS, int SW2>
+__device__ __forceinline__ Map<NS> dpp_map(const Map<NS>& m) {
   Map<NS> r;
-  r.p = shfl_up(m.p, d);
-  r.p2 = SW2 >= 0 ? shfl_up(m.p2, d) : 1.0;
+  r.p = dpp_move<CTRL, RM>(1.0, m.p);
+  r.p2 = SW2 >= 0 ? dpp_move<CTRL, RM>(1.0, m.p2) : 1.0;
 #pragma unroll
-  for (int c = 0; c < NS; ++c) r.b[c] = shfl_up(m.b[c], d);
-  r.seen = shfl_up(m.seen, d);
+  for (int c = 0; c < NS; ++c) r.b[c] = dpp_move<CTRL, RM>(0.0, m.b[c]);
+  r.seen = dpp_move<CTRL, RM>(0, m.seen);
   return r;
+}
+
+// Inclusive wave scan of maps in lane order (DPP row shifts by 1, 2, 4, 8,
+// then broadcasts of lanes 15 and 31 into the later rows), and the exclusive
+// scan by one whole-wave shift (lane 0 gets the identity).
+template <int NS, int SW2>
+__device__ __forceinline__ void wave_scan_maps(const Map<NS>& m, Map<NS>& incl, Map<NS>& excl) {
+  incl = m;
+  incl = compose<NS, SW2>(dpp_map<0x111, 0xf, NS, SW2>(incl), incl);
+  incl = compose<NS, SW2>(dpp_map<0x112, 0xf, NS, SW2>(incl), incl);
+  incl = compose<NS, SW2>(dpp_map<0x114, 0xf, NS, SW2>(incl), incl);
+  incl = compose<NS, SW2>(dpp_map<0x118, 0xf, NS, SW2>(incl), incl);
+  incl = compose<NS, SW2>(dpp_map<0x142, 0xa, NS, SW2>(incl), incl);
+  incl = compose<NS, SW2>(dpp_map<0x143, 0xc, NS, SW2>(incl), incl);
+  excl = dpp_map<0x138, 0xf, NS, SW2>(incl);  // wave_shr:1
 }
 
 // One step of the reference recurrence on state s (components per ExpSpec):
@@ -311,14 +328,8 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
 
   // 2. exclusive block scan of maps (composition in position order).
   const int lane = lane_id(), wave = wave_id();
-  Map<NS> incl = m;
-#pragma unroll
-  for (int dd = 1; dd < kWave; dd <<= 1) {
-    Map<NS> o = shfl_up_map<NS, Sp::SW2>(incl, dd);
-    if (lane >= dd) incl = compose<NS, Sp::SW2>(o, incl);
-  }
-  Map<NS> excl = shfl_up_map<NS, Sp::SW2>(incl, 1);
-  if (lane == 0) excl = Map<NS>::identity();
+  Map<NS> incl, excl;
+  wave_scan_maps<NS, Sp::SW2>(m, incl, excl);
   if (lane == kWave - 1) s_wave[wave] = incl;
   __syncthreads();
   Map<NS> wprefix = Map<NS>::identity();

@@ -26,10 +26,14 @@ struct D {
   __device__ __forceinline__ double to_double() const { return v; }
 };
 
-__device__ __forceinline__ D shfl_up_acc(const D& a, int d) { return D{shfl_up(a.v, d)}; }
-template <typename F>
-__device__ __forceinline__ Comp<F> shfl_up_acc(const Comp<F>& a, int d) {
-  return Comp<F>{shfl_up(a.hi, d), shfl_up(a.lo, d)};
+// An accumulator moved across lanes by DPP (zero where the move writes nothing).
+template <int CTRL, int RM>
+__device__ __forceinline__ D dpp_acc(const D& a) {
+  return D{dpp_move<CTRL, RM>(0.0, a.v)};
+}
+template <int CTRL, int RM, typename F>
+__device__ __forceinline__ Comp<F> dpp_acc(const Comp<F>& a) {
+  return Comp<F>{dpp_move<CTRL, RM>(F(0), a.hi), dpp_move<CTRL, RM>(F(0), a.lo)};
 }
 
 // a*b added to an accumulator: exact product via fma for double-double.
@@ -97,8 +101,9 @@ struct SumStats {
     c -= o.c;
     s.sub(o.s);
   }
-  __device__ __forceinline__ SumStats up(int d) const {
-    return SumStats{shfl_up(c, d), shfl_up_acc(s, d)};
+  template <int CTRL, int RM>
+  __device__ __forceinline__ SumStats dpp() const {
+    return SumStats{dpp_move<CTRL, RM>(0, c), dpp_acc<CTRL, RM>(s)};
   }
 };
 
@@ -162,15 +167,16 @@ struct MomStats {
       if (SQB) s22.sub(o.s22);
     }
   }
-  __device__ __forceinline__ MomStats up(int d) const {
+  template <int CTRL, int RM>
+  __device__ __forceinline__ MomStats dpp() const {
     MomStats r = zero();
-    r.c = shfl_up(c, d);
-    r.s1 = shfl_up_acc(s1, d);
-    if (SQA) r.s11 = shfl_up_acc(s11, d);
+    r.c = dpp_move<CTRL, RM>(0, c);
+    r.s1 = dpp_acc<CTRL, RM>(s1);
+    if (SQA) r.s11 = dpp_acc<CTRL, RM>(s11);
     if (B) {
-      r.s2 = shfl_up_acc(s2, d);
-      r.s12 = shfl_up_acc(s12, d);
-      if (SQB) r.s22 = shfl_up_acc(s22, d);
+      r.s2 = dpp_acc<CTRL, RM>(s2);
+      r.s12 = dpp_acc<CTRL, RM>(s12);
+      if (SQB) r.s22 = dpp_acc<CTRL, RM>(s22);
     }
     return r;
   }
@@ -347,18 +353,24 @@ struct BlockShared {
   S wave_tot[kMoveWaves];
 };
 
-// Inclusive wave scan of S (sum-like merge).
+// Inclusive wave scan of S (sum-like merge) with DPP lane moves: row shifts
+// by 1, 2, 4 and 8 scan each 16-lane row, then broadcasts of lane 15 (into
+// rows 1 and 3) and lane 31 (into rows 2 and 3) carry row totals forward.
+// A lane the move does not reach merges the zero statistics.
+template <int CTRL, int RM, typename S>
+__device__ __forceinline__ S dpp_scan_step(const S& v) {
+  S o = v.template dpp<CTRL, RM>();
+  o.merge(v);
+  return o;
+}
 template <typename S>
 __device__ __forceinline__ S wave_incl_scan(S v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    S o = v.up(d);
-    if (lane >= d) {
-      o.merge(v);
-      v = o;
-    }
-  }
+  v = dpp_scan_step<0x111, 0xf>(v);
+  v = dpp_scan_step<0x112, 0xf>(v);
+  v = dpp_scan_step<0x114, 0xf>(v);
+  v = dpp_scan_step<0x118, 0xf>(v);
+  v = dpp_scan_step<0x142, 0xa>(v);
+  v = dpp_scan_step<0x143, 0xc>(v);
   return v;
 }
 
@@ -383,6 +395,32 @@ __device__ __forceinline__ S block_excl_scan(S v, S& total, BlockShared<S>& sh) 
   return prefix;
 }
 
+// Exclusive block scan of v fused with the block total of w, behind one
+// barrier: wave scans, then the wave totals merged in wave order through LDS. The caller must pass a barrier before `sh` or
+// `sh_w` is written again.
+template <typename S>
+__device__ __forceinline__ S block_excl_scan_and_total(S v, S w, S& w_total, BlockShared<S>& sh,
+                                                       BlockShared<S>& sh_w) {
+  S incl = wave_incl_scan(v);
+  const S w_incl = wave_incl_scan(w);
+  if (lane_id() == kWave - 1) {
+    sh.wave_tot[wave_id()] = incl;
+    sh_w.wave_tot[wave_id()] = w_incl;
+  }
+  __syncthreads();
+  S prefix = S::zero();
+  w_total = S::zero();
+#pragma unroll
+  for (int i = 0; i < kMoveWaves; ++i) {
+    if (i < wave_id()) prefix.merge(sh.wave_tot[i]);
+    w_total.merge(sh_w.wave_tot[i]);
+  }
+  S excl = incl;
+  excl.unmerge(v);
+  prefix.merge(excl);
+  return prefix;
+}
+
 template <typename S>
 __device__ __forceinline__ S block_reduce(S v, BlockShared<S>& sh) {
   S total;
@@ -401,6 +439,7 @@ struct alignas(16) Vec16 {
 template <typename S>
 struct RollingShared {
   BlockShared<S> sh;
+  BlockShared<S> sh_w;  // halo totals of the fused scan
   int first[2];
 };
 
@@ -517,7 +556,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       if (valid(a, b)) w0.add_obs(a, b, ka, kb);
     }
   }
-  w0 = block_reduce(w0, sh);
+  // float32 sum/mean fold the halo reduction into the delta scan below.
+  constexpr bool FAST = !Tr::SHIFT && std::is_same<typename S::Acc, D>::value;
+  if constexpr (!FAST) w0 = block_reduce(w0, sh);
 
   // 3. head/tail chunks and the per-thread delta. Chunks are re-read from LDS
   //    (or global memory for long windows) rather than held in registers.
@@ -549,7 +590,7 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     }
   };
   T res[K];
-  if constexpr (!Tr::SHIFT && std::is_same<typename S::Acc, D>::value) {
+  if constexpr (FAST) {
     // float32 sum/mean with float64 accumulation, branch-free: exact
     // differences and count deltas are formed first, then one running float64
     // sum and int32 count produce the outputs (1/c from the LDS table).
@@ -573,9 +614,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     S delta = S::zero();
     delta.c = dcnt;
     delta.s.v = dsum;
-    S total;
-    S st = block_excl_scan(delta, total, sh);
-    st.merge(w0);  // window ending at c0 - 1
+    S w0_total;
+    S st = block_excl_scan_and_total(delta, w0, w0_total, sh, rs.sh_w);
+    st.merge(w0_total);  // window ending at c0 - 1 (the tile's closing barrier guards sh)
     const int64_t mce = KIND == kSum ? min_count : (min_count > 1 ? min_count : 1);
     const int mc32 = mce > 0x7fffffffLL ? 0x7fffffff : (int)mce;
     int cc[K];
