@@ -50,7 +50,7 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   if (span > tiles) span = tiles;
   const int64_t spans = (tiles + span - 1) / span;
   const int64_t blocks = rows * spans;
-  if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move: too many tiles");
+  if (blocks > 0x7fffffffLL || tiles > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move: too many tiles");
   const bool tail_lds = window <= lds_window_limit<T, Tr::NARGS>() - (KP - default_items<T>()) * kMoveThreads;
   int vec = pick_vec<T>(a, ld_a, rows);
   if (Tr::NARGS == 2) {

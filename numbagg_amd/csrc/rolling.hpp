@@ -464,15 +464,18 @@ __host__ __device__ inline int64_t rolling_stage_bytes(int items, int64_t window
 // the row), TILE = 256 threads x KP items. `smem_raw` holds the staged segment;
 // `lut` (mean only, may be null) holds 1/c for c = 0..window in LDS; `staged`
 // leading segment elements are already in LDS (span kernels carry the halo).
-// Ends with a barrier so the caller may reuse the shared memory for its next
-// tile.
+// `full` tells whether the tile lies wholly inside the row (1 / 0; -1: test
+// here); callers that know it from 32-bit tile counts keep the test on the
+// scalar unit. Ends with a barrier so the caller may reuse the shared memory
+// for its next tile.
 template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false>
 __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* __restrict__ rb,
                                              T* __restrict__ ro, int64_t n, int64_t window,
                                              int64_t min_count, int64_t tile,
                                              unsigned char* smem_raw,
                                              RollingShared<typename Traits<T, KIND>::S>& rs,
-                                             const double* lut = nullptr, int staged = 0) {
+                                             const double* lut = nullptr, int staged = 0,
+                                             int full = -1) {
   using Tr = Traits<T, KIND>;
   using S = typename Tr::S;
   constexpr int K = KP;
@@ -708,7 +711,20 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       res[j] = Tr::out(st, min_count, nullptr);
     }
   }
-  if (c0 + K <= n && VEC > 1) {
+  const bool tile_full = full >= 0 ? full != 0 : t0 + TILE <= n;
+  if (VEC > 1 && tile_full) {
+    // full tile (block-uniform test): no per-thread checks, 32-bit offsets
+    // from the tile's base
+    char* rot = reinterpret_cast<char*>(ro + t0);
+    const uint32_t b0 = (uint32_t)tid * (uint32_t)(K * sizeof(T));
+#pragma unroll
+    for (int j = 0; j < K; j += VEC) {
+      Vec16<T> vo;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) vo.v[v] = res[j + v];
+      *reinterpret_cast<Vec16<T>*>(rot + (b0 + (uint32_t)(j * sizeof(T)))) = vo;
+    }
+  } else if (c0 + K <= n && VEC > 1) {
 #pragma unroll
     for (int j = 0; j < K; j += VEC) {
       Vec16<T> vo;
@@ -778,11 +794,14 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(mo
   __shared__ RollingShared<typename Tr::S> rs;
   // Consecutive logical ids walk consecutive spans of a row on one XCD, so a
   // span's first halo (the end of the previous span) is read from that XCD's L2.
-  const int64_t spans_per_row = (tiles_per_row + span - 1) / span;
+  // Tile counts fit 32 bits (the host bounds rows x tiles by 2**31), so the
+  // span loop and its full-tile tests run on the scalar unit.
+  const uint32_t spans_per_row = (uint32_t)((tiles_per_row + span - 1) / span);
   const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
-  const int64_t row = (int64_t)(v / (uint64_t)spans_per_row);
-  const int64_t t_begin = (int64_t)(v % (uint64_t)spans_per_row) * span;
-  const int64_t t_end = t_begin + span < tiles_per_row ? t_begin + span : tiles_per_row;
+  const int64_t row = (int64_t)(v / spans_per_row);
+  const int t_begin = (int)(v % spans_per_row) * (int)span;
+  const int t_end = t_begin + (int)span < (int)tiles_per_row ? t_begin + (int)span : (int)tiles_per_row;
+  const int full_tiles = (int)(n / TILE);  // tiles lying wholly inside the row
   double* lut = nullptr;
   if constexpr (LUT) {
     lut = reinterpret_cast<double*>(smem_raw + rolling_stage_bytes<T>(KP, window, TAIL_LDS, Tr::NARGS));
@@ -805,8 +824,26 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(mo
   // itself: offsets H + tid*VEC + m*256*VEC + v.
   constexpr int NV = KP / VEC;  // vectors of VEC elements per thread and input
   T pa[KP], pb[(Tr::NARGS == 2) ? KP : 1];
-  auto prefetch = [&](int64_t next_tile) {
-    const int64_t t0n = next_tile * TILE;
+  auto prefetch = [&](int next_tile) {
+    const int64_t t0n = (int64_t)next_tile * TILE;
+    if (VEC > 1 && next_tile < full_tiles) {
+      // full tile (block-uniform test): no checks, 32-bit offsets
+      const char* rat = reinterpret_cast<const char*>(ra + t0n);
+      const char* rbt = (Tr::NARGS == 2) ? reinterpret_cast<const char*>(rb + t0n) : nullptr;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const uint32_t o = (threadIdx.x + (uint32_t)m * kMoveThreads) * 16u;  // bytes
+        const Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(rat + o);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) pa[m * VEC + v] = va.v[v];
+        if constexpr (Tr::NARGS == 2) {
+          const Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rbt + o);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) pb[m * VEC + v] = vb.v[v];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
       const int64_t p = t0n + ((int64_t)threadIdx.x + (int64_t)m * kMoveThreads) * VEC;
@@ -830,11 +867,11 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(mo
     }
   };
   int staged = 0;
-  for (int64_t tile = t_begin; tile < t_end; ++tile) {
+  for (int tile = t_begin; tile < t_end; ++tile) {
     const bool more = carry && tile + 1 < t_end;
     if (PF && more) prefetch(tile + 1);
     rolling_tile<T, KIND, TAIL_LDS, VEC, KP, LUT>(ra, rb, ro, n, window, min_count, tile, smem_raw,
-                                                  rs, lut, staged);
+                                                  rs, lut, staged, tile < full_tiles ? 1 : 0);
     if (more) {
       // The next tile's segment starts TILE elements later: its first H
       // elements are this segment's last H (source and destination disjoint,
