@@ -374,30 +374,9 @@ __device__ __forceinline__ S wave_incl_scan(S v) {
   return v;
 }
 
-// Exclusive block scan; also returns the block total.
-template <typename S>
-__device__ __forceinline__ S block_excl_scan(S v, S& total, BlockShared<S>& sh) {
-  S incl = wave_incl_scan(v);
-  if (lane_id() == kWave - 1) sh.wave_tot[wave_id()] = incl;
-  __syncthreads();
-  S prefix = S::zero();
-  total = S::zero();
-#pragma unroll
-  for (int w = 0; w < kMoveWaves; ++w) {
-    if (w < wave_id()) prefix.merge(sh.wave_tot[w]);
-    total.merge(sh.wave_tot[w]);
-  }
-  __syncthreads();
-  // exclusive = prefix + (incl - own)
-  S excl = incl;
-  excl.unmerge(v);
-  prefix.merge(excl);
-  return prefix;
-}
-
 // Exclusive block scan of v fused with the block total of w, behind one
-// barrier: wave scans, then the wave totals merged in wave order through LDS. The caller must pass a barrier before `sh` or
-// `sh_w` is written again.
+// barrier: wave scans, then the wave totals merged in wave order through
+// LDS. The caller must pass a barrier before `sh` or `sh_w` is written again.
 template <typename S>
 __device__ __forceinline__ S block_excl_scan_and_total(S v, S w, S& w_total, BlockShared<S>& sh,
                                                        BlockShared<S>& sh_w) {
@@ -421,14 +400,6 @@ __device__ __forceinline__ S block_excl_scan_and_total(S v, S w, S& w_total, Blo
   return prefix;
 }
 
-template <typename S>
-__device__ __forceinline__ S block_reduce(S v, BlockShared<S>& sh) {
-  S total;
-  (void)block_excl_scan(v, total, sh);
-  return total;
-}
-
-
 template <typename T>
 struct alignas(16) Vec16 {
   T v[16 / sizeof(T)];
@@ -440,12 +411,8 @@ template <typename S>
 struct RollingShared {
   BlockShared<S> sh;
   BlockShared<S> sh_w;  // halo totals of the fused scan
-  int first[2];
 };
 
-// One output tile [tile*TILE, tile*TILE + TILE) of one row (ra/rb/ro point at
-// the row). `smem_raw` holds the staged segment. Ends with a barrier so the
-// caller may reuse the shared memory for its next tile.
 // Default outputs per thread: 16 float32 / 8 float64 (64 bytes of input).
 template <typename T>
 constexpr int default_items() {
@@ -458,6 +425,18 @@ template <typename T>
 __host__ __device__ inline int64_t rolling_stage_bytes(int items, int64_t window, bool tail_lds, int nargs) {
   const int64_t seg = (int64_t)kMoveThreads * items + (tail_lds ? window + 16 : 0);
   return ((pad16(seg) + 16) * (int64_t)sizeof(T) * nargs + 15) / 16 * 16;
+}
+
+// The first non-NaN value among staged segment elements e0 .. e0 + 255 (0 if
+// none): one LDS read and a ballot per 64 elements; wave-uniform result.
+template <typename T>
+__device__ __forceinline__ T first_valid_staged(const T* seg, int e0) {
+#pragma unroll 1
+  for (int q = 0; q < kMoveThreads; q += kWave) {
+    const unsigned long long m = __ballot(!is_nan(seg[pad16(e0 + q + lane_id())]));
+    if (m != 0ull) return seg[pad16(e0 + q + __ffsll((long long)m) - 1)];
+  }
+  return T(0);
 }
 
 // One output tile [tile*TILE, tile*TILE + TILE) of one row (ra/rb/ro point at
@@ -495,9 +474,26 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   T* la = reinterpret_cast<T*>(smem_raw);
   T* lb = la + lds_elems;
 
+  const bool tile_full = full >= 0 ? full != 0 : t0 + TILE <= n;
+
   // 1. stage the segment (wave-striped vector loads, NaN past the row end).
   //    The first `staged` elements (a multiple of 16) may already be in LDS:
-  //    the halo carried over from the previous tile of a span.
+  //    the halo carried over from the previous tile of a span. A segment
+  //    wholly inside the row (block-uniform test) is staged without checks.
+  if (VEC > 1 && tile_full && s0a >= 0) {
+    const T* rsa = ra + s0a;
+    const T* rsb = (Tr::NARGS == 2) ? rb + s0a : nullptr;
+    for (int e = staged + tid * VEC; e < (int)seg_len; e += kMoveThreads * VEC) {
+      const Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(rsa + e);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
+      if (Tr::NARGS == 2) {
+        const Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rsb + e);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
+      }
+    }
+  } else
   for (int64_t e = (int64_t)staged + (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
     const int64_t p = s0a + e;
     if (VEC > 1 && p >= 0 && p + VEC <= n) {
@@ -524,18 +520,14 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   // Per-tile shifts for the moment kinds: the first valid value among the tile's
   // first 256 inputs (any constant works; one near the data keeps the float64
   // moments small).
+  // Every wave finds them itself from the staged segment (64 inputs per
+  // ballot), so no further barrier is needed.
   T ka = T(0), kb = T(0);
   if (Tr::SHIFT) {
-    int* s_first = rs.first;
-    if (tid < 2) s_first[tid] = 0x7fffffff;
-    __syncthreads();
     // first 256 inputs of the tile itself (always staged; NaN past the row end)
-    const int e = (int)(t0 - s0a) + tid;
-    if (!is_nan(la[pad16(e)])) atomicMin(&s_first[0], e);
-    if (Tr::NARGS == 2 && !is_nan(lb[pad16(e)])) atomicMin(&s_first[1], e);
-    __syncthreads();
-    ka = s_first[0] != 0x7fffffff ? la[pad16(s_first[0])] : T(0);
-    if (Tr::NARGS == 2) kb = s_first[1] != 0x7fffffff ? lb[pad16(s_first[1])] : T(0);
+    const int e0 = (int)(t0 - s0a);
+    ka = first_valid_staged(la, e0);
+    if (Tr::NARGS == 2) kb = first_valid_staged(lb, e0);
   }
 
   auto valid = [&](T a, T b) -> bool {
@@ -559,9 +551,8 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       if (valid(a, b)) w0.add_obs(a, b, ka, kb);
     }
   }
-  // float32 sum/mean fold the halo reduction into the delta scan below.
+  // The halo reduction is folded into the delta scan below (one barrier).
   constexpr bool FAST = !Tr::SHIFT && std::is_same<typename S::Acc, D>::value;
-  if constexpr (!FAST) w0 = block_reduce(w0, sh);
 
   // 3. head/tail chunks and the per-thread delta. Chunks are re-read from LDS
   //    (or global memory for long windows) rather than held in registers.
@@ -677,9 +668,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       delta.s.add(dj[j]);
       delta.c += (int)hv - (int)tv;
     }
-    S total;
-    S st = block_excl_scan(delta, total, sh);
-    st.merge(w0);  // window ending at c0 - 1
+    S w0_total;
+    S st = block_excl_scan_and_total(delta, w0, w0_total, sh, rs.sh_w);
+    st.merge(w0_total);  // window ending at c0 - 1
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       st.c += (int)((hbits >> j) & 1ull) - (int)((tbits >> j) & 1ull);
@@ -696,9 +687,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       tail(j, a, b);
       if (valid(a, b)) delta.sub_obs(a, b, ka, kb);
     }
-    S total;
-    S st = block_excl_scan(delta, total, sh);
-    st.merge(w0);  // window ending at c0 - 1
+    S w0_total;
+    S st = block_excl_scan_and_total(delta, w0, w0_total, sh, rs.sh_w);
+    st.merge(w0_total);  // window ending at c0 - 1
 
     // 4. sequential add/remove over the thread's chunk.
 #pragma unroll
@@ -711,7 +702,6 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       res[j] = Tr::out(st, min_count, nullptr);
     }
   }
-  const bool tile_full = full >= 0 ? full != 0 : t0 + TILE <= n;
   if (VEC > 1 && tile_full) {
     // full tile (block-uniform test): no per-thread checks, 32-bit offsets
     // from the tile's base
