@@ -20,8 +20,9 @@
 //
 // Inside a tile, loads are wave-striped vectors (every load instruction of a wave
 // covers 64*VEC contiguous elements); the carry "last valid (offset, value)" is
-// scanned per 64*VEC row with wave shuffles, across waves through LDS and across
-// tiles through one 64-bit status word per tile:
+// found per 64*VEC row from a ballot of the lanes holding a valid value,
+// across waves through LDS and across tiles through one 64-bit status word per
+// tile:
 //   bits 63..62  flag (1 = aggregate: tile holds no valid value;
 //                      2 = inclusive prefix)
 //   bits 61..0   1 + scan-order index of the last valid value (0 = none).
@@ -62,20 +63,50 @@ struct Last {
   T val;
 };
 
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int lane);
+template <>
+__device__ __forceinline__ int read_lane<int>(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+template <>
+__device__ __forceinline__ float read_lane<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
+__device__ __forceinline__ double read_lane<double>(double v, int lane) {
+  const long long x = __double_as_longlong(v);
+  const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)x, lane);
+  const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(x >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Wave step of the carry scan for one slot. Each lane holds `agg`, the last
+// valid value of its own elements (so < 0: none); scan order runs up the
+// lanes (ffill) or down them (bfill, REV), and scan offsets are monotonic in
+// the lane. A ballot of the lanes holding a valid value gives every lane its
+// nearest earlier such lane (`excl`, so < 0: none) with two lane reads, and
+// the wave's last one in scan order (`tail`) with two scalar reads.
 template <bool REV, typename T>
-__device__ __forceinline__ Last<T> wave_incl_last(Last<T> a) {
+__device__ __forceinline__ void wave_last_carry(const Last<T>& agg, Last<T>& excl, Last<T>& tail) {
   const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    int oso = REV ? shfl_down(a.so, d) : shfl_up(a.so, d);
-    T oval = REV ? shfl_down(a.val, d) : shfl_up(a.val, d);
-    bool in = REV ? (lane + d < kWave) : (lane >= d);
-    if (in && oso > a.so) {
-      a.so = oso;
-      a.val = oval;
-    }
-  }
-  return a;
+  const unsigned long long m = __ballot(agg.so >= 0);
+  unsigned long long before;
+  if (REV) before = lane == kWave - 1 ? 0ull : m & (~0ull << (lane + 1));
+  else before = m & ((1ull << lane) - 1ull);
+  const int src = before == 0ull ? -1 : (REV ? __ffsll((long long)before) - 1 : 63 - __clzll((long long)before));
+  // The lane reads run on every lane (a lane read from a lane outside the
+  // active set returns 0), and the result is selected afterwards.
+  const int from = src < 0 ? lane : src;
+  const int oso = __shfl(agg.so, from, kWave);
+  const T oval = __shfl(agg.val, from, kWave);
+  excl.so = src < 0 ? -1 : oso;
+  excl.val = oval;
+  const int t = m == 0ull ? 0 : (REV ? __ffsll((long long)m) - 1 : 63 - __clzll((long long)m));
+  const int tso = read_lane(agg.so, t);
+  const T tval = read_lane(agg.val, t);
+  tail.so = m == 0ull ? -1 : tso;
+  tail.val = tval;
 }
 
 // Dispatch order of a row's tiles: runs of `run` consecutive tiles of one row,
@@ -192,16 +223,10 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
         }
       }
     }
-    Last<T> incl = wave_incl_last<REV>(agg);
-    Last<T> excl;
-    excl.so = REV ? shfl_down(incl.so, 1) : shfl_up(incl.so, 1);
-    excl.val = REV ? shfl_down(incl.val, 1) : shfl_up(incl.val, 1);
-    const bool first_lane = REV ? (l == kWave - 1) : (l == 0);
-    if (first_lane || excl.so < 0) excl = wcarry;
+    Last<T> excl, tail;
+    wave_last_carry<REV>(agg, excl, tail);
+    if (excl.so < 0) excl = wcarry;
     cin[u] = excl;
-    Last<T> tail;
-    tail.so = shfl(incl.so, REV ? 0 : kWave - 1);
-    tail.val = shfl(incl.val, REV ? 0 : kWave - 1);
     if (tail.so >= 0) wcarry = tail;
   }
   if (l == 0) {
@@ -442,16 +467,10 @@ __global__ __launch_bounds__(kFillThreads) void fill_span_kernel(
           wfirst = so < wfirst ? so : wfirst;
         }
       }
-      Last<T> incl = wave_incl_last<REV>(agg);
-      Last<T> excl;
-      excl.so = REV ? shfl_down(incl.so, 1) : shfl_up(incl.so, 1);
-      excl.val = REV ? shfl_down(incl.val, 1) : shfl_up(incl.val, 1);
-      const bool first_lane = REV ? (l == kWave - 1) : (l == 0);
-      if (first_lane || excl.so < 0) excl = wcarry;
+      Last<T> excl, tail;
+      wave_last_carry<REV>(agg, excl, tail);
+      if (excl.so < 0) excl = wcarry;
       cin[u] = excl;
-      Last<T> tail;
-      tail.so = shfl(incl.so, REV ? 0 : kWave - 1);
-      tail.val = shfl(incl.val, REV ? 0 : kWave - 1);
       if (tail.so >= 0) wcarry = tail;
     }
 #pragma unroll
