@@ -49,7 +49,7 @@ enum GPass {
   kPKeyIdx = 4,   // min index among elements whose key equals the max
   kPFirst = 5,    // min index
   kPLast = 6,     // max index + 1
-  kPDev = 7,      // deviations from the label mean: sum d, sum d^2
+  kPDev = 7,      // sum of squared deviations from the label mean
 };
 
 template <typename T>
@@ -139,8 +139,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   SA* g_s = static_cast<SA*>(A.acc0) + row0 * L1;
   unsigned long long* g_u0 = static_cast<unsigned long long*>(A.acc0) + row0 * L1;
   unsigned long long* g_u1 = static_cast<unsigned long long*>(A.acc1) + row0 * L1;
-  double* g_d0 = static_cast<double*>(A.acc0) + row0 * L1;
-  double* g_d1 = static_cast<double*>(A.acc1) + row0 * L1;
+  double* g_sq = static_cast<double*>(A.acc1) + row0 * L1;
   const double* g_mean = A.means ? A.means + row0 * L1 : nullptr;
 
   // LDS accumulators.
@@ -148,9 +147,8 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   unsigned int* l_c = reinterpret_cast<unsigned int*>(smem_raw + (PASS == kPSum ? L * sizeof(SA) : 0));
   unsigned long long* l_u = reinterpret_cast<unsigned long long*>(smem_raw);
   unsigned int* l_u32 = reinterpret_cast<unsigned int*>(smem_raw);
-  double* l_d0 = reinterpret_cast<double*>(smem_raw);
-  double* l_d1 = l_d0 + L;
-  double* l_mean = l_d0 + 2 * L;
+  double* l_sq = reinterpret_cast<double*>(smem_raw);
+  double* l_mean = l_sq + L;
   unsigned long long* l_key = l_u + L;  // kPKeyIdx: staged per-label maxima
   const int64_t Lp = (int64_t)nr * L1;  // slots actually used
   if (LDS) {
@@ -168,8 +166,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       } else if constexpr (PASS == kPFirst) {
         l_u[l] = ~0ull;
       } else if constexpr (PASS == kPDev) {
-        l_d0[l] = 0.0;
-        l_d1[l] = 0.0;
+        l_sq[l] = 0.0;
         l_mean[l] = g_mean[l];
       }
     }
@@ -226,13 +223,8 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       else atomicMax(&g_u0[l], (unsigned long long)(i + 1));
     } else if constexpr (PASS == kPDev) {
       const double d = (double)v - (LDS ? l_mean[l] : g_mean[l]);
-      if (LDS) {
-        atomicAdd(&l_d0[l], d);
-        atomicAdd(&l_d1[l], d * d);
-      } else {
-        unsafeAtomicAdd(&g_d0[l], d);
-        unsafeAtomicAdd(&g_d1[l], d * d);
-      }
+      if (LDS) atomicAdd(&l_sq[l], d * d);
+      else unsafeAtomicAdd(&g_sq[l], d * d);
     }
   };
 
@@ -342,10 +334,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       } else if constexpr (PASS == kPFirst) {
         if (l_u[l] != ~0ull) atomicMin(&g_u0[l], l_u[l]);
       } else if constexpr (PASS == kPDev) {
-        if (l_d1[l] != 0.0 || l_d0[l] != 0.0) {
-          unsafeAtomicAdd(&g_d0[l], l_d0[l]);
-          unsafeAtomicAdd(&g_d1[l], l_d1[l]);
-        }
+        if (l_sq[l] != 0.0) unsafeAtomicAdd(&g_sq[l], l_sq[l]);
       }
     }
   }
@@ -409,15 +398,13 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
     } else if constexpr (KIND == kGLast) {
       const unsigned long long i = static_cast<const unsigned long long*>(A.acc0)[q];
       r = i ? vals[row * A.ld_v + (int64_t)(i - 1)] : missing_value<TV>();
-    } else {  // var / std from (count, sum d, sum d^2)
+    } else {  // var / std from (count, sum of squared deviations)
       const unsigned long long c = static_cast<const unsigned long long*>(A.acc2)[q];
-      const double d1 = static_cast<const double*>(A.acc0)[q];
-      const double d2 = static_cast<const double*>(A.acc1)[q];
+      const double m2 = static_cast<const double*>(A.acc1)[q];
       const double den = (double)((long long)c - A.ddof);
       if (!(den > 0)) {
         r = missing_value<TV>();
       } else {
-        const double m2 = d2 - d1 * d1 / (double)c;
         const double var = m2 / den;
         r = (TV)(KIND == kGStd ? sqrt(var) : var);
       }
@@ -434,7 +421,7 @@ int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
   switch (pass) {
     case kPSum: return sa_bytes + (count ? 4 : 0);
     case kPCount: return 4;
-    case kPDev: return 24;
+    case kPDev: return 16;
     case kPKeyIdx: return 16;
     default: return 8;
   }
@@ -538,7 +525,7 @@ int64_t lds_bytes_per_label(int kind, int vdtype) {
     case kGMean: return 12;
     case kGCount: return 4;
     case kGVar:
-    case kGStd: return 24;
+    case kGStd: return 16;
     case kGArgMax:
     case kGArgMin: return 16;
     default: return 8;
@@ -618,7 +605,10 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
       a.acc1 = nullptr;
       if ((rc = run_pass<TV, TL, kPLast>(a, lds, s))) return rc;
       return launch_finalize<TV, KIND>(a, nullptr, s);
-    default: {  // var / std: per-label means, then sums of deviations
+    default: {  // var / std: per-label means, then squared deviations
+      // The float64 means come from float64 sums, so the deviations' own sum
+      // (the usual two-pass correction term) is negligible: with n values
+      // per label its share of sum d^2 is about (n 2**-53 |mean| / std)^2.
       if ((rc = launch_init(acc0, 0ull, acc1, 0ull, per, s))) return rc;
       a.acc0 = acc0;
       a.acc1 = acc1;
@@ -627,10 +617,9 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
       if ((rc = launch_finalize<TV, KIND>(a, means, s))) return rc;
       GArgs b = a;
       b.means = means;
-      b.acc0 = acc0;  // sum of deviations (overwrites the sums)
       b.acc1 = acc3;  // sum of squared deviations
       b.acc2 = acc1;  // counts from the first pass
-      if ((rc = launch_init(acc0, 0ull, acc3, 0ull, per, s))) return rc;
+      if ((rc = launch_init(acc3, 0ull, nullptr, 0ull, per, s))) return rc;
       if ((rc = run_pass<TV, TL, kPDev>(b, lds, s))) return rc;
       return launch_finalize<TV, KIND>(b, nullptr, s);
     }

@@ -4,7 +4,8 @@ count, argmax, argmin, first and last are bit-exact against the oracles (C
 restatement and closed form). Float sums/means are compared to the float64
 oracle: float32 outputs within 1 float32 ULP of the float64 result (float64
 accumulation, one final rounding), float64 outputs within 1e-13 relative of
-the sum of |x|; var/std within 1e-12 relative (float64 corrected two-pass).
+the sum of |x|; var/std within 1e-12 relative (float64 two-pass: means, then
+squared deviations).
 Both group-by strategies (LDS-privatised and global atomics) are covered.
 """
 
@@ -127,6 +128,22 @@ def test_group_sum_pass_shapes(gpu_lib, monkeypatch, rows, unroll, pipe, func):
         v, lab = _data(9, 300_000, 1000, dtype, seed=int(rows) * 10 + int(unroll), zipf=True)
         got = getattr(nb, func)(v, lab, num_labels=1000, axis=-1)
         _check_float(func, dtype, v, lab, 1000, got)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_group_var_offset_data(gpu_lib, dtype):
+    """Data far from zero (mean 1e4, unit spread): the per-label means come
+    from float64 sums, so the squared deviations keep the variance to the
+    same bars as for centred data."""
+    import numbagg_amd as nb
+
+    rng = np.random.default_rng(7)
+    v = (1e4 + rng.standard_normal((3, 200_000))).astype(dtype)
+    v[rng.random(v.shape) < 0.05] = np.nan
+    lab = rng.integers(-1, 300, 200_000).astype(np.int32)
+    for func in ("group_nanvar", "group_nanstd"):
+        got = getattr(nb, func)(v, lab, num_labels=300, axis=-1)
+        _check_float(func, dtype, v, lab, 300, got)
 
 
 def test_group_ddof(gpu_lib):
