@@ -173,6 +173,33 @@ __device__ __forceinline__ void exp_step(double* s, int& seen, double a, double 
   }
 }
 
+// Division by a shared reciprocal. r: 1/b from the hardware estimate
+// refined by one Newton step (about 1 ULP of float64); div_rn then corrects
+// a * r with the exact remainder a - b * (a * r) from one fused multiply-add,
+// which gives the correctly rounded quotient a / b (the reference's division)
+// except in rare near-tie cases, where it is 1 ULP off. Valid while
+// |a|, |b| and |1/b| stay within 2**500 (no overflow, no subnormals).
+__device__ __forceinline__ double recip_refined(double b) {
+  const double r = __builtin_amdgcn_rcp(b);
+  return fma(r, fma(-b, r, 1.0), r);
+}
+__device__ __forceinline__ double div_rn(double a, double b, double r) {
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ bool div_range(double x) { return fabs(x) <= 0x1p500; }
+__device__ __forceinline__ bool recip_range(double b) { return fabs(b) >= 0x1p-500 && fabs(b) <= 0x1p500; }
+// 1/sqrt(v) for v in [2**-1000, 2**1000]: estimate plus one Newton step.
+__device__ __forceinline__ double rsqrt_refined(double v) {
+  const double y = __builtin_amdgcn_rsq(v);
+  return y * fma(-(0.5 * v) * y, y, 1.5);
+}
+
+// Outputs in the reference's formulas and operation order. corr, the
+// costliest, divides by the weight sum through one shared reciprocal with
+// corrected quotients (div_rn) when every operand is in range — the common
+// case — and forms cov / sqrt(v1 v2) with a refined 1/sqrt; the other kinds
+// keep ordinary divisions (a shared reciprocal measured no faster there).
 template <int KIND>
 __device__ __forceinline__ double exp_out(const double* s, int seen, double mw) {
   const double nan = canonical_nan<double>();
@@ -193,12 +220,22 @@ __device__ __forceinline__ double exp_out(const double* s, int seen, double mw) 
     return (s[5] >= mw && bias > 0 && seen >= 2) ? cov_biased / bias : nan;
   }
   // kECorr
-  const double cov = s[2] - s[0] * s[1] / s[3];
-  const double v1 = s[6] - s[0] * s[0] / s[3];
-  const double v2 = s[7] - s[1] * s[1] / s[3];
+  const double p01 = s[0] * s[1], p00 = s[0] * s[0], p11 = s[1] * s[1];
+  double cov, v1, v2;
+  if (recip_range(s[3]) && div_range(p01) && div_range(p00) && div_range(p11)) {
+    const double r = recip_refined(s[3]);
+    cov = s[2] - div_rn(p01, s[3], r);
+    v1 = s[6] - div_rn(p00, s[3], r);
+    v2 = s[7] - div_rn(p11, s[3], r);
+  } else {
+    cov = s[2] - p01 / s[3];
+    v1 = s[6] - p00 / s[3];
+    v2 = s[7] - p11 / s[3];
+  }
   if (!(s[5] >= mw && bias > 0 && seen >= 2)) return nan;
-  const double den = sqrt(v1 * v2);
-  return den > 0 ? cov / den : nan;
+  const double vv = v1 * v2;
+  if (!(vv > 0)) return nan;
+  return (vv >= 0x1p-1000 && vv <= 0x1p1000) ? cov * rsqrt_refined(vv) : cov / sqrt(vv);
 }
 
 
