@@ -63,24 +63,6 @@ struct Last {
   T val;
 };
 
-template <typename T>
-__device__ __forceinline__ T read_lane(T v, int lane);
-template <>
-__device__ __forceinline__ int read_lane<int>(int v, int lane) {
-  return __builtin_amdgcn_readlane(v, lane);
-}
-template <>
-__device__ __forceinline__ float read_lane<float>(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-template <>
-__device__ __forceinline__ double read_lane<double>(double v, int lane) {
-  const long long x = __double_as_longlong(v);
-  const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)x, lane);
-  const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(x >> 32), lane);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // Wave step of the carry scan for one slot. Each lane holds `agg`, the last
 // valid value of its own elements (so < 0: none); scan order runs up the
 // lanes (ffill) or down them (bfill, REV), and scan offsets are monotonic in

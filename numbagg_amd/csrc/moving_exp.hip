@@ -17,9 +17,11 @@
 //   * the tile's incoming state comes from a decoupled look-back over earlier
 //     tiles of the row (tile order is fixed by per-row tickets, so a tile only
 //     waits on tiles that are already running), which publish their aggregate
-//     map first and their inclusive state as soon as it is known;
-//   * each thread re-runs its K positions from its exact incoming state, with the
-//     reference's operation order (state *= decay; state += contribution).
+//     map first and their inclusive state as soon as it is known; one wave
+//     examines 64 predecessors per pass;
+//   * each thread re-runs its K positions from its exact incoming state, one
+//     fused multiply-add per state component (state = decay * state +
+//     contribution).
 // Squares and cross products of float32 inputs are formed in float64 (exact),
 // where the reference rounds them to float32 first.
 #include "common.hpp"
@@ -107,6 +109,18 @@ __device__ __forceinline__ Map<NS> dpp_map(const Map<NS>& m) {
 #pragma unroll
   for (int c = 0; c < NS; ++c) r.b[c] = dpp_move<CTRL, RM>(0.0, m.b[c]);
   r.seen = dpp_move<CTRL, RM>(0, m.seen);
+  return r;
+}
+
+// A map as held by one lane (wave-uniform index).
+template <int NS>
+__device__ __forceinline__ Map<NS> read_lane_map(const Map<NS>& m, int lane) {
+  Map<NS> r;
+  r.p = read_lane(m.p, lane);
+  r.p2 = read_lane(m.p2, lane);
+#pragma unroll
+  for (int c = 0; c < NS; ++c) r.b[c] = read_lane(m.b[c], lane);
+  r.seen = read_lane(m.seen, lane);
   return r;
 }
 
@@ -373,8 +387,102 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   for (int w = 0; w < wave; ++w) wprefix = compose<NS, Sp::SW2>(wprefix, s_wave[w]);
   excl = compose<NS, Sp::SW2>(wprefix, excl);
 
-  // 3. decoupled look-back for the tile's incoming state (thread 0).
-  if (tid == 0) {
+  // 3. decoupled look-back for the tile's incoming state, by wave 0: each
+  //    pass reads the flags of the 64 tiles before the latest unconsumed one
+  //    at once (later tiles in higher lanes), composes the run of published
+  //    aggregates above the highest inclusive state or unpublished tile with
+  //    one wave scan, and stops at the inclusive state (a zero state before
+  //    the row start). Used for the small maps of count/mean/sum; the
+  //    moment kinds' larger maps make the wave scan cost more registers and
+  //    time than the walk it saves.
+  constexpr bool WAVE_LOOKBACK = NS <= 3;
+  if (WAVE_LOOKBACK && wave == 0) {
+    Map<NS> agg = Map<NS>::identity();
+    for (int w = 0; w < kExpWaves; ++w) agg = compose<NS, Sp::SW2>(agg, s_wave[w]);
+    unsigned int* frow = A.flags + row * A.tiles_per_row;
+    double* prow = A.payload + row * A.tiles_per_row * kPayload;
+    double sin[NS];
+#pragma unroll
+    for (int c = 0; c < NS; ++c) sin[c] = 0.0;
+    int sin_seen = 0;
+    if (tile > 0) {
+      if (lane == 0) {  // publish the aggregate map
+        double* pl = prow + tile * kPayload;
+        st_agent_f64(pl + 0, agg.p);
+        st_agent_f64(pl + 1, agg.p2);
+#pragma unroll
+        for (int c = 0; c < NS; ++c) st_agent_f64(pl + 2 + c, agg.b[c]);
+        st_agent_f64(pl + 2 + NS, (double)agg.seen);
+        st_flag(&frow[tile], 1u);
+      }
+      Map<NS> acc = Map<NS>::identity();  // maps of the tiles (top, tile) composed
+      int64_t top = tile - 1;              // latest tile not yet composed
+      unsigned int spins = 0;
+      while (true) {
+        const int64_t j = top - (kWave - 1) + lane;
+        const unsigned int f = j >= 0 ? ld_flag(&frow[j]) : 2u;
+        const unsigned long long ready = __ballot(f != 0u);
+        const unsigned long long incl = __ballot(f == 2u);
+        const int hnr = ready == ~0ull ? -1 : 63 - __clzll((long long)~ready);  // highest unpublished
+        const int hp = incl == 0ull ? -1 : 63 - __clzll((long long)incl);      // highest inclusive
+        if (hnr == kWave - 1) {  // the latest tile has not published yet
+          spin_pause();
+          if (++spins > kSpinLimit) {
+            if (lane == 0) atomicAdd(A.timeouts, 1u);
+            acc = Map<NS>::identity();  // expired wait (counted): continue from zero state
+            break;
+          }
+          continue;
+        }
+        // lanes above lo hold published aggregates: compose them in tile order
+        const int lo = hp > hnr ? hp : hnr;
+        Map<NS> mj = Map<NS>::identity();
+        if (lane > lo) {
+          const double* pj = prow + j * kPayload;
+          mj.p = ld_agent_f64(pj + 0);
+          mj.p2 = ld_agent_f64(pj + 1);
+#pragma unroll
+          for (int c = 0; c < NS; ++c) mj.b[c] = ld_agent_f64(pj + 2 + c);
+          mj.seen = (int)ld_agent_f64(pj + 2 + NS);
+        }
+        Map<NS> run, unused;
+        wave_scan_maps<NS, Sp::SW2>(mj, run, unused);
+        acc = compose<NS, Sp::SW2>(read_lane_map(run, kWave - 1), acc);
+        if (hp > hnr) {  // inclusive state of tile top - 63 + hp (zero before the row)
+          const int64_t jp = top - (kWave - 1) + hp;
+          double sj[NS];
+          int seen_j = 0;
+#pragma unroll
+          for (int c = 0; c < NS; ++c) sj[c] = 0.0;
+          if (jp >= 0) {
+            const double* pj = prow + jp * kPayload;
+#pragma unroll
+            for (int c = 0; c < NS; ++c) sj[c] = ld_agent_f64(pj + kMaxNS + 3 + c);
+            seen_j = (int)ld_agent_f64(pj + kMaxNS + 3 + NS);
+          }
+#pragma unroll
+          for (int c = 0; c < NS; ++c) sin[c] = (c == Sp::SW2 ? acc.p2 : acc.p) * sj[c] + acc.b[c];
+          sin_seen = min(2, seen_j + acc.seen);
+          break;
+        }
+        top -= (kWave - 1) - hnr;  // lanes (hnr, 63] composed
+      }
+    }
+    if (lane == 0) {  // publish the inclusive state
+      double* pl = prow + tile * kPayload;
+#pragma unroll
+      for (int c = 0; c < NS; ++c)
+        st_agent_f64(pl + kMaxNS + 3 + c, (c == Sp::SW2 ? agg.p2 : agg.p) * sin[c] + agg.b[c]);
+      st_agent_f64(pl + kMaxNS + 3 + NS, (double)min(2, sin_seen + agg.seen));
+      st_flag(&frow[tile], 2u);
+#pragma unroll
+      for (int c = 0; c < NS; ++c) s_in[c] = sin[c];
+      s_in_seen = sin_seen;
+    }
+  }
+  // 3b. the moment kinds (larger maps): serial look-back by thread 0, one
+  //     predecessor at a time.
+  if (!WAVE_LOOKBACK && tid == 0) {
     Map<NS> agg = Map<NS>::identity();
     for (int w = 0; w < kExpWaves; ++w) agg = compose<NS, Sp::SW2>(agg, s_wave[w]);
     unsigned int* frow = A.flags + row * A.tiles_per_row;
