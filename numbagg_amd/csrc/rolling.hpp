@@ -493,35 +493,35 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
         for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
       }
     }
-  } else
-  for (int64_t e = (int64_t)staged + (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
-    const int64_t p = s0a + e;
-    if (VEC > 1 && p >= 0 && p + VEC <= n) {
-      Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(ra + p);
+  } else {
+    for (int64_t e = (int64_t)staged + (int64_t)tid * VEC; e < seg_len; e += (int64_t)kMoveThreads * VEC) {
+      const int64_t p = s0a + e;
+      if (VEC > 1 && p >= 0 && p + VEC <= n) {
+        Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(ra + p);
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
-      if (Tr::NARGS == 2) {
-        Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rb + p);
+        for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
+        if (Tr::NARGS == 2) {
+          Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rb + p);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
-      }
-    } else {
+          for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
+        }
+      } else {
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-        const int64_t q = p + v;
-        const bool in = q >= 0 && q < n;
-        la[pad16(e + v)] = in ? ra[q] : canonical_nan<T>();
-        if (Tr::NARGS == 2) lb[pad16(e + v)] = in ? rb[q] : canonical_nan<T>();
+        for (int v = 0; v < VEC; ++v) {
+          const int64_t q = p + v;
+          const bool in = q >= 0 && q < n;
+          la[pad16(e + v)] = in ? ra[q] : canonical_nan<T>();
+          if (Tr::NARGS == 2) lb[pad16(e + v)] = in ? rb[q] : canonical_nan<T>();
+        }
       }
     }
   }
   __syncthreads();
 
-  // Per-tile shifts for the moment kinds: the first valid value among the tile's
-  // first 256 inputs (any constant works; one near the data keeps the float64
-  // moments small).
-  // Every wave finds them itself from the staged segment (64 inputs per
-  // ballot), so no further barrier is needed.
+  // Per-tile shifts for the moment kinds: the first valid value among the
+  // tile's first 256 inputs (any constant works; one near the data keeps the
+  // float64 moments small). Every wave finds them itself from the staged
+  // segment (64 inputs per ballot), so no further barrier is needed.
   T ka = T(0), kb = T(0);
   if (Tr::SHIFT) {
     // first 256 inputs of the tile itself (always staged; NaN past the row end)
