@@ -130,6 +130,12 @@ __device__ __forceinline__ float read_lane<float>(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 template <>
+__device__ __forceinline__ long long read_lane<long long>(long long v, int lane) {
+  const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)v, lane);
+  const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+template <>
 __device__ __forceinline__ double read_lane<double>(double v, int lane) {
   const long long x = __double_as_longlong(v);
   const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)x, lane);
@@ -187,8 +193,6 @@ constexpr unsigned int kSpinLimit = 1u << 24;
 
 __device__ __forceinline__ void spin_pause() { __builtin_amdgcn_s_sleep(1); }
 
-// LDS index with one padding slot per 16 elements: threads that each own 16
-// consecutive elements then hit distinct banks on every strided access.
 // Workgroups of one launch are handed to the 8 XCDs round-robin (workgroup b
 // runs on XCD b % 8), and each XCD has its own L2. xcd_order(b, n) renumbers
 // the grid so that consecutive logical ids run on the same XCD one after the
@@ -203,6 +207,8 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n) {
   return b < per * kXcds ? (b % kXcds) * per + b / kXcds : b;
 }
 
+// LDS index with one padding slot per 16 elements: threads that each own 16
+// consecutive elements then hit distinct banks on every strided access.
 __host__ __device__ __forceinline__ int64_t pad16(int64_t e) { return e + (e >> 4); }
 __host__ __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 

@@ -91,6 +91,14 @@ __device__ __forceinline__ void wave_last_carry(const Last<T>& agg, Last<T>& exc
   tail.val = tval;
 }
 
+// Per-tile summary written by the independent first pass of the two-pass
+// fill, one 16-byte store per tile: scan-order offsets within the tile.
+struct alignas(16) FillSummary {
+  double val;  // the tile's last valid value
+  int last;    // its offset, -1: none
+  int first;   // offset of the tile's first valid value, -1: none
+};
+
 // Dispatch order of a row's tiles: runs of `run` consecutive tiles of one row,
 // rows taking turns (run = 1: row-interleaved, tile t of every row before tile
 // t + 1 of any; run = tiles: row-major). Block b gets its row and the tile it
@@ -113,21 +121,31 @@ __device__ __forceinline__ void fill_block_coords(uint32_t b, int64_t rows, int6
   }
 }
 
-template <typename T, int VEC, int U, bool REV, bool LOOKBACK, bool PACK = false>
+// INDEP: the first pass of the two-pass fill. Tiles take no ticket and no
+// carry (a tile's leading NaN run stays NaN) and write their FillSummary;
+// fill_fix_kernel then fills the leading runs.
+template <typename T, int VEC, int U, bool REV, bool LOOKBACK, bool PACK = false, bool INDEP = false>
 __global__ __launch_bounds__(kFillThreads) void fill_kernel(
     const T* __restrict__ x, T* __restrict__ out, int64_t rows, int64_t n, int64_t ld_x,
     int64_t ld_out, int64_t limit, int64_t tiles_per_row, unsigned int* __restrict__ counters,
-    unsigned long long* __restrict__ status, unsigned int* __restrict__ timeouts, int64_t run) {
+    unsigned long long* __restrict__ status, unsigned int* __restrict__ timeouts, int64_t run,
+    FillSummary* __restrict__ sums = nullptr) {
+  static_assert(!(INDEP && LOOKBACK), "the independent pass takes no look-back");
   constexpr int TILE = kFillThreads * U * VEC;
+  constexpr int kNone = 0x7fffffff;
   __shared__ int s_ticket;
+  __shared__ int s_wfirst[kFillWaves];
   __shared__ int s_wso[kFillWaves];
   __shared__ T s_wval[kFillWaves];
   __shared__ long long s_carry;  // scan-order absolute index of the carry, -1 = none
   __shared__ T s_carry_val;
   __shared__ int s_first_valid;
 
+  // INDEP (row-major): consecutive tiles run on one XCD, so the summaries of
+  // neighbouring tiles, which share cache lines, are written through one L2.
   int64_t row, guess;
-  fill_block_coords(blockIdx.x, rows, tiles_per_row, run, row, guess);
+  fill_block_coords(INDEP ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x, rows, tiles_per_row, run, row,
+                    guess);
   const T* xr = x + row * ld_x;
   T* orow = out + row * ld_out;
   const int w = wave_id();
@@ -191,10 +209,12 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
   // Per-slot exclusive carries (from earlier lanes / slots of this wave).
   Last<T> cin[U];
   Last<T> wcarry{-1, T(0)};
+  int wfirst = kNone;  // INDEP: the wave's first valid offset in scan order
 #pragma unroll
   for (int i = 0; i < U; ++i) {
     const int u = REV ? (U - 1 - i) : i;  // scan order over slots
     Last<T> agg{-1, T(0)};
+    int fso = kNone;
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
       if (!is_nan(vals[u][v])) {
@@ -203,6 +223,7 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
           agg.so = so;
           agg.val = vals[u][v];
         }
+        fso = so < fso ? so : fso;
       }
     }
     Last<T> excl, tail;
@@ -210,10 +231,19 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
     if (excl.so < 0) excl = wcarry;
     cin[u] = excl;
     if (tail.so >= 0) wcarry = tail;
+    if constexpr (INDEP) {
+      // slots run in scan order, so the first slot holding a valid value
+      // gives the wave's first: its earliest lane in scan order
+      const unsigned long long m = __ballot(fso != kNone);
+      const int f = m == 0ull ? 0 : (REV ? 63 - __clzll((long long)m) : __ffsll((long long)m) - 1);
+      const int ff = read_lane(fso, f);
+      if (wfirst == kNone && m != 0ull) wfirst = ff;
+    }
   }
   if (l == 0) {
     s_wso[w] = wcarry.so;
     s_wval[w] = wcarry.val;
+    if constexpr (INDEP) s_wfirst[w] = wfirst;
   }
   // The owner of scan offset 0 reports whether that element is valid.
   if (REV ? (w == kFillWaves - 1 && l == kWave - 1) : (w == 0 && l == 0))
@@ -234,6 +264,18 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
       tile_agg.so = s_wso[ww];
       tile_agg.val = s_wval[ww];
     }
+  }
+
+  // INDEP: the tile's summary, by thread 0.
+  if (INDEP && threadIdx.x == 0) {
+    int first = kNone;
+#pragma unroll
+    for (int i = 0; i < kFillWaves; ++i) first = s_wfirst[i] < first ? s_wfirst[i] : first;
+    FillSummary sm;
+    sm.val = (double)tile_agg.val;
+    sm.last = tile_agg.so;
+    sm.first = first != kNone ? first : -1;
+    sums[row * tiles_per_row + tile] = sm;
   }
 
   // Cross-tile carry (decoupled look-back), by thread 0.
@@ -344,6 +386,93 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
         if (p < n) orow[p] = res[v];
       }
     }
+  }
+}
+
+// ---- second pass of the two-pass fill -------------------------------------------
+// One workgroup per row walks the row's tile summaries in scan order, 256 at
+// a time (one per thread): each tile's carry (the last valid value before it)
+// comes from a ballot over the lanes holding one, the earlier waves' last
+// through LDS, and the running carry of the earlier chunks. Each wave then
+// fills the leading NaN runs of its own tiles that need it (positions
+// [tile start, first valid) within `limit` of the carry), one tile at a time
+// with whole-wave stores.
+template <typename T, bool REV>
+__global__ __launch_bounds__(kFillThreads) void fill_fix_kernel(T* __restrict__ out, int64_t n, int64_t ld_out,
+                                                                int64_t limit, int64_t tiles, int64_t tile_len,
+                                                                const FillSummary* __restrict__ sums) {
+  __shared__ long long s_wso[kFillWaves];
+  __shared__ T s_wval[kFillWaves];
+  const int64_t row = blockIdx.x;
+  const FillSummary* srow = sums + row * tiles;
+  T* orow = out + row * ld_out;
+  const int w = wave_id();
+  const int l = lane_id();
+  long long run_so = -1;  // carry from the earlier chunks (block-uniform)
+  T run_val = T(0);
+  for (int64_t c0 = 0; c0 < tiles; c0 += kFillThreads) {
+    const int64_t k = c0 + threadIdx.x;  // tile index in scan order
+    long long last = -1, beg = 0, end = 0;
+    T val = T(0);
+    if (k < tiles) {
+      const int64_t tl = REV ? tiles - 1 - k : k;
+      const FillSummary sm = srow[tl];
+      // scan-order position of the tile's first element (< 0 for the partial
+      // last tile of bfill) and end of its span
+      const int64_t p0 = tl * tile_len;
+      const int64_t base = REV ? n - p0 - tile_len : p0;
+      const int64_t stop = REV ? n - p0 : (p0 + tile_len < n ? p0 + tile_len : n);
+      last = sm.last >= 0 ? base + sm.last : -1;
+      end = sm.first >= 0 ? base + sm.first : stop;
+      val = (T)sm.val;
+      beg = base > 0 ? base : 0;
+    }
+    // nearest earlier lane (lower lane: earlier tile) holding a valid value;
+    // the lane reads run on every lane, results are selected afterwards
+    const unsigned long long m = __ballot(last >= 0);
+    const unsigned long long before = m & ((1ull << l) - 1ull);
+    const int src = before == 0ull ? -1 : 63 - __clzll((long long)before);
+    const int from = src < 0 ? l : src;
+    const long long oso = __shfl(last, from, kWave);
+    const T oval = __shfl(val, from, kWave);
+    const int t = m == 0ull ? 0 : 63 - __clzll((long long)m);
+    const long long tso = read_lane(last, t);
+    const T tval = read_lane(val, t);
+    if (l == 0) {
+      s_wso[w] = m == 0ull ? -1 : tso;
+      s_wval[w] = tval;
+    }
+    __syncthreads();
+    long long cso = run_so;
+    T cval = run_val;
+    for (int i = 0; i < w; ++i) {
+      if (s_wso[i] >= 0) {
+        cso = s_wso[i];
+        cval = s_wval[i];
+      }
+    }
+    if (src >= 0) {
+      cso = oso;
+      cval = oval;
+    }
+    for (int i = 0; i < kFillWaves; ++i) {  // the chunk's last valid value carries on
+      if (s_wso[i] >= 0) {
+        run_so = s_wso[i];
+        run_val = s_wval[i];
+      }
+    }
+    // leading run positions within `limit` of the carry
+    const long long lim_end = cso >= 0 ? (cso + limit + 1 < end ? cso + limit + 1 : end) : beg;
+    unsigned long long todo = __ballot(lim_end > beg);
+    while (todo != 0ull) {  // wave-uniform
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1ull;
+      const long long b = read_lane(beg, i);
+      const long long e = read_lane(lim_end, i);
+      const T v = read_lane(cval, i);
+      for (long long q = b + l; q < e; q += kWave) orow[REV ? n - 1 - q : q] = v;
+    }
+    __syncthreads();  // before the next chunk rewrites s_wso / s_wval
   }
 }
 
@@ -710,9 +839,67 @@ int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int
   return check_hip(hipGetLastError(), "fill_kernel launch");
 }
 
+// Two-pass fill for inputs with many rows: independent tiles of 16 B per
+// thread in row-major order (no tickets, no waiting: the pattern of a plain
+// copy), then one fix-up workgroup per row. NBG_FILL_TWO_PASS=0 / 1 forces
+// the choice; the workspace query asks here too.
+constexpr int64_t kFillTwoPassRows = 512;
+inline int64_t fill_pass1_tile(int elem) { return (int64_t)kFillThreads * 16 / elem; }
+bool fill_two_pass(int64_t rows, int64_t n, int elem) {
+  const int64_t t = fill_pass1_tile(elem);
+  if (rows <= 0 || (n + t - 1) / t <= 1) return false;
+  const char* v = getenv("NBG_FILL_TWO_PASS");
+  if (v != nullptr && strcmp(v, "0") == 0) return false;
+  if (v != nullptr && strcmp(v, "1") == 0) return true;
+  return rows >= kFillTwoPassRows;
+}
+
+template <typename T>
+int fill_two_pass_impl(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int64_t ld_x,
+                       int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, hipStream_t s) {
+  const T* x = static_cast<const T*>(xv);
+  T* out = static_cast<T*>(ov);
+  constexpr int MAXV = 16 / (int)sizeof(T);
+  constexpr int TILE = kFillThreads * MAXV;
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  const int64_t blocks = rows * tiles;
+  if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "fill: too many tiles (%lld)", (long long)blocks);
+  const size_t need = (size_t)blocks * sizeof(FillSummary);
+  if (ws == nullptr || ws_bytes < need)
+    return set_error(NBG_ERR_INVALID, "fill: workspace too small (%zu < %zu)", ws_bytes, need);
+  FillSummary* sums = static_cast<FillSummary*>(ws);
+  unsigned int* to = spin_counter_ptr();
+  if (to == nullptr) return set_error(NBG_ERR_HIP, "fill: cannot resolve device counter");
+  const int vx = pick_vec<T>(x, ld_x, rows), vo = pick_vec<T>(out, ld_out, rows);
+  const int vec = vx < vo ? vx : vo;
+  dim3 grid((unsigned)blocks), block(kFillThreads);
+  // run = tiles: row-major tile order, consecutive workgroups on consecutive memory
+#define NBG_FILL_PASS1(V, U, R)                                                                          \
+  hipLaunchKernelGGL((fill_kernel<T, V, U, R, false, false, true>), grid, block, 0, s, x, out, rows, n, \
+                     ld_x, ld_out, limit, tiles, nullptr, nullptr, to, tiles, sums)
+  if (vec == MAXV) {
+    if (rev) NBG_FILL_PASS1(MAXV, 1, true);
+    else NBG_FILL_PASS1(MAXV, 1, false);
+  } else {
+    if (rev) NBG_FILL_PASS1(1, MAXV, true);
+    else NBG_FILL_PASS1(1, MAXV, false);
+  }
+#undef NBG_FILL_PASS1
+  if (int rc = check_hip(hipGetLastError(), "fill_kernel (independent pass) launch")) return rc;
+  if (rev)
+    hipLaunchKernelGGL((fill_fix_kernel<T, true>), dim3((unsigned)rows), block, 0, s, out, n, ld_out, limit, tiles,
+                       (int64_t)TILE, sums);
+  else
+    hipLaunchKernelGGL((fill_fix_kernel<T, false>), dim3((unsigned)rows), block, 0, s, out, n, ld_out, limit, tiles,
+                       (int64_t)TILE, sums);
+  return check_hip(hipGetLastError(), "fill_fix_kernel launch");
+}
+
 template <typename T>
 int fill_impl(bool rev, const void* x, void* out, int64_t rows, int64_t n, int64_t ld_x,
               int64_t ld_out, int64_t limit, void* ws, size_t ws_bytes, hipStream_t s) {
+  if (fill_two_pass(rows, n, (int)sizeof(T)))
+    return fill_two_pass_impl<T>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
   switch (fill_bytes_per_thread()) {
     case 16: return fill_impl_b<T, 16>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
     case 32: return fill_impl_b<T, 32>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);
@@ -744,7 +931,12 @@ int fill_entry(const char* name, bool rev, int dtype, const void* x, void* out, 
 extern "C" {
 
 size_t nbg_fill_workspace_bytes(int64_t rows, int64_t n, int dtype) {
-  const int64_t tile = (int64_t)nbg::kFillThreads * nbg::fill_bytes_per_thread() / (dtype == NBG_F64 ? 8 : 4);
+  const int elem = dtype == NBG_F64 ? 8 : 4;
+  if (nbg::fill_two_pass(rows, n, elem)) {
+    const int64_t t = nbg::fill_pass1_tile(elem);
+    return (size_t)(rows * ((n + t - 1) / t)) * sizeof(nbg::FillSummary);
+  }
+  const int64_t tile = (int64_t)nbg::kFillThreads * nbg::fill_bytes_per_thread() / elem;
   const int64_t tiles = (n + tile - 1) / tile;
   if (rows <= 0 || tiles <= 1) return 0;
   return (size_t)(rows * tiles) * 8 + (size_t)((rows * 4 + 7) / 8) * 8;

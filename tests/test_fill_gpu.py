@@ -70,6 +70,33 @@ def test_fill_spans_bit_exact(gpu_lib, monkeypatch, span, prefetch, pack, dtype,
         np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
 
 
+@pytest.mark.parametrize("two_pass", ["1", "0"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("func", ["ffill", "bfill"])
+@pytest.mark.parametrize("limit", [None, 0, 3, 700, 5000])
+def test_fill_two_pass_bit_exact(gpu_lib, monkeypatch, two_pass, dtype, func, limit):
+    """The two-pass fill (independent tiles, then leading NaN runs filled from
+    the tile summaries) and the ordered single pass give the same bits,
+    including NaN stretches over many tiles, all-NaN rows, unaligned rows and
+    partial last tiles."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    monkeypatch.setenv("NBG_FILL_TWO_PASS", two_pass)
+    for shape in [(5, 1023), (3, 40_961), (16, 70_000), (600, 3000)]:
+        x = runs_array(*shape, seed=shape[1] + 11, dtype=dtype)
+        rows = shape[0]
+        x[1, 100 : min(30_000, shape[1] - 1)] = np.nan
+        x[2, :] = np.nan
+        x[rows - 1, :] = np.nan
+        x[rows - 1, shape[1] // 2] = 1.0
+        if rows > 4:
+            x[3, : shape[1] - 2] = np.nan
+            x[4, 2:] = np.nan
+        got = getattr(nb, func)(x, limit=limit)
+        np.testing.assert_array_equal(_bits(got), _bits(exact.fill(func, x, limit)))
+
+
 @pytest.mark.parametrize("run", ["2", "5", "16", "100000"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("func", ["ffill", "bfill"])
