@@ -13,8 +13,9 @@ inputs already resident in HBM:
 
 `value` is the whole step's algorithmic bytes (each input read once, each output
 written once) divided by its time. The roofline object covers the step's
-dominant kernel, timed with HIP events on the stream the kernels are launched
-on. `cpu_baseline` times the C restatement of the reference kernels (OpenMP over
+dominant function, timed with HIP events on the stream its kernels are
+launched on; for ffill these are the two kernels of one call (independent
+tiles, then the fix-up of the tiles' leading NaN runs). `cpu_baseline` times the C restatement of the reference kernels (OpenMP over
 rows, the stand-in for numba's parallel target) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--all] [--no-cpu]
@@ -149,7 +150,7 @@ def build_workloads(torch, nb, lib, want_all):
         return ("ffill", xs, dict(limit=None)), xs.nbytes * 2
 
     wl.append(Workload("ffill", {"shape": [1024, 1_000_000], "dtype": "f32", "limit": None, "nan": "30% geometric runs"},
-                       x.numel() * 8, run_ff, cpu_ff, kernel="fill_kernel"))
+                       x.numel() * 8, run_ff, cpu_ff, kernel="fill_kernel + fill_fix_kernel"))
     wl[-1].copy_src = x
     return wl
 
