@@ -119,3 +119,31 @@ def test_ragged_lds_labels_range(gpu_lib):
         RaggedMoveMeanGroupMean([100, 200], [5, 6], window=10, lds_labels=0)
     with pytest.raises(ValueError, match="lds_labels"):
         RaggedMoveMeanGroupMean([100, 200], [5, 6], window=10, lds_labels=100_000)
+
+
+def test_model_choice_with_measured_tie_break(gpu_lib, monkeypatch):
+    """The cost model ranks every LDS slot count; slot counts predicted within
+    the tie margin are timed on the batch and the faster kept, so the choice
+    is always one of the ranked candidates with its own strategies."""
+    from numbagg_amd import tuning
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    lengths, ks, v, lab = make_batch(60, 100, 40_000, 10, 6000, seed=4)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50)
+    model = tuning.CostModel(seed=2)  # untrained: arbitrary but valid predictions
+    skew, nanf = tuning.batch_statistics(rb, v, lab)
+    ranked = model.rank_launches(tuning.features(rb.lengths, rb.num_labels, skew, nanf, rb.window), rb.lengths)
+    assert sorted(c[1] for c in ranked) == sorted(tuning.CAPACITIES)
+    assert all(a[0] <= b[0] for a, b in zip(ranked, ranked[1:]))
+    for margin in (0.0, 1e9):  # no tie-break / every candidate timed
+        monkeypatch.setattr(tuning, "TIE_MARGIN", margin)
+        cap, st = tuning.choose_launch(rb, v, lab, method="model", model=model)
+        match = [c for c in ranked if c[1] == cap]
+        assert match and np.array_equal(st, match[0][2])
+        if margin == 0.0:
+            assert cap == ranked[0][1]
+        rb.set_strategy(st)
+        rb.set_lds_labels(cap)
+        mm, gm = rb(v, lab)
+        pm, pg = rb.plain(v, lab)
+        np.testing.assert_array_equal(_bits(mm), _bits(pm))
