@@ -40,11 +40,14 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   using Tr = Traits<T, KIND>;
   constexpr int TILE = kMoveThreads * KP;
   const int64_t tiles = (n + TILE - 1) / TILE;
-  // Spans of consecutive tiles per workgroup: as long as possible (up to
-  // kMaxSpan) while the grid keeps about 16 workgroups per CU.
-  // NBG_MOVE_SPAN=<k> forces k (results do not depend on it; tests use it to
-  // run carried halos on small inputs).
-  int64_t span = rows * tiles / (256 * 16);
+  // Spans of consecutive tiles per workgroup: about 96 spans per CU, at most
+  // 6 tiles each (measured on (256, 2M) float32 windows of 1000: spans of
+  // 4-6 tiles 0.843-0.848 ms, 8 tiles 0.854, 16 tiles 0.873, 2 tiles 0.948 —
+  // longer spans stage fewer halos but leave a longer tail of the grid).
+  // NBG_MOVE_SPAN=<k> forces k, up to kMaxSpan (results do not depend on it;
+  // tests use it to run carried halos on small inputs).
+  int64_t span = rows * tiles / (256 * 96);
+  span = span > 6 ? 6 : span;
   if (const char* e = getenv("NBG_MOVE_SPAN")) span = atoll(e);
   span = span < 1 ? 1 : (span > kMaxSpan ? kMaxSpan : span);
   if (span > tiles) span = tiles;
