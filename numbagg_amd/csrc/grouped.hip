@@ -427,11 +427,27 @@ int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
   }
 }
 
+int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return (v != nullptr && v[0] != '\0') ? atoi(v) : dflt;
+}
+
+// Workgroups per CU of the LDS strategy's grid. Every workgroup ends with one
+// global atomic per used label slot, and all of them end together, so the
+// grid is kept at the fewest workgroups that still keep enough loads in
+// flight: 4 for the float64 sum passes (2 rows x 2 vectors per thread), 2 for
+// the others (4 rows x 4 vectors per thread). Measured on MI355X for (64, 4M)
+// float32 x 1000 labels: count/argmax/first/last 5-8% faster at 2 than at the
+// occupancy limit, the sum pass alone (without init and finalize) 0.183 ms
+// at 4 against 0.194-0.220 ms at 8. Knob: NBG_GROUP_BLOCKS_PER_CU (0 = the
+// occupancy limit). Results do not depend on it.
+constexpr int group_blocks_per_cu(int pass) { return (pass == kPSum || pass == kPDev) ? 4 : 2; }
+
 // Chunking: the row groups' positions are split so that the whole grid is
 // one wave of resident workgroups (the device's capacity for this kernel and
-// LDS size): no partly filled second wave, and each workgroup flushes its
-// accumulators once. Chunks are multiples of 4096 positions (whole vector
-// super-steps) and at least 4096.
+// LDS size, capped per CU for the LDS strategy): no partly filled second
+// wave, and each workgroup flushes its accumulators once. Chunks are
+// multiples of 4096 positions (whole vector super-steps) and at least 4096.
 template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT, int U = kGroupUnroll,
           bool PIPE = false>
 int launch_pass(GArgs a, size_t lds, hipStream_t s) {
@@ -439,6 +455,14 @@ int launch_pass(GArgs a, size_t lds, hipStream_t s) {
   int cap = device_capacity_blocks((const void*)group_pass_kernel<TV, TL, PASS, LDS, R, COUNT, U, PIPE>,
                                    kGroupThreads, LDS ? lds : 0);
   if (cap <= 0) cap = 2048;
+  if (LDS) {
+    const int per_cu = env_int("NBG_GROUP_BLOCKS_PER_CU", group_blocks_per_cu(PASS));
+    int dev = 0, cus = 0;
+    if (per_cu > 0 && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0 &&
+        cap > per_cu * cus)
+      cap = per_cu * cus;
+  }
   const int64_t per_group = cap / groups > 1 ? cap / groups : 1;
   int64_t chunk = (a.n + per_group - 1) / per_group;
   chunk = (chunk + 4095) / 4096 * 4096;
@@ -463,11 +487,6 @@ constexpr int kGroupRowsPerBlock = 4;
 // folding the current one). Results do not depend on them.
 constexpr int kSumRows = 2;
 constexpr int kSumUnroll = 2;
-
-int env_int(const char* name, int dflt) {
-  const char* v = getenv(name);
-  return (v != nullptr && v[0] != '\0') ? atoi(v) : dflt;
-}
 
 template <typename TV, typename TL, int PASS, bool COUNT, int R>
 int launch_rows(GArgs a, size_t lds, hipStream_t s) {

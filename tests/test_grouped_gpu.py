@@ -130,6 +130,24 @@ def test_group_sum_pass_shapes(gpu_lib, monkeypatch, rows, unroll, pipe, func):
         _check_float(func, dtype, v, lab, 1000, got)
 
 
+@pytest.mark.parametrize("per_cu", ["0", "1", "3", "8"])
+def test_group_workgroups_per_cu(gpu_lib, monkeypatch, per_cu):
+    """Grids of any number of workgroups per CU (longer or shorter chunks per
+    workgroup) give the same bits for the exact functions and meet the same
+    bars for the float ones."""
+    import numbagg_amd as nb
+    from oracle import restated
+
+    monkeypatch.setenv("NBG_GROUP_BLOCKS_PER_CU", per_cu)
+    v, lab = _data(7, 500_000, 700, np.float32, seed=int(per_cu) + 40, zipf=True)
+    for func in EXACT_FUNCS:
+        got = getattr(nb, func)(v, lab, num_labels=700, axis=-1)
+        want = restated.group(func, v, lab.astype(np.int64), 700)
+        assert_array_equal(got.view(np.uint8), want.view(np.uint8))
+    for func in FLOAT_FUNCS:
+        _check_float(func, np.float32, v, lab, 700, getattr(nb, func)(v, lab, num_labels=700, axis=-1))
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_group_var_offset_data(gpu_lib, dtype):
     """Data far from zero (mean 1e4, unit spread): the per-label means come
