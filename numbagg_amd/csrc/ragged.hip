@@ -24,7 +24,11 @@
 namespace nbg {
 namespace {
 
-constexpr int kRaggedGrab = 8;       // work items (tiles) per queue grab
+// Work items (tiles) per queue grab. A series' LDS label slots are flushed
+// once per grab it appears in, so longer grabs flush less often; shorter ones
+// balance the grid's tail. Knob: NBG_RAGGED_GRAB.
+constexpr int kRaggedGrab = 8;
+constexpr int kRaggedMaxGrab = 1024;
 constexpr int64_t kRaggedLdsLabels = 1024;     // default LDS label slots per launch
 constexpr int64_t kRaggedMaxLdsLabels = 8192;  // 96 KB of slots
 
@@ -47,6 +51,7 @@ struct RaggedArgs {
   size_t stage_bytes;  // dynamic LDS used by the rolling tile
   size_t lut_bytes;    // 1/c table after the staged segment (0 when unused)
   int64_t lds_labels;  // LDS label slots (labels [0, lds_labels) of a series)
+  int grab;            // tiles per queue grab
 };
 
 __global__ void ragged_init_kernel(double* gsum, unsigned int* gcount, int64_t total_labels,
@@ -97,12 +102,12 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4)
   const float* la = reinterpret_cast<const float*>(smem_raw);  // the staged segment
 
   while (true) {
-    if (tid == 0) s_base = (long long)atomicAdd(A.queue, (unsigned)kRaggedGrab);
+    if (tid == 0) s_base = (long long)atomicAdd(A.queue, (unsigned)A.grab);
     __syncthreads();
     const int64_t base = s_base;
     __syncthreads();
     if (base >= A.total_tiles) break;  // uniform: every workgroup exits here
-    const int64_t stop = base + kRaggedGrab < A.total_tiles ? base + kRaggedGrab : A.total_tiles;
+    const int64_t stop = base + A.grab < A.total_tiles ? base + A.grab : A.total_tiles;
     // LDS label slots stay live across consecutive tiles of one series within
     // the grab and are flushed when the series changes or the grab ends.
     int cur = -1;
@@ -265,8 +270,17 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
   if (cap <= 0)
     return set_error(NBG_ERR_INVALID, "ragged: no workgroup fits (%zu bytes of LDS for window %lld and %lld label slots)",
                      lds, (long long)window, (long long)lds_labels);
+  int grab = kRaggedGrab;
+  if (const char* e = getenv("NBG_RAGGED_GRAB")) grab = atoi(e);
+  if (grab < 1 || grab > kRaggedMaxGrab)
+    return set_error(NBG_ERR_INVALID, "ragged: NBG_RAGGED_GRAB %d not in [1, %d]", grab, kRaggedMaxGrab);
+  a.grab = grab;
   int64_t grid = total_tiles < cap ? total_tiles : cap;
   if (grid < 1) grid = 1;
+  // The 32-bit queue counter ends at most one grab per workgroup past the
+  // last tile.
+  if (total_tiles + grid * (int64_t)grab > 0xffffffffLL)
+    return set_error(NBG_ERR_INVALID, "ragged: %lld tiles exceed the work-queue range", (long long)total_tiles);
   if (kp == 32) {
     if (lut) hipLaunchKernelGGL((ragged_kernel<32, true>), dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
     else hipLaunchKernelGGL((ragged_kernel<32, false>), dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);

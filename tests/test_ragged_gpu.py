@@ -55,6 +55,34 @@ def test_ragged_matches_plain_launches(gpu_lib, strategy, lds_labels):
     assert np.mean(a == b) > 0.999
 
 
+@pytest.mark.parametrize("grab", ["1", "3", "64", "1024"])
+def test_ragged_grab_sizes(gpu_lib, monkeypatch, grab):
+    """Any number of tiles per queue grab (series split over grabs or several
+    series in one grab, LDS slots flushed per grab) gives the same results."""
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    monkeypatch.setenv("NBG_RAGGED_GRAB", grab)
+    lengths, ks, v, lab = make_batch(150, 100, 80_000, 10, 9000, seed=int(grab) + 5)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50, strategy="lds", lds_labels=1024)
+    mm, gm = rb(v, lab)
+    pm, pg = rb.plain(v, lab)
+    np.testing.assert_array_equal(_bits(mm), _bits(pm))
+    a, b = _bits(gm).astype(np.int64), _bits(pg).astype(np.int64)
+    assert np.all(np.abs(a - b) <= 1)
+    assert np.mean(a == b) > 0.999
+
+
+def test_ragged_grab_out_of_range(gpu_lib, monkeypatch):
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    lengths, ks, v, lab = make_batch(4, 100, 1000, 10, 20, seed=3)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50)
+    for bad in ("0", "1025"):
+        monkeypatch.setenv("NBG_RAGGED_GRAB", bad)
+        with pytest.raises(RuntimeError, match="NBG_RAGGED_GRAB"):
+            rb(v, lab)
+
+
 @pytest.mark.parametrize("strategy,khi", [("heuristic", 2000), ("lds", 50_000), ("global", 50_000)])
 def test_ragged_against_oracle(gpu_lib, strategy, khi):
     """Includes series with more labels than the LDS slots: with "lds" their
