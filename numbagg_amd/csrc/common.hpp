@@ -229,4 +229,14 @@ inline int pick_vec(const void* p, int64_t ld, int64_t rows) {
   return 1;
 }
 
+// Widest vector for the rolling kernels, whose 16-byte global accesses need
+// element alignment only (see Vec16): MAXV elements when every pointer is
+// element-aligned, else 1.
+template <typename T>
+inline int pick_vec_elem(const void* a, const void* b, const void* out) {
+  const bool ok = ((uintptr_t)a % sizeof(T)) == 0 && ((uintptr_t)b % sizeof(T)) == 0 &&
+                  ((uintptr_t)out % sizeof(T)) == 0;
+  return ok ? 16 / (int)sizeof(T) : 1;
+}
+
 }  // namespace nbg

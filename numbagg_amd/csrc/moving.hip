@@ -55,17 +55,10 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   const int64_t blocks = rows * spans;
   if (blocks > 0x7fffffffLL || tiles > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move: too many tiles");
   const bool tail_lds = window <= lds_window_limit<T, Tr::NARGS>() - (KP - default_items<T>()) * kMoveThreads;
-  int vec = pick_vec<T>(a, ld_a, rows);
-  if (Tr::NARGS == 2) {
-    int vb = pick_vec<T>(b, ld_b, rows);
-    vec = vb < vec ? vb : vec;
-  }
-  {
-    int vo = pick_vec<T>(out, ld_out, rows);
-    vec = vo < vec ? vo : vec;
-  }
+  // 16-byte vector loads and stores need element alignment only, so rows at
+  // any element offset or stride take the vector path.
   constexpr int MAXV = 16 / (int)sizeof(T);
-  const int v_use = (vec == MAXV) ? MAXV : 1;
+  const int v_use = pick_vec_elem<T>(a, Tr::NARGS == 2 ? b : a, out);
   constexpr bool LUT_OK = (KIND == kMean && sizeof(T) == 4);
   const bool use_lut = LUT_OK && mean_uses_lut(window);
   size_t lds = (size_t)rolling_stage_bytes<T>(KP, window, tail_lds, Tr::NARGS);

@@ -11,7 +11,16 @@
 //     launches;
 //   * a persistent grid (one resident wave of workgroups) pulls items from an
 //     atomic work-queue counter, a few items per grab; every workgroup leaves
-//     the loop once the counter passes the last item;
+//     the loop once the counter passes the last item. The next grab is taken
+//     while the current one runs, and a grab's tile descriptors (series
+//     offsets, label range, strategy) are read into LDS once, so no tile waits
+//     on a chain of dependent lookups;
+//   * consecutive tiles of one series within a grab are software-pipelined as
+//     in the plain move kernel's spans: the next tile's inputs are loaded into
+//     registers while the current tile is evaluated, and its window halo is
+//     carried in LDS;
+//   * series sit at any element offset of the packed buffers: values, outputs
+//     and labels move 16 bytes per access at element alignment;
 //   * each item evaluates its rolling tile (halo staged in LDS) and then folds
 //     the tile's values into the series' label accumulators, either in
 //     LDS-privatised slots flushed once per tile (few labels, or skewed labels)
@@ -26,9 +35,11 @@ namespace {
 
 // Work items (tiles) per queue grab. A series' LDS label slots are flushed
 // once per grab it appears in, so longer grabs flush less often; shorter ones
-// balance the grid's tail. Knob: NBG_RAGGED_GRAB.
-constexpr int kRaggedGrab = 8;
-constexpr int kRaggedMaxGrab = 1024;
+// balance the grid's tail (measured on MI355X, C5-shaped batch of 2.1e9
+// elements with 4096 label slots: grabs of 4 / 8 / 16 tiles 11.0 / 10.0 /
+// 9.75 ms). Knob: NBG_RAGGED_GRAB.
+constexpr int kRaggedGrab = 16;
+constexpr int kRaggedMaxGrab = 64;
 constexpr int64_t kRaggedLdsLabels = 1024;     // default LDS label slots per launch
 constexpr int64_t kRaggedMaxLdsLabels = 8192;  // 96 KB of slots
 
@@ -64,13 +75,24 @@ __global__ void ragged_init_kernel(double* gsum, unsigned int* gcount, int64_t t
   if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
 }
 
+// A tile of the current grab, as read once from the batch description.
+struct RaggedTile {
+  int64_t off;   // series start in values / move_out
+  int64_t n;     // series length
+  int64_t lo;    // series start in the label accumulators
+  int64_t K;     // series label count
+  int tile;      // tile index within the series
+  int series;
+  int strategy;  // 0: LDS slots for labels below lds_labels, 1: global only
+};
+
 template <int VEC, int KP, bool LUT>
 __device__ __forceinline__ void ragged_rolling(const RaggedArgs& A, int64_t off, int64_t n,
                                                int64_t tile, unsigned char* smem,
                                                RollingShared<Traits<float, kMean>::S>& rs,
-                                               const double* lut) {
+                                               const double* lut, int staged, int full) {
   rolling_tile<float, kMean, true, VEC, KP, LUT>(A.values + off, nullptr, A.move_out + off, n,
-                                                 A.window, A.min_count, tile, smem, rs, lut);
+                                                 A.window, A.min_count, tile, smem, rs, lut, staged, full);
 }
 
 // Flushes the LDS label slots [0, nl) of series `s` into its global
@@ -88,42 +110,67 @@ __device__ __forceinline__ void ragged_flush(const RaggedArgs& A, int64_t lo, in
 }
 
 template <int KP, bool LUT>
-__global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4))) void ragged_kernel(RaggedArgs A) {
+// Held to 3 waves per SIMD (168 VGPRs): the prefetched tile stays in registers
+// without spills (measured on MI355X: a C5-shaped batch of 2.1e9 elements with
+// 4096 label slots takes 10.0 ms, against 12.1 ms at 4 waves with spills and
+// 12.3 ms without the prefetch).
+__global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3))) void ragged_kernel(RaggedArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ RollingShared<Traits<float, kMean>::S> rs;
-  __shared__ long long s_base;
-  __shared__ int s_series;
+  __shared__ RaggedTile s_tiles[kRaggedMaxGrab];
+  __shared__ long long s_next;
   constexpr int TILE = kMoveThreads * KP;
+  constexpr int NV = KP / 4;  // 16-byte vectors per thread of one tile
   const int tid = threadIdx.x;
   double* lut = reinterpret_cast<double*>(smem_raw + A.stage_bytes);
   if constexpr (LUT) fill_recip_lut(lut, A.window, A.min_count > 1 ? A.min_count : 1);  // ordered by the loop's first barrier
   double* l_sum = reinterpret_cast<double*>(smem_raw + A.stage_bytes + A.lut_bytes);
   unsigned int* l_cnt = reinterpret_cast<unsigned int*>(l_sum + A.lds_labels);
-  const float* la = reinterpret_cast<const float*>(smem_raw);  // the staged segment
+  float* la = reinterpret_cast<float*>(smem_raw);  // the staged segment
+  // The segment of tile t is [floor16(t0 - window), t0 + TILE): H = ceil16(window)
+  // halo elements, then the tile's own TILE inputs. The next tile's segment
+  // begins with this one's last H elements, so they are carried when H <= TILE.
+  const int H = (int)carried_halo(A.window);
+  const bool carry = H <= TILE;
 
+  if (tid == 0) s_next = (long long)atomicAdd(A.queue, (unsigned)A.grab);
   while (true) {
-    if (tid == 0) s_base = (long long)atomicAdd(A.queue, (unsigned)A.grab);
-    __syncthreads();
-    const int64_t base = s_base;
-    __syncthreads();
+    __syncthreads();  // s_next published; the previous grab's LDS reads done
+    const int64_t base = s_next;
     if (base >= A.total_tiles) break;  // uniform: every workgroup exits here
-    const int64_t stop = base + A.grab < A.total_tiles ? base + A.grab : A.total_tiles;
+    const int cnt = (int)(base + A.grab < A.total_tiles ? A.grab : A.total_tiles - base);
+    if (tid < cnt) {
+      const int64_t t = base + tid;
+      const int s = A.tile_series[t];
+      RaggedTile d;
+      d.off = A.offsets[s];
+      d.n = A.offsets[s + 1] - d.off;
+      d.lo = A.label_offsets[s];
+      d.K = A.label_offsets[s + 1] - d.lo;
+      d.tile = (int)(t - A.tile_offsets[s]);
+      d.series = s;
+      d.strategy = A.strategy[s];
+      s_tiles[tid] = d;
+    }
+    __syncthreads();  // descriptors visible; every thread has read s_next
+    if (tid == 0) s_next = (long long)atomicAdd(A.queue, (unsigned)A.grab);  // next grab, in flight meanwhile
+
     // LDS label slots stay live across consecutive tiles of one series within
     // the grab and are flushed when the series changes or the grab ends.
     int cur = -1;
     int64_t cur_lo = 0, cur_nl = 0;
-    for (int64_t t = base; t < stop; ++t) {
-      if (tid == 0) s_series = A.tile_series[t];
-      __syncthreads();
-      const int s = s_series;
-      const int64_t off = A.offsets[s];
-      const int64_t n = A.offsets[s + 1] - off;
-      const int64_t tile = t - A.tile_offsets[s];
-      const int64_t lo = A.label_offsets[s];
-      const int64_t K = A.label_offsets[s + 1] - lo;
+    int staged = 0;  // leading segment elements already in LDS (carried halo + prefetched tile)
+    float pa[KP];
+    for (int i = 0; i < cnt; ++i) {
+      const RaggedTile d = s_tiles[i];
+      const int s = d.series;
+      const int64_t n = d.n;
+      const int64_t tile = d.tile;
+      const int64_t lo = d.lo;
+      const int64_t K = d.K;
       // Strategy 0 keeps labels below lds_labels in LDS (all of them when K
       // fits) and sends the rest to global atomics; strategy 1 is global only.
-      const int64_t nl = A.strategy[s] == 0 ? (K < A.lds_labels ? K : A.lds_labels) : 0;
+      const int64_t nl = d.strategy == 0 ? (K < A.lds_labels ? K : A.lds_labels) : 0;
       if (s != cur) {
         if (cur_nl > 0) ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
         for (int64_t l = tid; l < nl; l += kMoveThreads) {
@@ -134,34 +181,58 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4)
         cur_lo = lo;
         cur_nl = nl;  // the zeroing is ordered before use by rolling_tile's barriers
       }
-      // 1. move_mean of this tile (identical arithmetic to the plain kernel).
-      if ((((uintptr_t)(A.values + off)) % 16 == 0) && (((uintptr_t)(A.move_out + off)) % 16 == 0))
-        ragged_rolling<4, KP, LUT>(A, off, n, tile, smem_raw, rs, lut);
-      else
-        ragged_rolling<1, KP, LUT>(A, off, n, tile, smem_raw, rs, lut);
-      // Labels of this thread's KP positions (vector loads when aligned).
+      const float* xs = A.values + d.off;
+      // 16-byte vector accesses need element alignment only (Vec16), so every
+      // series of a float32 batch takes the vector path.
+      const bool aligned = (((uintptr_t)xs) % 4 == 0) && (((uintptr_t)(A.move_out + d.off)) % 4 == 0);
       const int64_t t0 = tile * TILE;
+      const int full = t0 + TILE <= n ? 1 : 0;
       const int64_t c0 = t0 + (int64_t)tid * KP;
-      const int32_t* lab = A.labels + off;
+      // The next tile of this series, when it is the grab's next item: its
+      // inputs [t0 + TILE, t0 + 2 TILE) are loaded now (segment offsets
+      // [H, H + TILE); NaN past the series end), thread tid holding the
+      // elements it would stage itself.
+      const bool more = carry && aligned && i + 1 < cnt && s_tiles[i + 1].series == s &&
+                        s_tiles[i + 1].tile == d.tile + 1;
+      if (more) {
+        const int64_t t0n = t0 + TILE;
+        const bool nfull = t0n + TILE <= n;
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+          const int64_t p = t0n + ((int64_t)tid + (int64_t)m * kMoveThreads) * 4;
+          if (nfull || p + 4 <= n) {
+            const Vec16<float> q = *reinterpret_cast<const Vec16<float>*>(xs + p);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pa[m * 4 + v] = q.v[v];
+          } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pa[m * 4 + v] = p + v < n ? xs[p + v] : canonical_nan<float>();
+          }
+        }
+      }
+      // 1. move_mean of this tile (identical arithmetic to the plain kernel).
+      if (aligned)
+        ragged_rolling<4, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full);
+      else
+        ragged_rolling<1, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full);
+      // Labels of this thread's KP positions (16-byte loads inside the series).
+      const int32_t* lab = A.labels + d.off;
       int32_t lj[KP];
-      if (c0 + KP <= n && ((uintptr_t)(lab + c0)) % 16 == 0) {
+      if (c0 + KP <= n) {
 #pragma unroll
         for (int j = 0; j < KP; j += 4) {
-          const int4 q = *reinterpret_cast<const int4*>(lab + c0 + j);
-          lj[j] = q.x;
-          lj[j + 1] = q.y;
-          lj[j + 2] = q.z;
-          lj[j + 3] = q.w;
+          const Vec16<int32_t> q = *reinterpret_cast<const Vec16<int32_t>*>(lab + c0 + j);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) lj[j + v] = q.v[v];
         }
       } else {
 #pragma unroll
         for (int j = 0; j < KP; ++j) lj[j] = c0 + j < n ? lab[c0 + j] : -1;
       }
       // 2. group_nanmean accumulation of the thread's KP values, read from the
-      //    staged segment (staged from s0a = floor16(t0 - window); positions
-      //    past the series end hold NaN and carry label -1 here).
-      const int64_t s0a = (t0 - A.window) & ~(int64_t)15;
-      const int ehp = pad16((int)(c0 - s0a));
+      //    staged segment (offsets [H, H + TILE); positions past the series end
+      //    hold NaN and carry label -1 here).
+      const int ehp = pad16((int)(c0 - ((t0 - A.window) & ~(int64_t)15)));
 #pragma unroll
       for (int j = 0; j < KP; ++j) {
         const float x = la[ehp + j + (j >> 4)];
@@ -176,7 +247,24 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(4)
           }
         }
       }
-      __syncthreads();  // staged segment reads done before the next tile stages
+      if (more) {
+        // Carry the halo: segment offsets [TILE, TILE + H) -> [0, H). The
+        // group reads above use offsets [H, H + TILE) only, so the copy may
+        // overlap them; the barrier then orders the copy's reads and the
+        // group reads before the prefetched inputs overwrite [H, H + TILE).
+        for (int e = tid; e < H; e += kMoveThreads) la[pad16(e)] = la[pad16(e + TILE)];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+          const int e = H + (tid + m * kMoveThreads) * 4;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) la[pad16(e + v)] = pa[m * 4 + v];
+        }
+        staged = H + TILE;  // fully staged; rolling_tile's first barrier publishes these writes
+      } else {
+        __syncthreads();  // staged segment reads done before the next tile stages
+        staged = 0;
+      }
     }
     if (cur_nl > 0) ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
   }

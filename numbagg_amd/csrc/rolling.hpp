@@ -400,8 +400,11 @@ __device__ __forceinline__ S block_excl_scan_and_total(S v, S w, S& w_total, Blo
   return prefix;
 }
 
+// 16 bytes of global memory at element alignment: gfx9+ executes dword-aligned
+// multi-dword global accesses, so rows at any element offset (series packed
+// back to back, sliced views) still move 16 bytes per load/store instruction.
 template <typename T>
-struct alignas(16) Vec16 {
+struct Vec16 {
   T v[16 / sizeof(T)];
 };
 

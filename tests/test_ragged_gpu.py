@@ -55,7 +55,7 @@ def test_ragged_matches_plain_launches(gpu_lib, strategy, lds_labels):
     assert np.mean(a == b) > 0.999
 
 
-@pytest.mark.parametrize("grab", ["1", "3", "64", "1024"])
+@pytest.mark.parametrize("grab", ["1", "3", "17", "64"])
 def test_ragged_grab_sizes(gpu_lib, monkeypatch, grab):
     """Any number of tiles per queue grab (series split over grabs or several
     series in one grab, LDS slots flushed per grab) gives the same results."""
@@ -77,7 +77,7 @@ def test_ragged_grab_out_of_range(gpu_lib, monkeypatch):
 
     lengths, ks, v, lab = make_batch(4, 100, 1000, 10, 20, seed=3)
     rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50)
-    for bad in ("0", "1025"):
+    for bad in ("0", "65"):
         monkeypatch.setenv("NBG_RAGGED_GRAB", bad)
         with pytest.raises(RuntimeError, match="NBG_RAGGED_GRAB"):
             rb(v, lab)
