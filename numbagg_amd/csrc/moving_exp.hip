@@ -21,7 +21,12 @@
 //     examines 64 predecessors per pass;
 //   * each thread re-runs its K positions from its exact incoming state, one
 //     fused multiply-add per state component (state = decay * state +
-//     contribution).
+//     contribution);
+//   * with many tiles per row (long look-back chains) and the smallest maps the same
+//     tile code runs in two passes instead: independent tiles write their
+//     aggregate maps, one workgroup per row scans them into each tile's
+//     incoming state, and independent tiles re-run their positions from it
+//     (the input is read twice, but no tile waits on another).
 // Squares and cross products of float32 inputs are formed in float64 (exact),
 // where the reference rounds them to float32 first.
 #include "common.hpp"
@@ -269,7 +274,12 @@ struct ExpArgs {
   unsigned int* timeouts;
 };
 
-template <typename TX, typename TA, int KIND, bool SCALAR>
+// MODE 0: one pass, incoming states by decoupled look-back. MODE 1: aggregate
+// pass (each tile's aggregate map to the payload, no outputs). MODE 2: apply
+// pass (each tile's incoming state read from the payload, where
+// exp_carry_kernel put it). Modes 1 and 2 process tiles in row-major order on
+// XCD-consecutive workgroups and never wait.
+template <typename TX, typename TA, int KIND, bool SCALAR, int MODE = 0>
 __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   using Sp = ExpSpec<KIND>;
   constexpr int NS = Sp::NS;
@@ -286,7 +296,9 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   const TA* al = static_cast<const TA*>(A.alpha);
   TA* out = static_cast<TA*>(A.out);
 
-  const int64_t row = (int64_t)(blockIdx.x % (uint64_t)A.rows);
+  const uint32_t v_id = MODE == 0 ? 0u : xcd_order(blockIdx.x, gridDim.x);
+  const int64_t row = MODE == 0 ? (int64_t)(blockIdx.x % (uint64_t)A.rows)
+                                : (int64_t)(v_id / (uint64_t)A.tiles_per_row);
   const TX* ra = xa + row * A.ld_a;
   const TX* rb = (Sp::NARGS == 2) ? xb + row * A.ld_b : nullptr;
   const TA* rl = al ? al + row * A.ld_alpha : nullptr;
@@ -325,13 +337,20 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   // Blocks of a row are dispatched in order, so the ticket is almost always
   // blockIdx / rows: that tile is staged while the ticket atomic is in flight
   // and restaged only when the ticket differs.
-  const int64_t guess = (int64_t)(blockIdx.x / (uint64_t)A.rows);
-  if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&A.counters[row], 1u);
-  stage(guess);
-  __syncthreads();
-  const int64_t tile = s_ticket;
-  if (tile != guess) {
+  int64_t tile;
+  if constexpr (MODE == 0) {
+    const int64_t guess = (int64_t)(blockIdx.x / (uint64_t)A.rows);
+    if (threadIdx.x == 0) s_ticket = (int)atomicAdd(&A.counters[row], 1u);
+    stage(guess);
     __syncthreads();
+    tile = s_ticket;
+    if (tile != guess) {
+      __syncthreads();
+      stage(tile);
+      __syncthreads();
+    }
+  } else {
+    tile = (int64_t)(v_id % (uint64_t)A.tiles_per_row);
     stage(tile);
     __syncthreads();
   }
@@ -383,6 +402,19 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   wave_scan_maps<NS, Sp::SW2>(m, incl, excl);
   if (lane == kWave - 1) s_wave[wave] = incl;
   __syncthreads();
+  if constexpr (MODE == 1) {
+    if (tid == 0) {  // the tile's aggregate map
+      Map<NS> agg = Map<NS>::identity();
+      for (int w = 0; w < kExpWaves; ++w) agg = compose<NS, Sp::SW2>(agg, s_wave[w]);
+      double* pl = A.payload + (row * A.tiles_per_row + tile) * kPayload;
+      pl[0] = agg.p;
+      pl[1] = agg.p2;
+#pragma unroll
+      for (int c = 0; c < NS; ++c) pl[2 + c] = agg.b[c];
+      pl[2 + NS] = (double)agg.seen;
+    }
+    return;
+  }
   Map<NS> wprefix = Map<NS>::identity();
   for (int w = 0; w < wave; ++w) wprefix = compose<NS, Sp::SW2>(wprefix, s_wave[w]);
   excl = compose<NS, Sp::SW2>(wprefix, excl);
@@ -396,7 +428,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   //    moment kinds' larger maps make the wave scan cost more registers and
   //    time than the walk it saves.
   constexpr bool WAVE_LOOKBACK = NS <= 3;
-  if (WAVE_LOOKBACK && wave == 0) {
+  if (MODE == 0 && WAVE_LOOKBACK && wave == 0) {
     Map<NS> agg = Map<NS>::identity();
     for (int w = 0; w < kExpWaves; ++w) agg = compose<NS, Sp::SW2>(agg, s_wave[w]);
     unsigned int* frow = A.flags + row * A.tiles_per_row;
@@ -482,7 +514,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   }
   // 3b. the moment kinds (larger maps): serial look-back by thread 0, one
   //     predecessor at a time.
-  if (!WAVE_LOOKBACK && tid == 0) {
+  if (MODE == 0 && !WAVE_LOOKBACK && tid == 0) {
     Map<NS> agg = Map<NS>::identity();
     for (int w = 0; w < kExpWaves; ++w) agg = compose<NS, Sp::SW2>(agg, s_wave[w]);
     unsigned int* frow = A.flags + row * A.tiles_per_row;
@@ -555,10 +587,22 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   __syncthreads();
 
   // 4. re-run this thread's positions from its incoming state.
+  double sin_t[NS];
+  int sin_seen_t;
+  if constexpr (MODE == 2) {  // computed by exp_carry_kernel
+    const double* pl = A.payload + (row * A.tiles_per_row + tile) * kPayload + kMaxNS + 3;
+#pragma unroll
+    for (int c = 0; c < NS; ++c) sin_t[c] = pl[c];
+    sin_seen_t = (int)pl[NS];
+  } else {
+#pragma unroll
+    for (int c = 0; c < NS; ++c) sin_t[c] = s_in[c];
+    sin_seen_t = s_in_seen;
+  }
   double s[NS];
 #pragma unroll
-  for (int c = 0; c < NS; ++c) s[c] = (c == Sp::SW2 ? excl.p2 : excl.p) * s_in[c] + excl.b[c];
-  int seen = min(2, s_in_seen + excl.seen);
+  for (int c = 0; c < NS; ++c) s[c] = (c == Sp::SW2 ? excl.p2 : excl.p) * sin_t[c] + excl.b[c];
+  int seen = min(2, sin_seen_t + excl.seen);
   TA res[K];
   auto rerun = [&](int j) {
     const int64_t pos = t0 + e0 + j;
@@ -594,9 +638,72 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   }
 }
 
+// Two-pass middle step: one workgroup per row scans the aggregate maps of the
+// row's tiles (256 at a time, DPP wave scans and a running carry) and stores
+// each tile's incoming state, the composition of all earlier tiles' maps
+// applied to the zero state before the row start.
+template <int KIND>
+__global__ __launch_bounds__(kExpThreads) void exp_carry_kernel(ExpArgs A) {
+  using Sp = ExpSpec<KIND>;
+  constexpr int NS = Sp::NS;
+  __shared__ Map<NS> s_wave[kExpWaves];
+  const int64_t row = blockIdx.x;
+  const int64_t T = A.tiles_per_row;
+  double* prow = A.payload + row * T * kPayload;
+  const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+  Map<NS> carry = Map<NS>::identity();
+  for (int64_t c0 = 0; c0 < T; c0 += kExpThreads) {
+    const int64_t t = c0 + tid;
+    Map<NS> m = Map<NS>::identity();
+    if (t < T) {
+      const double* pl = prow + t * kPayload;
+      m.p = pl[0];
+      m.p2 = pl[1];
+#pragma unroll
+      for (int c = 0; c < NS; ++c) m.b[c] = pl[2 + c];
+      m.seen = (int)pl[2 + NS];
+    }
+    Map<NS> incl, excl;
+    wave_scan_maps<NS, Sp::SW2>(m, incl, excl);
+    if (lane == kWave - 1) s_wave[wave] = incl;
+    __syncthreads();
+    Map<NS> wprefix = Map<NS>::identity(), total = Map<NS>::identity();
+    for (int w = 0; w < kExpWaves; ++w) {
+      if (w < wave) wprefix = compose<NS, Sp::SW2>(wprefix, s_wave[w]);
+      total = compose<NS, Sp::SW2>(total, s_wave[w]);
+    }
+    const Map<NS> in = compose<NS, Sp::SW2>(carry, compose<NS, Sp::SW2>(wprefix, excl));
+    if (t < T) {
+      double* pl = prow + t * kPayload + kMaxNS + 3;
+#pragma unroll
+      for (int c = 0; c < NS; ++c) pl[c] = in.b[c];
+      pl[NS] = (double)in.seen;
+    }
+    carry = compose<NS, Sp::SW2>(carry, total);
+    __syncthreads();  // s_wave is rewritten by the next chunk
+  }
+}
+
 template <typename TX>
 constexpr int64_t exp_tile() {
   return kExpThreads * (64 / (int64_t)sizeof(TX));
+}
+
+// Two passes for rows of at least kExpTwoPassTiles tiles (long look-back
+// chains), for the kinds with two-component maps (count, sum). Measured on
+// MI355X for float32 (256, 2M), alpha 0.01: sum / count 1.99 / 2.00 ms in one
+// pass, 1.78 / 1.75 ms in two; mean 2.06 ms either way; var/std unchanged and
+// cov/corr slower (2.73 / 3.17 ms against 3.12 / 3.38): the moment kinds are
+// bound by their per-tile arithmetic, which two passes repeat.
+// NBG_EXP_TWO_PASS=0/1 forces either.
+constexpr int64_t kExpTwoPassTiles = 64;
+template <int KIND>
+bool exp_two_pass(int64_t tiles) {
+  if (const char* e = getenv("NBG_EXP_TWO_PASS")) {
+    if (e[0] == '0' && e[1] == '\0') return false;
+    if (e[0] == '1' && e[1] == '\0') return true;
+  }
+  return ExpSpec<KIND>::NS <= 2 && tiles >= kExpTwoPassTiles;
 }
 
 template <typename TX, typename TA, int KIND>
@@ -616,9 +723,21 @@ int exp_launch(ExpArgs a, void* ws, size_t ws_bytes, hipStream_t s) {
   a.tiles_per_row = tiles;
   a.timeouts = spin_counter_ptr();
   if (a.timeouts == nullptr) return set_error(NBG_ERR_HIP, "move_exp: cannot resolve device counter");
-  if (int rc = launch_zero(ws, zero_bytes, s)) return rc;
   constexpr int NARGS = ExpSpec<KIND>::NARGS;
   const size_t lds = (size_t)(pad16(exp_tile<TX>()) + 16) * sizeof(TX) * NARGS;
+  if (exp_two_pass<KIND>(tiles)) {
+    if (a.alpha == nullptr) {
+      hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, true, 1>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
+      hipLaunchKernelGGL((exp_carry_kernel<KIND>), dim3((unsigned)a.rows), dim3(kExpThreads), 0, s, a);
+      hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, true, 2>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
+    } else {
+      hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, false, 1>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
+      hipLaunchKernelGGL((exp_carry_kernel<KIND>), dim3((unsigned)a.rows), dim3(kExpThreads), 0, s, a);
+      hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, false, 2>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
+    }
+    return check_hip(hipGetLastError(), "exp_kernel launch");
+  }
+  if (int rc = launch_zero(ws, zero_bytes, s)) return rc;
   if (a.alpha == nullptr)
     hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, true>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
   else

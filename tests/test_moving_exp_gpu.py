@@ -50,6 +50,47 @@ def test_exp_float64_loop_against_exact(gpu_lib, func, shape, alpha):
         assert_allclose(got[ok], want[ok], rtol=1e-12, atol=1e-12 * scale)
 
 
+@pytest.mark.parametrize("two_pass", ["1", "0"])
+@pytest.mark.parametrize("func", FUNCS1 + FUNCS2)
+def test_exp_two_pass_against_exact(gpu_lib, monkeypatch, two_pass, func):
+    """The two-pass path (tile aggregates, per-row scan, apply) and the
+    single pass with look-back meet the same bars: rows of more than 256
+    tiles (several chunks of the row scan), partial last tiles, NaN stretches
+    longer than a tile, scalar and per-position alpha."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    monkeypatch.setenv("NBG_EXP_TWO_PASS", two_pass)
+    x = walk_array(2, 1_100_003, seed=11, offset=30.0)
+    y = walk_array(2, 1_100_003, seed=12, offset=-20.0)
+    x[0, 5000:20_000] = np.nan
+    y[1, 700_000:720_000] = np.nan
+    scale = float(np.nanmax(np.abs(x)))
+    for alpha in (0.01, np.linspace(0.005, 0.05, x.shape[1])):
+        got = getattr(nb, func)(*_args(func, x, y), alpha=alpha, min_weight=0.2)
+        want = exact.move_exp(func, x, alpha, 0.2, y=y if func in FUNCS2 else None, loop_dtype=np.float64)
+        assert np.array_equal(np.isnan(got), np.isnan(want)), func
+        ok = ~np.isnan(want)
+        if func in MOMENTS:
+            assert_allclose(got[ok], want[ok], rtol=1e-9, atol=1e-9 * scale**2)
+        else:
+            assert_allclose(got[ok], want[ok], rtol=1e-12, atol=1e-12 * scale)
+    # float32 loop (float32 alpha array), several rows of partial tiles
+    x32 = walk_array(9, 300_001, seed=13, offset=0.0)
+    y32 = walk_array(9, 300_001, seed=14, offset=0.0)
+    al = np.full(300_001, 0.03, dtype=np.float32)
+    got = getattr(nb, func)(*_args(func, x32, y32), alpha=al)
+    want = exact.move_exp(func, x32, al, y=y32 if func in FUNCS2 else None, loop_dtype=np.float32)
+    assert np.array_equal(np.isnan(got), np.isnan(want)), func
+    ok = ~np.isnan(want)
+    e = want[ok].astype(np.float32)
+    ulps = np.abs(got[ok].astype(np.float64) - want[ok]) / np.spacing(np.abs(e)).astype(np.float64)
+    if func in MOMENTS:
+        assert np.all((ulps <= 2) | (np.abs(got[ok] - want[ok]) <= 1e-6 * np.maximum(1, np.abs(want[ok])))), ulps.max()
+    else:
+        assert ulps.max() <= 2, ulps.max()
+
+
 @pytest.mark.parametrize("func", FUNCS1 + FUNCS2)
 def test_exp_float32_loop_against_exact(gpu_lib, func):
     import numbagg_amd as nb
