@@ -148,6 +148,24 @@ def test_group_workgroups_per_cu(gpu_lib, monkeypatch, per_cu):
         _check_float(func, np.float32, v, lab, 700, getattr(nb, func)(v, lab, num_labels=700, axis=-1))
 
 
+@pytest.mark.parametrize("hot", ["1", "0"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_group_hot_labels(gpu_lib, monkeypatch, hot, dtype):
+    """Labels 0 and 1 accumulated in registers (or not) meet the same bars:
+    skewed and uniform labels, a single label (no label 1), 5 rows (a partial
+    row group), out-of-range labels."""
+    import numbagg_amd as nb
+
+    monkeypatch.setenv("NBG_GROUP_HOT", hot)
+    for L, zipf in ((1000, True), (1000, False), (1, False), (2, True)):
+        v, lab = _data(5, 200_003, L, dtype, seed=L + (7 if zipf else 0), zipf=zipf)
+        if L == 1:
+            lab = np.where(lab > 0, 0, lab).astype(np.int32)
+        for func in FLOAT_FUNCS:
+            got = getattr(nb, func)(v, lab, num_labels=L, axis=-1)
+            _check_float(func, dtype, v, lab, L, got)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_group_var_offset_data(gpu_lib, dtype):
     """Data far from zero (mean 1e4, unit spread): the per-label means come
