@@ -57,7 +57,7 @@ int nbg_stream_copy(const void* src, void* dst, size_t bytes, void* stream);
 /* ---- fill (numbagg/funcs.py:294-326 bfill/ffill; ndfill.__call__ at
  *      numbagg/decorators.py:465-487) ------------------------------------- */
 /* limit: the reference's `lives_remaining` budget (>= 0; None -> n).
- * Workspace: per-tile summaries (two-pass path, 512 rows or more) or
+ * Workspace: per-tile summaries (two-pass path, 8 rows or more) or
  * look-back status words and tickets (ordered single pass); the query and the
  * call choose the same path for the same arguments and environment. */
 size_t nbg_fill_workspace_bytes(int64_t rows, int64_t n, int dtype);

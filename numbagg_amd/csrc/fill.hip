@@ -844,8 +844,12 @@ int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int
 // copy), then one fix-up workgroup per row. NBG_FILL_TWO_PASS=0 / 1 forces
 // the choice; the workspace query asks here too. (Measured on MI355X for the
 // first pass over float32 (1024, 1M): 16 B per thread 1.67 ms, 32 B per thread
-// 1.33 ms — with 4 KB tiles the per-workgroup cost shows.)
-constexpr int64_t kFillTwoPassRows = 512;
+// 1.33 ms — with 4 KB tiles the per-workgroup cost shows.) Two passes from 8
+// rows up: float32 ffill of 4.096e9 elements as rows x n = 1024 x 1M / 256 x
+// 4M / 64 x 16M / 16 x 64M / 4 x 256M took 1.42 / 1.36 / 1.44 / 1.73 / 2.99
+// ms in two passes against 1.64 / 1.65 / 1.80 / 2.97 / 3.11 ms in one (the
+// fix-up's per-row scan of tile summaries grows with fewer, longer rows).
+constexpr int64_t kFillTwoPassRows = 8;
 constexpr int kFillPass1Vecs = 2;  // 16-byte vectors per thread and tile
 inline int64_t fill_pass1_tile(int elem) { return (int64_t)kFillThreads * 16 * kFillPass1Vecs / elem; }
 bool fill_two_pass(int64_t rows, int64_t n, int elem) {
