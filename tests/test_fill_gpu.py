@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("func", ["ffill", "bfill"])
 @pytest.mark.parametrize("limit", [None, 0, 1, 16, 5000])
-@pytest.mark.parametrize("shape", [(3, 500), (7, 4097), (2, 50_001), (1, 200_000), (300, 333)])
+@pytest.mark.parametrize("shape", [(3, 500), (7, 4097), (2, 50_001), (1, 200_000), (300, 333), (12, 30_001)])
 def test_fill_bit_exact(gpu_lib, dtype, func, limit, shape):
     import numbagg_amd as nb
     from oracle import exact, restated
@@ -38,6 +38,7 @@ def test_fill_tile_sizes_bit_exact(gpu_lib, monkeypatch, pack, fill_bytes, dtype
     import numbagg_amd as nb
     from oracle import exact
 
+    monkeypatch.setenv("NBG_FILL_TWO_PASS", "0")  # the ordered single pass's knobs
     monkeypatch.setenv("NBG_FILL_BYTES", fill_bytes)
     monkeypatch.setenv("NBG_FILL_PACK", pack)
     for shape in [(5, 1023), (3, 40_961), (64, 3000)]:
@@ -58,6 +59,7 @@ def test_fill_spans_bit_exact(gpu_lib, monkeypatch, span, prefetch, pack, dtype,
     import numbagg_amd as nb
     from oracle import exact
 
+    monkeypatch.setenv("NBG_FILL_TWO_PASS", "0")  # the ordered single pass's knobs
     monkeypatch.setenv("NBG_FILL_SPAN", span)
     monkeypatch.setenv("NBG_FILL_PREFETCH", prefetch)
     monkeypatch.setenv("NBG_FILL_PACK", pack)
@@ -108,6 +110,7 @@ def test_fill_dispatch_runs_bit_exact(gpu_lib, monkeypatch, run, dtype, func, li
     import numbagg_amd as nb
     from oracle import exact
 
+    monkeypatch.setenv("NBG_FILL_TWO_PASS", "0")  # the ordered single pass's knobs
     monkeypatch.setenv("NBG_FILL_RUN", run)
     for shape in [(5, 1023), (3, 40_961), (16, 70_000), (2, 65_536)]:
         x = runs_array(*shape, seed=shape[1] + len(run), dtype=dtype)
@@ -118,7 +121,9 @@ def test_fill_dispatch_runs_bit_exact(gpu_lib, monkeypatch, run, dtype, func, li
 
 @pytest.mark.parametrize("fill_bytes", ["16", "64"])
 @pytest.mark.parametrize("func", ["ffill", "bfill"])
-def test_fill_long_nan_runs_cross_many_tiles(gpu_lib, monkeypatch, fill_bytes, func):
+@pytest.mark.parametrize("two_pass", ["1", "0"])
+def test_fill_long_nan_runs_cross_many_tiles(gpu_lib, monkeypatch, two_pass, fill_bytes, func):
+    monkeypatch.setenv("NBG_FILL_TWO_PASS", two_pass)
     monkeypatch.setenv("NBG_FILL_BYTES", fill_bytes)
     # All-NaN stretches longer than several tiles force the look-back to walk
     # past tiles that hold no valid value.
