@@ -22,8 +22,8 @@
 //   * each thread re-runs its K positions from its exact incoming state, one
 //     fused multiply-add per state component (state = decay * state +
 //     contribution);
-//   * with many tiles per row (long look-back chains) and the smallest maps the same
-//     tile code runs in two passes instead: independent tiles write their
+//   * on request (NBG_EXP_TWO_PASS=1) the same tile code runs in two passes
+//     instead: independent tiles write their
 //     aggregate maps, one workgroup per row scans them into each tile's
 //     incoming state, and independent tiles re-run their positions from it
 //     (the input is read twice, but no tile waits on another).
@@ -258,6 +258,17 @@ __device__ __forceinline__ double exp_out(const double* s, int seen, double mw) 
 }
 
 
+// Positions per thread: 64 bytes of input; 32 float32 positions for the
+// kinds with small maps (count, mean, sum: twice as many positions per tile,
+// half as many look-back hops per row). Measured on MI355X for float32
+// (256, 2M), alpha 0.01, 16 -> 32 positions: mean 2.06 -> 1.77 ms, sum /
+// count 1.99 / 2.00 -> 1.62 / 1.61 ms; var 2.51 -> 3.06 ms and corr 3.16 ->
+// 4.37 ms, so the moment kinds keep 16.
+template <typename TX, int KIND>
+constexpr int exp_items() {
+  return (sizeof(TX) == 4 && ExpSpec<KIND>::NS <= 3) ? 32 : 64 / (int)sizeof(TX);
+}
+
 struct ExpArgs {
   const void* xa;
   const void* xb;
@@ -283,7 +294,7 @@ template <typename TX, typename TA, int KIND, bool SCALAR, int MODE = 0>
 __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   using Sp = ExpSpec<KIND>;
   constexpr int NS = Sp::NS;
-  constexpr int K = 64 / (int)sizeof(TX);
+  constexpr int K = exp_items<TX, KIND>();
   constexpr int TILE = kExpThreads * K;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ Map<NS> s_wave[kExpWaves];
@@ -684,31 +695,36 @@ __global__ __launch_bounds__(kExpThreads) void exp_carry_kernel(ExpArgs A) {
   }
 }
 
-template <typename TX>
+template <typename TX, int KIND>
 constexpr int64_t exp_tile() {
+  return kExpThreads * (int64_t)exp_items<TX, KIND>();
+}
+// The smallest tile of any kind (the workspace is sized for it).
+template <typename TX>
+constexpr int64_t exp_min_tile() {
   return kExpThreads * (64 / (int64_t)sizeof(TX));
 }
 
-// Two passes for rows of at least kExpTwoPassTiles tiles (long look-back
-// chains), for the kinds with two-component maps (count, sum). Measured on
-// MI355X for float32 (256, 2M), alpha 0.01: sum / count 1.99 / 2.00 ms in one
-// pass, 1.78 / 1.75 ms in two; mean 2.06 ms either way; var/std unchanged and
-// cov/corr slower (2.73 / 3.17 ms against 3.12 / 3.38): the moment kinds are
-// bound by their per-tile arithmetic, which two passes repeat.
-// NBG_EXP_TWO_PASS=0/1 forces either.
-constexpr int64_t kExpTwoPassTiles = 64;
+// Two passes on request only (NBG_EXP_TWO_PASS=1; =0 or unset: one pass).
+// Measured on MI355X for float32 (256, 2M), alpha 0.01: with 16 positions per
+// thread sum / count took 1.99 / 2.00 ms in one pass and 1.78 / 1.75 ms in
+// two, but with 32 positions one pass takes 1.62 / 1.61 ms against 1.89 /
+// 1.85 ms in two; mean 1.77 ms in one pass, 2.12 in two; the moment kinds are
+// bound by their per-tile arithmetic, which two passes repeat (cov / corr
+// 2.73 / 3.17 ms in one pass, 3.12 / 3.38 in two).
 template <int KIND>
 bool exp_two_pass(int64_t tiles) {
   if (const char* e = getenv("NBG_EXP_TWO_PASS")) {
     if (e[0] == '0' && e[1] == '\0') return false;
     if (e[0] == '1' && e[1] == '\0') return true;
   }
-  return ExpSpec<KIND>::NS <= 2 && tiles >= kExpTwoPassTiles;
+  (void)tiles;
+  return false;
 }
 
 template <typename TX, typename TA, int KIND>
 int exp_launch(ExpArgs a, void* ws, size_t ws_bytes, hipStream_t s) {
-  const int64_t tiles = (a.n + exp_tile<TX>() - 1) / exp_tile<TX>();
+  const int64_t tiles = (a.n + exp_tile<TX, KIND>() - 1) / exp_tile<TX, KIND>();
   const int64_t blocks = a.rows * tiles;
   if (blocks > 0x7fffffffLL) return set_error(NBG_ERR_INVALID, "move_exp: too many tiles");
   const size_t need = nbg_move_exp_workspace_bytes(a.rows, a.n, sizeof(TX) == 4 ? NBG_F32 : NBG_F64);
@@ -724,7 +740,7 @@ int exp_launch(ExpArgs a, void* ws, size_t ws_bytes, hipStream_t s) {
   a.timeouts = spin_counter_ptr();
   if (a.timeouts == nullptr) return set_error(NBG_ERR_HIP, "move_exp: cannot resolve device counter");
   constexpr int NARGS = ExpSpec<KIND>::NARGS;
-  const size_t lds = (size_t)(pad16(exp_tile<TX>()) + 16) * sizeof(TX) * NARGS;
+  const size_t lds = (size_t)(pad16(exp_tile<TX, KIND>()) + 16) * sizeof(TX) * NARGS;
   if (exp_two_pass<KIND>(tiles)) {
     if (a.alpha == nullptr) {
       hipLaunchKernelGGL((exp_kernel<TX, TA, KIND, true, 1>), dim3((unsigned)blocks), dim3(kExpThreads), lds, s, a);
@@ -786,7 +802,7 @@ int exp_entry(const char* name, int x_dtype, int dtype, const void* xa, const vo
 extern "C" {
 
 size_t nbg_move_exp_workspace_bytes(int64_t rows, int64_t n, int x_dtype) {
-  const int64_t tile = (x_dtype == NBG_F64) ? nbg::exp_tile<double>() : nbg::exp_tile<float>();
+  const int64_t tile = (x_dtype == NBG_F64) ? nbg::exp_min_tile<double>() : nbg::exp_min_tile<float>();
   const int64_t tiles = (n + tile - 1) / tile;
   if (rows <= 0 || tiles <= 0) return 0;
   const size_t flag_bytes = (size_t)(rows * tiles + rows) * 4;
