@@ -23,8 +23,9 @@
 //     and labels move 16 bytes per access at element alignment;
 //   * each item evaluates its rolling tile (halo staged in LDS) and then folds
 //     the tile's values into the series' label accumulators, either in
-//     LDS-privatised slots flushed once per tile (few labels, or skewed labels)
-//     or straight into global float64/uint32 accumulators (many labels) — the
+//     LDS-privatised slots (flushed with global atomics when the workgroup
+//     moves to another series or its grab ends; labels beyond the slots go to
+//     global atomics) or straight into global float64/uint32 accumulators — the
 //     strategy is chosen per series by the host (heuristic, autotuner or cost
 //     model);
 //   * a finalize kernel writes group means (float64 sum / count, rounded once).
@@ -39,7 +40,7 @@ namespace {
 // elements with 4096 label slots: grabs of 4 / 8 / 16 tiles 11.0 / 10.0 /
 // 9.75 ms). Knob: NBG_RAGGED_GRAB.
 constexpr int kRaggedGrab = 16;
-constexpr int kRaggedMaxGrab = 64;
+constexpr int kRaggedMaxGrab = 64;  // a grab's tile descriptors are held in LDS
 constexpr int64_t kRaggedLdsLabels = 1024;     // default LDS label slots per launch
 constexpr int64_t kRaggedMaxLdsLabels = 8192;  // 96 KB of slots
 
