@@ -114,11 +114,12 @@ __device__ __forceinline__ void g_atomic_add(unsigned long long* p, unsigned lon
 // One accumulation pass over R rows x one chunk of positions. The R rows
 // share each label load. LDS (privatised strategy): per-(row, label)
 // accumulators, flushed once with global atomics.
-// HOT (float sum passes with LDS accumulators): labels 0 and 1 of every row
-// accumulate in registers, and each wave adds them to the LDS slots once at
-// the end. Skewed label distributions put most values on the first labels
-// (Zipf 1.1 over 1000 labels: 18% on label 0, 8% on label 1), whose LDS
-// atomics would otherwise serialise within a wave.
+// HOT (float sum passes with LDS accumulators): in chunks whose labels are
+// skewed towards labels 0 and 1, those two labels of every row accumulate in
+// registers, and each wave adds them to the LDS slots once at the end. Skewed
+// label distributions put most values on the first labels (Zipf 1.1 over 1000
+// labels: 18% on label 0, 8% on label 1), whose LDS atomics would otherwise
+// serialise within a wave.
 template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT, int U = kGroupUnroll,
           bool PIPE = false, bool HOT = false>
 __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
@@ -303,22 +304,22 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
         // are issued before any update, so one wait covers them.
         W ll[U];
         V vv[R][U];
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < U; ++u) ll[u] = *reinterpret_cast<const W*>(lab + base + u * STEP);
-  #pragma unroll
+#pragma unroll
         for (int r = 0; r < R; ++r) {
           if (r < nr) {
             const TV* vr = vbase + (int64_t)r * A.ld_v;
-  #pragma unroll
+#pragma unroll
             for (int u = 0; u < U; ++u) vv[r][u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
           }
         }
-  #pragma unroll
+#pragma unroll
         for (int r = 0; r < R; ++r) {
           if (r >= nr) break;
-  #pragma unroll
+#pragma unroll
           for (int u = 0; u < U; ++u)
-  #pragma unroll
+#pragma unroll
             for (int e = 0; e < VE; ++e) update(hot_c, vv[r][u].v[e], ll[u].v[e], base + u * STEP + e, r);
         }
       }
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       V vc[U], vn[U];
       {
         const int64_t b0 = c0 + (int64_t)tid * VE;
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < U; ++u) {
           ll[u] = *reinterpret_cast<const W*>(lab + b0 + u * STEP);
           vc[u] = *reinterpret_cast<const V*>(vbase + b0 + u * STEP);
@@ -337,35 +338,35 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       }
       for (int64_t k = 0; k < nfull; ++k) {
         const int64_t base = c0 + k * SUPER + (int64_t)tid * VE;
-  #pragma unroll
+#pragma unroll
         for (int r = 0; r < R; ++r) {
           if (r < nr) {
             if (r + 1 < nr) {
               const TV* vr = vbase + (int64_t)(r + 1) * A.ld_v;
-  #pragma unroll
+#pragma unroll
               for (int u = 0; u < U; ++u) vn[u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
             } else if (k + 1 < nfull) {
-  #pragma unroll
+#pragma unroll
               for (int u = 0; u < U; ++u) {
                 lln[u] = *reinterpret_cast<const W*>(lab + base + SUPER + u * STEP);
                 vn[u] = *reinterpret_cast<const V*>(vbase + base + SUPER + u * STEP);
               }
             }
-  #pragma unroll
+#pragma unroll
             for (int u = 0; u < U; ++u)
-  #pragma unroll
+#pragma unroll
               for (int e = 0; e < VE; ++e) update(hot_c, vc[u].v[e], ll[u].v[e], base + u * STEP + e, r);
-  #pragma unroll
+#pragma unroll
             for (int u = 0; u < U; ++u) vc[u] = vn[u];
           }
         }
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < U; ++u) ll[u] = lln[u];
       }
     }
     for (int64_t j = c0 + nfull * SUPER + tid; j < c1; j += kGroupThreads) {
       const TL lj = lab[j];
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < R; ++r)
         if (r < nr) update(hot_c, vbase[(int64_t)r * A.ld_v + j], lj, j, r);
     }
