@@ -6,6 +6,7 @@ ffill/bfill (numbagg/__init__.py). NumPy inputs are computed on the GPU and
 returned as NumPy arrays; torch tensors on the GPU stay on the GPU.
 """
 
+from ._lib import spin_timeouts
 from .funcs import bfill, ffill
 from .grouped import (
     group_nanargmax,
@@ -84,6 +85,7 @@ __all__ = [
     "move_std",
     "move_sum",
     "move_var",
+    "spin_timeouts",
     "GROUPED_FUNCS",
     "MOVE_EXP_FUNCS",
     "MOVE_FUNCS",

@@ -10,10 +10,12 @@ torch's caching allocator, so steady-state calls never hit hipMalloc.
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 import torch
 
+from . import _lib
 from ._lib import NBG_F32, NBG_F64, NBG_I32, NBG_I64
 
 _NBG_CODE = {
@@ -129,8 +131,40 @@ def ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr() if t is not None else 0)
 
 
+_spin_lock = threading.Lock()
+_spin_seen: dict[int, int] = {}
+
+
+def check_spin_timeouts() -> None:
+    """Raise if a bounded device-side wait expired since the last check.
+
+    The look-back kernels (fill, exponential scans) wait on other workgroups
+    with a bounded spin; an expiry means a broken cross-workgroup protocol and
+    invalid outputs. The counter is per device and only grows (until
+    ``nbg_reset_spin_timeouts``), so any increase since the last observation
+    on this device is reported, whichever call caused it.
+    """
+    dev_id = torch.cuda.current_device()
+    count = _lib.spin_timeouts()
+    with _spin_lock:
+        seen = _spin_seen.get(dev_id, 0)
+        _spin_seen[dev_id] = count
+    if count > seen:
+        raise RuntimeError(
+            f"numbagg_amd: {count - seen} bounded device-side wait(s) expired; "
+            "the results of the calls since the last check are invalid"
+        )
+
+
 def to_output(t: torch.Tensor, as_tensor: bool):
-    """Return ``t`` as a torch tensor (device inputs) or a NumPy array."""
+    """Return ``t`` as a torch tensor (device inputs) or a NumPy array.
+
+    The NumPy path synchronises anyway, so it also checks the spin-timeout
+    counter; callers that keep tensors on the GPU poll
+    ``numbagg_amd.spin_timeouts()`` themselves.
+    """
     if as_tensor:
         return t
-    return t.cpu().numpy()
+    res = t.cpu().numpy()
+    check_spin_timeouts()
+    return res

@@ -925,6 +925,10 @@ int fill_entry(const char* name, bool rev, int dtype, const void* x, void* out, 
   if (rows == 0 || n == 0) return NBG_OK;
   if (x == nullptr || out == nullptr) return set_error(NBG_ERR_INVALID, "%s: null pointer", name);
   if (rows > 1 && (ld_x < n || ld_out < n)) return set_error(NBG_ERR_INVALID, "%s: row stride < n", name);
+  // A gap within a row is at most n - 1 positions, so every limit >= n fills
+  // like limit = n; clamping keeps `carry + limit + 1` from overflowing for
+  // limits near INT64_MAX (the reference accepts any non-negative limit).
+  if (limit > n) limit = n;
   hipStream_t s = static_cast<hipStream_t>(stream);
   switch (dtype) {
     case NBG_F32: return fill_impl<float>(rev, x, out, rows, n, ld_x, ld_out, limit, ws, ws_bytes, s);

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   const bool aligned = (((uintptr_t)(vbase + c0)) % 16 == 0) && (((uintptr_t)(lab + c0)) % 16 == 0) &&
                        (nr == 1 || (A.ld_v % VE) == 0);
   const int64_t nfull = aligned ? (c1 - c0) / SUPER : 0;
-  struct alignas(16) V { TV v[VE]; };
+  struct alignas(VE * sizeof(TV)) V { TV v[VE]; };
   struct alignas(VE * sizeof(TL)) W { TL v[VE]; };
   // HOT: the register path runs when labels 0 and 1 are frequent in the
   // chunk, judged from its first 256 x VE labels (one block-wide count);
@@ -487,7 +487,7 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
       const unsigned long long c = static_cast<const unsigned long long*>(A.acc2)[q];
       const double m2 = static_cast<const double*>(A.acc1)[q];
       const double den = (double)((long long)c - A.ddof);
-      if (!(den > 0)) {
+      if (c == 0 || !(den > 0)) {  // an empty label is NaN whatever ddof (0/0 in the reference)
         r = missing_value<TV>();
       } else {
         const double var = m2 / den;
@@ -536,6 +536,7 @@ constexpr int group_blocks_per_cu(int pass) { return (pass == kPSum || pass == k
 template <typename TV, typename TL, int PASS, bool LDS, int R, bool COUNT, int U = kGroupUnroll,
           bool PIPE = false, bool HOT = false>
 int launch_pass(GArgs a, size_t lds, hipStream_t s) {
+  if (a.n == 0) return NBG_OK;  // nothing to accumulate: init and finalize give the empty-label outputs
   const int64_t groups = (a.rows + R - 1) / R;
   int cap = device_capacity_blocks((const void*)group_pass_kernel<TV, TL, PASS, LDS, R, COUNT, U, PIPE, HOT>,
                                    kGroupThreads, LDS ? lds : 0);

@@ -235,7 +235,9 @@ class exact:
                     m = s / np.where(cnt > 0, cnt, 1)
                     m2 = np.bincount(lr, weights=(xr - m[lr]) ** 2, minlength=num_labels)
                     den = cnt - ddof
-                    var = np.where(den > 0, m2 / np.where(den > 0, den, 1), np.nan)
+                    # an empty label is NaN whatever ddof: the reference's
+                    # (0 - 0**2/0) / (0 - ddof) (numbagg/grouped.py:166-174)
+                    var = np.where((den > 0) & (cnt > 0), m2 / np.where(den > 0, den, 1), np.nan)
                 out[r] = np.sqrt(var) if func == "group_nanstd" else var
             elif func in ("group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast"):
                 pos = np.nonzero(ok)[0]

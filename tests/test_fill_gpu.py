@@ -175,3 +175,23 @@ def test_fill_axes_strides_and_dtypes(gpu_lib):
     i = rng.integers(-5, 5, size=(4, 33)).astype(np.int32)
     np.testing.assert_array_equal(nb.bfill(i, limit=1), i)
 
+
+
+@pytest.mark.parametrize("two_pass", ["1", "0"])
+@pytest.mark.parametrize("func", ["ffill", "bfill"])
+def test_fill_huge_limit(gpu_lib, monkeypatch, two_pass, func):
+    """Limits up to INT64_MAX fill like limit=None (the reference accepts any
+    non-negative limit); on the two-pass path `carry + limit + 1` must not
+    overflow."""
+    import sys
+
+    import numbagg_amd as nb
+    from oracle import exact
+
+    monkeypatch.setenv("NBG_FILL_TWO_PASS", two_pass)
+    x = runs_array(16, 70_000, seed=5)
+    x[1, 100:30_000] = np.nan
+    want = exact.fill(func, x, None)
+    for limit in (np.iinfo(np.int64).max, sys.maxsize - 1, 70_000, 1 << 40):
+        got = getattr(nb, func)(x, limit=int(limit))
+        np.testing.assert_array_equal(_bits(got), _bits(want))

@@ -259,3 +259,44 @@ def test_group_zipf_bench_shape(gpu_lib):
             got = getattr(nb, func)(v, lab, num_labels=1000, axis=-1)
             want = exact.group(func, v, lab, 1000)
             assert_allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
+ALL_GROUP_FUNCS = EXACT_FUNCS + FLOAT_FUNCS
+
+
+@pytest.mark.parametrize("func", ALL_GROUP_FUNCS)
+def test_group_empty_core_axis(gpu_lib, func):
+    """An empty reduced axis with explicit num_labels gives the empty-label
+    outputs (0 for sum/count, NaN otherwise) in every axis mode, like the
+    reference (which reduces nothing)."""
+    import numbagg_amd as nb
+
+    empty_label = 0.0 if func in ("group_nansum", "group_nancount") else np.nan
+    for dtype in (np.float32, np.float64):
+        v = np.zeros((4, 0), dtype=dtype)
+        got = getattr(nb, func)(v, np.zeros(0, np.int64), axis=-1, num_labels=3)
+        assert got.shape == (4, 3) and got.dtype == dtype
+        np.testing.assert_array_equal(got, np.full((4, 3), empty_label, dtype=dtype))
+        got = getattr(nb, func)(np.zeros((2, 0, 5), dtype=dtype), np.zeros((0, 5), np.int32), axis=(1, 2), num_labels=2)
+        np.testing.assert_array_equal(got, np.full((2, 2), empty_label, dtype=dtype))
+        got = getattr(nb, func)(np.zeros((0, 5), dtype=dtype), np.zeros((0, 5), np.int32), num_labels=2)
+        np.testing.assert_array_equal(got, np.full(2, empty_label, dtype=dtype))
+
+
+def test_group_negative_ddof_empty_label(gpu_lib):
+    """ddof < 0 leaves count - ddof > 0 for an empty label; the reference's
+    (0 - 0**2/0) / (0 - ddof) is still NaN there."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    v, lab = _data(2, 5000, 30, np.float64, seed=2)
+    lab[lab == 7] = 3  # label 7 empty
+    for func in ("group_nanvar", "group_nanstd"):
+        for dtype in (np.float32, np.float64):
+            vv = v.astype(dtype)
+            got = getattr(nb, func)(vv, lab, num_labels=30, axis=-1, ddof=-1)
+            assert np.isnan(got[:, 7]).all()
+            want = exact.group(func, vv, lab, 30, ddof=-1)
+            ok = ~np.isnan(want)
+            assert np.array_equal(np.isnan(got), np.isnan(want))
+            assert_allclose(got[ok], want[ok], rtol=2e-7 if dtype == np.float32 else 1e-12)

@@ -67,3 +67,21 @@ def test_group_errors():
         nb.group_nansum(values2d, labels.astype(float), axis=1)
     with pytest.raises(TypeError, match="does not support boolean input"):
         nb.group_nanvar(np.array([True, False]), np.array([0, 0]), num_labels=1)
+
+
+def test_spin_timeout_increase_raises(monkeypatch):
+    # The synchronising NumPy path reports an expired device-side wait instead
+    # of returning its (invalid) outputs; the counter is simulated on the host.
+    import torch
+
+    from numbagg_amd import _device, _lib
+
+    counts = iter([0, 0, 2, 2])
+    monkeypatch.setattr(_lib, "spin_timeouts", lambda: next(counts))
+    monkeypatch.setattr(torch.cuda, "current_device", lambda: 0)
+    monkeypatch.setattr(_device, "_spin_seen", {})
+    _device.check_spin_timeouts()
+    _device.check_spin_timeouts()
+    with pytest.raises(RuntimeError, match="wait"):
+        _device.check_spin_timeouts()
+    _device.check_spin_timeouts()  # no new expiry since the last check

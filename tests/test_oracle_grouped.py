@@ -87,3 +87,16 @@ def test_planted_mistake_is_caught():
     # "last index wins" would give 2
     assert right[0, 0] == 0.0
     assert exact.group("group_nanargmax", values, labels, 1)[0, 0] == 0.0
+
+
+@pytest.mark.parametrize("func", ["group_nanvar", "group_nanstd"])
+def test_negative_ddof_empty_label_is_nan(func):
+    # count 0, ddof -1: denom 1 > 0, but (0 - 0**2/0) / 1 is NaN (grouped.py:166-174)
+    v = np.array([[1.0, 2.0, 4.0, np.nan]])
+    lab = np.array([0, 0, 2, 1])
+    for oracle_group in (restated.group, exact.group):
+        got = oracle_group(func, v, lab, 3, ddof=-1)
+        assert np.isnan(got[0, 1])
+        want0 = 0.5 / 3 if func == "group_nanvar" else np.sqrt(0.5 / 3)
+        assert_allclose(got[0, 0], want0)
+        assert got[0, 2] == 0.0
