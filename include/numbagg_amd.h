@@ -206,6 +206,21 @@ int nbg_group_nanstd(int vdtype, const void* values, int ldtype,
                      int strategy, void* ws, size_t ws_bytes,
                      void* stream); /* grouped.py:179-208 */
 
+/* No counterpart in the reference. Allocation-free variant of
+ * nbg_group_nansum for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * same arguments and results, but the scratch comes from a static per-device
+ * buffer of NBG_GROUP_SCRATCH_BYTES instead of a workspace argument. Calls
+ * whose nbg_group_workspace_bytes(...) exceeds it fail with NBG_ERR_INVALID
+ * (32 bytes per (row, label): rows x num_labels <= 524288). Every call on one
+ * device shares that device's buffer, so calls that overlap on one device
+ * (different streams or host threads without ordering) race. */
+#define NBG_GROUP_SCRATCH_BYTES (16 * 1024 * 1024)
+int nbg_group_nansum_noalloc(int vdtype, const void* values, int ldtype,
+                             const void* labels, void* out, int64_t rows,
+                             int64_t n, int64_t ld_v, int64_t num_labels,
+                             int strategy, void* stream);
+
 /* ---- ragged batched serving path (no reference counterpart: its reference
  *      semantics are per-series plain calls of move_mean, moving.py:12-55,
  *      and group_nanmean, grouped.py:7-27) ---------------------------------- */

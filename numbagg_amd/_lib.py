@@ -79,6 +79,10 @@ for _name in ("nansum", "nanmean", "nancount", "nanargmax", "nanargmin", "nanfir
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP
 for _name in ("nanvar", "nanstd"):
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP_DDOF
+_SIGNATURES["nbg_group_nansum_noalloc"] = (
+    _c_int,
+    [_c_int, _c_ptr, _c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_int, _c_ptr],
+)
 _SIGNATURES["nbg_ragged_tile_size"] = (_c_i64, [])
 _SIGNATURES["nbg_ragged_workspace_bytes"] = (_c_size, [_c_i64, _c_i64])
 _SIGNATURES["nbg_ragged_max_lds_labels"] = (_c_i64, [])

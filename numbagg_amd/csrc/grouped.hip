@@ -20,6 +20,9 @@
 // float32/int32 argmax/argmin so that the first index wins ties in one pass,
 // key-then-index passes for 64-bit values, index min/max for first/last, and a
 // corrected two-pass variance (per-label means, then sums of deviations).
+#include <mutex>
+#include <unordered_map>
+
 #include "common.hpp"
 
 namespace nbg {
@@ -790,9 +793,54 @@ int group_entry(const char* name, int vdtype, const void* values, int ldtype, co
 }
 
 }  // namespace
+
+// ---- allocation-free variant of group_nansum --------------------------------
+// Scratch of nbg_group_nansum_noalloc: one static array per device (each GPU
+// the library runs on holds it until the process ends). Host code never uses
+// the array's name as an address: the per-device address comes from
+// hipGetSymbolAddress with that device current.
+static __device__ __align__(256) unsigned char scratch[NBG_GROUP_SCRATCH_BYTES];
+
+// Device address of the current device's copy of `scratch`, resolved on the
+// first call on each device; nullptr (with the error recorded) on failure.
+static void* group_scratch_ptr() {
+  static std::mutex mu;
+  static std::unordered_map<int, void*> ptrs;
+  int dev = 0;
+  if (check_hip(hipGetDevice(&dev), "hipGetDevice") != NBG_OK) return nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = ptrs.find(dev);
+  if (it != ptrs.end()) return it->second;
+  void* ptr = nullptr;
+  if (check_hip(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") != NBG_OK)
+    return nullptr;
+  ptrs[dev] = ptr;
+  return ptr;
+}
+
 }  // namespace nbg
 
 extern "C" {
+
+int nbg_group_nansum_noalloc(int vdtype, const void* values, int ldtype, const void* labels, void* out,
+                             int64_t rows, int64_t n, int64_t ld_v, int64_t num_labels, int strategy,
+                             void* stream) {
+  if (rows > 0 && num_labels > 0) {
+    // rows * num_labels bounded first, so the size query cannot overflow
+    const size_t need = (rows > (int64_t)NBG_GROUP_SCRATCH_BYTES || num_labels > (int64_t)NBG_GROUP_SCRATCH_BYTES)
+                            ? (size_t)-1
+                            : nbg::group_ws_bytes(rows, num_labels);
+    if (rows * num_labels > (int64_t)NBG_GROUP_SCRATCH_BYTES || need > (size_t)NBG_GROUP_SCRATCH_BYTES)
+      return nbg::set_error(NBG_ERR_INVALID,
+                            "nbg_group_nansum_noalloc: %lld rows x %lld labels need more than the %d bytes "
+                            "of static scratch",
+                            (long long)rows, (long long)num_labels, NBG_GROUP_SCRATCH_BYTES);
+  }
+  void* ptr = nbg::group_scratch_ptr();
+  if (ptr == nullptr) return NBG_ERR_HIP;
+  return nbg::group_entry<nbg::kGSum>("nbg_group_nansum_noalloc", vdtype, values, ldtype, labels, out, rows, n,
+                                      ld_v, num_labels, 0, strategy, ptr, (size_t)NBG_GROUP_SCRATCH_BYTES, stream);
+}
 
 size_t nbg_group_workspace_bytes(int vdtype, int64_t rows, int64_t n, int64_t num_labels,
                                  int strategy) {
