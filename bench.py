@@ -15,10 +15,14 @@ inputs already resident in HBM:
 written once) divided by its time. The roofline object covers the step's
 dominant function, timed with HIP events on the stream its kernels are
 launched on; for ffill these are the two kernels of one call (independent
-tiles, then the fix-up of the tiles' leading NaN runs). `cpu_baseline` times the C restatement of the reference kernels (OpenMP over
-rows, the stand-in for numba's parallel target) on a bounded sample.
+tiles, then the fix-up of the tiles' leading NaN runs). Every other
+hot-path function is timed on the same inputs and reported beside the step
+(`other_functions`; --headline skips them). `cpu_baseline` times the C
+restatement of the reference kernels (OpenMP over rows, the stand-in for
+numba's parallel target), compiled for the host that runs it, on a bounded
+sample with at least as many rows as threads.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--all] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--headline] [--no-cpu]
 """
 
 from __future__ import annotations
@@ -112,6 +116,12 @@ def has_entry(nb_lib, name):
         return False
 
 
+def cpu_rows(total, per_thread):
+    """Rows of the CPU sample: `per_thread` rows for every host thread (so
+    OpenMP over rows keeps every thread busy), at most the whole input."""
+    return min(total, per_thread * len(os.sched_getaffinity(0)))
+
+
 def build_workloads(torch, nb, lib, want_all):
     wl = []
     if has_entry(lib, "nbg_move_mean"):
@@ -121,7 +131,7 @@ def build_workloads(torch, nb, lib, want_all):
             return nb.move_mean(x, window=1000, min_count=500)
 
         def cpu_mm(x=x):
-            xs = x[:8].cpu().numpy()
+            xs = x[:cpu_rows(256, 2)].cpu().numpy()
             return ("move_mean", xs, dict(window=1000, min_count=500)), xs.nbytes * 2
 
         wl.append(Workload("move_mean", {"shape": [256, 2_000_000], "dtype": "f32", "window": 1000, "min_count": 500},
@@ -134,7 +144,7 @@ def build_workloads(torch, nb, lib, want_all):
             return nb.group_nansum(v, lab, num_labels=1000, axis=-1)
 
         def cpu_gs(v=v, lab=lab):
-            vs = v[:8].cpu().numpy()
+            vs = v[:cpu_rows(64, 2)].cpu().numpy()
             return ("group_nansum", vs, dict(labels=lab.cpu().numpy(), num_labels=1000)), vs.nbytes + lab.numel() * 4
 
         wl.append(Workload("group_nansum", {"shape": [64, 4_000_000], "dtype": "f32", "num_labels": 1000, "labels": "uniform int32"},
@@ -146,7 +156,7 @@ def build_workloads(torch, nb, lib, want_all):
         return nb.ffill(x)
 
     def cpu_ff(x=x):
-        xs = x[:64].cpu().numpy()
+        xs = x[:cpu_rows(1024, 4)].cpu().numpy()
         return ("ffill", xs, dict(limit=None)), xs.nbytes * 2
 
     wl.append(Workload("ffill", {"shape": [1024, 1_000_000], "dtype": "f32", "limit": None, "nan": "30% geometric runs"},
@@ -167,6 +177,8 @@ def build_extra(torch, nb, wl):
         cfg = {"shape": [256, 2_000_000], "dtype": "f32", "window": 1000, "min_count": 500}
         for f in ("move_sum", "move_std", "move_var"):
             extra.append(Workload(f, cfg, n * 8, (lambda f=f, x=x: getattr(nb, f)(x, window=1000, min_count=500))))
+        extra.append(Workload("move_mean[window=20]", dict(cfg, window=20, min_count=20), n * 8,
+                              (lambda x=x: nb.move_mean(x, window=20, min_count=20))))
         for f in ("move_cov", "move_corr"):
             extra.append(Workload(f, cfg, n * 12, (lambda f=f, x=x, y=y: getattr(nb, f)(x, y, window=1000, min_count=500))))
         ecfg = {"shape": [256, 2_000_000], "dtype": "f32 in, f64 out (Python-float alpha)", "alpha": 0.01}
@@ -181,11 +193,16 @@ def build_extra(torch, nb, wl):
         for zname, lb in (("uniform", lab), ("zipf1.1", labz)):
             cfg = {"shape": [64, 4_000_000], "dtype": "f32", "num_labels": 1000, "labels": zname}
             for f in ("group_nansum", "group_nanmean", "group_nanvar", "group_nanstd", "group_nancount",
-                      "group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast"):
+                      "group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast", "group_nanmin",
+                      "group_nanmax", "group_nanprod", "group_nansum_of_squares", "group_nanany", "group_nanall"):
                 if zname == "uniform" and f == "group_nansum":
                     continue
                 extra.append(Workload(f"{f}[{zname}]", cfg, nbytes,
                                       (lambda f=f, v=v, lb=lb: getattr(nb, f)(v, lb, num_labels=1000, axis=-1))))
+        lab64 = lab.to(torch.int64)
+        cfg = {"shape": [64, 4_000_000], "dtype": "f32", "num_labels": 1000, "labels": "uniform int64"}
+        extra.append(Workload("group_nansum[uniform,int64 labels]", cfg, v.numel() * 4 + lab.numel() * 8 + 64 * 1000 * 4,
+                              (lambda v=v, lb=lab64: nb.group_nansum(v, lb, num_labels=1000, axis=-1))))
     src = [w.copy_src for w in wl if hasattr(w, "copy_src")]
     if src:
         x = src[0]
@@ -332,8 +349,9 @@ def measured_traffic(w):
     return int(rec["read_bytes"] + rec["write_bytes"])
 
 
-def time_cpu(spec, budget_s=4.0):
-    """Median wall time of the oracle restatement on the sample (>=3 runs)."""
+def time_cpu(spec, budget_s=6.0):
+    """Median wall time of the oracle restatement on the sample (>=5 runs
+    after one warm-up)."""
     from oracle import restated
 
     func, arr, kw = spec
@@ -346,7 +364,7 @@ def time_cpu(spec, budget_s=4.0):
     call()  # warm-up
     times = []
     t_end = time.perf_counter() + budget_s
-    while len(times) < 3 or (time.perf_counter() < t_end and len(times) < 15):
+    while len(times) < 5 or (time.perf_counter() < t_end and len(times) < 15):
         t0 = time.perf_counter()
         call()
         times.append(time.perf_counter() - t0)
@@ -358,8 +376,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--all", action="store_true",
-                    help="also time every other hot-path function on the same inputs (reported, not in value)")
+    ap.add_argument("--headline", action="store_true",
+                    help="time only the headline step (skip every other hot-path function)")
+    ap.add_argument("--all", action="store_true", help=argparse.SUPPRESS)  # the default; kept for old command lines
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-copy", action="store_true", help="skip the streaming-copy reference")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batched comparison")
@@ -388,7 +407,8 @@ def main(argv=None):
 
     lib = _lib.load()
     print(f"# device: {torch.cuda.get_device_name(0)}", file=sys.stderr, flush=True)
-    wl = build_workloads(torch, nb, lib, args.all)
+    want_all = not args.headline
+    wl = build_workloads(torch, nb, lib, want_all)
     if args.only:
         keep = set(args.only.split(","))
         wl = [w for w in wl if w.name in keep]
@@ -441,17 +461,17 @@ def main(argv=None):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        for _ in range(3):
+        for _ in range(10):
             copy_once()
         b.record(stream)
         torch.cuda.synchronize()
-        copy_ms = a.elapsed_time(b) / 3
+        copy_ms = a.elapsed_time(b) / 10
         copy_gbs = 2 * src.numel() * src.element_size() / (copy_ms * 1e-3) / 1e9
         del dst
         print(f"# copy reference: {copy_ms:.3f} ms  {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     all_rec = None
-    if args.all:
+    if want_all and not args.only:
         all_rec = {}
         for w in build_extra(torch, nb, wl):
             ms = time_workload(torch, w, max(3, args.steps), stream)
@@ -476,14 +496,21 @@ def main(argv=None):
 
     cpu = None
     if not args.no_cpu:
+        import tempfile
+
+        import oracle
+
+        oracle.use_native_build(tempfile.mkdtemp(prefix="nbg_oracle_"))
         threads = len(os.sched_getaffinity(0))
         omp = int(os.environ.get("OMP_NUM_THREADS", threads))
         cpu_per = {}
+        samples = []
         tot_b, tot_t = 0, 0.0
         for w in wl:
             if w.cpu is None:
                 continue
             spec, nbytes = w.cpu()
+            samples.append(f"{spec[1].shape[0]} rows of {w.name}")
             t = time_cpu(spec)
             cpu_per[w.name] = round(nbytes / t / 1e9, 2)
             tot_b += nbytes
@@ -494,8 +521,9 @@ def main(argv=None):
             "unit": "GB/s",
             "cores": min(omp, threads),
             "kind": "port",
-            "sample": "C restatement of the numbagg kernels (numba unavailable), OpenMP over rows, "
-            "on the first 8 rows of the move_mean and group_nansum inputs and 64 rows of the ffill input",
+            "sample": "C restatement of the numbagg kernels (numba unavailable) built with -O3 -march=native "
+            "-ffp-contract=off on this host, OpenMP over rows, median of >=5 runs on the first "
+            + ", ".join(samples),
             "per_function": cpu_per,
         }
 

@@ -149,14 +149,16 @@ int nbg_move_exp_nancorr(int x_dtype, int dtype, const void* a, const void* b,
                          double min_weight, void* ws, size_t ws_bytes,
                          void* stream); /* moving_exp.py:276-335 */
 
-/* ---- grouped reductions (numbagg/grouped.py:7-208; groupndreduce.__call__
+/* ---- grouped reductions (numbagg/grouped.py:7-270; groupndreduce.__call__
  *      at numbagg/decorators.py:558-674) ---------------------------------- */
 /* values: (rows, n) with row stride ld_v; labels: (n,) shared by all rows,
  * int32 or int64; out: (rows, num_labels) contiguous, in the values dtype.
  * Labels < 0 are skipped, and so are labels >= num_labels.
- * vdtype: NBG_F32/NBG_F64, or NBG_I32/NBG_I64 for sum, count, argmax,
- * argmin, first and last (mean/var/std take floats; the host widens integer
- * input to float64 as the reference does).
+ * vdtype: NBG_F32/NBG_F64, or NBG_I32/NBG_I64 for every function but mean,
+ * var and std (which take floats; the host widens integer input to float64 as
+ * the reference does). Integer sums, sums of squares and products wrap like
+ * the reference's fixed-width integer arithmetic; float ones are formed in
+ * float64 and rounded once.
  * strategy: per-label accumulators privatised in LDS, or global atomics. */
 #define NBG_GROUP_AUTO 0
 #define NBG_GROUP_LDS 1
@@ -205,6 +207,32 @@ int nbg_group_nanstd(int vdtype, const void* values, int ldtype,
                      int64_t ld_v, int64_t num_labels, int64_t ddof,
                      int strategy, void* ws, size_t ws_bytes,
                      void* stream); /* grouped.py:179-208 */
+
+int nbg_group_nanmin(int vdtype, const void* values, int ldtype,
+                     const void* labels, void* out, int64_t rows, int64_t n,
+                     int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                     size_t ws_bytes, void* stream); /* grouped.py:211-225 */
+int nbg_group_nanmax(int vdtype, const void* values, int ldtype,
+                     const void* labels, void* out, int64_t rows, int64_t n,
+                     int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                     size_t ws_bytes, void* stream); /* grouped.py:228-244 */
+int nbg_group_nanprod(int vdtype, const void* values, int ldtype,
+                      const void* labels, void* out, int64_t rows, int64_t n,
+                      int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                      size_t ws_bytes, void* stream); /* grouped.py:124-132 */
+int nbg_group_nansum_of_squares(int vdtype, const void* values, int ldtype,
+                                const void* labels, void* out, int64_t rows,
+                                int64_t n, int64_t ld_v, int64_t num_labels,
+                                int strategy, void* ws, size_t ws_bytes,
+                                void* stream); /* grouped.py:135-145 */
+int nbg_group_nanany(int vdtype, const void* values, int ldtype,
+                     const void* labels, void* out, int64_t rows, int64_t n,
+                     int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                     size_t ws_bytes, void* stream); /* grouped.py:247-257 */
+int nbg_group_nanall(int vdtype, const void* values, int ldtype,
+                     const void* labels, void* out, int64_t rows, int64_t n,
+                     int64_t ld_v, int64_t num_labels, int strategy, void* ws,
+                     size_t ws_bytes, void* stream); /* grouped.py:260-270 */
 
 /* No counterpart in the reference. Allocation-free variant of
  * nbg_group_nansum for callers that cannot allocate device memory (code under

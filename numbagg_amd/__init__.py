@@ -9,14 +9,20 @@ returned as NumPy arrays; torch tensors on the GPU stay on the GPU.
 from ._lib import spin_timeouts
 from .funcs import bfill, ffill
 from .grouped import (
+    group_nanall,
+    group_nanany,
     group_nanargmax,
     group_nanargmin,
     group_nancount,
     group_nanfirst,
     group_nanlast,
+    group_nanmax,
     group_nanmean,
+    group_nanmin,
+    group_nanprod,
     group_nanstd,
     group_nansum,
+    group_nansum_of_squares,
     group_nanvar,
 )
 from .moving import move_corr, move_cov, move_mean, move_std, move_sum, move_var
@@ -32,14 +38,20 @@ from .moving_exp import (
 
 # Family lists restricted to the accelerated functions (numbagg/__init__.py:64-127).
 GROUPED_FUNCS = [
+    group_nanall,
+    group_nanany,
     group_nanargmax,
     group_nanargmin,
     group_nancount,
     group_nanfirst,
     group_nanlast,
+    group_nanmax,
     group_nanmean,
+    group_nanmin,
+    group_nanprod,
     group_nanstd,
     group_nansum,
+    group_nansum_of_squares,
     group_nanvar,
 ]
 
@@ -63,14 +75,20 @@ __all__ = [
     "__version__",
     "bfill",
     "ffill",
+    "group_nanall",
+    "group_nanany",
     "group_nanargmax",
     "group_nanargmin",
     "group_nancount",
     "group_nanfirst",
     "group_nanlast",
+    "group_nanmax",
     "group_nanmean",
+    "group_nanmin",
+    "group_nanprod",
     "group_nanstd",
     "group_nansum",
+    "group_nansum_of_squares",
     "group_nanvar",
     "move_corr",
     "move_cov",

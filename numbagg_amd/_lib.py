@@ -75,7 +75,8 @@ _GROUP_DDOF = (
     [_c_int, _c_ptr, _c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_int, _c_ptr, _c_size,
      _c_ptr],
 )
-for _name in ("nansum", "nanmean", "nancount", "nanargmax", "nanargmin", "nanfirst", "nanlast"):
+for _name in ("nansum", "nanmean", "nancount", "nanargmax", "nanargmin", "nanfirst", "nanlast", "nanmin",
+              "nanmax", "nanprod", "nansum_of_squares", "nanany", "nanall"):
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP
 for _name in ("nanvar", "nanstd"):
     _SIGNATURES[f"nbg_group_{_name}"] = _GROUP_DDOF

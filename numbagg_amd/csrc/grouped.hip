@@ -1,12 +1,13 @@
-// Grouped reductions (group_nan{sum,mean,count,argmax,argmin,first,last,var,std})
-// for gfx950.
+// Grouped reductions (group_nan{sum,mean,count,argmax,argmin,first,last,var,std,
+// min,max,prod,sum_of_squares,any,all}) for gfx950.
 //
-// Reference semantics (numbagg/grouped.py:7-208): values (rows, n) share one
+// Reference semantics (numbagg/grouped.py:7-270): values (rows, n) share one
 // label vector (n,); for every row, each valid value (not NaN) whose label is
 // >= 0 is reduced into out[row, label]. Labels >= num_labels are skipped (the
-// reference leaves them undefined). Empty labels give 0 (sum, count), NaN
-// (mean, var, std, arg*, first, last; integer outputs get the minimum integer,
-// which is what numba's NaN-to-integer store produces).
+// reference leaves them undefined). Empty labels give 0 (sum, count,
+// sum_of_squares, any), 1 (prod, all), NaN (mean, var, std, arg*, first, last,
+// min, max; integer outputs get the minimum integer, which is what numba's
+// NaN-to-integer store produces).
 //
 // Design:
 //   * a workgroup reduces one (row, chunk) of values. With the LDS strategy its
@@ -17,9 +18,11 @@
 //   * a finalize kernel turns accumulators into the output dtype.
 // Accumulators: float sums in float64 (rounded once to the output dtype),
 // exact integer counts, 64-bit packed (orderable key, ~index) maxima for
-// float32/int32 argmax/argmin so that the first index wins ties in one pass,
-// key-then-index passes for 64-bit values, index min/max for first/last, and a
-// corrected two-pass variance (per-label means, then sums of deviations).
+// float32/int32 argmax/argmin/min/max so that the first index wins ties in
+// one pass (min/max then copy the value at that index), key-then-index passes
+// for 64-bit values, index min/max for first/last, float64 (or wrapping
+// integer) products by compare-and-swap, flags for any/all, and a corrected
+// two-pass variance (per-label means, then sums of deviations).
 #include <mutex>
 #include <unordered_map>
 
@@ -41,6 +44,12 @@ enum GKind {
   kGLast = 6,
   kGVar = 7,
   kGStd = 8,
+  kGMin = 9,
+  kGMax = 10,
+  kGProd = 11,
+  kGSumSq = 12,
+  kGAny = 13,
+  kGAll = 14,
 };
 
 // Internal accumulation passes.
@@ -53,6 +62,10 @@ enum GPass {
   kPFirst = 5,    // min index
   kPLast = 6,     // max index + 1
   kPDev = 7,      // sum of squared deviations from the label mean
+  kPSumSq = 8,    // sum of squares (f64 or wrapping u64)
+  kPProd = 9,     // product (f64 or wrapping u64), compare-and-swap updates
+  kPAny = 10,     // flag: a valid non-zero value was seen
+  kPAll = 11,     // flag: a valid zero was seen
 };
 
 template <typename T>
@@ -114,6 +127,33 @@ __device__ __forceinline__ void g_atomic_add(unsigned long long* p, unsigned lon
   atomicAdd(p, v);
 }
 
+// Multiply an accumulator (LDS or global) by v with a compare-and-swap loop:
+// no hardware atomic multiplies. Products are formed in float64 (float data)
+// or as wrapping 64-bit integers (integer data, narrowed at the end as the
+// reference's wrapping integer multiplies are).
+__device__ __forceinline__ double mul_acc(double a, double v) { return a * v; }
+__device__ __forceinline__ unsigned long long mul_acc(unsigned long long a, unsigned long long v) { return a * v; }
+template <typename SA>
+__device__ __forceinline__ void atomic_mul(SA* p, SA v) {
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+  unsigned long long old = *reinterpret_cast<volatile unsigned long long*>(q);
+  while (true) {
+    SA cur;
+    __builtin_memcpy(&cur, &old, 8);
+    const SA nv = mul_acc(cur, v);
+    unsigned long long nb;
+    __builtin_memcpy(&nb, &nv, 8);
+    const unsigned long long seen = atomicCAS(q, old, nb);
+    if (seen == old) break;
+    old = seen;
+  }
+}
+// The identity of the product accumulator, as a 64-bit word.
+template <typename SA>
+constexpr unsigned long long one_bits() {
+  return std::is_same<SA, double>::value ? 0x3FF0000000000000ull : 1ull;
+}
+
 // One accumulation pass over R rows x one chunk of positions. The R rows
 // share each label load. LDS (privatised strategy): per-(row, label)
 // accumulators, flushed once with global atomics.
@@ -162,10 +202,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   const int64_t Lp = (int64_t)nr * L1;  // slots actually used
   if (LDS) {
     for (int64_t l = tid; l < Lp; l += kGroupThreads) {
-      if constexpr (PASS == kPSum) {
+      if constexpr (PASS == kPSum || PASS == kPSumSq) {
         l_s[l] = SA(0);
-        if (COUNT) l_c[l] = 0u;
-      } else if constexpr (PASS == kPCount) {
+        if (PASS == kPSum && COUNT) l_c[l] = 0u;
+      } else if constexpr (PASS == kPProd) {
+        l_s[l] = SA(1);
+      } else if constexpr (PASS == kPCount || PASS == kPAny || PASS == kPAll) {
         l_u32[l] = 0u;
       } else if constexpr (PASS == kPPacked || PASS == kPKey || PASS == kPLast) {
         l_u[l] = 0ull;
@@ -239,6 +281,19 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       } else {
         g_atomic_add(&g_s[l], (SA)v);
         if (COUNT) atomicAdd(&g_u1[l], 1ull);
+      }
+    } else if constexpr (PASS == kPSumSq) {
+      const SA sq = (SA)v * (SA)v;  // exact float64 squares of float32 data; wrapping for integers
+      if (LDS) atomicAdd(&l_s[l], sq);
+      else g_atomic_add(&g_s[l], sq);
+    } else if constexpr (PASS == kPProd) {
+      if (LDS) atomic_mul(&l_s[l], (SA)v);
+      else atomic_mul(&g_s[l], (SA)v);
+    } else if constexpr (PASS == kPAny || PASS == kPAll) {
+      // any: a valid non-zero value; all: a valid zero (-0.0 counts as zero)
+      if ((PASS == kPAny) == (v != TV(0))) {
+        if (LDS) atomicOr(&l_u32[l], 1u);
+        else atomicOr(&g_u0[l], 1ull);
       }
     } else if constexpr (PASS == kPCount) {
       if (LDS) atomicAdd(&l_u32[l], 1u);
@@ -374,8 +429,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
         if (r < nr) update(hot_c, vbase[(int64_t)r * A.ld_v + j], lj, j, r);
     }
   };
-  if (HOT && hot_on) stream(std::true_type{});
-  else stream(std::false_type{});
+  if constexpr (HOT) {
+    if (hot_on) stream(std::true_type{});
+    else stream(std::false_type{});
+  } else {
+    stream(std::false_type{});
+  }
 
   if (LDS) {
     if (HOT && hot_on) {  // wave sums of the hot labels into their LDS slots (block-uniform)
@@ -413,6 +472,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
         } else if (l_s[l] != SA(0)) {
           g_atomic_add(&g_s[l], l_s[l]);
         }
+      } else if constexpr (PASS == kPSumSq) {
+        if (l_s[l] != SA(0)) g_atomic_add(&g_s[l], l_s[l]);
+      } else if constexpr (PASS == kPProd) {
+        if (l_s[l] != SA(1)) atomic_mul(&g_s[l], l_s[l]);
+      } else if constexpr (PASS == kPAny || PASS == kPAll) {
+        if (l_u32[l]) atomicOr(&g_u0[l], 1ull);
       } else if constexpr (PASS == kPCount) {
         if (l_u32[l]) atomicAdd(&g_u0[l], (unsigned long long)l_u32[l]);
       } else if constexpr (PASS == kPPacked || PASS == kPKey || PASS == kPLast) {
@@ -472,14 +537,30 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
       r = c ? (TV)(s / (double)c) : missing_value<TV>();
     } else if constexpr (KIND == kGCount) {
       r = (TV)static_cast<const unsigned long long*>(A.acc0)[q];
-    } else if constexpr (KIND == kGArgMax || KIND == kGArgMin) {
-      const unsigned long long w = static_cast<const unsigned long long*>(A.acc0)[q];
-      const unsigned long long idx1 = static_cast<const unsigned long long*>(A.acc1)[q];
-      if (A.acc1 == nullptr) {  // packed (key32, ~idx32)
-        r = w ? (TV)(double)(uint32_t)(~(uint32_t)(w & 0xffffffffull)) : missing_value<TV>();
+    } else if constexpr (KIND == kGArgMax || KIND == kGArgMin || KIND == kGMin || KIND == kGMax) {
+      // index of the extreme: packed (key32, ~idx32) words, or the key-index pass
+      bool has;
+      unsigned long long idx;
+      if (A.acc1 == nullptr) {
+        const unsigned long long w = static_cast<const unsigned long long*>(A.acc0)[q];
+        has = w != 0ull;
+        idx = (uint32_t)(~(uint32_t)(w & 0xffffffffull));
       } else {
-        r = (idx1 != ~0ull) ? (TV)(double)idx1 : missing_value<TV>();
+        idx = static_cast<const unsigned long long*>(A.acc1)[q];
+        has = idx != ~0ull;
       }
+      if constexpr (KIND == kGArgMax || KIND == kGArgMin) {
+        r = has ? (TV)(double)idx : missing_value<TV>();
+      } else {
+        // the value at that index, through float64 as the reference's float64
+        // running extreme stores it (64-bit integers round there)
+        r = has ? (TV)(double)vals[row * A.ld_v + (int64_t)idx] : missing_value<TV>();
+      }
+    } else if constexpr (KIND == kGSumSq || KIND == kGProd) {
+      r = (TV)static_cast<const SA*>(A.acc0)[q];
+    } else if constexpr (KIND == kGAny || KIND == kGAll) {
+      const bool seen = static_cast<const unsigned long long*>(A.acc0)[q] != 0ull;
+      r = (KIND == kGAny) == seen ? TV(1) : TV(0);
     } else if constexpr (KIND == kGFirst) {
       const unsigned long long i = static_cast<const unsigned long long*>(A.acc0)[q];
       r = (i != ~0ull) ? vals[row * A.ld_v + (int64_t)i] : missing_value<TV>();
@@ -508,9 +589,11 @@ constexpr int64_t kGroupLdsBudget = 48 * 1024;
 int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
   switch (pass) {
     case kPSum: return sa_bytes + (count ? 4 : 0);
-    case kPCount: return 4;
     case kPDev: return 16;
     case kPKeyIdx: return 16;
+    case kPCount:
+    case kPAny:
+    case kPAll: return 4;
     default: return 8;
   }
 }
@@ -646,7 +729,11 @@ int64_t lds_bytes_per_label(int kind, int vdtype) {
     case kGVar:
     case kGStd: return 16;
     case kGArgMax:
-    case kGArgMin: return 16;
+    case kGArgMin:
+    case kGMin:
+    case kGMax: return 16;
+    case kGAny:
+    case kGAll: return 4;
     default: return 8;
   }
   (void)vdtype;
@@ -673,7 +760,7 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
   if (strategy == NBG_GROUP_GLOBAL) lds = false;
 
   a.means = nullptr;
-  a.argmin = (KIND == kGArgMin);
+  a.argmin = (KIND == kGArgMin || KIND == kGMin);
   int rc = 0;
   const bool packed = a.n < 0xffffffffLL;  // indices fit the low 32 bits
   switch (KIND) {
@@ -695,8 +782,29 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
       a.acc1 = nullptr;
       if ((rc = run_pass<TV, TL, kPCount>(a, lds, s))) return rc;
       return launch_finalize<TV, KIND>(a, nullptr, s);
+    case kGSumSq:
+      if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = nullptr;
+      if ((rc = run_pass<TV, TL, kPSumSq>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
+    case kGProd:
+      if ((rc = launch_init(acc0, one_bits<typename SumAcc<TV>::type>(), nullptr, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = nullptr;
+      if ((rc = run_pass<TV, TL, kPProd>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
+    case kGAny:
+    case kGAll:
+      if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
+      a.acc0 = acc0;
+      a.acc1 = nullptr;
+      if ((rc = run_pass<TV, TL, KIND == kGAny ? kPAny : kPAll>(a, lds, s))) return rc;
+      return launch_finalize<TV, KIND>(a, nullptr, s);
     case kGArgMax:
     case kGArgMin:
+    case kGMin:
+    case kGMax:
       if constexpr (sizeof(TV) == 4) {
         if (packed) {
           if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
@@ -875,5 +983,11 @@ NBG_GROUP(nbg_group_nanfirst, nbg::kGFirst)
 NBG_GROUP(nbg_group_nanlast, nbg::kGLast)
 NBG_GROUP_DDOF(nbg_group_nanvar, nbg::kGVar)
 NBG_GROUP_DDOF(nbg_group_nanstd, nbg::kGStd)
+NBG_GROUP(nbg_group_nanmin, nbg::kGMin)
+NBG_GROUP(nbg_group_nanmax, nbg::kGMax)
+NBG_GROUP(nbg_group_nanprod, nbg::kGProd)
+NBG_GROUP(nbg_group_nansum_of_squares, nbg::kGSumSq)
+NBG_GROUP(nbg_group_nanany, nbg::kGAny)
+NBG_GROUP(nbg_group_nanall, nbg::kGAll)
 
 }  // extern "C"

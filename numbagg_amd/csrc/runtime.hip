@@ -92,20 +92,13 @@ int launch_zero(void* p, size_t bytes, hipStream_t s) {
 
 // Streaming copy used to calibrate bandwidth measurements: the practical HBM
 // ceiling of a read+write stream, and the scale of the FETCH_SIZE/WRITE_SIZE
-// counters for a kernel of known traffic.
+// counters for a kernel of known traffic. One 16-byte vector per thread and a
+// one-shot grid: short-lived independent workgroups are the fastest copy on
+// MI355X (a grid-stride loop keeps fewer loads in flight per byte moved).
 __global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src,
                                                           uint4* __restrict__ dst, int64_t n16) {
-  constexpr int U = 4;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + (U - 1) * stride < n16; i += U * stride) {
-    uint4 v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
-#pragma unroll
-    for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
-  }
-  for (; i < n16; i += stride) dst[i] = src[i];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
 }
 
 }  // namespace nbg
@@ -117,11 +110,16 @@ int nbg_stream_copy(const void* src, void* dst, size_t bytes, void* stream) {
     return nbg::set_error(NBG_ERR_INVALID, "nbg_stream_copy: 16-byte aligned pointers and size required");
   const int64_t n16 = (int64_t)(bytes / 16);
   if (n16 == 0) return NBG_OK;
-  int64_t blocks = (n16 + 255) / 256;
-  if (blocks > 256 * 32) blocks = 256 * 32;
-  hipLaunchKernelGGL(nbg::stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                     (const uint4*)src, (uint4*)dst, n16);
-  return nbg::check_hip(hipGetLastError(), "stream_copy_kernel launch");
+  // one-shot grids of at most 2**31 - 1 workgroups, one vector per thread
+  constexpr int64_t kMaxVecs = (int64_t)0x7fffffff * 256;
+  for (int64_t off = 0; off < n16; off += kMaxVecs) {
+    const int64_t cnt = n16 - off < kMaxVecs ? n16 - off : kMaxVecs;
+    const int64_t blocks = (cnt + 255) / 256;
+    hipLaunchKernelGGL(nbg::stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const uint4*)src + off, (uint4*)dst + off, cnt);
+    if (int rc = nbg::check_hip(hipGetLastError(), "stream_copy_kernel launch")) return rc;
+  }
+  return NBG_OK;
 }
 
 const char* nbg_version(void) { return "numbagg_amd 0.1.0 (gfx950)"; }

@@ -1,9 +1,11 @@
 """Grouped reductions (numbagg/grouped.py) on the gfx950 group-by kernels.
 
 Per-label accumulators are privatised in LDS when the label count fits, or kept
-in global memory otherwise. Float sums are accumulated in float64 and rounded
-once to the output dtype; counts and indices are exact integers; argmax/argmin
-keep the reference's first-index tie rule; first/last are exact value copies.
+in global memory otherwise. Float sums, sums of squares and products are
+accumulated in float64 and rounded once to the output dtype (integer ones wrap
+like the reference's fixed-width arithmetic); counts and indices are exact
+integers; argmax/argmin and min/max keep the reference's first-index tie rule;
+first/last/min/max are exact value copies.
 """
 
 from __future__ import annotations
@@ -56,6 +58,36 @@ def group_nanstd(values, labels, ddof, out):
     """Standard deviation of each label's non-NaN values (numbagg/grouped.py:179-208)."""
 
 
+@groupndreduce.wrap()
+def group_nanmin(values, labels, out):
+    """Minimum of the non-NaN values of each label; NaN for empty labels (numbagg/grouped.py:211-225)."""
+
+
+@groupndreduce.wrap()
+def group_nanmax(values, labels, out):
+    """Maximum of the non-NaN values of each label; NaN for empty labels (numbagg/grouped.py:228-244)."""
+
+
+@groupndreduce.wrap()
+def group_nanprod(values, labels, out):
+    """Product of the non-NaN values of each label; 1 for empty labels (numbagg/grouped.py:124-132)."""
+
+
+@groupndreduce.wrap()
+def group_nansum_of_squares(values, labels, out):
+    """Sum of the squares of the non-NaN values of each label (numbagg/grouped.py:135-145)."""
+
+
+@groupndreduce.wrap()
+def group_nanany(values, labels, out):
+    """1 where a label holds a non-NaN non-zero value, else 0 (numbagg/grouped.py:247-257)."""
+
+
+@groupndreduce.wrap()
+def group_nanall(values, labels, out):
+    """0 where a label holds a non-NaN zero, else 1 (numbagg/grouped.py:260-270)."""
+
+
 __all__ = [
     "group_nanmean",
     "group_nansum",
@@ -66,4 +98,10 @@ __all__ = [
     "group_nanlast",
     "group_nanvar",
     "group_nanstd",
+    "group_nanmin",
+    "group_nanmax",
+    "group_nanprod",
+    "group_nansum_of_squares",
+    "group_nanany",
+    "group_nanall",
 ]

@@ -38,6 +38,25 @@ def build() -> pathlib.Path:
     return LIB_PATH
 
 
+def use_native_build(out_dir: str | pathlib.Path) -> pathlib.Path:
+    """Compile the C oracle for the host it runs on (-O3 -march=native
+    -ffp-contract=off, OpenMP) into ``out_dir`` and make ``restated`` use it.
+
+    For the CPU baseline leg of bench.py: the portable build under _build/
+    serves the tests, this one is tuned for the machine that times it.
+    """
+    global _lib
+    out = pathlib.Path(out_dir) / "libnbg_oracle_native.so"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    cmd = ["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+           "-shared", "-o", str(out), str(ORACLE_DIR / "nbg_oracle.c"), "-lm"]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"native oracle build failed:\n{proc.stderr}")
+    _lib = ctypes.CDLL(str(out))
+    return out
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -210,7 +229,9 @@ class exact:
 
     @staticmethod
     def group(func: str, values: np.ndarray, labels: np.ndarray, num_labels: int, ddof: int = 1) -> np.ndarray:
-        """float64 closed forms (two-pass variance) per (row, label)."""
+        """float64 closed forms (two-pass variance) per (row, label); products are
+        float64 products in index order, min/max the value at the first index of
+        the extreme."""
         v = _c2(values)
         lab = np.asarray(labels, dtype=np.int64).ravel()
         rows, n = v.shape
@@ -239,15 +260,33 @@ class exact:
                     # (0 - 0**2/0) / (0 - ddof) (numbagg/grouped.py:166-174)
                     var = np.where((den > 0) & (cnt > 0), m2 / np.where(den > 0, den, 1), np.nan)
                 out[r] = np.sqrt(var) if func == "group_nanstd" else var
-            elif func in ("group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast"):
+            elif func == "group_nansum_of_squares":
+                out[r] = np.bincount(lr, weights=xr * xr, minlength=num_labels)
+            elif func in ("group_nanany", "group_nanall"):
+                hit = (xr != 0) if func == "group_nanany" else (xr == 0)
+                seen = np.bincount(lr[hit], minlength=num_labels) > 0
+                out[r] = seen if func == "group_nanany" else ~seen
+            elif func == "group_nanprod":
+                out[r] = 1.0
+                order = np.argsort(lr, kind="stable")
+                bounds = np.searchsorted(lr[order], np.arange(num_labels + 1))
+                for k in range(num_labels):
+                    out[r, k] = np.prod(xr[order[bounds[k] : bounds[k + 1]]])
+            elif func in ("group_nanargmax", "group_nanargmin", "group_nanfirst", "group_nanlast",
+                          "group_nanmin", "group_nanmax"):
                 pos = np.nonzero(ok)[0]
                 for k in range(num_labels):
                     sel = pos[lr == k]
                     if sel.size == 0:
                         continue
                     vals = vr[sel]
+                    # np.argmax/argmin return the first of equal extremes (-0.0 == 0.0)
                     if func == "group_nanargmax":
                         out[r, k] = sel[np.argmax(vals)]
+                    elif func == "group_nanmax":
+                        out[r, k] = vals[np.argmax(vals)]
+                    elif func == "group_nanmin":
+                        out[r, k] = vals[np.argmin(vals)]
                     elif func == "group_nanargmin":
                         out[r, k] = sel[np.argmin(vals)]
                     elif func == "group_nanfirst":

@@ -743,6 +743,104 @@ DEF_GROUP_NANLAST(double, f64, ISNAN_F, NAN_F(double))
 DEF_GROUP_NANLAST(int32_t, i32, ISNAN_I, NAN_I32)
 DEF_GROUP_NANLAST(int64_t, i64, ISNAN_I, NAN_I64)
 
+/* group_nanprod, numbagg/grouped.py:124-132: products in the values dtype
+ * (integer products wrap like numba's fixed-width multiplies; computed
+ * unsigned to stay defined C). */
+#define DEF_GROUP_NANPROD(T, U, SFX, ISNAN)                                             \
+  void orc_group_nanprod_##SFX(const T* values, const int64_t* labels, T* out,         \
+                               int64_t rows, int64_t n, int64_t ldv, int64_t num_labels) { \
+    GROUP_LOOP_HEAD(T, T)                                                               \
+    for (int64_t k = 0; k < num_labels; ++k) o[k] = 1;                                  \
+    for (int64_t i = 0; i < n; ++i) {                                                   \
+      int64_t label = labels[i];                                                        \
+      if (label < 0 || label >= num_labels) continue;                                   \
+      T value = v[i];                                                                   \
+      if (!ISNAN(value)) o[label] = (T)((U)o[label] * (U)value);                        \
+    }                                                                                   \
+  }                                                                                     \
+  }
+DEF_GROUP_NANPROD(float, float, f32, ISNAN_F)
+DEF_GROUP_NANPROD(double, double, f64, ISNAN_F)
+DEF_GROUP_NANPROD(int32_t, uint32_t, i32, ISNAN_I)
+DEF_GROUP_NANPROD(int64_t, uint64_t, i64, ISNAN_I)
+
+/* group_nansum_of_squares, numbagg/grouped.py:135-145: `value ** 2` in the
+ * values dtype (float32 squares round to float32), accumulated in it. */
+#define DEF_GROUP_NANSUMSQ(T, U, SFX, ISNAN)                                            \
+  void orc_group_nansum_of_squares_##SFX(const T* values, const int64_t* labels, T* out, \
+                                         int64_t rows, int64_t n, int64_t ldv,         \
+                                         int64_t num_labels) {                          \
+    GROUP_LOOP_HEAD(T, T)                                                               \
+    for (int64_t k = 0; k < num_labels; ++k) o[k] = 0;                                  \
+    for (int64_t i = 0; i < n; ++i) {                                                   \
+      int64_t label = labels[i];                                                        \
+      if (label < 0 || label >= num_labels) continue;                                   \
+      T value = v[i];                                                                   \
+      if (!ISNAN(value)) {                                                              \
+        U sq = (U)((U)value * (U)value);                                                \
+        o[label] = (T)((U)o[label] + sq);                                               \
+      }                                                                                 \
+    }                                                                                   \
+  }                                                                                     \
+  }
+DEF_GROUP_NANSUMSQ(float, float, f32, ISNAN_F)
+DEF_GROUP_NANSUMSQ(double, double, f64, ISNAN_F)
+DEF_GROUP_NANSUMSQ(int32_t, uint32_t, i32, ISNAN_I)
+DEF_GROUP_NANSUMSQ(int64_t, uint64_t, i64, ISNAN_I)
+
+/* group_nanmin (CMP <, numbagg/grouped.py:211-225) and group_nanmax (CMP >,
+ * :228-244): running extremes in a float64 array initialised to NaN, copied
+ * to the output at the end (NaN -> minimum integer for integer outputs). */
+#define DEF_GROUP_NANEXT(T, SFX, NAME, CMP, ISNAN, NANV)                                \
+  void orc_##NAME##_##SFX(const T* values, const int64_t* labels, T* out, int64_t rows, \
+                          int64_t n, int64_t ldv, int64_t num_labels) {                 \
+    GROUP_LOOP_HEAD(T, T)                                                               \
+    double* best = (double*)malloc(((size_t)num_labels + 1) * sizeof(double));          \
+    for (int64_t k = 0; k < num_labels; ++k) best[k] = NAN;                             \
+    for (int64_t i = 0; i < n; ++i) {                                                   \
+      int64_t label = labels[i];                                                        \
+      if (label < 0 || label >= num_labels) continue;                                   \
+      T value = v[i];                                                                   \
+      if (!ISNAN(value) && (isnan(best[label]) || (double)value CMP best[label]))       \
+        best[label] = (double)value;                                                    \
+    }                                                                                   \
+    for (int64_t k = 0; k < num_labels; ++k) o[k] = isnan(best[k]) ? NANV : (T)best[k]; \
+    free(best);                                                                         \
+  }                                                                                     \
+  }
+DEF_GROUP_NANEXT(float, f32, group_nanmin, <, ISNAN_F, NAN_F(float))
+DEF_GROUP_NANEXT(double, f64, group_nanmin, <, ISNAN_F, NAN_F(double))
+DEF_GROUP_NANEXT(int32_t, i32, group_nanmin, <, ISNAN_I, NAN_I32)
+DEF_GROUP_NANEXT(int64_t, i64, group_nanmin, <, ISNAN_I, NAN_I64)
+DEF_GROUP_NANEXT(float, f32, group_nanmax, >, ISNAN_F, NAN_F(float))
+DEF_GROUP_NANEXT(double, f64, group_nanmax, >, ISNAN_F, NAN_F(double))
+DEF_GROUP_NANEXT(int32_t, i32, group_nanmax, >, ISNAN_I, NAN_I32)
+DEF_GROUP_NANEXT(int64_t, i64, group_nanmax, >, ISNAN_I, NAN_I64)
+
+/* group_nanany (numbagg/grouped.py:247-257: 0, then 1 on a valid truthy
+ * value) and group_nanall (:260-270: 1, then 0 on a valid falsy value). */
+#define DEF_GROUP_NANANYALL(T, SFX, NAME, INIT, HIT, ISNAN)                             \
+  void orc_##NAME##_##SFX(const T* values, const int64_t* labels, T* out, int64_t rows, \
+                          int64_t n, int64_t ldv, int64_t num_labels) {                 \
+    GROUP_LOOP_HEAD(T, T)                                                               \
+    for (int64_t k = 0; k < num_labels; ++k) o[k] = INIT;                               \
+    for (int64_t i = 0; i < n; ++i) {                                                   \
+      int64_t label = labels[i];                                                        \
+      if (label < 0 || label >= num_labels) continue;                                   \
+      T value = v[i];                                                                   \
+      if (!ISNAN(value) && (HIT)) o[label] = 1 - INIT;                                  \
+    }                                                                                   \
+  }                                                                                     \
+  }
+DEF_GROUP_NANANYALL(float, f32, group_nanany, 0, value != 0, ISNAN_F)
+DEF_GROUP_NANANYALL(double, f64, group_nanany, 0, value != 0, ISNAN_F)
+DEF_GROUP_NANANYALL(int32_t, i32, group_nanany, 0, value != 0, ISNAN_I)
+DEF_GROUP_NANANYALL(int64_t, i64, group_nanany, 0, value != 0, ISNAN_I)
+DEF_GROUP_NANANYALL(float, f32, group_nanall, 1, value == 0, ISNAN_F)
+DEF_GROUP_NANANYALL(double, f64, group_nanall, 1, value == 0, ISNAN_F)
+DEF_GROUP_NANANYALL(int32_t, i32, group_nanall, 1, value == 0, ISNAN_I)
+DEF_GROUP_NANANYALL(int64_t, i64, group_nanall, 1, value == 0, ISNAN_I)
+
 /* ------------------------------------------------------------------------ */
 /* Exact oracles (long double, 64-bit significand) used to state the error   */
 /* bounds of float results. They follow the mathematical definitions the     */

@@ -92,6 +92,52 @@ def main() -> None:
     for ddof in (0, 1, 5):
         res[f"group_nanvar_ddof{ddof}"] = gb.var(ddof=ddof).T.values
         res[f"group_nanstd_ddof{ddof}"] = gb.std(ddof=ddof).T.values
+    # the rest of the grouped family (conftest.py:444-487)
+    res["group_nanmin"] = gb.min().T.values
+    res["group_nanmax"] = gb.max().T.values
+    res["group_nanprod"] = gb.prod().T.values
+    res["group_nanany"] = gb.any().T.values.astype(np.float64)
+    res["group_nanall"] = gb.all().T.values.astype(np.float64)
+    res["group_nansum_of_squares"] = df_of(g).pipe(lambda x: x**2).groupby(labels).sum().T.values
+
+    # test_grouped.py:67-135: labels rs.choice([-1, 0, .., 4], 200) and float64 /
+    # int32 / bool values drawn from a RandomState(0) stream, compared with
+    # pandas Series.groupby (NaN labels for -1)
+    def series_funcs():
+        return {
+            "group_nanall": lambda x: x.all(),
+            "group_nanany": lambda x: x.any(),
+            "group_nancount": lambda x: x.count(),
+            "group_nanfirst": lambda x: x.first(),
+            "group_nanlast": lambda x: x.last(),
+            "group_nanmax": lambda x: x.max(),
+            "group_nanmean": lambda x: x.mean(),
+            "group_nanmin": lambda x: x.min(),
+            "group_nanprod": lambda x: x.prod(),
+            "group_nanstd": lambda x: x.std(),
+            "group_nansum": lambda x: x.sum(),
+            "group_nanvar": lambda x: x.var(),
+            "group_nansum_of_squares": lambda x: x.agg(lambda y: (y**2).sum()),
+        }
+
+    for dt in ("float64", "int32", "bool"):
+        rs = np.random.RandomState(0)
+        labels200 = rs.choice([-1, 0, 1, 2, 3, 4], size=200)
+        if dt == "int32":
+            vals = rs.randint(-100, 100, size=200)
+        elif dt == "float64":
+            vals = rs.rand(200)
+            vals = np.where(vals > 0.1, vals, np.nan)
+            vals = np.where(labels200 != 2, vals, np.nan)
+        else:
+            vals = rs.choice([True, False], size=200)
+        res[f"series_{dt}_labels"] = labels200
+        res[f"series_{dt}_values"] = vals
+        plabels = np.where(labels200 >= 0, labels200, np.nan)
+        for name, fn in series_funcs().items():
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                res[f"series_{dt}_{name}"] = np.asarray(fn(pd.Series(vals).groupby(plabels)).values, dtype=np.float64)
 
     np.savez_compressed(OUT, **res)
     print(f"wrote {OUT} ({len(res)} arrays, pandas {pd.__version__})")

@@ -300,3 +300,115 @@ def test_group_negative_ddof_empty_label(gpu_lib):
             ok = ~np.isnan(want)
             assert np.array_equal(np.isnan(got), np.isnan(want))
             assert_allclose(got[ok], want[ok], rtol=2e-7 if dtype == np.float32 else 1e-12)
+
+
+REST_EXACT = ["group_nanmin", "group_nanmax", "group_nanany", "group_nanall"]
+
+
+@pytest.mark.parametrize("func", REST_EXACT)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32, np.int64])
+@pytest.mark.parametrize("shape,L,zipf", [((3, 1000), 12, False), ((4, 300_000), 1000, True), ((2, 70_001), 5000, False)])
+def test_group_min_max_any_all_exact(gpu_lib, strategy, func, dtype, shape, L, zipf):
+    """min/max copy the value at the first index of the extreme (so signed
+    zeros and 64-bit integers come out as the reference's float64 running
+    extreme leaves them); any/all are flags. All bit-exact."""
+    from oracle import restated
+
+    v, lab = _data(*shape, L, dtype, seed=shape[1] + 5 * L, zipf=zipf)
+    if np.issubdtype(dtype, np.floating):
+        v[:, 10:40] = 0.25  # ties
+        v[0, 100], v[0, 101], v[1, 102] = -0.0, 0.0, -0.0
+        v[:, 200:260] = 0.0  # zeros for any/all
+        v[:, 300:340] = np.nan
+    else:
+        v[:, 200:260] = 0
+        if dtype == np.int64:
+            v[0, 500:504] = [2**53 + 1, 2**53, -(2**60) - 1, 2**60 + 3]  # round through float64
+    got = _call(func, v, lab, num_labels=L, axis=-1)
+    want = restated.group(func, v, lab.astype(np.int64), L)
+    assert got.dtype == v.dtype
+    assert_array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32, np.int64])
+@pytest.mark.parametrize("shape,L,zipf", [((3, 1000), 12, False), ((4, 300_000), 1000, True), ((2, 70_001), 5000, False)])
+def test_group_prod_and_sum_of_squares(gpu_lib, strategy, dtype, shape, L, zipf):
+    """Integer products and sums of squares wrap like the reference's
+    fixed-width arithmetic (bit-exact against the restatement). Float ones are
+    formed in float64 and rounded once: float32 within 1 ULP of the float64
+    oracle plus 2 n 2**-53 of the value (float64 products of n factors in any
+    order), float64 within 4 n 2**-53 relative."""
+    from oracle import exact, restated
+
+    rng = np.random.default_rng(shape[1] + L)
+    v, lab = _data(*shape, L, dtype, seed=shape[1] + 7 * L, zipf=zipf)
+    if np.issubdtype(dtype, np.floating):
+        vp = (1.0 + 0.02 * rng.standard_normal(v.shape)).astype(dtype)  # products stay O(1)
+        vp[np.isnan(v)] = np.nan
+        vp[0, 7] = 0.0
+    else:
+        vp = v
+    for func, data in (("group_nanprod", vp), ("group_nansum_of_squares", v)):
+        got = _call(func, data, lab, num_labels=L, axis=-1)
+        assert got.dtype == data.dtype
+        if not np.issubdtype(dtype, np.floating):
+            want = restated.group(func, data, lab.astype(np.int64), L)
+            assert_array_equal(got, want)
+            continue
+        want = exact.group(func, data, lab, L)
+        cnt = exact.group("group_nancount", data, lab, L)
+        g64 = got.astype(np.float64)
+        err = np.abs(g64 - want)
+        if dtype == np.float32:
+            ulp = np.spacing(np.abs(want.astype(np.float32))).astype(np.float64)
+            assert np.all(err <= ulp + 2 * (cnt + 1) * 2.0**-53 * np.abs(want)), func
+        else:
+            assert np.all(err <= 4 * (cnt + 1) * 2.0**-53 * np.abs(want) + 1e-300), func
+
+
+def test_group_rest_of_family_pandas_vectors(gpu_lib, golden):
+    """The reference's pandas comparisons (conftest.py:444-487 on the (1, 500)
+    fixture; test_grouped.py:116-135 on float64 / int32 / bool series)."""
+    import numbagg_amd as nb
+
+    g = golden["group_array_1x500"]
+    codes = golden["group_codes"]
+    for func in ("group_nanmin", "group_nanmax", "group_nanprod", "group_nansum_of_squares", "group_nanany",
+                 "group_nanall"):
+        assert_allclose(getattr(nb, func)(g, codes, num_labels=12, axis=-1), golden[func], err_msg=func)
+    for dt in ("float64", "int32", "bool"):
+        vals = golden[f"series_{dt}_values"]
+        labels = golden[f"series_{dt}_labels"]
+        for func in ("group_nanall", "group_nanany", "group_nancount", "group_nanfirst", "group_nanlast",
+                     "group_nanmax", "group_nanmean", "group_nanmin", "group_nanprod", "group_nanstd",
+                     "group_nansum", "group_nanvar", "group_nansum_of_squares"):
+            if dt == "int32" and func == "group_nanprod":
+                continue  # "group_nanprod result too large" (test_grouped.py:127-128)
+            if dt == "bool" and func in ("group_nanstd", "group_nanvar"):
+                with pytest.raises(TypeError, match="does not support boolean input"):
+                    getattr(nb, func)(vals, labels)
+                continue
+            got = getattr(nb, func)(vals, labels)
+            np.testing.assert_almost_equal(got, golden[f"series_{dt}_{func}"], err_msg=f"{dt} {func}")
+
+
+@pytest.mark.parametrize("func", ["group_nanmin", "group_nanmax", "group_nanprod", "group_nansum_of_squares",
+                                  "group_nanany", "group_nanall"])
+def test_group_rest_of_family_edge_cases(gpu_lib, func):
+    """test_grouped.py:229-262 (all-NaN label, no valid labels, single NaN
+    label, all values NaN) and an empty reduced axis."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    cases = [
+        (np.array([1.0, 2.0, np.nan, np.nan]), np.array([0, 0, 1, 1])),
+        (np.array([1.0, 2.0, 3.0, 4.0]), np.array([-1, -1, -1, -1])),
+        (np.array([1.0, 2.0, np.nan]), np.array([0, 0, 1])),
+        (np.array([np.nan, np.nan, np.nan]), np.array([0, 1, 2])),
+    ]
+    for v, lab in cases:
+        L = max(int(lab.max()) + 1, 1)
+        np.testing.assert_array_equal(getattr(nb, func)(v, lab, num_labels=L), exact.group(func, v[None, :], lab, L)[0])
+    empty = {"group_nanprod": 1.0, "group_nanall": 1.0, "group_nansum_of_squares": 0.0, "group_nanany": 0.0}
+    got = getattr(nb, func)(np.zeros((3, 0), np.float32), np.zeros(0, np.int32), axis=-1, num_labels=2)
+    np.testing.assert_array_equal(got, np.full((3, 2), empty.get(func, np.nan), np.float32))

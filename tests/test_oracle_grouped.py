@@ -100,3 +100,75 @@ def test_negative_ddof_empty_label_is_nan(func):
         want0 = 0.5 / 3 if func == "group_nanvar" else np.sqrt(0.5 / 3)
         assert_allclose(got[0, 0], want0)
         assert got[0, 2] == 0.0
+
+
+NEW_FUNCS = ["group_nanmin", "group_nanmax", "group_nanprod", "group_nansum_of_squares", "group_nanany", "group_nanall"]
+
+
+@pytest.mark.parametrize("func", NEW_FUNCS)
+def test_rest_of_family_oracles_match_pandas(golden, func):
+    # conftest.py:444-487: pandas groupby min/max/prod/any/all and (x**2).sum()
+    g = golden["group_array_1x500"]
+    codes = golden["group_codes"]
+    want = golden[func]
+    assert_allclose(restated.group(func, g, codes, 12), want)
+    assert_allclose(exact.group(func, g, codes, 12), want)
+
+
+@pytest.mark.parametrize("dt", ["float64", "int32", "bool"])
+def test_series_comparisons(golden, dt):
+    # test_grouped.py:116-135: float64 / int32 / bool values, labels with -1
+    vals = golden[f"series_{dt}_values"]
+    labels = golden[f"series_{dt}_labels"]
+    kdt = np.int32 if dt == "bool" else vals.dtype  # bool input runs the int32 loop
+    v = vals.astype(kdt)[None, :]
+    for func in NEW_FUNCS + ["group_nansum", "group_nancount", "group_nanfirst", "group_nanlast"]:
+        if dt == "int32" and func == "group_nanprod":
+            continue  # "group_nanprod result too large" (test_grouped.py:127-128)
+        want = golden[f"series_{dt}_{func}"]
+        got = restated.group(func, v, labels, 5)[0]
+        assert_allclose(got.astype(np.float64), want, err_msg=func)
+        if dt == "float64":
+            assert_allclose(exact.group(func, v, labels, 5)[0], want, err_msg=func)
+
+
+@pytest.mark.parametrize("func", NEW_FUNCS)
+def test_rest_of_family_known_answers(func):
+    # test_grouped.py:229-262: all-NaN label, no valid labels, single NaN label
+    expect = {
+        "group_nanmin": ([1.0, np.nan], [np.nan], [1.0, np.nan], [np.nan] * 3),
+        "group_nanmax": ([2.0, np.nan], [np.nan], [2.0, np.nan], [np.nan] * 3),
+        "group_nanprod": ([2.0, 1.0], [1.0], [2.0, 1.0], [1.0] * 3),
+        "group_nansum_of_squares": ([5.0, 0.0], [0.0], [5.0, 0.0], [0.0] * 3),
+        "group_nanany": ([1.0, 0.0], [0.0], [1.0, 0.0], [0.0] * 3),
+        "group_nanall": ([1.0, 1.0], [1.0], [1.0, 1.0], [1.0] * 3),
+    }[func]
+    cases = [
+        (np.array([1.0, 2.0, np.nan, np.nan]), np.array([0, 0, 1, 1]), 2),
+        (np.array([1.0, 2.0, 3.0, 4.0]), np.array([-1, -1, -1, -1]), 1),
+        (np.array([1.0, 2.0, np.nan]), np.array([0, 0, 1]), 2),
+        (np.array([np.nan, np.nan, np.nan]), np.array([0, 1, 2]), 3),
+    ]
+    for (v, lab, L), want in zip(cases, expect):
+        assert_allclose(restated.group(func, v[None, :], lab, L)[0], want)
+        assert_allclose(exact.group(func, v[None, :], lab, L)[0], want)
+
+
+def test_min_max_keep_the_first_signed_zero():
+    # `value < min` is strict: among equal extremes (0.0 == -0.0) the first stays
+    v = np.array([[0.0, -0.0, 1.0, -0.0, 0.0, 2.0]])
+    lab = np.array([0, 0, 0, 1, 1, 1])
+    for oracle_group in (restated.group, exact.group):
+        mn = oracle_group("group_nanmin", v, lab, 2)[0]
+        assert np.signbit(mn).tolist() == [False, True]
+    # planted mistake: "last equal extreme wins" gives the other signs
+    assert np.signbit(v[0, [1, 4]]).tolist() == [True, False]
+
+
+def test_integer_products_and_squares_wrap():
+    v = np.array([[2**20, 2**20, 3, 2**16, 2**16]], dtype=np.int32)
+    lab = np.array([0, 0, 0, 1, 1])
+    prod = restated.group("group_nanprod", v, lab, 2)[0]
+    assert prod.tolist() == [np.int32(np.int64(2**40 * 3) % 2**32), 0]
+    sq = restated.group("group_nansum_of_squares", v, lab, 2)[0]
+    assert sq.tolist() == [np.int32((2 * 2**40 + 9) % 2**32), np.int32((2 * 2**32) % 2**32)]
