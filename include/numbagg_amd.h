@@ -234,6 +234,42 @@ int nbg_group_nanall(int vdtype, const void* values, int ldtype,
                      int64_t ld_v, int64_t num_labels, int strategy, void* ws,
                      size_t ws_bytes, void* stream); /* grouped.py:260-270 */
 
+/* ---- plain reductions (numbagg/funcs.py:23-228; ndaggregate.__call__ at
+ *      numbagg/decorators.py:231-256, ndreduce.__call__ at :1002-1028) ----- */
+/* x: (rows, n) with row stride ld; one output per row. dtype: NBG_F32,
+ * NBG_F64, NBG_I32 or NBG_I64 (mean/var/std: floats only). Outputs: sum in
+ * the input dtype (integers wrap); mean/var/std in the input dtype; count,
+ * argmax, argmin as int64 (argmax/argmin: -1 for a row without a valid value,
+ * which the host reports as the reference's ValueError); allnan/anynan as
+ * one byte (0/1); max/min in the input dtype, int64 for integers. n == 0 is
+ * rejected for max/min/argmax/argmin (the reference raises). Deterministic:
+ * fixed-order merges, no atomics. */
+size_t nbg_reduce_workspace_bytes(int64_t rows, int64_t n, int dtype);
+int nbg_nansum(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+               int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:69-82 */
+int nbg_nanmean(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:85-102 */
+int nbg_nancount(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                 int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:53-66 */
+int nbg_allnan(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+               int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:23-35 */
+int nbg_anynan(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+               int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:38-50 */
+int nbg_nanvar(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+               int64_t ld, int64_t ddof, void* ws, size_t ws_bytes,
+               void* stream); /* funcs.py:105-130 */
+int nbg_nanstd(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+               int64_t ld, int64_t ddof, void* ws, size_t ws_bytes,
+               void* stream); /* funcs.py:133-157 */
+int nbg_nanmax(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+               int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:200-216 */
+int nbg_nanmin(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+               int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:219-234 */
+int nbg_nanargmax(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                  int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:160-178 */
+int nbg_nanargmin(int dtype, const void* x, void* out, int64_t rows, int64_t n,
+                  int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:181-197 */
+
 /* No counterpart in the reference. Allocation-free variant of
  * nbg_group_nansum for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):

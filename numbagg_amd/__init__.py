@@ -7,7 +7,22 @@ returned as NumPy arrays; torch tensors on the GPU stay on the GPU.
 """
 
 from ._lib import spin_timeouts
-from .funcs import bfill, ffill
+from .funcs import (
+    allnan,
+    anynan,
+    bfill,
+    count,
+    ffill,
+    nanargmax,
+    nanargmin,
+    nancount,
+    nanmax,
+    nanmean,
+    nanmin,
+    nanstd,
+    nansum,
+    nanvar,
+)
 from .grouped import (
     group_nanall,
     group_nanany,
@@ -67,14 +82,41 @@ MOVE_EXP_FUNCS = [
 
 MOVE_FUNCS = [move_corr, move_cov, move_mean, move_std, move_sum, move_var]
 
+AGGREGATION_FUNCS = [
+    allnan,
+    anynan,
+    count,
+    nanargmax,
+    nanargmin,
+    nancount,
+    nanmax,
+    nanmean,
+    nanmin,
+    nanstd,
+    nansum,
+    nanvar,
+]
+
 OTHER_FUNCS = [bfill, ffill]
 
 __version__ = "0.1.0"
 
 __all__ = [
     "__version__",
+    "allnan",
+    "anynan",
     "bfill",
+    "count",
     "ffill",
+    "nanargmax",
+    "nanargmin",
+    "nancount",
+    "nanmax",
+    "nanmean",
+    "nanmin",
+    "nanstd",
+    "nansum",
+    "nanvar",
     "group_nanall",
     "group_nanany",
     "group_nanargmax",
@@ -104,6 +146,7 @@ __all__ = [
     "move_sum",
     "move_var",
     "spin_timeouts",
+    "AGGREGATION_FUNCS",
     "GROUPED_FUNCS",
     "MOVE_EXP_FUNCS",
     "MOVE_FUNCS",

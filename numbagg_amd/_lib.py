@@ -84,6 +84,13 @@ _SIGNATURES["nbg_group_nansum_noalloc"] = (
     _c_int,
     [_c_int, _c_ptr, _c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_int, _c_ptr],
 )
+_SIGNATURES["nbg_reduce_workspace_bytes"] = (_c_size, [_c_i64, _c_i64, _c_int])
+_RED = (_c_int, [_c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_size, _c_ptr])
+_RED_DDOF = (_c_int, [_c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_ptr, _c_size, _c_ptr])
+for _name in ("nansum", "nanmean", "nancount", "allnan", "anynan", "nanmax", "nanmin", "nanargmax", "nanargmin"):
+    _SIGNATURES[f"nbg_{_name}"] = _RED
+for _name in ("nanvar", "nanstd"):
+    _SIGNATURES[f"nbg_{_name}"] = _RED_DDOF
 _SIGNATURES["nbg_ragged_tile_size"] = (_c_i64, [])
 _SIGNATURES["nbg_ragged_workspace_bytes"] = (_c_size, [_c_i64, _c_i64])
 _SIGNATURES["nbg_ragged_max_lds_labels"] = (_c_i64, [])

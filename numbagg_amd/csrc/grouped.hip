@@ -66,6 +66,7 @@ enum GPass {
   kPProd = 9,     // product (f64 or wrapping u64), compare-and-swap updates
   kPAny = 10,     // flag: a valid non-zero value was seen
   kPAll = 11,     // flag: a valid zero was seen
+  kPMom = 12,     // shifted moments: count, sum and sum of squares of x - K (K: a value of the label)
 };
 
 template <typename T>
@@ -118,9 +119,41 @@ struct GArgs {
   void* acc0;
   void* acc1;
   void* acc2;           // variance finalize: counts
+  void* acc3;           // kPMom: per (row, label) shift (float64 bits, NaN until set)
   const double* means;  // kPDev: per (row, label) mean
   int argmin;           // kPPacked / kPKey / kPKeyIdx: complement keys
+  int mom;              // variance finalize: from shifted moments (kPMom) instead of two passes
 };
+
+// Canonical NaN bit patterns: the "no shift yet" marker of kPMom slots.
+template <typename TV>
+constexpr unsigned long long nan_bits() {
+  return sizeof(TV) == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
+}
+template <typename TV>
+__device__ __forceinline__ TV tv_from_bits(unsigned long long b) {
+  if constexpr (sizeof(TV) == 4) return __uint_as_float((unsigned int)b);
+  else return __longlong_as_double((long long)b);
+}
+template <typename TV>
+__device__ __forceinline__ unsigned long long tv_bits(TV v) {
+  if constexpr (sizeof(TV) == 4) return (unsigned long long)__float_as_uint(v);
+  else return (unsigned long long)__double_as_longlong(v);
+}
+// Claim the shift of an LDS slot: the first value to arrive becomes the
+// slot's shift; returns the shift in place afterwards.
+template <typename TV>
+__device__ __forceinline__ TV lds_claim_shift(TV* slot, TV v) {
+  if constexpr (sizeof(TV) == 4) {
+    const unsigned int old = atomicCAS(reinterpret_cast<unsigned int*>(slot), (unsigned int)nan_bits<TV>(),
+                                       __float_as_uint(v));
+    return old == (unsigned int)nan_bits<TV>() ? v : __uint_as_float(old);
+  } else {
+    const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long*>(slot), nan_bits<TV>(),
+                                             (unsigned long long)__double_as_longlong(v));
+    return old == nan_bits<TV>() ? v : __longlong_as_double((long long)old);
+  }
+}
 
 __device__ __forceinline__ void g_atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void g_atomic_add(unsigned long long* p, unsigned long long v) {
@@ -199,6 +232,11 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   double* l_sq = reinterpret_cast<double*>(smem_raw);
   double* l_mean = l_sq + L;
   unsigned long long* l_key = l_u + L;  // kPKeyIdx: staged per-label maxima
+  // kPMom: sums of y = x - K and y^2 (float64), shifts K (values dtype), counts
+  double* l_m1 = reinterpret_cast<double*>(smem_raw);
+  double* l_m2 = l_m1 + L;
+  TV* l_mk = reinterpret_cast<TV*>(l_m2 + L);
+  unsigned int* l_mc = reinterpret_cast<unsigned int*>(l_mk + L);
   const int64_t Lp = (int64_t)nr * L1;  // slots actually used
   if (LDS) {
     for (int64_t l = tid; l < Lp; l += kGroupThreads) {
@@ -219,23 +257,36 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       } else if constexpr (PASS == kPDev) {
         l_sq[l] = 0.0;
         l_mean[l] = g_mean[l];
+      } else if constexpr (PASS == kPMom) {
+        l_m1[l] = 0.0;
+        l_m2[l] = 0.0;
+        l_mc[l] = 0u;
+        l_mk[l] = tv_from_bits<TV>(nan_bits<TV>());
       }
     }
     __syncthreads();
   }
-  static_assert(!HOT || (LDS && (PASS == kPSum || PASS == kPDev) && std::is_same<SA, double>::value),
+  static_assert(!HOT || (LDS && (PASS == kPSum || PASS == kPDev || PASS == kPMom || PASS == kPSumSq ||
+                                 PASS == kPProd) &&
+                         std::is_same<SA, double>::value),
                 "hot labels: float sum passes with LDS accumulators only");
+  static_assert(PASS != kPMom || (LDS && std::is_floating_point<TV>::value),
+                "shifted moments: float values with LDS accumulators");
   constexpr int NH = HOT ? 2 : 1;
-  double h_s[R][NH];          // kPSum: sums; kPDev: sums of squared deviations
-  unsigned int h_c[R][NH];    // kPSum with COUNT: counts
+  double h_s[R][NH];          // kPSum: sums; kPDev: sums of squared deviations; kPMom: sums of y
+  unsigned int h_c[R][NH];    // kPSum with COUNT, kPMom: counts
   double h_mean[R][NH];       // kPDev: the hot labels' means
+  double h_s2[R][NH];         // kPMom: sums of y^2
+  TV h_k[R][NH];              // kPMom: this thread's shift per hot label (NaN: none yet)
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-      h_s[r][h] = 0.0;
+      h_s[r][h] = PASS == kPProd ? 1.0 : 0.0;
       h_c[r][h] = 0u;
       h_mean[r][h] = 0.0;
+      h_s2[r][h] = 0.0;
+      h_k[r][h] = tv_from_bits<TV>(nan_bits<TV>());
       if constexpr (HOT && PASS == kPDev) {
         if (r < nr && h < L1) h_mean[r][h] = l_mean[r * L1 + h];
       }
@@ -264,6 +315,26 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
             h_c[r][0] += first ? 1u : 0u;
             h_c[r][1] += first ? 0u : 1u;
           }
+        } else if constexpr (PASS == kPMom) {
+          // the thread's first value of the label is its shift
+          const TV kc = first ? h_k[r][0] : h_k[r][1];
+          const TV k = is_nan(kc) ? v : kc;
+          h_k[r][0] = first ? k : h_k[r][0];
+          h_k[r][1] = first ? h_k[r][1] : k;
+          const double y = (double)v - (double)k;
+          h_s[r][0] += first ? y : 0.0;
+          h_s[r][1] += first ? 0.0 : y;
+          h_s2[r][0] += first ? y * y : 0.0;
+          h_s2[r][1] += first ? 0.0 : y * y;
+          h_c[r][0] += first ? 1u : 0u;
+          h_c[r][1] += first ? 0u : 1u;
+        } else if constexpr (PASS == kPSumSq) {
+          const double sq = (double)v * (double)v;
+          h_s[r][0] += first ? sq : 0.0;
+          h_s[r][1] += first ? 0.0 : sq;
+        } else if constexpr (PASS == kPProd) {
+          h_s[r][0] *= first ? (double)v : 1.0;
+          h_s[r][1] *= first ? 1.0 : (double)v;
         } else {
           const double d = (double)v - (first ? h_mean[r][0] : h_mean[r][1]);
           h_s[r][0] += first ? d * d : 0.0;
@@ -326,6 +397,17 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
       const double d = (double)v - (LDS ? l_mean[l] : g_mean[l]);
       if (LDS) atomicAdd(&l_sq[l], d * d);
       else unsafeAtomicAdd(&g_sq[l], d * d);
+    } else if constexpr (PASS == kPMom) {
+      // y = x - K with K the slot's shift (the first value of the label to
+      // reach this workgroup): exact in float64 for float32 data, and small
+      // where the label's values are, so y^2 sums do not cancel against the
+      // squared mean at the end
+      TV k = l_mk[l];
+      if (is_nan(k)) k = lds_claim_shift(&l_mk[l], v);
+      const double y = (double)v - (double)k;
+      atomicAdd(&l_m1[l], y);
+      atomicAdd(&l_m2[l], y * y);
+      atomicAdd(&l_mc[l], 1u);
     }
   };
 
@@ -437,7 +519,40 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   }
 
   if (LDS) {
-    if (HOT && hot_on) {  // wave sums of the hot labels into their LDS slots (block-uniform)
+    if constexpr (HOT && PASS == kPMom) {
+      if (hot_on) {  // the hot labels' moments, moved to the slot's shift, then wave sums
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int h = 0; h < NH; ++h) {
+            const unsigned long long act = __ballot(h_c[r][h] != 0u);
+            if (act == 0ull || r >= nr || h >= L1) continue;  // wave-uniform
+            const int l = r * (int)L1 + h;
+            const int src = __ffsll((long long)act) - 1;
+            TV kslot = l_mk[l];
+            if (lane_id() == src && is_nan(kslot)) kslot = lds_claim_shift(&l_mk[l], h_k[r][h]);
+            kslot = __shfl(kslot, src, kWave);
+            double c = (double)h_c[r][h];
+            const double dl = h_c[r][h] ? (double)h_k[r][h] - (double)kslot : 0.0;
+            double x1 = h_s[r][h] + c * dl;
+            double x2 = h_s2[r][h] + 2.0 * dl * h_s[r][h] + c * dl * dl;
+            unsigned int cc = h_c[r][h];
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+              x1 += __shfl_xor(x1, o);
+              x2 += __shfl_xor(x2, o);
+              cc += __shfl_xor(cc, o);
+            }
+            (void)c;
+            if (lane_id() == 0) {
+              atomicAdd(&l_m1[l], x1);
+              atomicAdd(&l_m2[l], x2);
+              atomicAdd(&l_mc[l], cc);
+            }
+          }
+      }
+    } else if constexpr (HOT) {
+      if (hot_on) {  // wave sums of the hot labels into their LDS slots (block-uniform)
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -446,7 +561,8 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
           unsigned int c = h_c[r][h];
 #pragma unroll
           for (int o = kWave / 2; o > 0; o >>= 1) {
-            x += __shfl_xor(x, o);
+            if constexpr (PASS == kPProd) x *= __shfl_xor(x, o);
+            else x += __shfl_xor(x, o);
             if (COUNT) c += __shfl_xor(c, o);
           }
           if (lane_id() == 0 && r < nr && h < L1) {
@@ -454,11 +570,16 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
             if constexpr (PASS == kPSum) {
               atomicAdd(&l_s[l], x);
               if (COUNT) atomicAdd(&l_c[l], c);
+            } else if constexpr (PASS == kPSumSq) {
+              atomicAdd(&l_s[l], x);
+            } else if constexpr (PASS == kPProd) {
+              if (x != 1.0) atomic_mul(&l_s[l], x);
             } else {
               atomicAdd(&l_sq[l], x);
             }
           }
         }
+      }
     }
     __syncthreads();
     for (int64_t l = tid; l < Lp; l += kGroupThreads) {
@@ -488,6 +609,23 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
         if (l_u[l] != ~0ull) atomicMin(&g_u0[l], l_u[l]);
       } else if constexpr (PASS == kPDev) {
         if (l_sq[l] != 0.0) unsafeAtomicAdd(&g_sq[l], l_sq[l]);
+      } else if constexpr (PASS == kPMom) {
+        // moved to the row's global shift Kg (the first workgroup to flush
+        // the label sets it): sums of x - Kg = s1 + c d, of (x - Kg)^2 =
+        // s2 + 2 d s1 + c d^2, with d = K - Kg
+        const unsigned int c = l_mc[l];
+        if (c) {
+          const double k = (double)l_mk[l];
+          unsigned long long* gk = static_cast<unsigned long long*>(A.acc3) + row0 * L1 + l;
+          const unsigned long long kb = (unsigned long long)__double_as_longlong(k);
+          const unsigned long long old = atomicCAS(gk, 0x7FF8000000000000ull, kb);
+          const double kg = old == 0x7FF8000000000000ull ? k : __longlong_as_double((long long)old);
+          const double d = k - kg;
+          const double s1 = l_m1[l], s2 = l_m2[l], dc = (double)c;
+          unsafeAtomicAdd(&static_cast<double*>(A.acc0)[row0 * L1 + l], s1 + dc * d);
+          unsafeAtomicAdd(&static_cast<double*>(A.acc1)[row0 * L1 + l], s2 + 2.0 * d * s1 + dc * d * d);
+          atomicAdd(&static_cast<unsigned long long*>(A.acc2)[row0 * L1 + l], (unsigned long long)c);
+        }
       }
     }
   }
@@ -569,7 +707,13 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
       r = i ? vals[row * A.ld_v + (int64_t)(i - 1)] : missing_value<TV>();
     } else {  // var / std from (count, sum of squared deviations)
       const unsigned long long c = static_cast<const unsigned long long*>(A.acc2)[q];
-      const double m2 = static_cast<const double*>(A.acc1)[q];
+      double m2 = static_cast<const double*>(A.acc1)[q];
+      if (A.mom && c) {
+        // shifted moments: M2 = sum (x-Kg)^2 - (sum (x-Kg))^2 / n, kept >= 0
+        const double s1 = static_cast<const double*>(A.acc0)[q];
+        m2 = m2 - s1 * s1 / (double)c;
+        m2 = m2 > 0.0 ? m2 : 0.0;
+      }
       const double den = (double)((long long)c - A.ddof);
       if (c == 0 || !(den > 0)) {  // an empty label is NaN whatever ddof (0/0 in the reference)
         r = missing_value<TV>();
@@ -612,7 +756,7 @@ int env_int(const char* name, int dflt) {
 // occupancy limit, the sum pass alone (without init and finalize) 0.183 ms
 // at 4 against 0.194-0.220 ms at 8. Knob: NBG_GROUP_BLOCKS_PER_CU (0 = the
 // occupancy limit). Results do not depend on it.
-constexpr int group_blocks_per_cu(int pass) { return (pass == kPSum || pass == kPDev) ? 4 : 2; }
+constexpr int group_blocks_per_cu(int pass) { return (pass == kPSum || pass == kPDev || pass == kPMom) ? 4 : 2; }
 
 // Chunking: the row groups' positions are split so that the whole grid is
 // one wave of resident workgroups (the device's capacity for this kernel and
@@ -665,7 +809,8 @@ constexpr int kGroupHotDefault = 1;
 
 template <typename TV, typename TL, int PASS, bool COUNT, int R>
 int launch_rows(GArgs a, size_t lds, hipStream_t s) {
-  constexpr bool HOT_OK = std::is_floating_point<TV>::value && (PASS == kPSum || PASS == kPDev);
+  constexpr bool HOT_OK = std::is_floating_point<TV>::value &&
+                         (PASS == kPSum || PASS == kPDev || PASS == kPMom || PASS == kPSumSq || PASS == kPProd);
   const bool hot = HOT_OK && env_int("NBG_GROUP_HOT", kGroupHotDefault) == 1;
   if constexpr (PASS == kPSum) {
     const int u = env_int("NBG_GROUP_UNROLL", kSumUnroll);
@@ -688,8 +833,12 @@ int launch_rows(GArgs a, size_t lds, hipStream_t s) {
 template <typename TV, typename TL, int PASS, bool COUNT = true>
 int run_pass(GArgs a, bool lds, hipStream_t s) {
   const int64_t L = a.num_labels;
-  const int64_t slot = pass_slot_bytes(PASS, COUNT, sizeof(typename SumAcc<TV>::type));
-  if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
+  // kPMom: two float64 sums, a shift in the values dtype and a count
+  const int64_t slot = PASS == kPMom ? 20 + (int64_t)sizeof(TV)
+                                     : pass_slot_bytes(PASS, COUNT, sizeof(typename SumAcc<TV>::type));
+  if constexpr (PASS != kPMom) {  // shifted moments run in LDS only
+    if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
+  }
   const int rmax = PASS == kPSum ? env_int("NBG_GROUP_ROWS", kSumRows) : kGroupRowsPerBlock;
   if (rmax >= 4 && a.rows >= 4 && L * slot * 4 <= kGroupLdsBudget)
     return launch_rows<TV, TL, PASS, COUNT, 4>(a, (size_t)(L * slot * 4), s);
@@ -727,7 +876,7 @@ int64_t lds_bytes_per_label(int kind, int vdtype) {
     case kGMean: return 12;
     case kGCount: return 4;
     case kGVar:
-    case kGStd: return 16;
+    case kGStd: return 28;  // shifted-moment slots (the two-pass fallback needs 16)
     case kGArgMax:
     case kGArgMin:
     case kGMin:
@@ -832,7 +981,23 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
       a.acc1 = nullptr;
       if ((rc = run_pass<TV, TL, kPLast>(a, lds, s))) return rc;
       return launch_finalize<TV, KIND>(a, nullptr, s);
-    default: {  // var / std: per-label means, then squared deviations
+    default: {  // var / std
+      if constexpr (std::is_floating_point<TV>::value) {
+        if (lds) {
+          // One pass: per-(row, label) shifted moments in LDS, moved to a
+          // global shift per (row, label) when each workgroup flushes them.
+          if ((rc = launch_init(acc0, 0ull, acc1, 0ull, per, s))) return rc;
+          if ((rc = launch_init(acc2, 0ull, acc3, 0x7FF8000000000000ull, per, s))) return rc;
+          a.acc0 = acc0;
+          a.acc1 = acc1;
+          a.acc2 = acc2;
+          a.acc3 = acc3;
+          a.mom = 1;
+          if ((rc = run_pass<TV, TL, kPMom>(a, true, s))) return rc;
+          return launch_finalize<TV, KIND>(a, nullptr, s);
+        }
+      }
+      // Global accumulators: per-label means, then squared deviations.
       // The float64 means come from float64 sums, so the deviations' own sum
       // (the usual two-pass correction term) is negligible: with n values
       // per label its share of sum d^2 is about (n 2**-53 |mean| / std)^2.

@@ -28,7 +28,10 @@
 //     incoming state, and independent tiles re-run their positions from it
 //     (the input is read twice, but no tile waits on another).
 // Squares and cross products of float32 inputs are formed in float64 (exact),
-// where the reference rounds them to float32 first.
+// where the reference rounds them to float32 first; the moment kinds shift
+// each row by its first finite value, so data far from zero (a random walk
+// offset by 1e4) does not lose its variance to the cancellation of
+// sum(w x^2)/sum(w) against the squared mean.
 #include "common.hpp"
 
 namespace nbg {
@@ -369,6 +372,26 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   const bool full = t0 + TILE <= n;
 
   const int64_t e0 = (int64_t)tid * K;
+  // The moment kinds (var, std, cov, corr) run the recurrence on x - ka (and
+  // y - kb) with ka the row's first finite value among its first 64
+  // positions (0 if there is none). Every wave of every tile of the row finds
+  // the same shift from the same positions, so no tile waits on another for
+  // it. The squares and cross products in the state then scale with the
+  // data's spread around the shift rather than with its offset from zero,
+  // and the outputs (variances, covariance, correlation) are unchanged by a
+  // common shift. For float32 data x - ka is exact in float64.
+  double ka = 0.0, kb = 0.0;
+  if constexpr (KIND == kEVar || KIND == kEStd || KIND == kECov || KIND == kECorr) {
+    const int ln = lane_id();
+    const double va = ln < n ? (double)ra[ln] : 0.0;
+    const unsigned long long ma = __ballot(ln < n && isfinite(va));
+    if (ma != 0ull) ka = read_lane(va, __ffsll((long long)ma) - 1);
+    if constexpr (Sp::NARGS == 2) {
+      const double vb = ln < n ? (double)rb[ln] : 0.0;
+      const unsigned long long mb = __ballot(ln < n && isfinite(vb));
+      if (mb != 0ull) kb = read_lane(vb, __ffsll((long long)mb) - 1);
+    }
+  }
   // SCALAR: one alpha for every position (kernel-uniform decay factors).
   const double alpha_s = (double)(TA)A.alpha_scalar;
   auto alpha_at = [&](int64_t p) -> double {
@@ -390,8 +413,8 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
       const double d = 1.0 - alpha;
       p *= d;
       if (Sp::SW2 >= 0) p2 *= d * d;
-      const double a = (double)la[pad16(e0 + j)];
-      const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
+      const double a = (double)la[pad16(e0 + j)] - ka;
+      const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] - kb : 0.0;
       exp_step<KIND>(s, seen, a, b, alpha);
     };
     if (full) {
@@ -618,8 +641,8 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
   auto rerun = [&](int j) {
     const int64_t pos = t0 + e0 + j;
     const double alpha = alpha_at(pos);
-    const double a = (double)la[pad16(e0 + j)];
-    const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] : 0.0;
+    const double a = (double)la[pad16(e0 + j)] - ka;
+    const double b = (Sp::NARGS == 2) ? (double)lb[pad16(e0 + j)] - kb : 0.0;
     exp_step<KIND>(s, seen, a, b, alpha);
     res[j] = (TA)exp_out<KIND>(s, seen, A.min_weight);
   };

@@ -22,6 +22,7 @@ from typing import Any
 
 import numpy as np
 import torch
+from numpy.lib.array_utils import normalize_axis_tuple
 
 from . import _device as dev
 from . import _lib
@@ -471,3 +472,130 @@ class groupndreduce(_Op):
             return np.dtype(np.int64)
         raise TypeError(f"numbagg_amd: unsupported values dtype {dt}")
 
+
+def _first_safe_loop(dt: np.dtype, loops: list[np.dtype], func_name: str) -> np.dtype:
+    """NumPy's loop choice for a gufunc with the given loops: the first one the
+    input casts to safely (what numba's guvectorize registers, in order)."""
+    for loop in loops:
+        if np.can_cast(dt, loop, casting="safe"):
+            return loop
+    raise TypeError(
+        f"ufunc '{func_name}' not supported for the input types ({dt},), and the inputs "
+        "could not be safely coerced to any supported types according to the casting rule ''safe''"
+    )
+
+
+def _reduce_rows(t: torch.Tensor, axes: tuple[int, ...]):
+    """Move `axes` (in that order) last and flatten them: (rows, n) view plus
+    the outer shape (numbagg/utils.py:20-32)."""
+    ndim = t.dim()
+    moved = t.movedim(axes, tuple(range(ndim - len(axes), ndim))) if axes else t
+    outer = tuple(moved.shape[: ndim - len(axes)])
+    n = int(np.prod(moved.shape[ndim - len(axes):], dtype=np.int64))
+    flat = moved.reshape(*outer, n) if outer else moved.reshape(n)
+    v2, rows, n2, ld = dev.rows_view(flat)
+    return v2, rows, n2, ld, outer
+
+
+def _scalar_or_array(res: torch.Tensor, outer: tuple, as_tensor: bool):
+    out = dev.to_output(res.view(outer) if outer else res.view(()), as_tensor)
+    if not as_tensor and not outer:
+        return out[()]  # NumPy gufuncs return a scalar for a 0-d result
+    return out
+
+
+class ndaggregate(_Op):
+    """Plain aggregation over one or more axes (numbagg/decorators.py:188-272).
+
+    ``axis=None`` reduces every axis; the reduced axes are moved last and
+    flattened (numbagg/utils.py:20-32). Calls ``nbg_<name>(dtype, x, out, rows,
+    n, ld, [ddof,] ws, ws_bytes, stream)``.
+    """
+
+    def __init__(self, func: Callable[..., Any], *, loops, out_dtype=None, supports_ddof: bool = False) -> None:
+        super().__init__(func)
+        self.loops = [np.dtype(d) for d in loops]
+        self.out_dtype = None if out_dtype is None else np.dtype(out_dtype)
+        self.supports_ddof = supports_ddof
+        if not supports_ddof:
+            params = [p for p in self.__signature__.parameters.values() if p.name != "ddof"]
+            self.__signature__ = self.__signature__.replace(parameters=params)
+
+    def __call__(self, *arrays, ddof: int = 1, axis: int | tuple[int, ...] | None = None):
+        if not all(isinstance(a, (np.ndarray, torch.Tensor)) for a in arrays):
+            raise TypeError(f"All positional arguments to {self} must be arrays: {arrays}")
+        if len(arrays) != 1:
+            raise TypeError(f"{self.__name__} takes one array, got {len(arrays)}")
+        (arr,) = arrays
+        ndim = arr.ndim
+        if axis is None:
+            axis = tuple(range(ndim))
+        elif not isinstance(axis, tuple):
+            axis = (axis,)
+        axes = normalize_axis_tuple(axis, ndim)
+        as_tensor = dev.is_tensor(arr)
+        loop = _first_safe_loop(dev.np_dtype(arr), self.loops, self.__name__)
+        t = dev.to_device(arr, loop)
+        v2, rows, n, ld, outer = _reduce_rows(t, axes)
+        odt = self.out_dtype or loop
+        out = torch.empty(rows, dtype=dev.torch_dtype(odt), device=t.device)
+        if rows:
+            ws_bytes = _lib.query("nbg_reduce_workspace_bytes", rows, n, dev.nbg_code(v2))
+            ws, ws_bytes = dev.workspace(ws_bytes, t.device)
+            args = [ctypes.c_int(dev.nbg_code(v2)), dev.ptr(v2), dev.ptr(out), rows, n, ld]
+            if self.supports_ddof:
+                args.append(int(ddof))
+            args += [dev.ptr(ws), ws_bytes, dev.stream_handle()]
+            _lib.call(self.entry, *args)
+        return _scalar_or_array(out, outer, as_tensor)
+
+
+class ndreduce(_Op):
+    """Reduction with an index or extreme result (numbagg/decorators.py:904-1028).
+
+    nanmax/nanmin return the extreme in the input dtype (int64 for integers,
+    as the reference's int64 signatures), nanargmax/nanargmin the int64 flat
+    index over the reduced axes. Zero-size and all-NaN slices raise the
+    reference's ValueError.
+    """
+
+    def __init__(self, func: Callable[..., Any], *, index: bool) -> None:
+        super().__init__(func)
+        self.index = index
+
+    _LOOPS = [np.dtype(np.int32), np.dtype(np.int64), np.dtype(np.float32), np.dtype(np.float64)]
+
+    def __call__(self, arr, *args, axis: int | tuple[int, ...] | None = None):
+        if args:
+            raise TypeError(f"{self.__name__} takes no extra positional arguments")
+        if not isinstance(arr, (np.ndarray, torch.Tensor)):
+            arr = np.asarray(arr)
+        ndim = arr.ndim
+        if axis is None:
+            axes = tuple(range(ndim))
+        elif isinstance(axis, (int, np.integer)):
+            axes = normalize_axis_tuple(int(axis), ndim)
+        else:
+            axes = normalize_axis_tuple(tuple(axis), ndim)
+        as_tensor = dev.is_tensor(arr)
+        loop = _first_safe_loop(dev.np_dtype(arr), self._LOOPS, self.__name__)
+        t = dev.to_device(arr, loop)
+        v2, rows, n, ld, outer = _reduce_rows(t, axes)
+        if self.index:
+            odt = np.dtype(np.int64)
+        else:
+            odt = np.dtype(np.int64) if loop.kind == "i" else loop
+        out = torch.empty(rows, dtype=dev.torch_dtype(odt), device=t.device)
+        if rows and n == 0:
+            if self.index:
+                raise ValueError("All-NaN slice encountered")
+            op = "fmax" if self.__name__ == "nanmax" else "fmin"
+            raise ValueError(f"zero-size array to reduction operation {op} which has no identity")
+        if rows:
+            ws_bytes = _lib.query("nbg_reduce_workspace_bytes", rows, n, dev.nbg_code(v2))
+            ws, ws_bytes = dev.workspace(ws_bytes, t.device)
+            _lib.call(self.entry, ctypes.c_int(dev.nbg_code(v2)), dev.ptr(v2), dev.ptr(out), rows, n, ld,
+                      dev.ptr(ws), ws_bytes, dev.stream_handle())
+            if self.index and bool((out < 0).any().item()):
+                raise ValueError("All-NaN slice encountered")
+        return _scalar_or_array(out, outer, as_tensor)

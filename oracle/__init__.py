@@ -19,6 +19,7 @@ All functions take (rows, n) NumPy arrays and return NumPy arrays.
 from __future__ import annotations
 
 import ctypes
+import math
 import pathlib
 import subprocess
 
@@ -150,6 +151,28 @@ class restated:
             args.append(I64(ddof))
         fn(*args)
         return out
+
+    @staticmethod
+    def reduce(func: str, x: np.ndarray, ddof: int = 1) -> np.ndarray:
+        """Plain reductions over the last axis (numbagg/funcs.py:23-228); the
+        nanargmax/nanargmin rows without a valid value give -1."""
+        v = _c2(x)
+        rows, n = v.shape
+        if func in ("allnan", "anynan"):
+            odt = np.uint8
+        elif func in ("nancount", "nanargmax", "nanargmin"):
+            odt = np.int64
+        elif func in ("nanmax", "nanmin") and v.dtype.kind == "i":
+            odt = np.int64
+        else:
+            odt = v.dtype
+        out = np.empty(rows, dtype=odt)
+        fn = getattr(lib(), f"orc_{func}_{_SFX[v.dtype]}")
+        args = [_p(v), _p(out), I64(rows), I64(n), I64(n)]
+        if func in ("nanvar", "nanstd"):
+            args.append(I64(ddof))
+        fn(*args)
+        return out.astype(bool) if odt == np.uint8 else out
 
 
 _MOVE_KIND = {"move_sum": 0, "move_mean": 1, "move_var": 2, "move_std": 3, "move_cov": 4, "move_corr": 5}
@@ -293,6 +316,27 @@ class exact:
                         out[r, k] = vr[sel[0]]
                     else:
                         out[r, k] = vr[sel[-1]]
+            else:
+                raise ValueError(func)
+        return out
+
+    @staticmethod
+    def reduce(func: str, x: np.ndarray, ddof: int = 1) -> np.ndarray:
+        """float64 closed forms over the last axis: exactly rounded sums
+        (math.fsum), two-pass variance about the fsum mean."""
+        v = _c2(x).astype(np.float64)
+        out = np.full(v.shape[0], np.nan)
+        for r, row in enumerate(v):
+            ok = row[~np.isnan(row)]
+            if func == "nansum":
+                out[r] = math.fsum(ok)
+            elif func == "nanmean":
+                out[r] = math.fsum(ok) / ok.size if ok.size else np.nan
+            elif func in ("nanvar", "nanstd"):
+                if ok.size > ddof:
+                    m = math.fsum(ok) / ok.size
+                    var = math.fsum((ok - m) ** 2) / (ok.size - ddof)
+                    out[r] = np.sqrt(var) if func == "nanstd" else var
             else:
                 raise ValueError(func)
         return out

@@ -964,3 +964,182 @@ void orc_exact_move_exp(int kind, const double* x, const double* y, const double
     }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* plain reductions over the last axis of (rows, n): numbagg/funcs.py       */
+/* ------------------------------------------------------------------------ */
+
+/* allnan (numbagg/funcs.py:23-35) and anynan (:38-50): bool outputs */
+#define DEF_ALLANY(T, SFX, ISNAN)                                                       \
+  void orc_allnan_##SFX(const T* x, uint8_t* out, int64_t rows, int64_t n, int64_t ld) { \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      out[r] = 1;                                                                       \
+      for (int64_t i = 0; i < n; ++i)                                                   \
+        if (!ISNAN(a[i])) {                                                             \
+          out[r] = 0;                                                                   \
+          break;                                                                        \
+        }                                                                               \
+    }                                                                                   \
+  }                                                                                     \
+  void orc_anynan_##SFX(const T* x, uint8_t* out, int64_t rows, int64_t n, int64_t ld) { \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      out[r] = 0;                                                                       \
+      for (int64_t i = 0; i < n; ++i)                                                   \
+        if (ISNAN(a[i])) {                                                              \
+          out[r] = 1;                                                                   \
+          break;                                                                        \
+        }                                                                               \
+    }                                                                                   \
+  }
+DEF_ALLANY(float, f32, ISNAN_F)
+DEF_ALLANY(double, f64, ISNAN_F)
+DEF_ALLANY(int32_t, i32, ISNAN_I)
+DEF_ALLANY(int64_t, i64, ISNAN_I)
+
+/* nancount (numbagg/funcs.py:53-66): int64 output */
+#define DEF_NANCOUNT(T, SFX, ISNAN)                                                     \
+  void orc_nancount_##SFX(const T* x, int64_t* out, int64_t rows, int64_t n, int64_t ld) { \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      int64_t c = 0;                                                                    \
+      for (int64_t i = 0; i < n; ++i) c += !ISNAN(a[i]);                                \
+      out[r] = c;                                                                       \
+    }                                                                                   \
+  }
+DEF_NANCOUNT(float, f32, ISNAN_F)
+DEF_NANCOUNT(double, f64, ISNAN_F)
+DEF_NANCOUNT(int32_t, i32, ISNAN_I)
+DEF_NANCOUNT(int64_t, i64, ISNAN_I)
+
+/* nansum (numbagg/funcs.py:69-82): accumulates in the input dtype (integer sums
+ * wrap like numba's fixed-width adds; computed unsigned to stay defined C) */
+#define DEF_NANSUM(T, U, SFX, ISNAN)                                                    \
+  void orc_nansum_##SFX(const T* x, T* out, int64_t rows, int64_t n, int64_t ld) {      \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      U s = 0;                                                                          \
+      for (int64_t i = 0; i < n; ++i)                                                   \
+        if (!ISNAN(a[i])) s = (U)(s + (U)a[i]);                                         \
+      out[r] = (T)s;                                                                    \
+    }                                                                                   \
+  }
+DEF_NANSUM(float, float, f32, ISNAN_F)
+DEF_NANSUM(double, double, f64, ISNAN_F)
+DEF_NANSUM(int32_t, uint32_t, i32, ISNAN_I)
+DEF_NANSUM(int64_t, uint64_t, i64, ISNAN_I)
+
+/* nanmean (numbagg/funcs.py:85-102): float64 sum (`asum = 0.0`), int count */
+#define DEF_NANMEAN(T, SFX)                                                             \
+  void orc_nanmean_##SFX(const T* x, T* out, int64_t rows, int64_t n, int64_t ld) {     \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      double s = 0.0;                                                                   \
+      int64_t c = 0;                                                                    \
+      for (int64_t i = 0; i < n; ++i)                                                   \
+        if (!isnan(a[i])) {                                                             \
+          s += (double)a[i];                                                            \
+          c += 1;                                                                       \
+        }                                                                               \
+      out[r] = c > 0 ? (T)(s / (double)c) : (T)NAN;                                     \
+    }                                                                                   \
+  }
+DEF_NANMEAN(float, f32)
+DEF_NANMEAN(double, f64)
+
+/* nanvar (SQRT=0, numbagg/funcs.py:105-130) and nanstd (SQRT=1, :133-157): two
+ * passes in float64 (the loop variable unifies to float64 after `ai -= amean`) */
+#define DEF_NANVAR(T, SFX, NAME, SQRT)                                                  \
+  void orc_##NAME##_##SFX(const T* x, T* out, int64_t rows, int64_t n, int64_t ld,      \
+                          int64_t ddof) {                                               \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      double s = 0.0;                                                                   \
+      int64_t c = 0;                                                                    \
+      for (int64_t i = 0; i < n; ++i)                                                   \
+        if (!isnan(a[i])) {                                                             \
+          s += (double)a[i];                                                            \
+          c += 1;                                                                       \
+        }                                                                               \
+      if (c > ddof) {                                                                   \
+        const double m = s / (double)c;                                                 \
+        double q = 0.0;                                                                 \
+        for (int64_t i = 0; i < n; ++i)                                                 \
+          if (!isnan(a[i])) {                                                           \
+            const double d = (double)a[i] - m;                                          \
+            q += d * d;                                                                 \
+          }                                                                             \
+        const double v = q / (double)(c - ddof);                                        \
+        out[r] = (T)(SQRT ? sqrt(v) : v);                                               \
+      } else {                                                                          \
+        out[r] = (T)NAN;                                                                \
+      }                                                                                 \
+    }                                                                                   \
+  }
+DEF_NANVAR(float, f32, nanvar, 0)
+DEF_NANVAR(double, f64, nanvar, 0)
+DEF_NANVAR(float, f32, nanstd, 1)
+DEF_NANVAR(double, f64, nanstd, 1)
+
+/* nanmax (CMP >=, INIT -inf, numbagg/funcs.py:200-216) and nanmin (CMP <=, INIT
+ * +inf, :219-234): float64 running extreme replaced on >= / <=; all-NaN -> NaN;
+ * integer inputs return int64 (the float64 extreme converted). n > 0. */
+#define DEF_NANEXT(T, OT, SFX, NAME, CMP, INIT)                                         \
+  void orc_##NAME##_##SFX(const T* x, OT* out, int64_t rows, int64_t n, int64_t ld) {   \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      double best = INIT;                                                               \
+      int missing = 1;                                                                  \
+      for (int64_t i = 0; i < n; ++i)                                                   \
+        if ((double)a[i] CMP best) {                                                    \
+          best = (double)a[i];                                                          \
+          missing = 0;                                                                  \
+        }                                                                               \
+      out[r] = missing ? (OT)NAN : (OT)best;                                            \
+    }                                                                                   \
+  }
+DEF_NANEXT(float, float, f32, nanmax, >=, -INFINITY)
+DEF_NANEXT(double, double, f64, nanmax, >=, -INFINITY)
+DEF_NANEXT(int32_t, int64_t, i32, nanmax, >=, -INFINITY)
+DEF_NANEXT(int64_t, int64_t, i64, nanmax, >=, -INFINITY)
+DEF_NANEXT(float, float, f32, nanmin, <=, INFINITY)
+DEF_NANEXT(double, double, f64, nanmin, <=, INFINITY)
+DEF_NANEXT(int32_t, int64_t, i32, nanmin, <=, INFINITY)
+DEF_NANEXT(int64_t, int64_t, i64, nanmin, <=, INFINITY)
+
+/* nanargmax (CMP >, INIT -inf, numbagg/funcs.py:160-178) and nanargmin (CMP <,
+ * INIT +inf, :181-197): the first index of the extreme; -1 for an all-NaN row
+ * (the reference raises ValueError there). */
+#define DEF_NANARGEXT(T, SFX, NAME, CMP, INIT)                                          \
+  void orc_##NAME##_##SFX(const T* x, int64_t* out, int64_t rows, int64_t n, int64_t ld) { \
+    ORC_PRAGMA_OMP                                                                      \
+    for (int64_t r = 0; r < rows; ++r) {                                                \
+      const T* a = x + r * ld;                                                          \
+      double best = INIT;                                                               \
+      int64_t idx = -1;                                                                 \
+      for (int64_t i = 0; i < n; ++i) {                                                 \
+        const double v = (double)a[i];                                                  \
+        if (v CMP best || (idx == -1 && !isnan(v))) {                                   \
+          best = v;                                                                     \
+          idx = i;                                                                      \
+        }                                                                               \
+      }                                                                                 \
+      out[r] = idx;                                                                     \
+    }                                                                                   \
+  }
+DEF_NANARGEXT(float, f32, nanargmax, >, -INFINITY)
+DEF_NANARGEXT(double, f64, nanargmax, >, -INFINITY)
+DEF_NANARGEXT(int32_t, i32, nanargmax, >, -INFINITY)
+DEF_NANARGEXT(int64_t, i64, nanargmax, >, -INFINITY)
+DEF_NANARGEXT(float, f32, nanargmin, <, INFINITY)
+DEF_NANARGEXT(double, f64, nanargmin, <, INFINITY)
+DEF_NANARGEXT(int32_t, i32, nanargmin, <, INFINITY)
+DEF_NANARGEXT(int64_t, i64, nanargmin, <, INFINITY)

@@ -25,6 +25,12 @@ pytestmark = pytest.mark.gpu
 
 REPORT: dict[str, dict] = {}
 
+# EWMA variance / covariance / correlation on a random walk offset by 1e4
+# (alpha 0.01): relative error of var and std, and error of cov and corr in
+# units of their scale (sqrt(var_x var_y) and 1), against the long-double
+# oracle. The moment kinds work on data shifted by each row's first value.
+EXP_MOMENT_BAR = 1e-8
+
 
 def _rel(got, want):
     ok = ~np.isnan(want)
@@ -100,8 +106,27 @@ def test_report_exp(gpu_lib, func):
     got = getattr(nb, func)(*args, alpha=0.01)
     want = exact.move_exp(func, x, 0.01, y=y, loop_dtype=np.float64)
     ref = restated.move_exp(func, x.astype(np.float64), 0.01, y=None if y is None else y.astype(np.float64))
-    moments = func in ("move_exp_nanvar", "move_exp_nanstd", "move_exp_nancov", "move_exp_nancorr")
-    _record(f"{func} alpha0.01 (float64 loop)", got, ref, want, rel_bar=None if moments else 1e-11)
+    name = f"{func} alpha0.01 (float64 loop)"
+    if func in ("move_exp_nanvar", "move_exp_nanstd"):
+        # the variance is positive everywhere here: a relative bar
+        _record(name, got, ref, want, rel_bar=EXP_MOMENT_BAR)
+    elif func in ("move_exp_nancov", "move_exp_nancorr"):
+        # cov and corr pass through zero: the bar is relative to the data
+        # scale, sqrt(var_x var_y) for cov (the exponentially weighted
+        # variances) and 1 for corr
+        _record(name, got, ref, want)
+        ok = ~np.isnan(want)
+        if func == "move_exp_nancov":
+            vx = exact.move_exp("move_exp_nanvar", x, 0.01, loop_dtype=np.float64)
+            vy = exact.move_exp("move_exp_nanvar", y, 0.01, loop_dtype=np.float64)
+            scale = np.sqrt(np.abs(vx * vy))[ok]
+        else:
+            scale = np.ones(int(ok.sum()))
+        err = np.abs(np.asarray(got, dtype=np.float64)[ok] - want[ok])
+        REPORT[name]["gpu_max_err_in_scale"] = float(np.max(err / np.maximum(scale, np.abs(want[ok]))))
+        assert np.all(err <= EXP_MOMENT_BAR * np.maximum(scale, np.abs(want[ok]))), (name, REPORT[name])
+    else:
+        _record(name, got, ref, want, rel_bar=1e-11)
 
 
 @pytest.mark.parametrize("labels", ["uniform", "zipf"])

@@ -1,0 +1,78 @@
+"""Thread-safety contract (numbagg/test/test_multithreading.py:1-14): every
+public function, called once serially and then 5 times from a 2-worker
+ThreadPoolExecutor, returns the serial result each time.
+
+Bit-exact functions and the rolling/EWMA/fill kernels (no cross-workgroup
+float merges) must repeat their bits; grouped float reductions merge
+per-workgroup float64 partials with atomics, so their float32/float64 results
+may differ in the last place between runs and are held to 1 ULP.
+"""
+
+from __future__ import annotations
+
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ATOMIC_FLOAT = {"group_nansum", "group_nanmean", "group_nanvar", "group_nanstd", "group_nanprod",
+                "group_nansum_of_squares"}
+
+
+def _cases():
+    import numbagg_amd as nb
+
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((3, 20_000))
+    x[rng.random(x.shape) < 0.1] = np.nan
+    y = rng.standard_normal((3, 20_000))
+    lab = rng.integers(-1, 30, 20_000)
+    cases = []
+    for f in nb.MOVE_FUNCS:
+        args = (x, y) if f.__name__ in ("move_cov", "move_corr") else (x,)
+        cases.append((f.__name__, lambda f=f, args=args: f(*args, window=50, min_count=5)))
+    for f in nb.MOVE_EXP_FUNCS:
+        args = (x, y) if f.__name__ in ("move_exp_nancov", "move_exp_nancorr") else (x,)
+        cases.append((f.__name__, lambda f=f, args=args: f(*args, alpha=0.1)))
+    for f in nb.GROUPED_FUNCS:
+        cases.append((f.__name__, lambda f=f: f(x, lab, num_labels=30, axis=-1)))
+    for f in nb.OTHER_FUNCS:
+        cases.append((f.__name__, lambda f=f: f(x, limit=7)))
+    for f in getattr(nb, "AGGREGATE_FUNCS", []):
+        cases.append((f.__name__, lambda f=f: f(x, axis=-1)))
+    return cases
+
+
+def _same(name, a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if name in ATOMIC_FLOAT and a.dtype.kind == "f":
+        assert np.array_equal(np.isnan(a), np.isnan(b)), name
+        ok = ~np.isnan(a)
+        ulp = np.spacing(np.abs(a[ok]).astype(a.dtype))
+        assert np.all(np.abs(a[ok] - b[ok]) <= ulp), name
+    else:
+        assert a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+
+
+def test_every_function_from_two_threads(gpu_lib):
+    for name, call in _cases():
+        want = call()
+        with ThreadPoolExecutor(max_workers=2) as ex:
+            futs = [ex.submit(call) for _ in range(5)]
+            got = [f.result() for f in futs]
+        assert len(got) == 5
+        for g in got:
+            _same(name, g, want)
+
+
+def test_interleaved_functions_from_four_threads(gpu_lib):
+    """All functions at once from 4 threads: per-call workspaces and per-device
+    caches stay separate."""
+    cases = _cases()
+    want = {name: call() for name, call in cases}
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        futs = [(name, ex.submit(call)) for _ in range(2) for name, call in cases]
+        for name, f in futs:
+            _same(name, f.result(), want[name])
