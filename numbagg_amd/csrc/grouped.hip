@@ -32,6 +32,11 @@ namespace nbg {
 namespace {
 
 constexpr int kGroupThreads = 256;
+// LDS slots past the label slots: an element that must not be counted (NaN,
+// or a label outside [0, num_labels)) updates slot L + lane instead of taking
+// a branch, so the element loop of the flag, count, extreme and sum passes
+// runs without exec-mask branches; those slots are never flushed.
+constexpr int kSpare = 64;
 constexpr int kGroupUnroll = 4;  // default vectors per thread and super-step
 
 enum GKind {
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
 
   // LDS accumulators.
   SA* l_s = reinterpret_cast<SA*>(smem_raw);
-  unsigned int* l_c = reinterpret_cast<unsigned int*>(smem_raw + (PASS == kPSum ? L * sizeof(SA) : 0));
+  unsigned int* l_c = reinterpret_cast<unsigned int*>(smem_raw + (PASS == kPSum ? (L + kSpare) * sizeof(SA) : 0));
   unsigned long long* l_u = reinterpret_cast<unsigned long long*>(smem_raw);
   unsigned int* l_u32 = reinterpret_cast<unsigned int*>(smem_raw);
   double* l_sq = reinterpret_cast<double*>(smem_raw);
@@ -299,6 +304,37 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   // hot_c: std::true_type on the register path for labels 0 and 1.
   auto update = [&](auto hot_c, TV v, TL lbl, int64_t i, int r) {
     constexpr bool H = decltype(hot_c)::value;
+    constexpr bool SPARE = LDS && !H &&
+                           (PASS == kPSum || PASS == kPSumSq || PASS == kPCount || PASS == kPAny ||
+                            PASS == kPAll || PASS == kPPacked || PASS == kPFirst || PASS == kPLast);
+    if constexpr (SPARE) {
+      bool ok;
+      if constexpr (sizeof(TL) == 4) ok = (uint32_t)lbl < lim32;
+      else ok = (uint64_t)lbl < (uint64_t)L1;
+      ok = ok && gvalid(v);
+      if constexpr (PASS == kPAny) ok = ok && v != TV(0);
+      if constexpr (PASS == kPAll) ok = ok && v == TV(0);
+      const int l = ok ? r * (int)L1 + (int)lbl : (int)L + lane_id();
+      if constexpr (PASS == kPSum) {
+        atomicAdd(&l_s[l], ok ? (SA)v : SA(0));
+        if (COUNT) atomicAdd(&l_c[l], 1u);
+      } else if constexpr (PASS == kPSumSq) {
+        atomicAdd(&l_s[l], ok ? (SA)v * (SA)v : SA(0));
+      } else if constexpr (PASS == kPCount) {
+        atomicAdd(&l_u32[l], 1u);
+      } else if constexpr (PASS == kPAny || PASS == kPAll) {
+        atomicOr(&l_u32[l], 1u);
+      } else if constexpr (PASS == kPPacked) {
+        uint32_t k = okey32(v);
+        if (A.argmin) k = ~k;
+        atomicMax(&l_u[l], ((unsigned long long)k << 32) | (unsigned long long)(~(uint32_t)i));
+      } else if constexpr (PASS == kPFirst) {
+        atomicMin(&l_u[l], (unsigned long long)i);
+      } else {
+        atomicMax(&l_u[l], (unsigned long long)(i + 1));
+      }
+      return;
+    }
     if constexpr (sizeof(TL) == 4) {
       if ((uint32_t)lbl >= lim32) return;
     } else {
@@ -729,6 +765,16 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
 // LDS budget for privatised accumulators (bytes per workgroup).
 constexpr int64_t kGroupLdsBudget = 48 * 1024;
 
+// The most LDS one workgroup may use on the current device (the forced LDS
+// strategy may go up to it; the automatic choice keeps to kGroupLdsBudget).
+int64_t group_lds_max() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    return kGroupLdsBudget;
+  return v;
+}
+
 // LDS bytes per (row, label) slot of a pass.
 int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
   switch (pass) {
@@ -840,11 +886,13 @@ int run_pass(GArgs a, bool lds, hipStream_t s) {
     if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
   }
   const int rmax = PASS == kPSum ? env_int("NBG_GROUP_ROWS", kSumRows) : kGroupRowsPerBlock;
-  if (rmax >= 4 && a.rows >= 4 && L * slot * 4 <= kGroupLdsBudget)
-    return launch_rows<TV, TL, PASS, COUNT, 4>(a, (size_t)(L * slot * 4), s);
-  if (rmax >= 2 && a.rows >= 2 && L * slot * 2 <= kGroupLdsBudget)
-    return launch_rows<TV, TL, PASS, COUNT, 2>(a, (size_t)(L * slot * 2), s);
-  return launch_rows<TV, TL, PASS, COUNT, 1>(a, (size_t)(L * slot), s);
+  // R rows of label slots plus the spare slots
+  auto bytes = [&](int64_t r) { return (L * r + kSpare) * slot; };
+  if (rmax >= 4 && a.rows >= 4 && bytes(4) <= kGroupLdsBudget)
+    return launch_rows<TV, TL, PASS, COUNT, 4>(a, (size_t)bytes(4), s);
+  if (rmax >= 2 && a.rows >= 2 && bytes(2) <= kGroupLdsBudget)
+    return launch_rows<TV, TL, PASS, COUNT, 2>(a, (size_t)bytes(2), s);
+  return launch_rows<TV, TL, PASS, COUNT, 1>(a, (size_t)bytes(1), s);
 }
 
 int launch_init(void* acc0, unsigned long long init0, void* acc1, unsigned long long init1,
@@ -903,9 +951,14 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
   unsigned long long* acc2 = acc1 + per;
   unsigned long long* acc3 = acc2 + per;
   a.acc2 = nullptr;
-  bool lds = L * lds_bytes_per_label(KIND, 0) <= kGroupLdsBudget;
-  if (strategy == NBG_GROUP_LDS && !lds)
-    return set_error(NBG_ERR_INVALID, "group: %lld labels do not fit the LDS strategy", (long long)L);
+  const int64_t lds_need = (L + kSpare) * lds_bytes_per_label(KIND, 0);
+  bool lds = lds_need <= kGroupLdsBudget;
+  if (strategy == NBG_GROUP_LDS && !lds) {
+    // forced: one row per workgroup, up to the device's LDS per workgroup
+    if (lds_need > group_lds_max())
+      return set_error(NBG_ERR_INVALID, "group: %lld labels do not fit the LDS strategy", (long long)L);
+    lds = true;
+  }
   if (strategy == NBG_GROUP_GLOBAL) lds = false;
 
   a.means = nullptr;

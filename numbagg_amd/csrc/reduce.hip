@@ -53,7 +53,8 @@ constexpr bool first_wins(int k) { return k == kRArgMax || k == kRArgMin; }
 
 // Partial state of one (row, chunk): 32 bytes.
 //   sum/mean: w0 sum (float64 bits, or wrapping u64 for integers), w1 the
-//             double-double low part (float64 data), w2 count;
+//             double-double low part (float64 data), w2 count, w3 the sum of
+//             the infinite values (float64 data: +-inf or NaN, else 0);
 //   count/allnan/anynan: w2 count of valid values, w3 count of NaNs;
 //   var/std: w0 mean, w1 M2 (float64 bits), w2 count;
 //   extremes: w0 key (0 = no valid value), w1 index.
@@ -94,7 +95,8 @@ struct Chan {
 template <typename T, int KIND>
 struct Lane {
   double s = 0.0;                                 // float32 data: float64 sum
-  Comp<double> cs{0.0, 0.0};                      // float64 data: double-double sum
+  Comp<double> cs{0.0, 0.0};                      // float64 data: double-double sum of finite values
+  double sinf = 0.0;                              // float64 data: sum of the infinite values
   unsigned long long us = 0ull;                   // integers: wrapping sum
   unsigned long long cnt = 0ull, nans = 0ull;
   Chan ch = Chan{0.0, 0.0, 0.0};
@@ -107,7 +109,12 @@ struct Lane {
       if constexpr (!std::is_floating_point<T>::value) {
         us += (unsigned long long)(long long)v;
       } else if constexpr (sizeof(T) == 8) {
-        if (valid) cs.add((double)v);
+        // infinities stay out of the double-double pair, whose error terms
+        // would turn them into NaN
+        if (valid) {
+          if (isfinite((double)v)) cs.add((double)v);
+          else sinf += (double)v;
+        }
       } else {
         s += valid ? (double)v : 0.0;
       }
@@ -159,6 +166,7 @@ struct Lane {
       } else if constexpr (sizeof(T) == 8) {
         p.w0 = bits_of(cs.hi);
         p.w1 = bits_of(cs.lo);
+        p.w3 = bits_of(sinf);
       } else {
         p.w0 = bits_of(s);
       }
@@ -189,6 +197,7 @@ __device__ __forceinline__ void merge_part(RPart& a, const RPart& b) {
       x.add(Comp<double>{f64_of(b.w0), f64_of(b.w1)});
       a.w0 = bits_of(x.hi);
       a.w1 = bits_of(x.lo);
+      a.w3 = bits_of(f64_of(a.w3) + f64_of(b.w3));
     } else {
       a.w0 = bits_of(f64_of(a.w0) + f64_of(b.w0));
     }
@@ -301,10 +310,12 @@ __global__ void reduce_finalize_kernel(const RPart* __restrict__ parts, int64_t 
     if constexpr (!std::is_floating_point<T>::value) {
       r = (TO)(T)(long long)p.w0;  // modular narrowing
     } else {
-      r = (TO)(f64_of(p.w0) + f64_of(p.w1));
+      const double inf = f64_of(p.w3);
+      r = (TO)(inf != 0.0 || inf != inf ? inf : f64_of(p.w0) + f64_of(p.w1));
     }
   } else if constexpr (KIND == kRMean) {
-    const double s = f64_of(p.w0) + f64_of(p.w1);
+    const double inf = f64_of(p.w3);
+    const double s = inf != 0.0 || inf != inf ? inf : f64_of(p.w0) + f64_of(p.w1);
     r = p.w2 ? (TO)(s / (double)p.w2) : (TO)canonical_nan<double>();
   } else if constexpr (KIND == kRCount) {
     r = (TO)p.w2;

@@ -11,7 +11,12 @@
 namespace nbg {
 namespace {
 
-constexpr int kMoveThreads = 256;
+// Threads per rolling tile (a build-time choice: NBG_MOVE_THREADS).
+#ifndef NBG_MOVE_THREADS
+#define NBG_MOVE_THREADS 256
+#endif
+constexpr int kMoveThreads = NBG_MOVE_THREADS;
+static_assert(kMoveThreads % 64 == 0 && kMoveThreads >= 64 && kMoveThreads <= 1024, "whole wavefronts");
 
 enum MoveKind { kSum = 0, kMean = 1, kVar = 2, kStd = 3, kCov = 4, kCorr = 5 };
 
@@ -50,6 +55,15 @@ __device__ __forceinline__ void sub_prod(Comp<double>& acc, double a, double b) 
   double e = fma(a, b, -p);
   acc.add(-p);
   acc.add(-e);
+}
+
+// A float32 value with NaN replaced by 0, selected in float32 and kept opaque
+// to the optimiser, so that a later widening to float64 is not pushed ahead of
+// the select (which would then act on both halves of the float64 value).
+__device__ __forceinline__ float nan_to_zero(float v) {
+  float r = is_nan(v) ? 0.0f : v;
+  __asm__("" : "+v"(r));
+  return r;
 }
 
 // ---- per-kind window statistics ---------------------------------------------
@@ -538,8 +552,34 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   };
 
   // 2. halo statistics W0 = window ending at t0-1.
+  constexpr bool FAST = !Tr::SHIFT && std::is_same<typename S::Acc, D>::value;
   S w0 = S::zero();
-  if (TAIL_LDS) {
+  if constexpr (FAST && TAIL_LDS) {
+    // float32 sum/mean: four halo inputs per iteration at fixed LDS offsets
+    // (pad16(e + T) = pad16(e) + T + T / 16 for T a multiple of 16), NaN -> 0
+    // selected in float32.
+    constexpr int PS = kMoveThreads + kMoveThreads / 16;  // padded stride of kMoveThreads elements
+    const int e_end = (int)(t0 - s0a);
+    int e = (int)(t0 - window - s0a) + tid;
+    double hs = 0.0;
+    int hc = 0;
+    for (; e + 3 * kMoveThreads < e_end; e += 4 * kMoveThreads) {
+      const T* q = la + pad16(e);
+      const T v0 = q[0], v1 = q[PS], v2 = q[2 * PS], v3 = q[3 * PS];
+      hs += (double)nan_to_zero(v0);
+      hs += (double)nan_to_zero(v1);
+      hs += (double)nan_to_zero(v2);
+      hs += (double)nan_to_zero(v3);
+      hc += (int)!is_nan(v0) + (int)!is_nan(v1) + (int)!is_nan(v2) + (int)!is_nan(v3);
+    }
+    for (; e < e_end; e += kMoveThreads) {
+      const T v0 = la[pad16(e)];
+      hs += (double)nan_to_zero(v0);
+      hc += (int)!is_nan(v0);
+    }
+    w0.c = hc;
+    w0.s.v = hs;
+  } else if (TAIL_LDS) {
     const int e_end = (int)(t0 - s0a);
     for (int e = (int)(t0 - window - s0a) + tid; e < e_end; e += kMoveThreads) {
       const T a = la[pad16(e)];
@@ -555,7 +595,6 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     }
   }
   // The halo reduction is folded into the delta scan below (one barrier).
-  constexpr bool FAST = !Tr::SHIFT && std::is_same<typename S::Acc, D>::value;
 
   // 3. head/tail chunks and the per-thread delta. Chunks are re-read from LDS
   //    (or global memory for long windows) rather than held in registers.
@@ -602,7 +641,7 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       tail(j, t, u);
       const bool hv = !is_nan(h), tv = !is_nan(t);
       // NaN -> 0 selected in float32 before widening (one select each)
-      const T h0 = hv ? h : T(0), t0v = tv ? t : T(0);
+      const T h0 = nan_to_zero(h), t0v = nan_to_zero(t);
       dj[j] = (double)h0 - (double)t0v;
       dc[j] = (int)hv - (int)tv;
       dsum += dj[j];

@@ -13,6 +13,8 @@ float32 sums within 1 ULP of the float64 sum, and bitwise repeatability.
 
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 from numpy.testing import assert_allclose, assert_array_almost_equal, assert_array_equal, assert_equal
@@ -22,6 +24,41 @@ pytestmark = pytest.mark.gpu
 
 def _slow_count(x, axis=None):
     return np.sum(~np.isnan(x), axis=axis)
+
+
+def _fsum(a, axis):
+    """Exactly rounded float64 sums of the non-NaN values over `axis` (and of
+    their magnitudes); +-inf handled as NumPy does."""
+    a = np.asarray(a, dtype=np.float64)
+    axes = tuple(range(a.ndim)) if axis is None else ((axis,) if np.isscalar(axis) else tuple(axis))
+    axes = tuple(ax % max(a.ndim, 1) for ax in axes) if a.ndim else ()
+    m = np.moveaxis(a, axes, tuple(range(a.ndim - len(axes), a.ndim))) if axes else a
+    outer = m.shape[: a.ndim - len(axes)]
+    flat = m.reshape(int(np.prod(outer, dtype=np.int64)), -1) if a.ndim else m.reshape(1, 1)
+    s = np.empty(flat.shape[0])
+    sa = np.empty(flat.shape[0])
+    for i, row in enumerate(flat):
+        row = row[~np.isnan(row)]
+        fin = row[np.isfinite(row)]
+        inf = row[~np.isfinite(row)]
+        s[i] = np.sum(inf) if inf.size else math.fsum(fin)
+        sa[i] = math.fsum(np.abs(fin))
+    return s.reshape(outer), sa.reshape(outer)
+
+
+def _check_sum(actual, want, sabs, tag):
+    """float32: within 1 ULP of the exact sum; float64: within 2**-52 of it
+    relative plus 2**-100 of the sum of |x| (double-double accumulation)."""
+    g = np.asarray(actual, dtype=np.float64)
+    want = np.asarray(want)
+    fin = np.isfinite(want)
+    assert np.array_equal(g[~fin], want[~fin], equal_nan=True), tag
+    err = np.abs(g[fin] - want[fin])
+    if np.asarray(actual).dtype == np.float32:
+        bar = np.spacing(np.abs(want[fin]).astype(np.float32)).astype(np.float64) + 1e-14 * np.asarray(sabs)[fin]
+    else:
+        bar = 2.0**-52 * np.abs(want[fin]) + 2.0**-100 * np.asarray(sabs)[fin]
+    assert np.all(err <= bar), (tag, float(np.max(err - bar)))
 
 
 def _functions():
@@ -118,11 +155,10 @@ def test_numerical_results_identical(gpu_lib, k):
                 assert actualraised == desiredraised, tag
                 actual, desired = np.asarray(actual), np.asarray(desired)
                 if numbagg_func.__name__ == "nansum" and actual.dtype.kind == "f":
-                    # float sums: the reference compares to NumPy's pairwise sum with
-                    # assert_array_equal; here both are checked against the float64 sum
-                    want = np.asarray(np.nansum(desired_arr.astype(np.float64), axis=axis))
-                    assert_allclose(actual.astype(np.float64), want, rtol=2e-7 if actual.dtype == np.float32 else 1e-14,
-                                    err_msg=tag)
+                    # float sums: the reference compares with NumPy's pairwise sum;
+                    # here the sum is held to the exactly rounded one (math.fsum)
+                    want, sabs = _fsum(desired_arr, axis)
+                    _check_sum(actual, want, sabs, tag)
                 elif decimal < np.inf and np.isfinite(arr).sum() > 0:
                     assert_array_almost_equal(actual, desired, decimal, tag)
                 else:
@@ -191,7 +227,8 @@ def test_multi_axis_and_layouts(gpu_lib):
         assert_array_equal(nb.nancount(x, axis=axis), _slow_count(x, axis=axis))
     assert_allclose(nb.nanstd(x, axis=(0, 2), ddof=0), np.nanstd(x, axis=(0, 2)), rtol=1e-11)
     xt = np.transpose(x, (2, 0, 3, 1))[:, ::2]
-    assert_allclose(nb.nansum(xt, axis=(1, 3)), np.nansum(xt, axis=(1, 3)), rtol=1e-13)
+    want, sabs = _fsum(xt, (1, 3))
+    _check_sum(nb.nansum(xt, axis=(1, 3)), want, sabs, "nansum transposed view")
     t = torch.from_numpy(x).cuda()
     r = nb.nanmean(t, axis=-1)
     assert isinstance(r, torch.Tensor) and r.is_cuda

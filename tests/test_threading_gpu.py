@@ -2,10 +2,11 @@
 public function, called once serially and then 5 times from a 2-worker
 ThreadPoolExecutor, returns the serial result each time.
 
-Bit-exact functions and the rolling/EWMA/fill kernels (no cross-workgroup
-float merges) must repeat their bits; grouped float reductions merge
-per-workgroup float64 partials with atomics, so their float32/float64 results
-may differ in the last place between runs and are held to 1 ULP.
+Bit-exact functions, the rolling/EWMA/fill kernels and the plain reductions
+(no cross-workgroup float merges in arbitrary order) must repeat their bits;
+grouped float reductions merge per-workgroup float64 partials with atomics, so
+their results may differ in the last place between runs: float32 within 1 ULP,
+float64 within 1e-12 relative.
 """
 
 from __future__ import annotations
@@ -48,10 +49,15 @@ def _cases():
 def _same(name, a, b):
     a, b = np.asarray(a), np.asarray(b)
     if name in ATOMIC_FLOAT and a.dtype.kind == "f":
+        # float64 sums merged in any order: within the float64 parity bar of
+        # the grouped tests (1e-12 relative); float32: one rounding, 1 ULP
         assert np.array_equal(np.isnan(a), np.isnan(b)), name
         ok = ~np.isnan(a)
-        ulp = np.spacing(np.abs(a[ok]).astype(a.dtype))
-        assert np.all(np.abs(a[ok] - b[ok]) <= ulp), name
+        if a.dtype == np.float32:
+            bar = np.spacing(np.abs(a[ok]).astype(a.dtype))
+        else:
+            bar = 1e-12 * np.maximum(np.abs(a[ok]), 1e-300)
+        assert np.all(np.abs(a[ok] - b[ok]) <= bar), name
     else:
         assert a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
 
