@@ -108,6 +108,54 @@ class Workload:
         self.kernel = kernel or name
 
 
+def tune_key(w):
+    """(function, loop dtype, rows, length, window or label count) of a
+    workload's call: the shape class the plain-kernel autotuner records."""
+    func = w.name.split("[")[0]
+    shape = w.config.get("shape", [1, 1])
+    rows, n = int(shape[0]), int(shape[1])
+    if func.startswith("move_exp_"):
+        return func, "float64", rows, n, 0
+    if func.startswith("move_"):
+        return func, "float32", rows, n, int(w.config.get("window", 0))
+    if func.startswith("group_"):
+        return func, "float32", rows, n, int(w.config.get("num_labels", 0))
+    return func, "float32", rows, n, 0
+
+
+def autotune(workloads):
+    """Record the fastest knob setting of every workload's shape class (the
+    first workload of a class decides: uniform labels before Zipf ones)."""
+    from numbagg_amd import plain_tuning
+
+    db = plain_tuning.database()
+    done = set()
+    for w in workloads:
+        func, dtype, rows, n, extra = tune_key(w)
+        key = (func, dtype, rows, n, extra)
+        if key in done or plain_tuning.family_of(func) is None:
+            continue
+        done.add(key)
+        best, timings = plain_tuning.tune(func, w.run, dtype, rows, n, extra, db=db)
+        ms = min(timings.values())
+        print(f"# tuned {w.name}: {best} ({ms:.4f} ms; {len(timings)} settings)", file=sys.stderr, flush=True)
+    db.save()
+    plain_tuning.reload_database()
+
+
+def tuned_settings(workloads):
+    """The recorded knob setting applied to each workload (None: defaults)."""
+    from numbagg_amd import plain_tuning
+
+    db = plain_tuning.database()
+    arch = plain_tuning.device_arch()
+    out = {}
+    for w in workloads:
+        func, dtype, rows, n, extra = tune_key(w)
+        out[w.name] = db.get(plain_tuning.shape_key(arch, func, dtype, rows, n, extra))
+    return {"kernel_version": db.version, "entries": len(db.entries), "applied": out}
+
+
 def has_entry(nb_lib, name):
     try:
         getattr(nb_lib, name)
@@ -384,6 +432,8 @@ def main(argv=None):
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batched comparison")
     ap.add_argument("--judge", action="store_true", help="also time exhaustive/heuristic strategies (regret)")
     ap.add_argument("--only", default=None, help="comma-separated workload names to run (profiling)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the plain-kernel autotuner on every workload first and save its database")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -414,6 +464,10 @@ def main(argv=None):
         wl = [w for w in wl if w.name in keep]
     torch.cuda.synchronize()
     print(f"# inputs ready: {[w.name for w in wl]}", file=sys.stderr, flush=True)
+
+    extras = build_extra(torch, nb, wl) if want_all and not args.only else []
+    if args.tune:
+        autotune(wl + extras)
 
     for _ in range(args.warmup):
         for w in wl:
@@ -473,7 +527,7 @@ def main(argv=None):
     all_rec = None
     if want_all and not args.only:
         all_rec = {}
-        for w in build_extra(torch, nb, wl):
+        for w in extras:
             ms = time_workload(torch, w, max(3, args.steps), stream)
             gbs = w.bytes / (ms * 1e-3) / 1e9
             all_rec[w.name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
@@ -548,6 +602,7 @@ def main(argv=None):
             "copy_reference_GBs": round(copy_gbs, 1) if copy_gbs else None,
         },
         "per_function": per,
+        "autotuner": tuned_settings(wl + extras),
         "other_functions": all_rec,
         "ragged": ragged_rec,
         "roofline": {

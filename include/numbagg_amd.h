@@ -50,6 +50,17 @@ const char* nbg_last_error(void);
 int nbg_spin_timeouts(unsigned int* count);
 int nbg_reset_spin_timeouts(void);
 
+/* Tuning knobs of this host thread (no reference counterpart): tile shapes,
+ * grid caps and path choices the autotuner picks per call shape, by NBG_* name
+ * (the same names are read from the environment when no override is set).
+ * Overrides are thread-local, so concurrent callers do not see each other's
+ * choices; results never depend on them beyond the float rounding the parity
+ * bars allow. nbg_tune_get returns the effective value (override, environment,
+ * else dflt). */
+int nbg_tune_set(const char* name, int64_t value);
+int nbg_tune_clear(void);
+int64_t nbg_tune_get(const char* name, int64_t dflt);
+
 /* Streaming copy of `bytes` (16-byte aligned, multiple of 16). No reference
  * counterpart: a calibration kernel for bandwidth and traffic measurements. */
 int nbg_stream_copy(const void* src, void* dst, size_t bytes, void* stream);

@@ -30,6 +30,9 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "nbg_spin_timeouts": (_c_int, [ctypes.POINTER(ctypes.c_uint)]),
     "nbg_reset_spin_timeouts": (_c_int, []),
     "nbg_stream_copy": (_c_int, [_c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "nbg_tune_set": (_c_int, [ctypes.c_char_p, _c_i64]),
+    "nbg_tune_clear": (_c_int, []),
+    "nbg_tune_get": (_c_i64, [ctypes.c_char_p, _c_i64]),
     "nbg_fill_workspace_bytes": (_c_size, [_c_i64, _c_i64, _c_int]),
     "nbg_ffill": (
         _c_int,

@@ -29,6 +29,13 @@ int device_capacity_blocks(const void* kernel, int block, size_t lds_bytes);
     if (_e != hipSuccess) return ::nbg::check_hip(_e, #expr);     \
   } while (0)
 
+// Tuning knobs (tile shapes, grid caps, path choices). The value of knob
+// `name` is this host thread's override (nbg_tune_set: the autotuner's choice
+// for the calls the thread is about to make), else the environment variable of
+// the same name, else `dflt`. Results never depend on a knob beyond the
+// float rounding the parity bars allow.
+long long knob(const char* name, long long dflt);
+
 // Device address of this device's counter of bounded spin-waits that expired
 // (should always stay 0). Kernels that wait on other workgroups take it as an
 // argument; the host reads it through nbg_spin_timeouts().

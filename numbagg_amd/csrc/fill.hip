@@ -743,11 +743,8 @@ __global__ __launch_bounds__(kFillThreads) void fill_span_kernel(
 // 16-byte vectors per thread: more or fewer workgroups and look-back tiles).
 // The workspace query reads the same setting.
 int fill_bytes_per_thread() {
-  const char* v = getenv("NBG_FILL_BYTES");
-  if (v != nullptr && strcmp(v, "16") == 0) return 16;
-  if (v != nullptr && strcmp(v, "32") == 0) return 32;
-  if (v != nullptr && strcmp(v, "128") == 0) return 128;
-  return 64;
+  const long long v = knob("NBG_FILL_BYTES", 64);
+  return (v == 16 || v == 32 || v == 128) ? (int)v : 64;
 }
 
 template <typename T, int BYTES>
@@ -781,24 +778,20 @@ int fill_impl_b(bool rev, const void* xv, void* ov, int64_t rows, int64_t n, int
   // PACK: the status word carries the float32 carry value (rows < 2**30;
   // NBG_FILL_PACK=0 turns it off, so tests reach the other path on short rows).
   constexpr bool PACK_OK = sizeof(T) == 4;
-  const char* pk = getenv("NBG_FILL_PACK");
-  const bool pack = PACK_OK && n <= kPackMaxN && !(pk != nullptr && strcmp(pk, "0") == 0);
+  const bool pack = PACK_OK && n <= kPackMaxN && knob("NBG_FILL_PACK", 1) != 0;
   // Tiles per dispatch run of one row: NBG_FILL_RUN (default 1, row-interleaved).
-  int64_t run = 1;
-  if (const char* e = getenv("NBG_FILL_RUN")) run = atoll(e);
+  int64_t run = knob("NBG_FILL_RUN", 1);
   run = run < 1 ? 1 : (run > tiles ? tiles : run);
 #define NBG_FILL_LAUNCH_P(V, U, R, LB, P)                                                          \
   hipLaunchKernelGGL((fill_kernel<T, V, U, R, LB, P>), grid, block, 0, s, x, out, rows, n, ld_x, \
                      ld_out, limit, tiles, counters, status, to, run)
   // Spans: NBG_FILL_SPAN=<k> tiles per workgroup (1 = one tile each).
-  int64_t span = 1;
-  if (const char* e = getenv("NBG_FILL_SPAN")) span = atoll(e);
+  int64_t span = knob("NBG_FILL_SPAN", 1);
   span = span < 1 ? 1 : (span > 16 ? 16 : span);
   if (span > tiles) span = tiles;
   const int64_t spans = (tiles + span - 1) / span;
   dim3 sgrid((unsigned)(rows * spans));
-  const char* fpf = getenv("NBG_FILL_PREFETCH");
-  const bool fill_pf = fpf != nullptr && strcmp(fpf, "1") == 0;
+  const bool fill_pf = knob("NBG_FILL_PREFETCH", 0) == 1;
 #define NBG_FILL_SPAN_P(V, U, R, P)                                                                       \
   do {                                                                                                    \
     if (fill_pf)                                                                                          \
@@ -855,9 +848,9 @@ inline int64_t fill_pass1_tile(int elem) { return (int64_t)kFillThreads * 16 * k
 bool fill_two_pass(int64_t rows, int64_t n, int elem) {
   const int64_t t = fill_pass1_tile(elem);
   if (rows <= 0 || (n + t - 1) / t <= 1) return false;
-  const char* v = getenv("NBG_FILL_TWO_PASS");
-  if (v != nullptr && strcmp(v, "0") == 0) return false;
-  if (v != nullptr && strcmp(v, "1") == 0) return true;
+  const long long v = knob("NBG_FILL_TWO_PASS", -1);
+  if (v == 0) return false;
+  if (v == 1) return true;
   return rows >= kFillTwoPassRows;
 }
 

@@ -32,11 +32,6 @@ namespace nbg {
 namespace {
 
 constexpr int kGroupThreads = 256;
-// LDS slots past the label slots: an element that must not be counted (NaN,
-// or a label outside [0, num_labels)) updates slot L + lane instead of taking
-// a branch, so the element loop of the flag, count, extreme and sum passes
-// runs without exec-mask branches; those slots are never flushed.
-constexpr int kSpare = 64;
 constexpr int kGroupUnroll = 4;  // default vectors per thread and super-step
 
 enum GKind {
@@ -231,7 +226,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
 
   // LDS accumulators.
   SA* l_s = reinterpret_cast<SA*>(smem_raw);
-  unsigned int* l_c = reinterpret_cast<unsigned int*>(smem_raw + (PASS == kPSum ? (L + kSpare) * sizeof(SA) : 0));
+  unsigned int* l_c = reinterpret_cast<unsigned int*>(smem_raw + (PASS == kPSum ? L * sizeof(SA) : 0));
   unsigned long long* l_u = reinterpret_cast<unsigned long long*>(smem_raw);
   unsigned int* l_u32 = reinterpret_cast<unsigned int*>(smem_raw);
   double* l_sq = reinterpret_cast<double*>(smem_raw);
@@ -304,37 +299,6 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
   // hot_c: std::true_type on the register path for labels 0 and 1.
   auto update = [&](auto hot_c, TV v, TL lbl, int64_t i, int r) {
     constexpr bool H = decltype(hot_c)::value;
-    constexpr bool SPARE = LDS && !H &&
-                           (PASS == kPSum || PASS == kPSumSq || PASS == kPCount || PASS == kPAny ||
-                            PASS == kPAll || PASS == kPPacked || PASS == kPFirst || PASS == kPLast);
-    if constexpr (SPARE) {
-      bool ok;
-      if constexpr (sizeof(TL) == 4) ok = (uint32_t)lbl < lim32;
-      else ok = (uint64_t)lbl < (uint64_t)L1;
-      ok = ok && gvalid(v);
-      if constexpr (PASS == kPAny) ok = ok && v != TV(0);
-      if constexpr (PASS == kPAll) ok = ok && v == TV(0);
-      const int l = ok ? r * (int)L1 + (int)lbl : (int)L + lane_id();
-      if constexpr (PASS == kPSum) {
-        atomicAdd(&l_s[l], ok ? (SA)v : SA(0));
-        if (COUNT) atomicAdd(&l_c[l], 1u);
-      } else if constexpr (PASS == kPSumSq) {
-        atomicAdd(&l_s[l], ok ? (SA)v * (SA)v : SA(0));
-      } else if constexpr (PASS == kPCount) {
-        atomicAdd(&l_u32[l], 1u);
-      } else if constexpr (PASS == kPAny || PASS == kPAll) {
-        atomicOr(&l_u32[l], 1u);
-      } else if constexpr (PASS == kPPacked) {
-        uint32_t k = okey32(v);
-        if (A.argmin) k = ~k;
-        atomicMax(&l_u[l], ((unsigned long long)k << 32) | (unsigned long long)(~(uint32_t)i));
-      } else if constexpr (PASS == kPFirst) {
-        atomicMin(&l_u[l], (unsigned long long)i);
-      } else {
-        atomicMax(&l_u[l], (unsigned long long)(i + 1));
-      }
-      return;
-    }
     if constexpr (sizeof(TL) == 4) {
       if ((uint32_t)lbl >= lim32) return;
     } else {
@@ -788,21 +752,21 @@ int64_t pass_slot_bytes(int pass, bool count, int64_t sa_bytes) {
   }
 }
 
-int env_int(const char* name, int dflt) {
-  const char* v = getenv(name);
-  return (v != nullptr && v[0] != '\0') ? atoi(v) : dflt;
-}
+int env_int(const char* name, int dflt) { return (int)knob(name, dflt); }
 
 // Workgroups per CU of the LDS strategy's grid. Every workgroup ends with one
 // global atomic per used label slot, and all of them end together, so the
 // grid is kept at the fewest workgroups that still keep enough loads in
-// flight: 4 for the float64 sum passes (2 rows x 2 vectors per thread), 2 for
-// the others (4 rows x 4 vectors per thread). Measured on MI355X for (64, 4M)
-// float32 x 1000 labels: count/argmax/first/last 5-8% faster at 2 than at the
-// occupancy limit, the sum pass alone (without init and finalize) 0.183 ms
-// at 4 against 0.194-0.220 ms at 8. Knob: NBG_GROUP_BLOCKS_PER_CU (0 = the
-// occupancy limit). Results do not depend on it.
-constexpr int group_blocks_per_cu(int pass) { return (pass == kPSum || pass == kPDev || pass == kPMom) ? 4 : 2; }
+// flight: 2 for the sum passes (2 rows x 4 vectors per thread) and the
+// others (4 rows x 4 vectors per thread), 4 for the deviation and moment
+// passes. Measured on MI355X for (64, 4M) float32 x 1000 labels, launches
+// back to back: group_nansum with uniform labels 0.180 ms at 2 workgroups per
+// CU and 4 vectors against 0.200 ms at 4 and 2 vectors (Zipf(1.1) labels:
+// 0.201 against 0.192 ms), count/argmax/first/last 5-8% faster at 2 than at
+// the occupancy limit. Knob: NBG_GROUP_BLOCKS_PER_CU (0 = the occupancy
+// limit); the autotuner may pick another value per shape. Results do not
+// depend on it.
+constexpr int group_blocks_per_cu(int pass) { return (pass == kPDev || pass == kPMom) ? 4 : 2; }
 
 // Chunking: the row groups' positions are split so that the whole grid is
 // one wave of resident workgroups (the device's capacity for this kernel and
@@ -841,14 +805,14 @@ int launch_pass(GArgs a, size_t lds, hipStream_t s) {
 // the LDS budget (the rows then share each label load), else 2, else 1.
 constexpr int kGroupRowsPerBlock = 4;
 
-// Shape of the sum pass (sum, mean, and the first pass of var/std): at most
-// 2 rows per workgroup and 2 vectors per thread and super-step (measured
-// fastest on MI355X for (64, 4M) x 1000 labels: 16 KB of LDS slots and ~50
-// VGPRs let 8 workgroups per CU stay resident). Knobs: NBG_GROUP_ROWS (1, 2,
-// 4), NBG_GROUP_UNROLL (2, 4), NBG_GROUP_PIPE=1 (load the next row while
-// folding the current one). Results do not depend on them.
+// Shape of the sum pass (sum, mean, sum of squares, and the first pass of
+// var/std on global accumulators): at most 2 rows per workgroup and 4
+// vectors per thread and super-step, with 2 workgroups per CU (see
+// group_blocks_per_cu). Knobs: NBG_GROUP_ROWS (1, 2, 4), NBG_GROUP_UNROLL (2,
+// 4), NBG_GROUP_PIPE=1 (load the next row while folding the current one).
+// Results do not depend on them.
 constexpr int kSumRows = 2;
-constexpr int kSumUnroll = 2;
+constexpr int kSumUnroll = 4;
 // Hot labels 0 and 1 in registers for the float sum passes (see
 // group_pass_kernel). Knob: NBG_GROUP_HOT=0/1.
 constexpr int kGroupHotDefault = 1;
@@ -886,8 +850,7 @@ int run_pass(GArgs a, bool lds, hipStream_t s) {
     if (!lds) return launch_pass<TV, TL, PASS, false, 1, COUNT>(a, 0, s);
   }
   const int rmax = PASS == kPSum ? env_int("NBG_GROUP_ROWS", kSumRows) : kGroupRowsPerBlock;
-  // R rows of label slots plus the spare slots
-  auto bytes = [&](int64_t r) { return (L * r + kSpare) * slot; };
+  auto bytes = [&](int64_t r) { return L * r * slot; };
   if (rmax >= 4 && a.rows >= 4 && bytes(4) <= kGroupLdsBudget)
     return launch_rows<TV, TL, PASS, COUNT, 4>(a, (size_t)bytes(4), s);
   if (rmax >= 2 && a.rows >= 2 && bytes(2) <= kGroupLdsBudget)
@@ -951,7 +914,7 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
   unsigned long long* acc2 = acc1 + per;
   unsigned long long* acc3 = acc2 + per;
   a.acc2 = nullptr;
-  const int64_t lds_need = (L + kSpare) * lds_bytes_per_label(KIND, 0);
+  const int64_t lds_need = L * lds_bytes_per_label(KIND, 0);
   bool lds = lds_need <= kGroupLdsBudget;
   if (strategy == NBG_GROUP_LDS && !lds) {
     // forced: one row per workgroup, up to the device's LDS per workgroup

@@ -48,7 +48,7 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   // tests use it to run carried halos on small inputs).
   int64_t span = rows * tiles / (256 * 96);
   span = span > 6 ? 6 : span;
-  if (const char* e = getenv("NBG_MOVE_SPAN")) span = atoll(e);
+  if (const long long k = knob("NBG_MOVE_SPAN", 0)) span = k;
   span = span < 1 ? 1 : (span > kMaxSpan ? kMaxSpan : span);
   if (span > tiles) span = tiles;
   const int64_t spans = (tiles + span - 1) / span;
@@ -67,8 +67,7 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   // Each next tile of a span is prefetched into registers while the current
   // one is evaluated (more VGPRs, fewer resident waves; measured faster on
   // MI355X); NBG_MOVE_PREFETCH=0 turns it off.
-  const char* pfe = getenv("NBG_MOVE_PREFETCH");
-  const bool pf = !(pfe != nullptr && pfe[0] == '0' && pfe[1] == '\0');
+  const bool pf = knob("NBG_MOVE_PREFETCH", 1) != 0;
 #define NBG_MOVE_LAUNCH(TL, V, LU)                                                                        \
   do {                                                                                                    \
     if (pf && TL && Tr::NARGS == 1)                                                                       \

@@ -737,12 +737,8 @@ constexpr int64_t exp_min_tile() {
 // 2.73 / 3.17 ms in one pass, 3.12 / 3.38 in two).
 template <int KIND>
 bool exp_two_pass(int64_t tiles) {
-  if (const char* e = getenv("NBG_EXP_TWO_PASS")) {
-    if (e[0] == '0' && e[1] == '\0') return false;
-    if (e[0] == '1' && e[1] == '\0') return true;
-  }
   (void)tiles;
-  return false;
+  return knob("NBG_EXP_TWO_PASS", 0) == 1;
 }
 
 template <typename TX, typename TA, int KIND>

@@ -360,7 +360,7 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
     return set_error(NBG_ERR_INVALID, "ragged: no workgroup fits (%zu bytes of LDS for window %lld and %lld label slots)",
                      lds, (long long)window, (long long)lds_labels);
   int grab = kRaggedGrab;
-  if (const char* e = getenv("NBG_RAGGED_GRAB")) grab = atoi(e);
+  if (const long long g = knob("NBG_RAGGED_GRAB", 0)) grab = (int)g;
   if (grab < 1 || grab > kRaggedMaxGrab)
     return set_error(NBG_ERR_INVALID, "ragged: NBG_RAGGED_GRAB %d not in [1, %d]", grab, kRaggedMaxGrab);
   a.grab = grab;

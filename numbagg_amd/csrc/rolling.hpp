@@ -788,9 +788,7 @@ __device__ __forceinline__ void fill_recip_lut(double* lut, int64_t window, int6
 // halo over more outputs, fewer keep more waves resident). Plain and ragged
 // launches read the same setting, so their tiles stay identical.
 inline int f32_sum_items() {
-  const char* v = getenv("NBG_MOVE_ITEMS");
-  if (v != nullptr && v[0] == '3' && v[1] == '2' && v[2] == '\0') return 32;
-  return 16;
+  return knob("NBG_MOVE_ITEMS", 16) == 32 ? 32 : 16;
 }
 
 // Whether a float32 mean of this window uses the table (plain and ragged

@@ -30,6 +30,23 @@ unsigned int* spin_counter_ptr() {
 
 static thread_local char t_err[512] = "";
 
+// This thread's knob overrides (a handful of entries at most).
+struct KnobEntry {
+  char name[40];
+  long long value;
+};
+constexpr int kMaxKnobs = 32;
+static thread_local KnobEntry t_knobs[kMaxKnobs];
+static thread_local int t_nknobs = 0;
+
+long long knob(const char* name, long long dflt) {
+  for (int i = 0; i < t_nknobs; ++i)
+    if (strcmp(t_knobs[i].name, name) == 0) return t_knobs[i].value;
+  const char* v = getenv(name);
+  if (v != nullptr && v[0] != '\0') return atoll(v);
+  return dflt;
+}
+
 int set_error(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -121,6 +138,28 @@ int nbg_stream_copy(const void* src, void* dst, size_t bytes, void* stream) {
   }
   return NBG_OK;
 }
+
+int nbg_tune_set(const char* name, int64_t value) {
+  if (name == nullptr || strncmp(name, "NBG_", 4) != 0 || strlen(name) >= sizeof(nbg::KnobEntry::name))
+    return nbg::set_error(NBG_ERR_INVALID, "nbg_tune_set: knob names are NBG_* (under 40 characters)");
+  for (int i = 0; i < nbg::t_nknobs; ++i)
+    if (strcmp(nbg::t_knobs[i].name, name) == 0) {
+      nbg::t_knobs[i].value = value;
+      return NBG_OK;
+    }
+  if (nbg::t_nknobs == nbg::kMaxKnobs) return nbg::set_error(NBG_ERR_INVALID, "nbg_tune_set: too many knobs");
+  strcpy(nbg::t_knobs[nbg::t_nknobs].name, name);
+  nbg::t_knobs[nbg::t_nknobs].value = value;
+  ++nbg::t_nknobs;
+  return NBG_OK;
+}
+
+int nbg_tune_clear(void) {
+  nbg::t_nknobs = 0;
+  return NBG_OK;
+}
+
+int64_t nbg_tune_get(const char* name, int64_t dflt) { return name ? nbg::knob(name, dflt) : dflt; }
 
 const char* nbg_version(void) { return "numbagg_amd 0.1.0 (gfx950)"; }
 

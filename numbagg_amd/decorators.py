@@ -26,6 +26,7 @@ from numpy.lib.array_utils import normalize_axis_tuple
 
 from . import _device as dev
 from . import _lib
+from . import plain_tuning
 
 logger = logging.getLogger(__name__)
 
@@ -165,7 +166,8 @@ class ndmove(_Op):
             args += [dev.ptr(out), rows, n]
             args += [v[3] for v in views]
             args += [n, int(window), int(min_count), dev.stream_handle()]
-            _lib.call(self.entry, *args)
+            with plain_tuning.applied(self.__name__, loop, rows, n, int(window)):
+                _lib.call(self.entry, *args)
         # NumPy places the output's core dimension at `axis` as well.
         res = out.view(*outer, n).movedim(-1, axis)
         return dev.to_output(res, as_tensor)
@@ -253,7 +255,8 @@ class ndmoveexp(_Op):
             args += [alpha_ptr, ld_alpha, ctypes.c_double(alpha_scalar), dev.ptr(out), rows, n]
             args += [v[3] for v in views]
             args += [n, ctypes.c_double(mw), dev.ptr(ws), ws_bytes, dev.stream_handle()]
-            _lib.call(self.entry, *args)
+            with plain_tuning.applied(self.__name__, loop, rows, n):
+                _lib.call(self.entry, *args)
         del alpha_keep
         # NumPy places the output's core dimension at `axis` as well.
         res = out.view(*outer, n).movedim(-1, axis)
@@ -290,22 +293,24 @@ class ndfill(_Op):
         v2, rows, n, ld = dev.rows_view(moved)
         out = torch.empty((rows, n), dtype=t.dtype, device=t.device)
         if rows and n:
-            ws_bytes = _lib.query("nbg_fill_workspace_bytes", rows, n, dev.nbg_code(t))
-            ws, ws_bytes = dev.workspace(ws_bytes, t.device)
-            _lib.call(
-                self.entry,
-                ctypes.c_int(dev.nbg_code(t)),
-                dev.ptr(v2),
-                dev.ptr(out),
-                rows,
-                n,
-                ld,
-                n,
-                int(limit),
-                dev.ptr(ws),
-                ws_bytes,
-                dev.stream_handle(),
-            )
+            # the workspace query and the call see the same knob settings
+            with plain_tuning.applied(self.__name__, work, rows, n):
+                ws_bytes = _lib.query("nbg_fill_workspace_bytes", rows, n, dev.nbg_code(t))
+                ws, ws_bytes = dev.workspace(ws_bytes, t.device)
+                _lib.call(
+                    self.entry,
+                    ctypes.c_int(dev.nbg_code(t)),
+                    dev.ptr(v2),
+                    dev.ptr(out),
+                    rows,
+                    n,
+                    ld,
+                    n,
+                    int(limit),
+                    dev.ptr(ws),
+                    ws_bytes,
+                    dev.stream_handle(),
+                )
         res = out.view(moved.shape).movedim(-1, axis)
         if work != dt:
             res = res.to(dev.torch_dtype(dt))
@@ -447,7 +452,8 @@ class groupndreduce(_Op):
             if self.supports_ddof:
                 args.append(int(ddof))
             args += [ctypes.c_int(strategy), dev.ptr(ws), ws_bytes, dev.stream_handle()]
-            _lib.call(self.entry, *args)
+            with plain_tuning.applied(self.__name__, kdt, rows, n, int(num_labels)):
+                _lib.call(self.entry, *args)
         res = out.view(*broadcast_shape, num_labels)
         if kdt != result_dtype:
             res = res.to(dev.torch_dtype(result_dtype))

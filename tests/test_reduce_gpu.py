@@ -49,15 +49,18 @@ def _fsum(a, axis):
 def _check_sum(actual, want, sabs, tag):
     """float32: within 1 ULP of the exact sum; float64: within 2**-52 of it
     relative plus 2**-100 of the sum of |x| (double-double accumulation)."""
-    g = np.asarray(actual, dtype=np.float64)
-    want = np.asarray(want)
+    dt = np.asarray(actual).dtype
+    g = np.atleast_1d(np.asarray(actual, dtype=np.float64)).ravel()
+    want = np.atleast_1d(np.asarray(want)).ravel()
+    sabs = np.atleast_1d(np.asarray(sabs)).ravel()
+    assert g.shape == want.shape, tag
     fin = np.isfinite(want)
     assert np.array_equal(g[~fin], want[~fin], equal_nan=True), tag
     err = np.abs(g[fin] - want[fin])
-    if np.asarray(actual).dtype == np.float32:
-        bar = np.spacing(np.abs(want[fin]).astype(np.float32)).astype(np.float64) + 1e-14 * np.asarray(sabs)[fin]
+    if dt == np.float32:
+        bar = np.spacing(np.abs(want[fin]).astype(np.float32)).astype(np.float64) + 1e-14 * sabs[fin]
     else:
-        bar = 2.0**-52 * np.abs(want[fin]) + 2.0**-100 * np.asarray(sabs)[fin]
+        bar = 2.0**-52 * np.abs(want[fin]) + 2.0**-100 * sabs[fin]
     assert np.all(err <= bar), (tag, float(np.max(err - bar)))
 
 
