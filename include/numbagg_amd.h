@@ -281,6 +281,27 @@ int nbg_nanargmax(int dtype, const void* x, void* out, int64_t rows, int64_t n,
 int nbg_nanargmin(int dtype, const void* x, void* out, int64_t rows, int64_t n,
                   int64_t ld, void* ws, size_t ws_bytes, void* stream); /* funcs.py:181-197 */
 
+/* ---- label factorisation (the caller-side step before the grouped
+ *      reductions: numbagg/decorators.py:570-573 requires factorised labels,
+ *      numbagg/test/conftest.py:175-189 factorises with pd.factorize(sort=True))
+ * codes[i] = position of values[i] among the uniques (-1 for NaN); uniques
+ * ascending (sort != 0) or in order of first appearance; -0.0 and +0.0 are one
+ * value, represented by whichever comes first. Two calls: _count leaves the
+ * number of uniques in *num_uniques (device memory; the caller reads it and
+ * allocates the uniques), _finish writes codes (int64, n) and uniques (values
+ * dtype). The workspace must be left untouched between the two calls and
+ * num_uniques must be the count _count wrote for the same values: on a
+ * mismatch _finish writes nothing; a value absent from the table (values
+ * changed in between) gets code -1. dtype: NBG_F32, NBG_F64, NBG_I32, NBG_I64. */
+size_t nbg_factorize_workspace_bytes(int64_t n);
+int nbg_factorize_count(int dtype, const void* values, int64_t n, int sort,
+                        void* ws, size_t ws_bytes, int64_t* num_uniques,
+                        void* stream);
+int nbg_factorize_finish(int dtype, const void* values, int64_t n,
+                         int64_t num_uniques, int sort, void* codes,
+                         void* uniques, void* ws, size_t ws_bytes,
+                         void* stream);
+
 /* No counterpart in the reference. Allocation-free variant of
  * nbg_group_nansum for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):

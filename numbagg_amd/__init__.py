@@ -7,6 +7,7 @@ returned as NumPy arrays; torch tensors on the GPU stay on the GPU.
 """
 
 from ._lib import spin_timeouts
+from .factorize import factorize
 from .funcs import (
     allnan,
     anynan,
@@ -107,6 +108,7 @@ __all__ = [
     "anynan",
     "bfill",
     "count",
+    "factorize",
     "ffill",
     "nanargmax",
     "nanargmin",

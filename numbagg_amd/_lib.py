@@ -94,6 +94,12 @@ for _name in ("nansum", "nanmean", "nancount", "allnan", "anynan", "nanmax", "na
     _SIGNATURES[f"nbg_{_name}"] = _RED
 for _name in ("nanvar", "nanstd"):
     _SIGNATURES[f"nbg_{_name}"] = _RED_DDOF
+_SIGNATURES["nbg_factorize_workspace_bytes"] = (_c_size, [_c_i64])
+_SIGNATURES["nbg_factorize_count"] = (_c_int, [_c_int, _c_ptr, _c_i64, _c_int, _c_ptr, _c_size, _c_ptr, _c_ptr])
+_SIGNATURES["nbg_factorize_finish"] = (
+    _c_int,
+    [_c_int, _c_ptr, _c_i64, _c_i64, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr],
+)
 _SIGNATURES["nbg_ragged_tile_size"] = (_c_i64, [])
 _SIGNATURES["nbg_ragged_workspace_bytes"] = (_c_size, [_c_i64, _c_i64])
 _SIGNATURES["nbg_ragged_max_lds_labels"] = (_c_i64, [])

@@ -1,0 +1,344 @@
+// Device label factorisation for gfx950: codes and uniques of a label array,
+// the caller-side step before the grouped reductions (the reference requires
+// labels already factorised, numbagg/decorators.py:570-573, and its benchmarks
+// factorise with pd.factorize(labels, sort=True), numbagg/test/conftest.py:175-189).
+//
+// Semantics (pandas.factorize): codes[i] = position of values[i] among the
+// uniques, -1 for NaN; uniques sorted ascending (sort) or in order of first
+// appearance (no sort); -0.0 and +0.0 are one value.
+//
+// Design (hash, then sort the uniques):
+//   1. insert: every value's 64-bit key goes into an open-addressing table
+//      (linear probing, compare-and-swap on an empty marker); each slot also
+//      records the first position of its key (atomic minimum);
+//   2. compact: occupied slots append (sort key, slot) pairs to a list;
+//   3. sort: a bitonic network sorts the list by sort key (the order-preserving
+//      image of the value, or the first position), padded to a power of two;
+//   4. rank: each sorted entry writes its rank into its slot and its value into
+//      the uniques;
+//   5. codes: every value finds its slot again and reads the rank.
+// The empty marker is the all-ones word: no float key can equal it (it is a
+// NaN pattern, and NaN is never inserted); the integer -1 (sign-extended to
+// the same bits) is tracked by a flag outside the table.
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace nbg {
+namespace {
+
+constexpr unsigned long long kEmpty = ~0ull;
+constexpr int kFzThreads = 256;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
+  // finaliser of a 64-bit multiplicative hash (xor-shift / multiply rounds)
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// The table key of a value (-0.0 folded onto +0.0) and whether it is NaN.
+template <typename T>
+__device__ __forceinline__ unsigned long long fz_key(T v, bool& nan) {
+  if constexpr (std::is_floating_point<T>::value) {
+    nan = v != v;
+    const double d = (double)v + 0.0;
+    return (unsigned long long)__double_as_longlong(d);
+  } else {
+    nan = false;
+    return (unsigned long long)(long long)v;
+  }
+}
+
+// Order-preserving image of a key as an unsigned word (floats by their
+// sign-magnitude order, integers by offsetting the sign).
+template <typename T>
+__device__ __forceinline__ unsigned long long fz_order(unsigned long long k) {
+  if constexpr (std::is_floating_point<T>::value) {
+    return (k & 0x8000000000000000ull) ? ~k : (k | 0x8000000000000000ull);
+  } else {
+    return k ^ 0x8000000000000000ull;
+  }
+}
+
+struct FzArgs {
+  int64_t n;
+  unsigned long long mask;             // table size - 1 (a power of two)
+  unsigned long long* keys;            // table keys (kEmpty: free)
+  unsigned long long* first;           // first position of the key
+  long long* rank;                     // the key's code
+  unsigned int* count;                 // [0] uniques in the table, [1] minus-one flag
+  unsigned long long* minus_one_first; // first position of an int64 -1
+  unsigned long long* skey;            // sort keys (padded list)
+  unsigned long long* sval;            // slot of each list entry (kEmpty: the int64 -1)
+  int64_t padded;                      // list length, a power of two
+};
+
+template <typename T>
+__global__ __launch_bounds__(kFzThreads) void fz_insert_kernel(const T* __restrict__ x, FzArgs A) {
+  for (int64_t i = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; i < A.n; i += (int64_t)gridDim.x * kFzThreads) {
+    bool nan;
+    const unsigned long long k = fz_key(x[i], nan);
+    if (nan) continue;
+    if (k == kEmpty) {  // int64 -1: outside the table
+      atomicOr(&A.count[1], 1u);
+      atomicMin(A.minus_one_first, (unsigned long long)i);
+      continue;
+    }
+    unsigned long long s = mix64(k) & A.mask;
+    for (;;) {  // the table holds at most half its slots, so a free slot is found
+      const unsigned long long old = atomicCAS(&A.keys[s], kEmpty, k);
+      if (old == kEmpty || old == k) {
+        atomicMin(&A.first[s], (unsigned long long)i);
+        break;
+      }
+      s = (s + 1) & A.mask;
+    }
+  }
+}
+
+// Occupied slots append (sort key, slot) to the list (positions from an
+// atomic counter; the bitonic sort fixes the order).
+template <typename T>
+__global__ __launch_bounds__(kFzThreads) void fz_compact_kernel(FzArgs A, int by_value) {
+  const int64_t size = (int64_t)A.mask + 1;
+  for (int64_t s = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; s < size; s += (int64_t)gridDim.x * kFzThreads) {
+    const unsigned long long k = A.keys[s];
+    if (k == kEmpty) continue;
+    const unsigned int p = atomicAdd(&A.count[0], 1u);
+    A.skey[p] = by_value ? fz_order<T>(k) : A.first[s];
+    A.sval[p] = (unsigned long long)s;
+  }
+}
+
+// The number of uniques (table entries plus an int64 -1).
+__device__ __forceinline__ int64_t fz_total(const FzArgs& A) {
+  return (int64_t)A.count[0] + (A.count[1] ? 1 : 0);
+}
+
+__global__ void fz_total_kernel(FzArgs A, long long* total) { *total = fz_total(A); }
+
+// Entry K - 1 for an int64 -1 (value kEmpty, outside the table), and the
+// padding [K, padded): the largest key with a value above every slot, so
+// that (key, value) pairs order padding after every real entry.
+template <typename T>
+__global__ void fz_pad_kernel(FzArgs A, int64_t k_total, int by_value) {
+  if (fz_total(A) != k_total) return;  // not the count of these values: write nothing
+  const int64_t table_entries = (int64_t)A.count[0];
+  for (int64_t p = table_entries + (int64_t)blockIdx.x * kFzThreads + threadIdx.x; p < A.padded;
+       p += (int64_t)gridDim.x * kFzThreads) {
+    if (p < k_total) {  // the int64 -1
+      A.skey[p] = by_value ? fz_order<T>(kEmpty) : *A.minus_one_first;
+      A.sval[p] = kEmpty;
+    } else {
+      A.skey[p] = kEmpty;
+      A.sval[p] = kEmpty - 1;
+    }
+  }
+}
+
+// One compare-exchange stage of the bitonic network (ascending overall).
+__global__ __launch_bounds__(kFzThreads) void fz_bitonic_kernel(unsigned long long* key, unsigned long long* val,
+                                                               int64_t padded, int64_t j, int64_t k) {
+  for (int64_t i = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; i < padded; i += (int64_t)gridDim.x * kFzThreads) {
+    const int64_t p = i ^ j;
+    if (p <= i) continue;
+    const bool up = (i & k) == 0;
+    const unsigned long long a = key[i], b = key[p];
+    const bool greater = a > b || (a == b && val[i] > val[p]);  // (key, value) order
+    if (greater == up) {
+      key[i] = b;
+      key[p] = a;
+      const unsigned long long t = val[i];
+      val[i] = val[p];
+      val[p] = t;
+    }
+  }
+}
+
+// Ranks into the table, and into the uniques the value at each key's first
+// position (so -0.0 or +0.0, whichever came first, as pandas keeps it).
+template <typename T>
+__global__ __launch_bounds__(kFzThreads) void fz_rank_kernel(const T* __restrict__ x, FzArgs A, int64_t k_total,
+                                                            T* __restrict__ uniques,
+                                                            long long* __restrict__ minus_one_rank) {
+  if (fz_total(A) != k_total) return;
+  for (int64_t r = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; r < k_total; r += (int64_t)gridDim.x * kFzThreads) {
+    const unsigned long long s = A.sval[r];
+    unsigned long long pos;
+    if (s == kEmpty) {
+      *minus_one_rank = r;
+      pos = *A.minus_one_first;
+    } else {
+      A.rank[s] = r;
+      pos = A.first[s];
+    }
+    uniques[r] = x[pos];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kFzThreads) void fz_codes_kernel(const T* __restrict__ x, FzArgs A,
+                                                             const long long* __restrict__ minus_one_rank,
+                                                             long long* __restrict__ codes, int64_t k_total) {
+  if (fz_total(A) != k_total) return;
+  for (int64_t i = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; i < A.n; i += (int64_t)gridDim.x * kFzThreads) {
+    bool nan;
+    const unsigned long long k = fz_key(x[i], nan);
+    long long c = -1;
+    if (!nan) {
+      if (k == kEmpty) {
+        c = *minus_one_rank;
+      } else {
+        // The table is at most half full, so the probe ends at the key or at a
+        // free slot (a key not inserted: values changed since the count).
+        for (unsigned long long s = mix64(k) & A.mask;; s = (s + 1) & A.mask) {
+          const unsigned long long ks = A.keys[s];
+          if (ks == k) {
+            c = A.rank[s];
+            break;
+          }
+          if (ks == kEmpty) break;
+        }
+      }
+    }
+    codes[i] = c;
+  }
+}
+
+__global__ void fz_init_kernel(FzArgs A) {
+  const int64_t size = (int64_t)A.mask + 1;
+  for (int64_t s = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; s < size; s += (int64_t)gridDim.x * kFzThreads) {
+    A.keys[s] = kEmpty;
+    A.first[s] = kEmpty;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.count[0] = 0u;
+    A.count[1] = 0u;
+    *A.minus_one_first = kEmpty;
+  }
+}
+
+int64_t next_pow2(int64_t v) {
+  int64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Table of 2 n slots (rounded up to a power of two, at least 1024): load
+// factor at most one half.
+int64_t fz_table_size(int64_t n) { return next_pow2(2 * (n > 512 ? n : 512)); }
+
+size_t fz_ws_bytes(int64_t n) {
+  const int64_t size = fz_table_size(n);
+  const int64_t padded = next_pow2(n + 1);
+  return (size_t)size * 24 + (size_t)padded * 16 + 32;
+}
+
+// Workspace: keys | first | rank (table size each, 8 bytes) | sort keys |
+// sort values (n + 1 rounded up to a power of two each) | counters.
+FzArgs fz_layout(int64_t n, void* ws) {
+  FzArgs a{};
+  const int64_t size = fz_table_size(n);
+  const int64_t padded_max = next_pow2(n + 1);
+  char* p = static_cast<char*>(ws);
+  a.n = n;
+  a.mask = (unsigned long long)(size - 1);
+  a.keys = reinterpret_cast<unsigned long long*>(p);
+  a.first = a.keys + size;
+  a.rank = reinterpret_cast<long long*>(a.first + size);
+  a.skey = reinterpret_cast<unsigned long long*>(a.rank + size);
+  a.sval = a.skey + padded_max;
+  a.count = reinterpret_cast<unsigned int*>(a.sval + padded_max);
+  a.minus_one_first = reinterpret_cast<unsigned long long*>(a.count + 2);
+  a.padded = padded_max;
+  return a;
+}
+long long* fz_minus_one_rank(int64_t n, void* ws) {
+  return reinterpret_cast<long long*>(fz_layout(n, ws).minus_one_first + 1);
+}
+
+int fz_grid(int64_t work) {
+  int64_t b = (work + kFzThreads - 1) / kFzThreads;
+  if (b > 4096) b = 4096;
+  return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+}  // namespace nbg
+
+extern "C" {
+
+size_t nbg_factorize_workspace_bytes(int64_t n) { return n < 0 ? 0 : nbg::fz_ws_bytes(n); }
+
+int nbg_factorize_count(int dtype, const void* values, int64_t n, int sort, void* ws, size_t ws_bytes,
+                        int64_t* num_uniques, void* stream) {
+  if (int rc = nbg::injected_error("nbg_factorize_count")) return rc;
+  if (n < 0) return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_count: negative length");
+  if ((values == nullptr && n > 0) || num_uniques == nullptr || ws == nullptr)
+    return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_count: null pointer");
+  if (ws_bytes < nbg::fz_ws_bytes(n))
+    return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_count: workspace too small");
+  nbg::FzArgs a = nbg::fz_layout(n, ws);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t size = (int64_t)a.mask + 1;
+  hipLaunchKernelGGL(nbg::fz_init_kernel, dim3(nbg::fz_grid(size)), dim3(nbg::kFzThreads), 0, s, a);
+#define NBG_FZ1(T)                                                                                          \
+  hipLaunchKernelGGL((nbg::fz_insert_kernel<T>), dim3(nbg::fz_grid(n)), dim3(nbg::kFzThreads), 0, s,          \
+                     (const T*)values, a);                                                                  \
+  hipLaunchKernelGGL((nbg::fz_compact_kernel<T>), dim3(nbg::fz_grid(size)), dim3(nbg::kFzThreads), 0, s, a,   \
+                     sort ? 1 : 0)
+  switch (dtype) {
+    case NBG_F32: NBG_FZ1(float); break;
+    case NBG_F64: NBG_FZ1(double); break;
+    case NBG_I32: NBG_FZ1(int32_t); break;
+    case NBG_I64: NBG_FZ1(int64_t); break;
+    default: return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_count: unsupported dtype code %d", dtype);
+  }
+#undef NBG_FZ1
+  hipLaunchKernelGGL(nbg::fz_total_kernel, dim3(1), dim3(1), 0, s, a, (long long*)num_uniques);
+  return nbg::check_hip(hipGetLastError(), "factorize launch");
+}
+
+int nbg_factorize_finish(int dtype, const void* values, int64_t n, int64_t num_uniques, int sort, void* codes,
+                         void* uniques, void* ws, size_t ws_bytes, void* stream) {
+  if (int rc = nbg::injected_error("nbg_factorize_finish")) return rc;
+  if (n < 0 || num_uniques < 0 || num_uniques > n)
+    return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_finish: bad sizes");
+  if ((codes == nullptr && n > 0) || (uniques == nullptr && num_uniques > 0) || ws == nullptr)
+    return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_finish: null pointer");
+  if (ws_bytes < nbg::fz_ws_bytes(n))
+    return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_finish: workspace too small");
+  if (n == 0) return NBG_OK;
+  nbg::FzArgs a = nbg::fz_layout(n, ws);
+  a.padded = nbg::next_pow2(num_uniques > 1 ? num_uniques : 1);
+  hipStream_t s = (hipStream_t)stream;
+  long long* minus_one_rank = nbg::fz_minus_one_rank(n, ws);
+#define NBG_FZ2(T)                                                                                            \
+  do {                                                                                                        \
+    hipLaunchKernelGGL((nbg::fz_pad_kernel<T>), dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0, s, a,   \
+                       num_uniques, sort ? 1 : 0);                                                            \
+    for (int64_t k = 2; k <= a.padded; k <<= 1)                                                               \
+      for (int64_t j = k >> 1; j > 0; j >>= 1)                                                                \
+        hipLaunchKernelGGL(nbg::fz_bitonic_kernel, dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0, s,  \
+                           a.skey, a.sval, a.padded, j, k);                                                   \
+    hipLaunchKernelGGL((nbg::fz_rank_kernel<T>), dim3(nbg::fz_grid(num_uniques)), dim3(nbg::kFzThreads), 0, s,  \
+                       (const T*)values, a, num_uniques, (T*)uniques, minus_one_rank);                        \
+    hipLaunchKernelGGL((nbg::fz_codes_kernel<T>), dim3(nbg::fz_grid(n)), dim3(nbg::kFzThreads), 0, s,           \
+                       (const T*)values, a, (const long long*)minus_one_rank, (long long*)codes, num_uniques);             \
+  } while (0)
+  switch (dtype) {
+    case NBG_F32: NBG_FZ2(float); break;
+    case NBG_F64: NBG_FZ2(double); break;
+    case NBG_I32: NBG_FZ2(int32_t); break;
+    case NBG_I64: NBG_FZ2(int64_t); break;
+    default: return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_finish: unsupported dtype code %d", dtype);
+  }
+#undef NBG_FZ2
+  return nbg::check_hip(hipGetLastError(), "factorize launch");
+}
+
+}  // extern "C"

@@ -359,11 +359,10 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
   if (cap <= 0)
     return set_error(NBG_ERR_INVALID, "ragged: no workgroup fits (%zu bytes of LDS for window %lld and %lld label slots)",
                      lds, (long long)window, (long long)lds_labels);
-  int grab = kRaggedGrab;
-  if (const long long g = knob("NBG_RAGGED_GRAB", 0)) grab = (int)g;
+  const long long grab = knob("NBG_RAGGED_GRAB", kRaggedGrab);
   if (grab < 1 || grab > kRaggedMaxGrab)
-    return set_error(NBG_ERR_INVALID, "ragged: NBG_RAGGED_GRAB %d not in [1, %d]", grab, kRaggedMaxGrab);
-  a.grab = grab;
+    return set_error(NBG_ERR_INVALID, "ragged: NBG_RAGGED_GRAB %lld not in [1, %d]", grab, kRaggedMaxGrab);
+  a.grab = (int)grab;
   int64_t grid = total_tiles < cap ? total_tiles : cap;
   if (grid < 1) grid = 1;
   // The 32-bit queue counter ends at most one grab per workgroup past the

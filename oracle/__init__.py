@@ -340,3 +340,23 @@ class exact:
             else:
                 raise ValueError(func)
         return out
+
+    @staticmethod
+    def factorize(values: np.ndarray, sort: bool = True):
+        """Closed form of pandas.factorize for 1-D labels: np.unique over the
+        non-NaN values (-0.0 folded onto +0.0), the first occurrence of each
+        value as its representative; codes -1 for NaN; uniques ascending or in
+        order of first appearance."""
+        v = np.asarray(values).ravel()
+        nan = np.isnan(v) if v.dtype.kind == "f" else np.zeros(v.shape, bool)
+        pos = np.nonzero(~nan)[0]
+        keys = v[pos] + 0 if v.dtype.kind == "f" else v[pos]
+        uk, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+        if not sort:
+            order = np.argsort(first, kind="stable")
+            rank = np.empty_like(order)
+            rank[order] = np.arange(order.size)
+            inv, first = rank[inv], first[order]
+        codes = np.full(v.shape, -1, dtype=np.int64)
+        codes[pos] = inv
+        return codes, v[pos[first]]

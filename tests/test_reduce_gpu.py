@@ -34,7 +34,8 @@ def _fsum(a, axis):
     axes = tuple(ax % max(a.ndim, 1) for ax in axes) if a.ndim else ()
     m = np.moveaxis(a, axes, tuple(range(a.ndim - len(axes), a.ndim))) if axes else a
     outer = m.shape[: a.ndim - len(axes)]
-    flat = m.reshape(int(np.prod(outer, dtype=np.int64)), -1) if a.ndim else m.reshape(1, 1)
+    inner = int(np.prod(m.shape[a.ndim - len(axes):], dtype=np.int64))
+    flat = m.reshape(int(np.prod(outer, dtype=np.int64)), inner) if a.ndim else m.reshape(1, 1)
     s = np.empty(flat.shape[0])
     sa = np.empty(flat.shape[0])
     for i, row in enumerate(flat):
