@@ -153,7 +153,7 @@ def tuned_settings(workloads):
     for w in workloads:
         func, dtype, rows, n, extra = tune_key(w)
         out[w.name] = db.get(plain_tuning.shape_key(arch, func, dtype, rows, n, extra))
-    return {"kernel_version": db.version, "entries": len(db.entries), "applied": out}
+    return {"kernel_versions": db.versions, "entries": len(db.entries), "applied": out}
 
 
 def has_entry(nb_lib, name):

@@ -1,10 +1,11 @@
 """Moving-window functions (numbagg/moving.py) on the gfx950 rolling kernels.
 
-Window statistics are computed from compensated window sums (FP32 TwoSum pairs
-for first moments of float32 data, float64 second/co-moments around a tile
-shift; double-double for float64 data), so results are closer to the exact
-values than the reference's running add/subtract accumulators, while the
-valid-count masking (`min_count`) is identical.
+Window statistics are computed from float64 running sums for float32 data
+(first moments; FP32 TwoSum pairs are a build-time alternative), float64
+second/co-moments around a per-tile shift, and double-double sums for float64
+data, so results are closer to the exact values than the reference's running
+add/subtract accumulators, while the valid-count masking (`min_count`) is
+identical.
 """
 
 from __future__ import annotations

@@ -5,12 +5,13 @@ Each family of kernels exposes tuning knobs (``NBG_*`` names, read by the C
 library through ``nbg_tune_set`` overrides of the calling thread, else the
 environment, else the kernel's default). ``tune`` times every candidate
 setting of a family's knobs on the given call with HIP events and records
-the fastest in a JSON database keyed by GPU architecture, a hash of the
-kernel sources, the function, dtype and the call's shape class (log2 rows and
-length, and the window or label count). The host operators apply the
-recorded setting of a call's shape class around the C call (``applied``);
-shapes without an entry run the kernels' defaults. A database written for
-other kernel sources or another architecture is ignored.
+the fastest in a JSON database keyed by GPU architecture, the function,
+dtype and the call's shape class (log2 rows and length, and the window or
+label count), and versioned per family by a hash of that family's kernel
+sources. The host operators apply the recorded setting of a call's shape
+class around the C call (``applied``); shapes without an entry run the
+kernels' defaults. Entries of a family whose sources changed since they were
+measured are ignored, as are entries of another architecture.
 
 Knob settings change launch shapes only: results stay within the parity
 bars (bit-exact functions stay bit-exact; float sums and moments may differ
@@ -63,14 +64,33 @@ def family_of(func: str) -> str | None:
     return None
 
 
-def kernel_version() -> str:
-    """Hash of the kernel sources the recorded settings were measured on."""
+# The sources each family's kernels are built from.
+FAMILY_SOURCES: dict[str, tuple[str, ...]] = {
+    "move": ("moving.hip", "rolling.hpp", "common.hpp"),
+    "group_sum": ("grouped.hip", "common.hpp"),
+    "group": ("grouped.hip", "common.hpp"),
+    "fill": ("fill.hip", "common.hpp"),
+    "exp": ("moving_exp.hip", "common.hpp"),
+}
+
+
+def kernel_version(family: str) -> str:
+    """Hash of the sources of ``family``'s kernels."""
     h = hashlib.sha256()
     csrc = pathlib.Path(__file__).resolve().parent / "csrc"
-    for p in sorted(csrc.glob("*.h*")):
-        h.update(p.name.encode())
-        h.update(p.read_bytes())
+    for name in FAMILY_SOURCES[family]:
+        h.update(name.encode())
+        h.update((csrc / name).read_bytes())
     return h.hexdigest()[:16]
+
+
+def kernel_versions() -> dict[str, str]:
+    return {fam: kernel_version(fam) for fam in FAMILY_SOURCES}
+
+
+def _func_of(key: str) -> str:
+    parts = key.split("|")
+    return parts[1] if len(parts) > 1 else ""
 
 
 def _bucket(v: int) -> int:
@@ -84,11 +104,12 @@ def shape_key(arch: str, func: str, dtype: str, rows: int, n: int, extra: int = 
 
 
 class PlainTuningDB:
-    """The recorded settings (JSON: {"kernel_version", "entries": {key: {...}}})."""
+    """The recorded settings (JSON: {"kernel_versions": {family: hash},
+    "entries": {key: {...}}})."""
 
     def __init__(self, path: pathlib.Path | str = DB_PATH) -> None:
         self.path = pathlib.Path(path)
-        self.version = kernel_version()
+        self.versions = kernel_versions()
         self.entries: dict[str, dict] = {}
         self._lock = threading.Lock()
         try:
@@ -96,8 +117,13 @@ class PlainTuningDB:
                 raw = json.load(f)
         except (OSError, ValueError):
             raw = {}
-        if raw.get("kernel_version") == self.version:
-            self.entries = dict(raw.get("entries", {}))
+        if not isinstance(raw, dict):
+            raw = {}
+        recorded = raw.get("kernel_versions") or {}
+        for key, rec in dict(raw.get("entries") or {}).items():
+            fam = family_of(_func_of(key))
+            if fam is not None and recorded.get(fam) == self.versions[fam]:
+                self.entries[key] = rec
 
     def get(self, key: str) -> dict | None:
         rec = self.entries.get(key)
@@ -110,7 +136,7 @@ class PlainTuningDB:
     def save(self) -> None:
         self.path.parent.mkdir(parents=True, exist_ok=True)
         with open(self.path, "w") as f:
-            json.dump({"kernel_version": self.version, "entries": self.entries}, f, indent=1, sort_keys=True)
+            json.dump({"kernel_versions": self.versions, "entries": self.entries}, f, indent=1, sort_keys=True)
 
 
 _db: PlainTuningDB | None = None
@@ -247,6 +273,7 @@ __all__ = [
     "database",
     "family_of",
     "kernel_version",
+    "kernel_versions",
     "knobs_set",
     "reload_database",
     "set_knobs",

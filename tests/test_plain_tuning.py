@@ -44,14 +44,20 @@ def test_database_versioning_and_keys(tmp_path):
     assert key == "gfx950|move_mean|float32|r8|n20|x9"
     path = tmp_path / "db.json"
     db = plain_tuning.PlainTuningDB(path)
+    fill_key = plain_tuning.shape_key("gfx950", "ffill", "float64", 1, 1 << 24)
     db.put(key, {"NBG_MOVE_SPAN": 4}, {"{}": 1.0})
+    db.put(fill_key, {"NBG_FILL_TWO_PASS": 0}, {"{}": 1.0})
     db.save()
     again = plain_tuning.PlainTuningDB(path)
     assert again.get(key) == {"NBG_MOVE_SPAN": 4}
     raw = json.loads(path.read_text())
-    raw["kernel_version"] = "stale"
+    raw["kernel_versions"]["move"] = "stale"
     path.write_text(json.dumps(raw))
-    assert plain_tuning.PlainTuningDB(path).get(key) is None  # other kernel sources: ignored
+    stale = plain_tuning.PlainTuningDB(path)
+    assert stale.get(key) is None  # the move kernels' sources changed: ignored
+    assert stale.get(fill_key) == {"NBG_FILL_TWO_PASS": 0}  # other families keep theirs
+    assert plain_tuning.kernel_version("group") == plain_tuning.kernel_version("group_sum")
+    assert plain_tuning.kernel_version("move") != plain_tuning.kernel_version("fill")
     path.write_text("{corrupt")
     assert plain_tuning.PlainTuningDB(path).entries == {}
 
