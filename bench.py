@@ -234,6 +234,12 @@ def build_extra(torch, nb, wl):
             extra.append(Workload(f, ecfg, n * 12, (lambda f=f, x=x: getattr(nb, f)(x, alpha=0.01))))
         for f in ("move_exp_nancorr", "move_exp_nancov"):
             extra.append(Workload(f, ecfg, n * 16, (lambda f=f, x=x, y=y: getattr(nb, f)(x, y, alpha=0.01))))
+        if hasattr(nb, "nansum"):
+            rcfg = {"shape": [256, 2_000_000], "dtype": "f32", "axis": -1}
+            for f in ("nansum", "nanmean", "nanvar", "nanstd", "nancount", "nanmax", "nanmin", "nanargmax",
+                      "nanargmin", "anynan", "allnan"):
+                extra.append(Workload(f, rcfg, n * 4 + 256 * 8, (lambda f=f, x=x: getattr(nb, f)(x, axis=-1))))
+            extra.append(Workload("nansum[axis=None]", dict(rcfg, axis=None), n * 4, lambda x=x: nb.nansum(x)))
     if "group_nansum" in ins:
         v, lab = ins["group_nansum"]["v"], ins["group_nansum"]["lab"]
         _, labz = gen_group(torch, 1, 4_000_000, 1000, seed=22, zipf=1.1)
@@ -251,6 +257,13 @@ def build_extra(torch, nb, wl):
         cfg = {"shape": [64, 4_000_000], "dtype": "f32", "num_labels": 1000, "labels": "uniform int64"}
         extra.append(Workload("group_nansum[uniform,int64 labels]", cfg, v.numel() * 4 + lab.numel() * 8 + 64 * 1000 * 4,
                               (lambda v=v, lb=lab64: nb.group_nansum(v, lb, num_labels=1000, axis=-1))))
+        if hasattr(nb, "factorize"):
+            # 16M int64 labels with 1000 uniques (pd.factorize(sort=True) of a
+            # label column): one read of the labels, one write of the codes.
+            keys = (lab[0].to(torch.int64) * 7919 - 500_000).contiguous()
+            keys = torch.cat([keys] * 4)
+            fcfg = {"shape": [keys.numel()], "dtype": "int64", "uniques": 1000, "sort": True}
+            extra.append(Workload("factorize", fcfg, keys.numel() * 16, lambda k=keys: nb.factorize(k, sort=True)))
     src = [w.copy_src for w in wl if hasattr(w, "copy_src")]
     if src:
         x = src[0]
