@@ -209,7 +209,9 @@ class Autotuner:
         self.entries.append(entry)
         return entry
 
-    def run_grid(self, lengths=None, ks=None, zipfs=(0.0, 1.1, 2.0), nans=(0.05, 0.4), windows=(20, 200), seed=0):
+    def run_grid(self, lengths=None, ks=None, zipfs=(0.0, 1.1, 2.0), nans=(0.05, 0.4), windows=(20, 200), seed=0,
+                 progress=None):
+        """Measure every grid point; ``progress(i, entry)`` after each."""
         lengths = lengths or [2**e for e in range(7, 21, 2)]
         ks = ks or [8, 32, 128, 512, 1024, 4096, 16384, 65536]
         i = 0
@@ -218,8 +220,10 @@ class Autotuner:
                 for z in zipfs:
                     for f in nans:
                         for w in windows:
-                            self.measure(L, k, z, f, w, seed=seed + i)
+                            entry = self.measure(L, k, z, f, w, seed=seed + i)
                             i += 1
+                            if progress is not None:
+                                progress(i, entry)
         return self.entries
 
     def training_set(self):

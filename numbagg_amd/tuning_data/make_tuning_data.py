@@ -38,7 +38,12 @@ def main(argv=None) -> int:
     t0 = time.time()
     tuner = Autotuner(db_path=out / "ragged_timing_db.json")
     tuner.entries = []
-    tuner.run_grid()
+    def progress(i, entry):
+        if i % 24 == 0:
+            print(f"# autotuner: {i} points, {time.time() - t0:.0f}s (length {entry['length']}, "
+                  f"labels {entry['num_labels']})", file=sys.stderr, flush=True)
+
+    tuner.run_grid(progress=progress)
     tuner.save()
     print(f"# autotuner: {len(tuner.entries)} points in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
