@@ -260,8 +260,7 @@ def build_extra(torch, nb, wl):
         if hasattr(nb, "factorize"):
             # 16M int64 labels with 1000 uniques (pd.factorize(sort=True) of a
             # label column): one read of the labels, one write of the codes.
-            keys = (lab[0].to(torch.int64) * 7919 - 500_000).contiguous()
-            keys = torch.cat([keys] * 4)
+            keys = torch.cat([lab.to(torch.int64) * 7919 - 500_000] * 4).contiguous()
             fcfg = {"shape": [keys.numel()], "dtype": "int64", "uniques": 1000, "sort": True}
             extra.append(Workload("factorize", fcfg, keys.numel() * 16, lambda k=keys: nb.factorize(k, sort=True)))
     src = [w.copy_src for w in wl if hasattr(w, "copy_src")]

@@ -302,6 +302,54 @@ int nbg_factorize_finish(int dtype, const void* values, int64_t n,
                          void* uniques, void* ws, size_t ws_bytes,
                          void* stream);
 
+/* ---- correlation / covariance matrices (numbagg/funcs.py:338-532;
+ *      numbagg/moving_matrix.py:16-432; ndmatrix / ndmovematrix /
+ *      ndmoveexpmatrix at numbagg/decorators.py:677-818, :1031-1112)
+ * Pairwise complete: an observation counts for (i, j) where both variables
+ * are valid. float64 sums of values shifted by a per-variable constant.
+ * Static: x (batch, nvars, nobs) contiguous -> out (batch, nvars, nvars),
+ * NaN where a pair has fewer than 2 observations (corr also where a variance
+ * is not positive); nobs must be > 0 (every pair is NaN otherwise). Moving and
+ * exponential: x (batch, nobs, nvars) contiguous -> out (batch, nobs, nvars,
+ * nvars). dtype NBG_F32 or NBG_F64; results are deterministic. */
+size_t nbg_matrix_workspace_bytes(int64_t batch, int64_t nvars, int64_t nobs);
+int nbg_nancorrmatrix(int dtype, const void* x, void* out, int64_t batch,
+                      int64_t nvars, int64_t nobs, void* ws, size_t ws_bytes,
+                      void* stream); /* funcs.py:338-441 */
+int nbg_nancovmatrix(int dtype, const void* x, void* out, int64_t batch,
+                     int64_t nvars, int64_t nobs, void* ws, size_t ws_bytes,
+                     void* stream); /* funcs.py:444-532 */
+/* min_count as the caller passes it (>= 0); the kernels apply the reference's
+ * max(min_count, 1) and, for corr, the two-observation minimum. */
+size_t nbg_move_matrix_workspace_bytes(int64_t batch, int64_t nobs,
+                                       int64_t nvars, int64_t window);
+int nbg_move_corrmatrix(int dtype, const void* x, void* out, int64_t batch,
+                        int64_t nobs, int64_t nvars, int64_t window,
+                        int64_t min_count, void* ws, size_t ws_bytes,
+                        void* stream); /* moving_matrix.py:16-114 */
+int nbg_move_covmatrix(int dtype, const void* x, void* out, int64_t batch,
+                       int64_t nobs, int64_t nvars, int64_t window,
+                       int64_t min_count, void* ws, size_t ws_bytes,
+                       void* stream); /* moving_matrix.py:117-201 */
+/* x_dtype / dtype as for the exponential windows above (float32 data on the
+ * float64 loop allowed); alpha: loop dtype, (batch, nobs) with row stride
+ * ld_alpha (0: one row shared by the batch); min_weight already rounded to
+ * the loop dtype. */
+size_t nbg_move_exp_matrix_workspace_bytes(int64_t batch, int64_t nobs,
+                                           int64_t nvars);
+int nbg_move_exp_nancorrmatrix(int x_dtype, int dtype, const void* x,
+                               const void* alpha, int64_t ld_alpha,
+                               double min_weight, void* out, int64_t batch,
+                               int64_t nobs, int64_t nvars, void* ws,
+                               size_t ws_bytes,
+                               void* stream); /* moving_matrix.py:204-328 */
+int nbg_move_exp_nancovmatrix(int x_dtype, int dtype, const void* x,
+                              const void* alpha, int64_t ld_alpha,
+                              double min_weight, void* out, int64_t batch,
+                              int64_t nobs, int64_t nvars, void* ws,
+                              size_t ws_bytes,
+                              void* stream); /* moving_matrix.py:331-432 */
+
 /* No counterpart in the reference. Allocation-free variant of
  * nbg_group_nansum for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):

@@ -3,7 +3,9 @@
 Same public names, signatures and argument semantics as numbagg for the grouped
 reductions, the moving-window and exponentially weighted scans, and
 ffill/bfill (numbagg/__init__.py). NumPy inputs are computed on the GPU and
-returned as NumPy arrays; torch tensors on the GPU stay on the GPU.
+returned as NumPy arrays; torch tensors on the GPU stay on the GPU. Also the
+plain reductions, the correlation / covariance matrices and a device label
+factorisation.
 """
 
 from ._lib import spin_timeouts
@@ -16,7 +18,9 @@ from .funcs import (
     ffill,
     nanargmax,
     nanargmin,
+    nancorrmatrix,
     nancount,
+    nancovmatrix,
     nanmax,
     nanmean,
     nanmin,
@@ -42,6 +46,7 @@ from .grouped import (
     group_nanvar,
 )
 from .moving import move_corr, move_cov, move_mean, move_std, move_sum, move_var
+from .moving_matrix import move_corrmatrix, move_covmatrix, move_exp_nancorrmatrix, move_exp_nancovmatrix
 from .moving_exp import (
     move_exp_nancorr,
     move_exp_nancount,
@@ -81,7 +86,9 @@ MOVE_EXP_FUNCS = [
     move_exp_nanvar,
 ]
 
-MOVE_FUNCS = [move_corr, move_cov, move_mean, move_std, move_sum, move_var]
+MOVE_EXP_MATRIX_FUNCS = [move_exp_nancorrmatrix, move_exp_nancovmatrix]
+
+MOVE_FUNCS = [move_corr, move_cov, move_mean, move_corrmatrix, move_covmatrix, move_std, move_sum, move_var]
 
 AGGREGATION_FUNCS = [
     allnan,
@@ -98,6 +105,8 @@ AGGREGATION_FUNCS = [
     nanvar,
 ]
 
+MATRIX_FUNCS = [nancorrmatrix, nancovmatrix]
+
 OTHER_FUNCS = [bfill, ffill]
 
 __version__ = "0.1.0"
@@ -112,7 +121,9 @@ __all__ = [
     "ffill",
     "nanargmax",
     "nanargmin",
+    "nancorrmatrix",
     "nancount",
+    "nancovmatrix",
     "nanmax",
     "nanmean",
     "nanmin",
@@ -135,10 +146,14 @@ __all__ = [
     "group_nansum_of_squares",
     "group_nanvar",
     "move_corr",
+    "move_corrmatrix",
     "move_cov",
+    "move_covmatrix",
     "move_exp_nancorr",
+    "move_exp_nancorrmatrix",
     "move_exp_nancount",
     "move_exp_nancov",
+    "move_exp_nancovmatrix",
     "move_exp_nanmean",
     "move_exp_nanstd",
     "move_exp_nansum",
@@ -150,7 +165,9 @@ __all__ = [
     "spin_timeouts",
     "AGGREGATION_FUNCS",
     "GROUPED_FUNCS",
+    "MATRIX_FUNCS",
     "MOVE_EXP_FUNCS",
+    "MOVE_EXP_MATRIX_FUNCS",
     "MOVE_FUNCS",
     "OTHER_FUNCS",
 ]

@@ -100,6 +100,22 @@ _SIGNATURES["nbg_factorize_finish"] = (
     _c_int,
     [_c_int, _c_ptr, _c_i64, _c_i64, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr],
 )
+_SIGNATURES["nbg_matrix_workspace_bytes"] = (_c_size, [_c_i64, _c_i64, _c_i64])
+for _name in ("nancorrmatrix", "nancovmatrix"):
+    _SIGNATURES[f"nbg_{_name}"] = (_c_int, [_c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_size, _c_ptr])
+_SIGNATURES["nbg_move_matrix_workspace_bytes"] = (_c_size, [_c_i64, _c_i64, _c_i64, _c_i64])
+for _name in ("move_corrmatrix", "move_covmatrix"):
+    _SIGNATURES[f"nbg_{_name}"] = (
+        _c_int,
+        [_c_int, _c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_ptr, _c_size, _c_ptr],
+    )
+_SIGNATURES["nbg_move_exp_matrix_workspace_bytes"] = (_c_size, [_c_i64, _c_i64, _c_i64])
+for _name in ("move_exp_nancorrmatrix", "move_exp_nancovmatrix"):
+    _SIGNATURES[f"nbg_{_name}"] = (
+        _c_int,
+        [_c_int, _c_int, _c_ptr, _c_ptr, _c_i64, ctypes.c_double, _c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_size,
+         _c_ptr],
+    )
 _SIGNATURES["nbg_ragged_tile_size"] = (_c_i64, [])
 _SIGNATURES["nbg_ragged_workspace_bytes"] = (_c_size, [_c_i64, _c_i64])
 _SIGNATURES["nbg_ragged_max_lds_labels"] = (_c_i64, [])

@@ -1,0 +1,879 @@
+// Correlation / covariance matrices for gfx950 (numbagg/funcs.py:338-532,
+// numbagg/moving_matrix.py:16-432).
+//
+// Every statistic is pairwise complete: an observation counts for the pair
+// (i, j) only where both variables are valid. All sums are float64 over values
+// shifted by a per-variable constant (the first valid value of the variable in
+// the span a kernel covers), so a covariance is formed from sums of O(spread)
+// terms rather than by cancelling sums of O(level^2) terms; every statistic is
+// shift invariant, and the shift is exact to apply for float32 data.
+//
+// Static matrices (x: (batch, vars, obs) -> (batch, vars, vars)) are six
+// pairwise contractions over the observations, with y = valid ? x - shift : 0
+// and m = valid ? 1 : 0 per element:
+//   P = Y Y^T, Si = Y M^T, Sj = M Y^T, N = M M^T, Qi = Y^2 M^T, Qj = M (Y^2)^T
+// (covariance needs the first four). They run on the float64 matrix cores
+// (V_MFMA_F64_16X16X4_F64): a wave owns a 16 TW x 16 TW tile of variable pairs
+// over a range of observations; each lane loads four consecutive observations
+// of one row of the i block and of the j block, and the four MFMA steps of a
+// 16-observation group take element e of every lane's quadruple, so that the
+// k slot (lane / 16) of step e is observation 4 (lane / 16) + e for both
+// operands. Only tiles on or above the diagonal are computed. Observation
+// ranges are split over waves when there are few tiles; splits write float64
+// partial sums that a second kernel adds in split order (deterministic), and a
+// single split finalises in the main kernel.
+//
+// Moving matrices (x: (batch, obs, vars) -> (batch, obs, vars, vars)) assign a
+// lane to one ordered pair (i, j): a wave covers 64 consecutive pairs of the
+// output matrix (or, for fewer than 64 pairs, several consecutive time ranges
+// of all pairs), so every store writes consecutive output elements. Each lane
+// walks its time range with running float64 window sums (add the new
+// observation, remove the one leaving the window), after warming up over the
+// window - 1 observations before the range; shifts are taken per time tile.
+//
+// Exponentially weighted matrices carry the same per-pair state as
+// move_exp_nancov/nancorr (numbagg/moving_exp.py:227-335) across the whole
+// series: the series is cut into chunks, a first kernel forms each chunk's
+// state from zero, a second carries the chunk states along the series (state
+// at a chunk's start = decay product of the previous chunk x state at its
+// start + its local state), and a third replays each chunk from its carried
+// state and writes the outputs. Shifts are per variable over the whole series.
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace nbg {
+namespace {
+
+typedef double mx_d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMxThreads = 256;
+constexpr int kMxWaves = kMxThreads / kWave;
+
+template <typename T>
+__device__ __forceinline__ T mx_nan() {
+  return canonical_nan<T>();
+}
+
+// ---------------------------------------------------------------------------
+// Shifts: the first valid value of each (outer, variable) series within
+// [lo, hi) of its time axis (0 when none is valid). One wave per series.
+// ---------------------------------------------------------------------------
+struct MxShiftArgs {
+  int64_t outer;      // series groups: batch x tiles
+  int64_t nv;         // variables
+  int64_t tiles;      // time tiles per batch row (1: whole series)
+  int64_t tile_len;   // steps per tile
+  int64_t halo;       // steps before a tile that its windows reach
+  int64_t len;        // time axis length
+  int64_t ld_b;       // batch stride (elements)
+  int64_t vstride;    // variable stride
+  int64_t tstride;    // time stride
+};
+
+template <typename T>
+__global__ __launch_bounds__(kMxThreads) void mx_shift_kernel(const T* __restrict__ x, double* __restrict__ shift,
+                                                             MxShiftArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t s = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  if (s >= a.outer * a.nv) return;  // whole wave
+  const int64_t o = s / a.nv, v = s % a.nv;
+  const int64_t b = o / a.tiles, tile = o % a.tiles;
+  int64_t lo = tile * a.tile_len - a.halo;
+  if (lo < 0) lo = 0;
+  int64_t hi = tile * a.tile_len + a.tile_len;
+  if (hi > a.len) hi = a.len;
+  const T* p = x + b * a.ld_b + v * a.vstride;
+  double found = 0.0;
+  for (int64_t t0 = lo; t0 < hi; t0 += kWave) {
+    const int64_t t = t0 + lane;
+    const T val = t < hi ? p[t * a.tstride] : mx_nan<T>();
+    const unsigned long long m = __ballot(val == val);
+    if (m) {
+      const int first = __ffsll((long long)m) - 1;
+      found = __shfl((double)val, first);
+      break;
+    }
+  }
+  if (lane == 0) shift[s] = found;
+}
+
+// ---------------------------------------------------------------------------
+// Static matrices on the float64 matrix cores.
+// ---------------------------------------------------------------------------
+struct MxStaticArgs {
+  const void* x;
+  void* out;
+  const double* shift;  // (batch, nv)
+  double* part;         // partial sums when splits > 1
+  int64_t batch, nv, nobs;
+  int64_t ntile;        // tiles per side
+  int64_t tiles;        // tiles on or above the diagonal
+  int64_t splits;       // observation ranges per tile
+  int64_t span;         // observations per split (a multiple of 16)
+};
+
+template <bool CORR>
+struct StaticSpec {
+  static constexpr int NPROD = CORR ? 6 : 4;  // P, Si, Sj, N[, Qi, Qj]
+};
+
+// The output of one pair from its sums (i == j: the variance itself, so the
+// diagonal of a correlation matrix is exactly 1).
+template <bool CORR>
+__device__ __forceinline__ double static_value(double P, double Si, double Sj, double N, double Qi, double Qj,
+                                               bool diag) {
+  const double nan = __builtin_nan("");
+  if (!(N > 1.0)) return nan;
+  const double cv = P - Si * Sj / N;
+  if (!CORR) return cv / (N - 1.0);
+  const double vi = Qi - Si * Si / N, vj = Qj - Sj * Sj / N;
+  if (!(vi > 0.0 && vj > 0.0)) return nan;
+  return (diag ? vi : cv) / sqrt(vi * vj);
+}
+
+__device__ __forceinline__ void tile_of(int64_t t, int64_t ntile, int64_t& I, int64_t& J) {
+  int64_t i = 0;
+  while (t >= ntile - i) {
+    t -= ntile - i;
+    ++i;
+  }
+  I = i;
+  J = i + t;
+}
+
+template <typename T, bool VEC>
+__device__ __forceinline__ void load4(const T* __restrict__ row, int64_t k, int64_t k_hi, bool ok, T (&v)[4]) {
+  if (VEC && ok && k + 4 <= k_hi) {
+    if constexpr (sizeof(T) == 4) {
+      const float4 f = *reinterpret_cast<const float4*>(row + k);
+      v[0] = f.x;
+      v[1] = f.y;
+      v[2] = f.z;
+      v[3] = f.w;
+    } else {
+      const double2 f0 = *reinterpret_cast<const double2*>(row + k);
+      const double2 f1 = *reinterpret_cast<const double2*>(row + k + 2);
+      v[0] = f0.x;
+      v[1] = f0.y;
+      v[2] = f1.x;
+      v[3] = f1.y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = (ok && k + e < k_hi) ? row[k + e] : mx_nan<T>();
+  }
+}
+
+template <typename T, bool CORR, int TW, bool VEC>
+__global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
+  constexpr int NP = StaticSpec<CORR>::NPROD;
+  constexpr int TS = 16 * TW;
+  const int64_t item = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  if (item >= a.batch * a.tiles * a.splits) return;  // whole wave
+  const int lane = threadIdx.x & (kWave - 1);
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t split = item % a.splits;
+  const int64_t tile = (item / a.splits) % a.tiles;
+  const int64_t b = item / (a.splits * a.tiles);
+  int64_t I, J;
+  tile_of(tile, a.ntile, I, J);
+  const T* x = static_cast<const T*>(a.x) + b * a.nv * a.nobs;
+  const T* irow[TW];
+  const T* jrow[TW];
+  bool iok[TW], jok[TW];
+  double ci[TW], cj[TW];
+#pragma unroll
+  for (int t = 0; t < TW; ++t) {
+    const int64_t i = I * TS + t * 16 + r, j = J * TS + t * 16 + r;
+    iok[t] = i < a.nv;
+    jok[t] = j < a.nv;
+    irow[t] = x + (iok[t] ? i : 0) * a.nobs;
+    jrow[t] = x + (jok[t] ? j : 0) * a.nobs;
+    ci[t] = iok[t] ? a.shift[b * a.nv + i] : 0.0;
+    cj[t] = jok[t] ? a.shift[b * a.nv + j] : 0.0;
+  }
+  mx_d4 acc[NP][TW][TW];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int u = 0; u < TW; ++u)
+#pragma unroll
+      for (int w = 0; w < TW; ++w) acc[p][u][w] = mx_d4{0.0, 0.0, 0.0, 0.0};
+
+  const int64_t k_lo = split * a.span;
+  const int64_t k_hi = k_lo + a.span < a.nobs ? k_lo + a.span : a.nobs;
+  for (int64_t k0 = k_lo; k0 < k_hi; k0 += 16) {
+    const int64_t k = k0 + 4 * q;
+    T vi[TW][4], vj[TW][4];
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+      load4<T, VEC>(irow[t], k, k_hi, iok[t], vi[t]);
+      load4<T, VEC>(jrow[t], k, k_hi, jok[t], vj[t]);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double yi[TW], mi[TW], yj[TW], mj[TW], qi[TW], qj[TW];
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+        const bool vai = vi[t][e] == vi[t][e], vaj = vj[t][e] == vj[t][e];
+        yi[t] = vai ? (double)vi[t][e] - ci[t] : 0.0;
+        yj[t] = vaj ? (double)vj[t][e] - cj[t] : 0.0;
+        mi[t] = vai ? 1.0 : 0.0;
+        mj[t] = vaj ? 1.0 : 0.0;
+        qi[t] = yi[t] * yi[t];
+        qj[t] = yj[t] * yj[t];
+      }
+#pragma unroll
+      for (int u = 0; u < TW; ++u)
+#pragma unroll
+        for (int w = 0; w < TW; ++w) {
+          acc[0][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(yi[u], yj[w], acc[0][u][w], 0, 0, 0);
+          acc[1][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(yi[u], mj[w], acc[1][u][w], 0, 0, 0);
+          acc[2][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(mi[u], yj[w], acc[2][u][w], 0, 0, 0);
+          acc[3][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(mi[u], mj[w], acc[3][u][w], 0, 0, 0);
+          if constexpr (CORR) {
+            acc[4][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(qi[u], mj[w], acc[4][u][w], 0, 0, 0);
+            acc[5][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(mi[u], qj[w], acc[5][u][w], 0, 0, 0);
+          }
+        }
+    }
+  }
+
+  // Lane l holds rows 4 (l / 16) + c, column l % 16 of each 16 x 16 block.
+  if (a.splits == 1) {
+    T* out = static_cast<T*>(a.out) + b * a.nv * a.nv;
+#pragma unroll
+    for (int u = 0; u < TW; ++u)
+#pragma unroll
+      for (int w = 0; w < TW; ++w)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int64_t i = I * TS + u * 16 + 4 * q + c, j = J * TS + w * 16 + r;
+          if (i >= a.nv || j >= a.nv || (I == J && i > j)) continue;
+          const double Qi = CORR ? acc[4 % NP][u][w][c] : 0.0;
+          const double Qj = CORR ? acc[5 % NP][u][w][c] : 0.0;
+          const T v = (T)static_value<CORR>(acc[0][u][w][c], acc[1][u][w][c], acc[2][u][w][c], acc[3][u][w][c],
+                                            Qi, Qj, i == j);
+          out[i * a.nv + j] = v;
+          out[j * a.nv + i] = v;
+        }
+  } else {
+    double* part = a.part + item * (int64_t)(NP * TS * TS);
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int u = 0; u < TW; ++u)
+#pragma unroll
+        for (int w = 0; w < TW; ++w)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) part[(p * TS + u * 16 + 4 * q + c) * TS + w * 16 + r] = acc[p][u][w][c];
+  }
+}
+
+// One thread per output element: sums the partials of its pair (the lower
+// index first) in split order.
+template <typename T, bool CORR, int TW>
+__global__ __launch_bounds__(kMxThreads) void mx_static_finish_kernel(MxStaticArgs a) {
+  constexpr int NP = StaticSpec<CORR>::NPROD;
+  constexpr int TS = 16 * TW;
+  const int64_t e = (int64_t)blockIdx.x * kMxThreads + threadIdx.x;
+  if (e >= a.batch * a.nv * a.nv) return;
+  const int64_t b = e / (a.nv * a.nv);
+  const int64_t i = (e / a.nv) % a.nv, j = e % a.nv;
+  const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  const int64_t I = lo / TS, J = hi / TS;
+  const int64_t tile = I * a.ntile - I * (I - 1) / 2 + (J - I);
+  const int64_t off = (lo % TS) * TS + hi % TS;
+  double s[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) s[p] = 0.0;
+  const double* part = a.part + (b * a.tiles + tile) * a.splits * (int64_t)(NP * TS * TS) + off;
+  for (int64_t sp = 0; sp < a.splits; ++sp) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) s[p] += part[p * TS * TS];
+    part += NP * TS * TS;
+  }
+  const double Qi = CORR ? s[4 % NP] : 0.0, Qj = CORR ? s[5 % NP] : 0.0;
+  static_cast<T*>(a.out)[e] = (T)static_value<CORR>(s[0], s[1], s[2], s[3], Qi, Qj, lo == hi);
+}
+
+// Launch plan: wave tile, observation splits, workspace layout.
+struct StaticPlan {
+  int tw;
+  int64_t ntile, tiles, splits, span;
+  size_t shift_bytes, part_bytes;
+};
+
+// Waves the static kernel aims for (several per SIMD of 256 CUs).
+constexpr int64_t kStaticTargetWaves = 2048;
+// Fewest observations a split covers (keeps partials small next to the input).
+constexpr int64_t kStaticMinSpan = 4096;
+
+StaticPlan static_plan(int64_t batch, int64_t nv, int64_t nobs, bool corr) {
+  StaticPlan p{};
+  p.tw = nv > 16 ? 2 : 1;
+  const int64_t ts = 16 * p.tw;
+  p.ntile = (nv + ts - 1) / ts;
+  p.tiles = p.ntile * (p.ntile + 1) / 2;
+  const int64_t items = batch * p.tiles;
+  int64_t splits = (kStaticTargetWaves + items - 1) / (items > 0 ? items : 1);
+  const int64_t max_splits = (nobs + kStaticMinSpan - 1) / kStaticMinSpan;
+  if (splits > max_splits) splits = max_splits;
+  if (splits < 1) splits = 1;
+  int64_t span = (nobs + splits - 1) / splits;
+  span = (span + 15) / 16 * 16;
+  if (span < 16) span = 16;
+  p.splits = (nobs + span - 1) / span;
+  if (p.splits < 1) p.splits = 1;
+  p.span = span;
+  p.shift_bytes = (size_t)(batch * nv) * 8;
+  const int np = corr ? 6 : 4;
+  p.part_bytes = p.splits > 1 ? (size_t)(items * p.splits) * np * ts * ts * 8 : 0;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// Moving-window matrices: a lane per ordered pair and time range.
+// ---------------------------------------------------------------------------
+struct MxMoveArgs {
+  const void* x;
+  void* out;
+  const double* shift;  // (batch, tiles, nv)
+  int64_t batch, nobs, nv;
+  int64_t window, min_count;  // min_count as the reference adjusts it (>= 1)
+  int64_t tile_len, tiles;
+  int64_t pair_waves;         // waves per (batch row, tile)
+  int64_t groups;             // time ranges per wave (1 when nv * nv >= 64)
+  int64_t sub_len;            // steps per time range
+};
+
+template <bool CORR>
+struct MoveState {
+  double si, sj, p, qi, qj;
+  int64_t n;
+};
+
+template <typename T, bool CORR>
+__global__ __launch_bounds__(kMxThreads) void mx_move_kernel(MxMoveArgs a) {
+  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  if (wv >= a.batch * a.tiles * a.pair_waves) return;  // whole wave
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t pw = wv % a.pair_waves;
+  const int64_t tile = (wv / a.pair_waves) % a.tiles;
+  const int64_t b = wv / (a.pair_waves * a.tiles);
+  const int64_t np = a.nv * a.nv;
+  int64_t qq, g;
+  if (a.groups == 1) {
+    qq = pw * kWave + lane;
+    g = 0;
+  } else {
+    qq = lane % np;
+    g = lane / np;
+  }
+  if (qq >= np || g >= a.groups) return;
+  const int64_t i = qq / a.nv, j = qq % a.nv;
+  const int64_t tile_lo = tile * a.tile_len;
+  int64_t tile_hi = tile_lo + a.tile_len;
+  if (tile_hi > a.nobs) tile_hi = a.nobs;
+  const int64_t t_start = tile_lo + g * a.sub_len;
+  int64_t t_end = t_start + a.sub_len;
+  if (t_end > tile_hi) t_end = tile_hi;
+  if (t_start >= t_end) return;
+  const int64_t t_w = t_start - a.window + 1 > 0 ? t_start - a.window + 1 : 0;
+  const T* x = static_cast<const T*>(a.x) + b * a.nobs * a.nv;
+  T* out = static_cast<T*>(a.out) + b * a.nobs * np + qq;
+  const double ci = a.shift[(b * a.tiles + tile) * a.nv + i];
+  const double cj = a.shift[(b * a.tiles + tile) * a.nv + j];
+  const bool diag = i == j;
+  const int64_t mcorr = a.min_count > 2 ? a.min_count : 2;
+  double si = 0.0, sj = 0.0, sp = 0.0, qi = 0.0, qj = 0.0;
+  int64_t n = 0;
+  for (int64_t t = t_w; t < t_end; ++t) {
+    {
+      const T xi = x[t * a.nv + i], xj = x[t * a.nv + j];
+      if (xi == xi && xj == xj) {
+        const double yi = (double)xi - ci, yj = (double)xj - cj;
+        si += yi;
+        sj += yj;
+        sp += yi * yj;
+        if (CORR) {
+          qi += yi * yi;
+          qj += yj * yj;
+        }
+        n += 1;
+      }
+    }
+    if (t - a.window >= t_w) {
+      const int64_t to = t - a.window;
+      const T xi = x[to * a.nv + i], xj = x[to * a.nv + j];
+      if (xi == xi && xj == xj) {
+        const double yi = (double)xi - ci, yj = (double)xj - cj;
+        si -= yi;
+        sj -= yj;
+        sp -= yi * yj;
+        if (CORR) {
+          qi -= yi * yi;
+          qj -= yj * yj;
+        }
+        n -= 1;
+      }
+    }
+    if (t < t_start) continue;
+    double v = __builtin_nan("");
+    const double dn = (double)n;
+    if (CORR) {
+      if (n >= mcorr) {
+        const double vi = qi - si * si / dn, vj = qj - sj * sj / dn;
+        if (vi > 0.0 && vj > 0.0) v = (diag ? vi : sp - si * sj / dn) / sqrt(vi * vj);
+      }
+    } else if (n >= a.min_count && n > 1) {
+      v = (sp - si * sj / dn) / (dn - 1.0);
+    }
+    out[t * np] = (T)v;
+  }
+}
+
+struct MovePlan {
+  int64_t tile_len, tiles, pair_waves, groups, sub_len;
+  size_t shift_bytes;
+};
+
+MovePlan move_plan(int64_t batch, int64_t nobs, int64_t nv, int64_t window) {
+  MovePlan p{};
+  const int64_t np = nv * nv;
+  if (np >= kWave) {
+    p.groups = 1;
+    p.pair_waves = (np + kWave - 1) / kWave;
+  } else {
+    p.groups = kWave / np;
+    p.pair_waves = 1;
+  }
+  // Time tiles: enough waves to fill the device, but every time range at
+  // least twice the window (the warm-up is repeated per range) and 64 steps.
+  const int64_t waves_per_tile = batch * p.pair_waves;
+  int64_t tiles = (kStaticTargetWaves + waves_per_tile - 1) / (waves_per_tile > 0 ? waves_per_tile : 1);
+  int64_t min_range = 2 * window > 64 ? 2 * window : 64;
+  int64_t max_tiles = nobs / (min_range * p.groups);
+  if (max_tiles < 1) max_tiles = 1;
+  if (tiles > max_tiles) tiles = max_tiles;
+  if (tiles < 1) tiles = 1;
+  p.tile_len = (nobs + tiles - 1) / tiles;
+  if (p.tile_len < 1) p.tile_len = 1;
+  p.tiles = (nobs + p.tile_len - 1) / p.tile_len;
+  if (p.tiles < 1) p.tiles = 1;
+  p.sub_len = (p.tile_len + p.groups - 1) / p.groups;
+  p.shift_bytes = (size_t)(batch * p.tiles * nv) * 8;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// Exponentially weighted matrices: chunk states, carries, replay.
+// State order (as move_exp_nancov / nancorr): s1, s2, s12, sw, sw2, weight,
+// s11, s22; `seen` counts valid pair observations, saturating at 2.
+// ---------------------------------------------------------------------------
+template <bool CORR>
+struct ExpM {
+  static constexpr int NS = CORR ? 8 : 6;
+  static constexpr int REC = NS + 1;  // + seen
+};
+
+struct MxExpArgs {
+  const void* x;
+  const void* alpha;   // loop dtype, row stride ld_alpha (0: shared)
+  void* out;
+  const double* shift; // (batch, nv)
+  double* state;       // (batch, chunks, nv * nv, REC)
+  double* decay;       // (batch, chunks, 2): product of d and of d^2 over the chunk
+  int64_t ld_alpha;
+  double min_weight;
+  int64_t batch, nobs, nv;
+  int64_t chunk, chunks;
+  int64_t pair_waves, groups;
+};
+
+template <typename TA>
+__device__ __forceinline__ double mx_alpha(const MxExpArgs& a, int64_t b, int64_t t) {
+  return (double)static_cast<const TA*>(a.alpha)[b * a.ld_alpha + t];
+}
+
+template <bool CORR>
+__device__ __forceinline__ void exp_step(double (&s)[ExpM<CORR>::NS], int& seen, double d, double al, bool valid,
+                                         double yi, double yj) {
+  constexpr int NS = ExpM<CORR>::NS;
+#pragma unroll
+  for (int c = 0; c < NS; ++c) s[c] *= (c == 4 ? d * d : d);
+  if (valid) {
+    s[0] += yi;
+    s[1] += yj;
+    s[2] += yi * yj;
+    s[3] += 1.0;
+    s[4] += 1.0;
+    s[5] += al;
+    if (CORR) {
+      s[6 % NS] += yi * yi;
+      s[7 % NS] += yj * yj;
+    }
+    seen += seen < 2;
+  }
+}
+
+// Work item of lane `lane` in wave `wv`: (batch row, chunk, pair); false when
+// the lane has none.
+__device__ __forceinline__ bool exp_item(const MxExpArgs& a, int64_t wv, int lane, int64_t& b, int64_t& c,
+                                         int64_t& qq) {
+  const int64_t np = a.nv * a.nv;
+  if (a.groups == 1) {
+    const int64_t pw = wv % a.pair_waves;
+    c = (wv / a.pair_waves) % a.chunks;
+    b = wv / (a.pair_waves * a.chunks);
+    qq = pw * kWave + lane;
+  } else {
+    const int64_t cgs = (a.chunks + a.groups - 1) / a.groups;
+    const int64_t cg = wv % cgs;
+    b = wv / cgs;
+    qq = lane % np;
+    const int64_t g = lane / np;
+    if (g >= a.groups) return false;
+    c = cg * a.groups + g;
+  }
+  return b < a.batch && c < a.chunks && qq < np;
+}
+
+template <typename TX, typename TA, bool CORR, bool REPLAY>
+__global__ __launch_bounds__(kMxThreads) void mx_exp_chunk_kernel(MxExpArgs a) {
+  using TO = TA;
+  constexpr int NS = ExpM<CORR>::NS, REC = ExpM<CORR>::REC;
+  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  int64_t b, c, qq;
+  if (!exp_item(a, wv, lane, b, c, qq)) return;
+  const int64_t np = a.nv * a.nv;
+  const int64_t i = qq / a.nv, j = qq % a.nv;
+  const int64_t t0 = c * a.chunk;
+  const int64_t t1 = t0 + a.chunk < a.nobs ? t0 + a.chunk : a.nobs;
+  const TX* x = static_cast<const TX*>(a.x) + b * a.nobs * a.nv;
+  const double ci = a.shift[b * a.nv + i], cj = a.shift[b * a.nv + j];
+  double* rec = a.state + ((b * a.chunks + c) * np + qq) * REC;
+  double s[NS];
+  int seen = 0;
+  if (REPLAY) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = rec[k];
+    seen = (int)rec[NS];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = 0.0;
+  }
+  double dp = 1.0, dp2 = 1.0;
+  const double mw = a.min_weight;
+  const bool diag = i == j;
+  TO* out = static_cast<TO*>(a.out) + b * a.nobs * np + qq;
+  for (int64_t t = t0; t < t1; ++t) {
+    const double al = mx_alpha<TA>(a, b, t);
+    const double d = 1.0 - al;
+    const TX xi = x[t * a.nv + i], xj = x[t * a.nv + j];
+    const bool valid = xi == xi && xj == xj;
+    exp_step<CORR>(s, seen, d, al, valid, valid ? (double)xi - ci : 0.0, valid ? (double)xj - cj : 0.0);
+    if (!REPLAY) {
+      dp *= d;
+      dp2 *= d * d;
+      continue;
+    }
+    const double bias = 1.0 - s[4] / (s[3] * s[3]);
+    const bool ok = s[5] >= mw && bias > 0.0 && seen >= 2;
+    double v = __builtin_nan("");
+    if (ok) {
+      if (!CORR) {
+        v = (s[2] - s[0] * s[1] / s[3]) / s[3] / bias;
+      } else {
+        const double vi = s[6 % NS] - s[0] * s[0] / s[3], vj = s[7 % NS] - s[1] * s[1] / s[3];
+        if (vi > 0.0 && vj > 0.0) v = (diag ? vi : s[2] - s[0] * s[1] / s[3]) / sqrt(vi * vj);
+      }
+    }
+    out[t * np] = (TO)v;
+  }
+  if (!REPLAY) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rec[k] = s[k];
+    rec[NS] = (double)seen;
+    if (qq == 0) {
+      a.decay[(b * a.chunks + c) * 2] = dp;
+      a.decay[(b * a.chunks + c) * 2 + 1] = dp2;
+    }
+  }
+}
+
+// Thread per (batch row, pair): replaces each chunk's local state by the state
+// at the chunk's start.
+template <bool CORR>
+__global__ __launch_bounds__(kMxThreads) void mx_exp_carry_kernel(MxExpArgs a) {
+  constexpr int NS = ExpM<CORR>::NS, REC = ExpM<CORR>::REC;
+  const int64_t np = a.nv * a.nv;
+  const int64_t e = (int64_t)blockIdx.x * kMxThreads + threadIdx.x;
+  if (e >= a.batch * np) return;
+  const int64_t b = e / np, qq = e % np;
+  double carry[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) carry[k] = 0.0;
+  int seen = 0;
+  for (int64_t c = 0; c < a.chunks; ++c) {
+    double* rec = a.state + ((b * a.chunks + c) * np + qq) * REC;
+    const double d = a.decay[(b * a.chunks + c) * 2], d2 = a.decay[(b * a.chunks + c) * 2 + 1];
+    double loc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) loc[k] = rec[k];
+    const int ls = (int)rec[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rec[k] = carry[k];
+    rec[NS] = (double)seen;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) carry[k] = (k == 4 ? d2 : d) * carry[k] + loc[k];
+    seen = seen + ls < 2 ? seen + ls : 2;
+  }
+}
+
+struct ExpPlan {
+  int64_t chunk, chunks, pair_waves, groups, waves;
+  size_t shift_bytes, state_bytes, decay_bytes;
+};
+
+ExpPlan exp_plan(int64_t batch, int64_t nobs, int64_t nv, bool corr) {
+  ExpPlan p{};
+  const int64_t np = nv * nv;
+  if (np >= kWave) {
+    p.groups = 1;
+    p.pair_waves = (np + kWave - 1) / kWave;
+  } else {
+    p.groups = kWave / np;
+    p.pair_waves = 1;
+  }
+  // Chunks: enough lanes busy to fill the device, at least 64 steps each.
+  const int64_t lanes_per_chunk = batch * (p.groups == 1 ? p.pair_waves * kWave : np);
+  int64_t chunks = (kStaticTargetWaves * kWave + lanes_per_chunk - 1) / (lanes_per_chunk > 0 ? lanes_per_chunk : 1);
+  int64_t max_chunks = nobs / 64;
+  if (max_chunks < 1) max_chunks = 1;
+  if (chunks > max_chunks) chunks = max_chunks;
+  if (chunks < 1) chunks = 1;
+  p.chunk = (nobs + chunks - 1) / chunks;
+  if (p.chunk < 1) p.chunk = 1;
+  p.chunks = (nobs + p.chunk - 1) / p.chunk;
+  if (p.chunks < 1) p.chunks = 1;
+  p.waves = p.groups == 1 ? batch * p.chunks * p.pair_waves : batch * ((p.chunks + p.groups - 1) / p.groups);
+  p.shift_bytes = (size_t)(batch * nv) * 8;
+  p.state_bytes = (size_t)(batch * p.chunks * np) * (corr ? ExpM<true>::REC : ExpM<false>::REC) * 8;
+  p.decay_bytes = (size_t)(batch * p.chunks) * 2 * 8;
+  return p;
+}
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+int mx_grid(int64_t waves) {
+  const int64_t blocks = (waves + kMxWaves - 1) / kMxWaves;
+  return (int)(blocks < 1 ? 1 : blocks);
+}
+
+template <typename T>
+int static_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nv, int64_t nobs, void* ws,
+                  hipStream_t s) {
+  const StaticPlan p = static_plan(batch, nv, nobs, corr);
+  char* w = static_cast<char*>(ws);
+  double* shift = reinterpret_cast<double*>(w);
+  double* part = reinterpret_cast<double*>(w + align256(p.shift_bytes));
+  MxShiftArgs sa{batch, nv, 1, nobs, 0, nobs, nv * nobs, nobs, 1};
+  hipLaunchKernelGGL(mx_shift_kernel<T>, dim3(mx_grid(batch * nv)), dim3(kMxThreads), 0, s,
+                     static_cast<const T*>(x), shift, sa);
+  MxStaticArgs a{x, out, shift, part, batch, nv, nobs, p.ntile, p.tiles, p.splits, p.span};
+  const bool vec = ((uintptr_t)x % 16) == 0 && (nobs % (16 / (int64_t)sizeof(T))) == 0;
+  const int grid = mx_grid(batch * p.tiles * p.splits);
+#define NBG_MX_STATIC(C, TW)                                                                                   \
+  do {                                                                                                         \
+    if (vec) hipLaunchKernelGGL((mx_static_kernel<T, C, TW, true>), dim3(grid), dim3(kMxThreads), 0, s, a);   \
+    else hipLaunchKernelGGL((mx_static_kernel<T, C, TW, false>), dim3(grid), dim3(kMxThreads), 0, s, a);      \
+    if (p.splits > 1)                                                                                          \
+      hipLaunchKernelGGL((mx_static_finish_kernel<T, C, TW>),                                                  \
+                         dim3((unsigned)((batch * nv * nv + kMxThreads - 1) / kMxThreads)), dim3(kMxThreads), 0, \
+                         s, a);                                                                                \
+  } while (0)
+  if (corr) {
+    if (p.tw == 2) NBG_MX_STATIC(true, 2);
+    else NBG_MX_STATIC(true, 1);
+  } else {
+    if (p.tw == 2) NBG_MX_STATIC(false, 2);
+    else NBG_MX_STATIC(false, 1);
+  }
+#undef NBG_MX_STATIC
+  return check_hip(hipGetLastError(), "matrix launch");
+}
+
+size_t static_ws(int64_t batch, int64_t nv, int64_t nobs, bool corr) {
+  const StaticPlan p = static_plan(batch, nv, nobs, corr);
+  return align256(p.shift_bytes) + p.part_bytes;
+}
+
+template <typename T>
+int move_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nobs, int64_t nv, int64_t window,
+                int64_t min_count, void* ws, hipStream_t s) {
+  const MovePlan p = move_plan(batch, nobs, nv, window);
+  double* shift = static_cast<double*>(ws);
+  MxShiftArgs sa{batch * p.tiles, nv, p.tiles, p.tile_len, window - 1, nobs, nobs * nv, 1, nv};
+  hipLaunchKernelGGL(mx_shift_kernel<T>, dim3(mx_grid(batch * p.tiles * nv)), dim3(kMxThreads), 0, s,
+                     static_cast<const T*>(x), shift, sa);
+  MxMoveArgs a{x, out, shift, batch, nobs, nv, window, min_count > 1 ? min_count : 1,
+               p.tile_len, p.tiles, p.pair_waves, p.groups, p.sub_len};
+  const int grid = mx_grid(batch * p.tiles * p.pair_waves);
+  if (corr) hipLaunchKernelGGL((mx_move_kernel<T, true>), dim3(grid), dim3(kMxThreads), 0, s, a);
+  else hipLaunchKernelGGL((mx_move_kernel<T, false>), dim3(grid), dim3(kMxThreads), 0, s, a);
+  return check_hip(hipGetLastError(), "matrix launch");
+}
+
+template <typename TX, typename TA>
+int exp_launch(bool corr, const void* x, const void* alpha, int64_t ld_alpha, double min_weight, void* out,
+               int64_t batch, int64_t nobs, int64_t nv, void* ws, hipStream_t s) {
+  const ExpPlan p = exp_plan(batch, nobs, nv, corr);
+  char* w = static_cast<char*>(ws);
+  double* shift = reinterpret_cast<double*>(w);
+  double* state = reinterpret_cast<double*>(w + align256(p.shift_bytes));
+  double* decay = reinterpret_cast<double*>(w + align256(p.shift_bytes) + align256(p.state_bytes));
+  MxShiftArgs sa{batch, nv, 1, nobs, 0, nobs, nobs * nv, 1, nv};
+  hipLaunchKernelGGL(mx_shift_kernel<TX>, dim3(mx_grid(batch * nv)), dim3(kMxThreads), 0, s,
+                     static_cast<const TX*>(x), shift, sa);
+  MxExpArgs a{x, alpha, out, shift, state, decay, ld_alpha, min_weight, batch, nobs, nv,
+              p.chunk, p.chunks, p.pair_waves, p.groups};
+  const int grid = mx_grid(p.waves);
+  const unsigned carry_grid = (unsigned)((batch * nv * nv + kMxThreads - 1) / kMxThreads);
+  if (corr) {
+    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, true, false>), dim3(grid), dim3(kMxThreads), 0, s, a);
+    hipLaunchKernelGGL((mx_exp_carry_kernel<true>), dim3(carry_grid), dim3(kMxThreads), 0, s, a);
+    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, true, true>), dim3(grid), dim3(kMxThreads), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, false, false>), dim3(grid), dim3(kMxThreads), 0, s, a);
+    hipLaunchKernelGGL((mx_exp_carry_kernel<false>), dim3(carry_grid), dim3(kMxThreads), 0, s, a);
+    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, false, true>), dim3(grid), dim3(kMxThreads), 0, s, a);
+  }
+  return check_hip(hipGetLastError(), "matrix launch");
+}
+
+size_t exp_ws(int64_t batch, int64_t nobs, int64_t nv, bool corr) {
+  const ExpPlan p = exp_plan(batch, nobs, nv, corr);
+  return align256(p.shift_bytes) + align256(p.state_bytes) + p.decay_bytes;
+}
+
+bool sizes_ok(int64_t a, int64_t b, int64_t c) { return a >= 0 && b >= 0 && c >= 0; }
+
+}  // namespace
+}  // namespace nbg
+
+extern "C" {
+
+size_t nbg_matrix_workspace_bytes(int64_t batch, int64_t nvars, int64_t nobs) {
+  if (!nbg::sizes_ok(batch, nvars, nobs) || batch * nvars * nobs == 0) return 0;
+  return nbg::static_ws(batch, nvars, nobs, true);
+}
+
+static int static_entry(const char* name, bool corr, int dtype, const void* x, void* out, int64_t batch,
+                        int64_t nvars, int64_t nobs, void* ws, size_t ws_bytes, void* stream) {
+  if (int rc = nbg::injected_error(name)) return rc;
+  if (!nbg::sizes_ok(batch, nvars, nobs)) return nbg::set_error(NBG_ERR_INVALID, "%s: negative size", name);
+  if (batch == 0 || nvars == 0) return NBG_OK;
+  if (x == nullptr && nobs > 0) return nbg::set_error(NBG_ERR_INVALID, "%s: null input", name);
+  if (out == nullptr) return nbg::set_error(NBG_ERR_INVALID, "%s: null output", name);
+  if (nobs == 0) {  // every pair has no observation: NaN
+    return nbg::set_error(NBG_ERR_INVALID, "%s: no observations (the host fills NaN)", name);
+  }
+  if (ws == nullptr || ws_bytes < nbg::static_ws(batch, nvars, nobs, corr))
+    return nbg::set_error(NBG_ERR_INVALID, "%s: workspace too small", name);
+  hipStream_t s = (hipStream_t)stream;
+  switch (dtype) {
+    case NBG_F32: return nbg::static_launch<float>(corr, x, out, batch, nvars, nobs, ws, s);
+    case NBG_F64: return nbg::static_launch<double>(corr, x, out, batch, nvars, nobs, ws, s);
+    default: return nbg::set_error(NBG_ERR_INVALID, "%s: unsupported dtype code %d", name, dtype);
+  }
+}
+
+int nbg_nancorrmatrix(int dtype, const void* x, void* out, int64_t batch, int64_t nvars, int64_t nobs, void* ws,
+                      size_t ws_bytes, void* stream) {
+  return static_entry("nbg_nancorrmatrix", true, dtype, x, out, batch, nvars, nobs, ws, ws_bytes, stream);
+}
+
+int nbg_nancovmatrix(int dtype, const void* x, void* out, int64_t batch, int64_t nvars, int64_t nobs, void* ws,
+                     size_t ws_bytes, void* stream) {
+  return static_entry("nbg_nancovmatrix", false, dtype, x, out, batch, nvars, nobs, ws, ws_bytes, stream);
+}
+
+size_t nbg_move_matrix_workspace_bytes(int64_t batch, int64_t nobs, int64_t nvars, int64_t window) {
+  if (!nbg::sizes_ok(batch, nobs, nvars) || window < 1 || batch * nobs * nvars == 0) return 0;
+  return nbg::move_plan(batch, nobs, nvars, window).shift_bytes;
+}
+
+static int move_entry(const char* name, bool corr, int dtype, const void* x, void* out, int64_t batch, int64_t nobs,
+                      int64_t nvars, int64_t window, int64_t min_count, void* ws, size_t ws_bytes, void* stream) {
+  if (int rc = nbg::injected_error(name)) return rc;
+  if (!nbg::sizes_ok(batch, nobs, nvars)) return nbg::set_error(NBG_ERR_INVALID, "%s: negative size", name);
+  if (window < 1 || window > nobs) return nbg::set_error(NBG_ERR_INVALID, "%s: window not in valid range: %lld", name,
+                                                         (long long)window);
+  if (min_count < 0) return nbg::set_error(NBG_ERR_INVALID, "%s: min_count must be positive", name);
+  if (batch == 0 || nobs == 0 || nvars == 0) return NBG_OK;
+  if (x == nullptr || out == nullptr) return nbg::set_error(NBG_ERR_INVALID, "%s: null pointer", name);
+  if (ws == nullptr || ws_bytes < nbg::move_plan(batch, nobs, nvars, window).shift_bytes)
+    return nbg::set_error(NBG_ERR_INVALID, "%s: workspace too small", name);
+  hipStream_t s = (hipStream_t)stream;
+  switch (dtype) {
+    case NBG_F32: return nbg::move_launch<float>(corr, x, out, batch, nobs, nvars, window, min_count, ws, s);
+    case NBG_F64: return nbg::move_launch<double>(corr, x, out, batch, nobs, nvars, window, min_count, ws, s);
+    default: return nbg::set_error(NBG_ERR_INVALID, "%s: unsupported dtype code %d", name, dtype);
+  }
+}
+
+int nbg_move_corrmatrix(int dtype, const void* x, void* out, int64_t batch, int64_t nobs, int64_t nvars,
+                        int64_t window, int64_t min_count, void* ws, size_t ws_bytes, void* stream) {
+  return move_entry("nbg_move_corrmatrix", true, dtype, x, out, batch, nobs, nvars, window, min_count, ws, ws_bytes,
+                    stream);
+}
+
+int nbg_move_covmatrix(int dtype, const void* x, void* out, int64_t batch, int64_t nobs, int64_t nvars,
+                       int64_t window, int64_t min_count, void* ws, size_t ws_bytes, void* stream) {
+  return move_entry("nbg_move_covmatrix", false, dtype, x, out, batch, nobs, nvars, window, min_count, ws, ws_bytes,
+                    stream);
+}
+
+size_t nbg_move_exp_matrix_workspace_bytes(int64_t batch, int64_t nobs, int64_t nvars) {
+  if (!nbg::sizes_ok(batch, nobs, nvars) || batch * nobs * nvars == 0) return 0;
+  return nbg::exp_ws(batch, nobs, nvars, true);
+}
+
+static int exp_entry(const char* name, bool corr, int x_dtype, int dtype, const void* x, const void* alpha,
+                     int64_t ld_alpha, double min_weight, void* out, int64_t batch, int64_t nobs, int64_t nvars,
+                     void* ws, size_t ws_bytes, void* stream) {
+  if (int rc = nbg::injected_error(name)) return rc;
+  if (!nbg::sizes_ok(batch, nobs, nvars) || ld_alpha < 0)
+    return nbg::set_error(NBG_ERR_INVALID, "%s: negative size", name);
+  if (batch == 0 || nobs == 0 || nvars == 0) return NBG_OK;
+  if (x == nullptr || out == nullptr || alpha == nullptr) return nbg::set_error(NBG_ERR_INVALID, "%s: null pointer", name);
+  if (ws == nullptr || ws_bytes < nbg::exp_ws(batch, nobs, nvars, corr))
+    return nbg::set_error(NBG_ERR_INVALID, "%s: workspace too small", name);
+  hipStream_t s = (hipStream_t)stream;
+  if (x_dtype == NBG_F32 && dtype == NBG_F32)
+    return nbg::exp_launch<float, float>(corr, x, alpha, ld_alpha, min_weight, out, batch, nobs, nvars, ws, s);
+  if (x_dtype == NBG_F32 && dtype == NBG_F64)
+    return nbg::exp_launch<float, double>(corr, x, alpha, ld_alpha, min_weight, out, batch, nobs, nvars, ws, s);
+  if (x_dtype == NBG_F64 && dtype == NBG_F64)
+    return nbg::exp_launch<double, double>(corr, x, alpha, ld_alpha, min_weight, out, batch, nobs, nvars, ws, s);
+  return nbg::set_error(NBG_ERR_INVALID, "%s: unsupported dtype pair (%d, %d)", name, x_dtype, dtype);
+}
+
+int nbg_move_exp_nancorrmatrix(int x_dtype, int dtype, const void* x, const void* alpha, int64_t ld_alpha,
+                               double min_weight, void* out, int64_t batch, int64_t nobs, int64_t nvars, void* ws,
+                               size_t ws_bytes, void* stream) {
+  return exp_entry("nbg_move_exp_nancorrmatrix", true, x_dtype, dtype, x, alpha, ld_alpha, min_weight, out, batch,
+                   nobs, nvars, ws, ws_bytes, stream);
+}
+
+int nbg_move_exp_nancovmatrix(int x_dtype, int dtype, const void* x, const void* alpha, int64_t ld_alpha,
+                              double min_weight, void* out, int64_t batch, int64_t nobs, int64_t nvars, void* ws,
+                              size_t ws_bytes, void* stream) {
+  return exp_entry("nbg_move_exp_nancovmatrix", false, x_dtype, dtype, x, alpha, ld_alpha, min_weight, out, batch,
+                   nobs, nvars, ws, ws_bytes, stream);
+}
+
+}  // extern "C"

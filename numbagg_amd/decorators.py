@@ -605,3 +605,142 @@ class ndreduce(_Op):
             if self.index and bool((out < 0).any().item()):
                 raise ValueError("All-NaN slice encountered")
         return _scalar_or_array(out, outer, as_tensor)
+
+
+def _matrix_loop(a, func_name: str) -> np.dtype:
+    """Loop dtype of the float32/float64 matrix signatures for ``a``."""
+    return _loop_dtype([dev.np_dtype(a)], func_name)
+
+
+def _batched(t: torch.Tensor) -> tuple[torch.Tensor, tuple[int, ...], int]:
+    """(batch, d0, d1) contiguous view of ``t`` (..., d0, d1)."""
+    outer = tuple(t.shape[:-2])
+    batch = int(np.prod(outer, dtype=np.int64)) if outer else 1
+    return t.contiguous().view(batch, t.shape[-2], t.shape[-1]), outer, batch
+
+
+class ndmatrix(_Op):
+    """Static matrix operator (numbagg/decorators.py:677-744): fixed core
+    dimensions (..., vars, obs) -> (..., vars, vars), no axis argument.
+
+    Calls ``nbg_<name>(dtype, x, out, batch, nvars, nobs, ws, ws_bytes, stream)``.
+    """
+
+    def __call__(self, a, **kwargs):
+        if "axis" in kwargs or "axes" in kwargs:
+            raise TypeError(f"{self.__name__}() got an unexpected keyword argument 'axis'")
+        self._no_extra(kwargs)
+        if len(_shape(a)) < 2:
+            raise ValueError(
+                f"{self.__name__} requires at least a 2D array with shape (..., vars, obs). "
+                "For 1D arrays, use nanvar for variance calculations."
+            )
+        as_tensor = dev.is_tensor(a)
+        loop = _matrix_loop(a, self.__name__)
+        t = dev.to_device(a, loop)
+        x, outer, batch = _batched(t)
+        nv, nobs = x.shape[1], x.shape[2]
+        out = torch.empty((batch, nv, nv), dtype=dev.torch_dtype(loop), device=t.device)
+        if batch and nv:
+            if nobs == 0:  # no pair has an observation
+                out.fill_(float("nan"))
+            else:
+                ws_bytes = _lib.query("nbg_matrix_workspace_bytes", batch, nv, nobs)
+                ws, ws_bytes = dev.workspace(ws_bytes, t.device)
+                _lib.call(self.entry, ctypes.c_int(dev.nbg_code(out)), dev.ptr(x), dev.ptr(out), batch, nv, nobs,
+                          dev.ptr(ws), ws_bytes, dev.stream_handle())
+        return dev.to_output(out.view(*outer, nv, nv), as_tensor)
+
+
+class ndmovematrix(_Op):
+    """Moving-window matrix operator (numbagg/decorators.py:747-818): fixed core
+    dimensions (..., obs, vars) -> (..., obs, vars, vars).
+
+    Calls ``nbg_<name>(dtype, x, out, batch, nobs, nvars, window, min_count, ws,
+    ws_bytes, stream)``.
+    """
+
+    def __call__(self, a, window: int, min_count: int | None = None, **kwargs):
+        self._no_extra(kwargs)
+        shape = _shape(a)
+        if len(shape) < 2:
+            raise ValueError(f"{self.__name__} requires at least a 2D array with shape (..., obs, vars).")
+        if min_count is None:
+            min_count = window
+        elif min_count < 0:
+            raise ValueError(f"min_count must be positive: {min_count}")
+        if not 0 < window <= shape[-2]:
+            raise ValueError(f"window not in valid range: {window}")
+        if not _is_int_scalar(window) or not _is_int_scalar(min_count):
+            raise TypeError(
+                f"ufunc '{self.__name__}' not supported for the input types: window and min_count must be integers"
+            )
+        as_tensor = dev.is_tensor(a)
+        loop = _matrix_loop(a, self.__name__)
+        t = dev.to_device(a, loop)
+        x, outer, batch = _batched(t)
+        nobs, nv = x.shape[1], x.shape[2]
+        out = torch.empty((batch, nobs, nv, nv), dtype=dev.torch_dtype(loop), device=t.device)
+        if batch and nobs and nv:
+            ws_bytes = _lib.query("nbg_move_matrix_workspace_bytes", batch, nobs, nv, int(window))
+            ws, ws_bytes = dev.workspace(ws_bytes, t.device)
+            _lib.call(self.entry, ctypes.c_int(dev.nbg_code(out)), dev.ptr(x), dev.ptr(out), batch, nobs, nv,
+                      int(window), int(min_count), dev.ptr(ws), ws_bytes, dev.stream_handle())
+        return dev.to_output(out.view(*outer, nobs, nv, nv), as_tensor)
+
+
+class ndmoveexpmatrix(_Op):
+    """Exponentially weighted matrix operator (numbagg/decorators.py:1031-1112):
+    fixed core dimensions (..., obs, vars), (obs) alpha -> (..., obs, vars, vars).
+
+    A non-array alpha is broadcast along the observations (as a float64 array
+    for a Python float, so float32 data runs on the float64 loop, as NumPy's
+    loop selection does in the reference); an array alpha has its last axis
+    along the observations and broadcasts over the leading dimensions.
+    """
+
+    def __call__(self, a, alpha, min_weight: float = 0, **kwargs):
+        self._no_extra(kwargs)
+        shape = _shape(a)
+        if len(shape) < 2:
+            raise ValueError(f"{self.__name__} requires at least a 2D array with shape (..., obs, vars).")
+        as_tensor = dev.is_tensor(a)
+        nobs = shape[-2]
+        if isinstance(alpha, (np.ndarray, torch.Tensor)):
+            alpha_dtype = dev.np_dtype(alpha)
+            alpha_shape = tuple(alpha.shape)
+        else:
+            alpha_dtype = np.asarray(alpha).dtype
+            alpha_shape = ()
+        dtypes = [dev.np_dtype(a), alpha_dtype]
+        if isinstance(min_weight, np.generic):
+            dtypes.append(min_weight.dtype)
+        loop = _loop_dtype(dtypes, self.__name__)
+        mw = float(np.asarray(min_weight, dtype=loop))
+        fused = loop == np.float64 and dev.np_dtype(a) == np.float32
+        x_dtype = np.dtype(np.float32) if fused else loop
+        t = dev.to_device(a, x_dtype)
+        x, outer, batch = _batched(t)
+        nv = x.shape[2]
+        if alpha_shape and alpha_shape[-1] != nobs:
+            raise ValueError(
+                f"{self.__name__}: alpha has length {alpha_shape[-1]} but the observations dimension has "
+                f"length {nobs}"
+            )
+        if len(alpha_shape) <= 1:
+            a_row = torch.full((nobs,), float(np.asarray(alpha, dtype=loop)), dtype=dev.torch_dtype(loop),
+                               device=t.device) if not alpha_shape else dev.to_device(alpha, loop).contiguous()
+            alpha_t, ld_alpha = a_row, 0
+        else:
+            a_full = dev.to_device(alpha, loop)
+            alpha_t = a_full.expand(*outer, nobs).contiguous().view(max(batch, 1), nobs)
+            ld_alpha = nobs
+        out = torch.empty((batch, nobs, nv, nv), dtype=dev.torch_dtype(loop), device=t.device)
+        if batch and nobs and nv:
+            code_in = _lib.NBG_F32 if x_dtype == np.float32 else _lib.NBG_F64
+            ws_bytes = _lib.query("nbg_move_exp_matrix_workspace_bytes", batch, nobs, nv)
+            ws, ws_bytes = dev.workspace(ws_bytes, t.device)
+            _lib.call(self.entry, ctypes.c_int(code_in), ctypes.c_int(dev.nbg_code(out)), dev.ptr(x),
+                      dev.ptr(alpha_t), ld_alpha, ctypes.c_double(mw), dev.ptr(out), batch, nobs, nv, dev.ptr(ws),
+                      ws_bytes, dev.stream_handle())
+        return dev.to_output(out.view(*outer, nobs, nv, nv), as_tensor)

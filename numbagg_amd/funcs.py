@@ -5,7 +5,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .decorators import ndaggregate, ndfill, ndreduce
+from .decorators import ndaggregate, ndfill, ndmatrix, ndreduce
 
 _NUMERIC = [np.int32, np.int64, np.float32, np.float64]
 _FLOAT = [np.float32, np.float64]
@@ -75,6 +75,19 @@ def nanmin(a):
 count = nancount
 
 
+@ndmatrix.wrap()
+def nancorrmatrix(a, out):
+    """Correlation matrix of (..., vars, obs) data over pairwise-complete
+    observations: float64 sums on the matrix cores, NaN where a pair has fewer
+    than two observations or a variance is not positive (numbagg/funcs.py:338-441)."""
+
+
+@ndmatrix.wrap()
+def nancovmatrix(a, out):
+    """Covariance matrix (ddof 1) of (..., vars, obs) data over
+    pairwise-complete observations (numbagg/funcs.py:444-532)."""
+
+
 @ndfill.wrap()
 def bfill(a, limit, out):
     """Backward fill missing values.
@@ -101,7 +114,9 @@ __all__ = [
     "ffill",
     "nanargmax",
     "nanargmin",
+    "nancorrmatrix",
     "nancount",
+    "nancovmatrix",
     "nanmax",
     "nanmean",
     "nanmin",

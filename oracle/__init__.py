@@ -175,6 +175,42 @@ class restated:
         return out.astype(bool) if odt == np.uint8 else out
 
 
+    @staticmethod
+    def matrix(func: str, x: np.ndarray, window: int | None = None, min_count: int | None = None, alpha=None,
+               min_weight: float = 0.0) -> np.ndarray:
+        """Matrix functions on a 3-D batch: static (batch, vars, obs) ->
+        (batch, vars, vars); moving and exponential (batch, obs, vars) ->
+        (batch, obs, vars, vars). alpha: scalar, (obs,) or (batch, obs)."""
+        x = np.ascontiguousarray(x)
+        if x.ndim == 2:
+            x = x[None]
+        sfx = _SFX[x.dtype]
+        fn = getattr(lib(), f"orc_{func}_{sfx}")
+        bsz = x.shape[0]
+        if func in ("nancorrmatrix", "nancovmatrix"):
+            nv, nobs = x.shape[1:]
+            out = np.empty((bsz, nv, nv), dtype=x.dtype)
+            fn(_p(x), _p(out), I64(bsz), I64(nv), I64(nobs))
+            return out
+        nobs, nv = x.shape[1:]
+        out = np.empty((bsz, nobs, nv, nv), dtype=x.dtype)
+        if func in ("move_corrmatrix", "move_covmatrix"):
+            mc = window if min_count is None else min_count
+            fn(_p(x), _p(out), I64(bsz), I64(nobs), I64(nv), I64(window), I64(mc))
+            return out
+        al = np.asarray(alpha, dtype=x.dtype)
+        if al.ndim == 0:
+            al, lda = np.full((1, nobs), al, dtype=x.dtype), 0
+        elif al.ndim == 1:
+            al, lda = al[None, :], 0
+        else:
+            lda = nobs
+        al = np.ascontiguousarray(al)
+        mw = ctypes.c_float(min_weight) if x.dtype == np.float32 else ctypes.c_double(min_weight)
+        fn(_p(x), _p(al), I64(lda), _p(out), I64(bsz), I64(nobs), I64(nv), mw)
+        return out
+
+
 _MOVE_KIND = {"move_sum": 0, "move_mean": 1, "move_var": 2, "move_std": 3, "move_cov": 4, "move_corr": 5}
 _EXP_KIND = {
     "move_exp_nancount": 0,
@@ -340,6 +376,43 @@ class exact:
             else:
                 raise ValueError(func)
         return out
+
+    @staticmethod
+    def matrix(func: str, x: np.ndarray, window: int | None = None, min_count: int | None = None, alpha=None,
+               min_weight: float = 0.0, loop_dtype=None) -> np.ndarray:
+        """Matrix functions pair by pair (float64 results): static ones by a
+        two-pass long-double evaluation over each pair's complete observations;
+        moving and exponential ones by the two-series oracles above applied to
+        every (i, j) pair (a pair's statistics are those of move_cov/move_corr
+        and move_exp_nancov/nancorr on columns i and j)."""
+        x0 = np.ascontiguousarray(x)
+        if x0.ndim == 2:
+            x0 = x0[None]
+        loop = np.dtype(loop_dtype or x0.dtype)
+        xf = x0.astype(np.float64)
+        bsz = xf.shape[0]
+        if func in ("nancorrmatrix", "nancovmatrix"):
+            nv, nobs = xf.shape[1:]
+            out = np.empty((bsz, nv, nv))
+            lib().orc_exact_matrix(ctypes.c_int(1 if func == "nancorrmatrix" else 0), _p(xf), _p(out), I64(bsz),
+                                   I64(nv), I64(nobs))
+            return out
+        nobs, nv = xf.shape[1:]
+        cols = np.moveaxis(xf, 2, 1)  # (batch, vars, obs)
+        xi = np.repeat(cols[:, :, None, :], nv, axis=2).reshape(-1, nobs)
+        xj = np.repeat(cols[:, None, :, :], nv, axis=1).reshape(-1, nobs)
+        if func in ("move_corrmatrix", "move_covmatrix"):
+            mc = window if min_count is None else min_count
+            if func == "move_corrmatrix":
+                mc = max(mc, 2)
+            res = exact.move("move_corr" if func == "move_corrmatrix" else "move_cov", xi, window, mc, y=xj)
+        else:
+            al = np.asarray(alpha, dtype=loop)
+            if al.ndim == 2:
+                al = np.repeat(al, nv * nv, axis=0)
+            res = exact.move_exp("move_exp_nancorr" if func == "move_exp_nancorrmatrix" else "move_exp_nancov",
+                                 xi, al, min_weight, y=xj, loop_dtype=loop)
+        return np.moveaxis(res.reshape(bsz, nv, nv, nobs), 3, 1).copy()
 
     @staticmethod
     def factorize(values: np.ndarray, sort: bool = True):

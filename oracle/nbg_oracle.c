@@ -1143,3 +1143,273 @@ DEF_NANARGEXT(float, f32, nanargmin, <, INFINITY)
 DEF_NANARGEXT(double, f64, nanargmin, <, INFINITY)
 DEF_NANARGEXT(int32_t, i32, nanargmin, <, INFINITY)
 DEF_NANARGEXT(int64_t, i64, nanargmin, <, INFINITY)
+
+/* ------------------------------------------------------------------------ */
+/* matrix functions: numbagg/funcs.py:338-532, numbagg/moving_matrix.py      */
+/* Pairwise-complete statistics per (i, j): an observation counts for a pair */
+/* only where both variables are valid. Accumulators are the data dtype, as  */
+/* numba types the np.zeros(..., dtype=a.dtype) arrays; counts are int64.   */
+/* ------------------------------------------------------------------------ */
+
+/* nancorrmatrix (numbagg/funcs.py:338-441) and nancovmatrix (:444-532):
+ * x (batch, nv, nobs) -> out (batch, nv, nv); upper triangle accumulated in
+ * observation order, mirrored at the end. */
+#define DEF_STATIC_MATRIX(T, SFX)                                                         \
+  static void orc_static_matrix_##SFX(const T* x, T* out, int64_t batch, int64_t nv,      \
+                                      int64_t nobs, int corr) {                           \
+    ORC_PRAGMA_OMP                                                                        \
+    for (int64_t b = 0; b < batch; ++b) {                                                 \
+      const T* a = x + b * nv * nobs;                                                     \
+      T* o = out + b * nv * nv;                                                           \
+      const int64_t P = nv * nv;                                                          \
+      T* si = (T*)calloc((size_t)P * 5, sizeof(T));                                       \
+      T *sj = si + P, *sqi = si + 2 * P, *sqj = si + 3 * P, *sij = si + 4 * P;             \
+      int64_t* cnt = (int64_t*)calloc((size_t)P, sizeof(int64_t));                        \
+      for (int64_t k = 0; k < nobs; ++k) {                                                \
+        for (int64_t i = 0; i < nv; ++i) {                                                \
+          const T vi = a[i * nobs + k];                                                   \
+          if (isnan(vi)) continue;                                                        \
+          for (int64_t j = i; j < nv; ++j) {                                              \
+            const T vj = a[j * nobs + k];                                                 \
+            if (isnan(vj)) continue;                                                      \
+            const int64_t p = i * nv + j;                                                 \
+            si[p] += vi;                                                                  \
+            sj[p] += vj;                                                                  \
+            sqi[p] += (T)(vi * vi);                                                       \
+            sqj[p] += (T)(vj * vj);                                                       \
+            sij[p] += (T)(vi * vj);                                                       \
+            cnt[p] += 1;                                                                  \
+          }                                                                               \
+        }                                                                                 \
+      }                                                                                   \
+      for (int64_t i = 0; i < nv; ++i) {                                                  \
+        for (int64_t j = i; j < nv; ++j) {                                                \
+          const int64_t p = i * nv + j;                                                   \
+          const int64_t c = cnt[p];                                                       \
+          double r = NAN;                                                                 \
+          if (c > 1) {                                                                    \
+            const double mi = (double)si[p] / (double)c, mj = (double)sj[p] / (double)c;  \
+            const double cv = ((double)sij[p] / (double)c - mi * mj) * (double)c / (double)(c - 1); \
+            if (!corr) {                                                                  \
+              r = cv;                                                                     \
+            } else {                                                                      \
+              const double vi = ((double)sqi[p] / (double)c - mi * mi) * (double)c / (double)(c - 1); \
+              const double vj = ((double)sqj[p] / (double)c - mj * mj) * (double)c / (double)(c - 1); \
+              if (vi > 0 && vj > 0) r = cv / sqrt(vi * vj);                               \
+            }                                                                             \
+          }                                                                               \
+          o[i * nv + j] = (T)r;                                                           \
+          o[j * nv + i] = (T)r;                                                           \
+        }                                                                                 \
+      }                                                                                   \
+      free(si);                                                                           \
+      free(cnt);                                                                          \
+    }                                                                                     \
+  }                                                                                       \
+  void orc_nancorrmatrix_##SFX(const T* x, T* out, int64_t batch, int64_t nv, int64_t nobs) { \
+    orc_static_matrix_##SFX(x, out, batch, nv, nobs, 1);                                  \
+  }                                                                                       \
+  void orc_nancovmatrix_##SFX(const T* x, T* out, int64_t batch, int64_t nv, int64_t nobs) { \
+    orc_static_matrix_##SFX(x, out, batch, nv, nobs, 0);                                  \
+  }
+DEF_STATIC_MATRIX(float, f32)
+DEF_STATIC_MATRIX(double, f64)
+
+/* move_corrmatrix (numbagg/moving_matrix.py:16-114) and move_covmatrix
+ * (:117-201): x (batch, nobs, nv) -> out (batch, nobs, nv, nv). Each step
+ * removes the observation leaving the window, then adds the new one, for every
+ * ordered pair. */
+#define DEF_MOVE_MATRIX(T, SFX)                                                           \
+  static void orc_move_matrix_##SFX(const T* x, T* out, int64_t batch, int64_t nobs,      \
+                                    int64_t nv, int64_t window, int64_t min_count, int corr) { \
+    ORC_PRAGMA_OMP                                                                        \
+    for (int64_t b = 0; b < batch; ++b) {                                                 \
+      const T* a = x + b * nobs * nv;                                                     \
+      T* o = out + b * nobs * nv * nv;                                                    \
+      const int64_t P = nv * nv;                                                          \
+      const int64_t mc = min_count > 1 ? min_count : 1;                                   \
+      T* si = (T*)calloc((size_t)P * 5, sizeof(T));                                       \
+      T *sj = si + P, *sqi = si + 2 * P, *sqj = si + 3 * P, *sij = si + 4 * P;             \
+      int64_t* cnt = (int64_t*)calloc((size_t)P, sizeof(int64_t));                        \
+      for (int64_t t = 0; t < nobs; ++t) {                                                \
+        if (t >= window) {                                                                \
+          const T* old = a + (t - window) * nv;                                           \
+          for (int64_t i = 0; i < nv; ++i) {                                              \
+            if (isnan(old[i])) continue;                                                  \
+            for (int64_t j = 0; j < nv; ++j) {                                            \
+              if (isnan(old[j])) continue;                                                \
+              const int64_t p = i * nv + j;                                               \
+              si[p] -= old[i];                                                            \
+              sj[p] -= old[j];                                                            \
+              sqi[p] -= (T)(old[i] * old[i]);                                             \
+              sqj[p] -= (T)(old[j] * old[j]);                                             \
+              sij[p] -= (T)(old[i] * old[j]);                                             \
+              cnt[p] -= 1;                                                                \
+            }                                                                             \
+          }                                                                               \
+        }                                                                                 \
+        const T* nw = a + t * nv;                                                         \
+        for (int64_t i = 0; i < nv; ++i) {                                                \
+          if (isnan(nw[i])) continue;                                                     \
+          for (int64_t j = 0; j < nv; ++j) {                                              \
+            if (isnan(nw[j])) continue;                                                   \
+            const int64_t p = i * nv + j;                                                 \
+            si[p] += nw[i];                                                               \
+            sj[p] += nw[j];                                                               \
+            sqi[p] += (T)(nw[i] * nw[i]);                                                 \
+            sqj[p] += (T)(nw[j] * nw[j]);                                                 \
+            sij[p] += (T)(nw[i] * nw[j]);                                                 \
+            cnt[p] += 1;                                                                  \
+          }                                                                               \
+        }                                                                                 \
+        T* ot = o + t * P;                                                                \
+        for (int64_t p = 0; p < P; ++p) {                                                 \
+          const int64_t c = cnt[p];                                                       \
+          double r = NAN;                                                                 \
+          if (corr) {                                                                     \
+            if (c >= (mc > 2 ? mc : 2)) {                                                 \
+              const double mi = (double)si[p] / (double)c, mj = (double)sj[p] / (double)c; \
+              const double vi = (double)sqi[p] / (double)c - mi * mi;                     \
+              const double vj = (double)sqj[p] / (double)c - mj * mj;                     \
+              const double cv = (double)sij[p] / (double)c - mi * mj;                     \
+              if (vi > 0 && vj > 0) r = cv / sqrt(vi * vj);                               \
+            }                                                                             \
+          } else if (c >= mc && c > 1) {                                                  \
+            const double mi = (double)si[p] / (double)c, mj = (double)sj[p] / (double)c;  \
+            r = ((double)sij[p] / (double)c - mi * mj) * (double)c / (double)(c - 1);     \
+          }                                                                               \
+          ot[p] = (T)r;                                                                   \
+        }                                                                                 \
+      }                                                                                   \
+      free(si);                                                                           \
+      free(cnt);                                                                          \
+    }                                                                                     \
+  }                                                                                       \
+  void orc_move_corrmatrix_##SFX(const T* x, T* out, int64_t batch, int64_t nobs, int64_t nv, \
+                                 int64_t window, int64_t min_count) {                     \
+    orc_move_matrix_##SFX(x, out, batch, nobs, nv, window, min_count, 1);                 \
+  }                                                                                       \
+  void orc_move_covmatrix_##SFX(const T* x, T* out, int64_t batch, int64_t nobs, int64_t nv, \
+                                int64_t window, int64_t min_count) {                      \
+    orc_move_matrix_##SFX(x, out, batch, nobs, nv, window, min_count, 0);                 \
+  }
+DEF_MOVE_MATRIX(float, f32)
+DEF_MOVE_MATRIX(double, f64)
+
+/* move_exp_nancorrmatrix (numbagg/moving_matrix.py:204-328) and
+ * move_exp_nancovmatrix (:331-432): x (batch, nobs, nv), alpha (nobs) per
+ * batch row (ld_alpha 0: shared) in the data dtype -> out (batch, nobs, nv,
+ * nv). Every statistic decays each step (decay = 1.0 - alpha, float64), the
+ * squared-weight sum by decay**2; a pair adds its observation where both
+ * values are valid. */
+#define DEF_EXP_MATRIX(T, SFX)                                                            \
+  static void orc_exp_matrix_##SFX(const T* x, const T* alpha, int64_t ld_alpha, T* out,  \
+                                   int64_t batch, int64_t nobs, int64_t nv, T min_weight, int corr) { \
+    ORC_PRAGMA_OMP                                                                        \
+    for (int64_t b = 0; b < batch; ++b) {                                                 \
+      const T* a = x + b * nobs * nv;                                                     \
+      const T* al = alpha + b * ld_alpha;                                                 \
+      T* o = out + b * nobs * nv * nv;                                                    \
+      const int64_t P = nv * nv;                                                          \
+      T* si = (T*)calloc((size_t)P * 8, sizeof(T));                                       \
+      T *sj = si + P, *sqi = si + 2 * P, *sqj = si + 3 * P, *sij = si + 4 * P;             \
+      T *pw = si + 5 * P, *psw = si + 6 * P, *psw2 = si + 7 * P;                          \
+      for (int64_t t = 0; t < nobs; ++t) {                                                \
+        const T alpha_t = al[t];                                                          \
+        const double decay = 1.0 - (double)alpha_t;                                       \
+        for (int64_t p = 0; p < P; ++p) {                                                 \
+          si[p] = (T)(si[p] * decay);                                                     \
+          sj[p] = (T)(sj[p] * decay);                                                     \
+          sqi[p] = (T)(sqi[p] * decay);                                                   \
+          sqj[p] = (T)(sqj[p] * decay);                                                   \
+          sij[p] = (T)(sij[p] * decay);                                                   \
+          pw[p] = (T)(pw[p] * decay);                                                     \
+          psw[p] = (T)(psw[p] * decay);                                                   \
+          psw2[p] = (T)(psw2[p] * (decay * decay));                                       \
+        }                                                                                 \
+        const T* nw = a + t * nv;                                                         \
+        for (int64_t i = 0; i < nv; ++i) {                                                \
+          if (isnan(nw[i])) continue;                                                     \
+          for (int64_t j = 0; j < nv; ++j) {                                              \
+            if (isnan(nw[j])) continue;                                                   \
+            const int64_t p = i * nv + j;                                                 \
+            si[p] += nw[i];                                                               \
+            sj[p] += nw[j];                                                               \
+            sqi[p] += (T)(nw[i] * nw[i]);                                                 \
+            sqj[p] += (T)(nw[j] * nw[j]);                                                 \
+            sij[p] += (T)(nw[i] * nw[j]);                                                 \
+            pw[p] += alpha_t;                                                             \
+            psw[p] = (T)(psw[p] + 1.0);                                                   \
+            psw2[p] = (T)(psw2[p] + 1.0);                                                 \
+          }                                                                               \
+        }                                                                                 \
+        T* ot = o + t * P;                                                                \
+        for (int64_t p = 0; p < P; ++p) {                                                 \
+          const double bias = psw[p] > 0 ? 1.0 - (double)(T)(psw2[p] / (T)(psw[p] * psw[p])) : 0.0; \
+          double r = NAN;                                                                 \
+          if (pw[p] >= min_weight && bias > 0) {                                          \
+            const T n = psw[p];                                                           \
+            const T mi = si[p] / n, mj = sj[p] / n;                                       \
+            const T cb = (T)(sij[p] / n - mi * mj);                                       \
+            if (!corr) {                                                                  \
+              r = (double)cb / bias;                                                      \
+            } else {                                                                      \
+              const double vi = (double)(T)(sqi[p] / n - mi * mi) / bias;                 \
+              const double vj = (double)(T)(sqj[p] / n - mj * mj) / bias;                 \
+              if (vi > 0 && vj > 0) r = ((double)cb / bias) / sqrt(vi * vj);              \
+            }                                                                             \
+          }                                                                               \
+          ot[p] = (T)r;                                                                   \
+        }                                                                                 \
+      }                                                                                   \
+      free(si);                                                                           \
+    }                                                                                     \
+  }                                                                                       \
+  void orc_move_exp_nancorrmatrix_##SFX(const T* x, const T* alpha, int64_t ld_alpha, T* out, \
+                                        int64_t batch, int64_t nobs, int64_t nv, T min_weight) { \
+    orc_exp_matrix_##SFX(x, alpha, ld_alpha, out, batch, nobs, nv, min_weight, 1);        \
+  }                                                                                       \
+  void orc_move_exp_nancovmatrix_##SFX(const T* x, const T* alpha, int64_t ld_alpha, T* out, \
+                                       int64_t batch, int64_t nobs, int64_t nv, T min_weight) { \
+    orc_exp_matrix_##SFX(x, alpha, ld_alpha, out, batch, nobs, nv, min_weight, 0);        \
+  }
+DEF_EXP_MATRIX(float, f32)
+DEF_EXP_MATRIX(double, f64)
+
+/* Exact static matrices: per pair, two-pass long-double mean and
+ * (co)variance over the pairwise-complete observations. kind 0 cov, 1 corr.
+ * x (batch, nv, nobs) float64 -> out (batch, nv, nv) float64. */
+void orc_exact_matrix(int kind, const double* x, double* out, int64_t batch, int64_t nv, int64_t nobs) {
+  ORC_PRAGMA_OMP
+  for (int64_t bp = 0; bp < batch * nv * nv; ++bp) {
+    const int64_t b = bp / (nv * nv), i = (bp / nv) % nv, j = bp % nv;
+    const double* xi = x + (b * nv + i) * nobs;
+    const double* xj = x + (b * nv + j) * nobs;
+    long double s1 = 0, s2 = 0;
+    int64_t c = 0;
+    for (int64_t k = 0; k < nobs; ++k) {
+      if (isnan(xi[k]) || isnan(xj[k])) continue;
+      s1 += xi[k];
+      s2 += xj[k];
+      c += 1;
+    }
+    double r = NAN;
+    if (c > 1) {
+      const long double m1 = s1 / c, m2 = s2 / c;
+      long double s11 = 0, s22 = 0, s12 = 0;
+      for (int64_t k = 0; k < nobs; ++k) {
+        if (isnan(xi[k]) || isnan(xj[k])) continue;
+        const long double d1 = xi[k] - m1, d2 = xj[k] - m2;
+        s11 += d1 * d1;
+        s22 += d2 * d2;
+        s12 += d1 * d2;
+      }
+      if (kind == 0) {
+        r = (double)(s12 / (c - 1));
+      } else if (s11 > 0 && s22 > 0) {
+        r = (double)(s12 / sqrtl(s11 * s22));
+      }
+    }
+    out[bp] = r;
+  }
+}

@@ -139,6 +139,26 @@ def main() -> None:
                 warnings.simplefilter("ignore")
                 res[f"series_{dt}_{name}"] = np.asarray(fn(pd.Series(vals).groupby(plabels)).values, dtype=np.float64)
 
+    # matrix functions (conftest.py:108-156, :204-211): static on (vars, obs)
+    # arrays, moving / exponential on the (obs, vars) transpose; pandas' pairwise
+    # DataFrame statistics use pairwise-complete observations.
+    for shape in ((3, 500), (20, 1000)):
+        m = a if shape == (3, 500) else fixture_array(shape)
+        tag = f"{shape[0]}x{shape[1]}"
+        res[f"matrix_array_{tag}"] = m
+        res[f"nancorrmatrix_{tag}"] = pd.DataFrame(m.T).corr().values
+        res[f"nancovmatrix_{tag}"] = pd.DataFrame(m.T).cov().values
+    mt = np.ascontiguousarray(a.T)  # (500, 3)
+    for window, mc in ((20, None), (20, 3), (50, 1)):
+        tag = f"w{window}_mc{mc}"
+        r = pd.DataFrame(mt).rolling(window, min_periods=mc)
+        res[f"move_corrmatrix_{tag}"] = r.corr().values.reshape(500, 3, 3)
+        res[f"move_covmatrix_{tag}"] = r.cov().values.reshape(500, 3, 3)
+    for alpha in (0.5, 0.1):
+        e = pd.DataFrame(mt).ewm(alpha=alpha)
+        res[f"move_exp_nancorrmatrix_a{alpha}"] = e.corr().values.reshape(500, 3, 3)
+        res[f"move_exp_nancovmatrix_a{alpha}"] = e.cov().values.reshape(500, 3, 3)
+
     np.savez_compressed(OUT, **res)
     print(f"wrote {OUT} ({len(res)} arrays, pandas {pd.__version__})")
 

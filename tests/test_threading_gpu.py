@@ -31,18 +31,27 @@ def _cases():
     y = rng.standard_normal((3, 20_000))
     lab = rng.integers(-1, 30, 20_000)
     cases = []
+    xm = np.ascontiguousarray(x.T[:2000])  # (obs, vars) for the moving matrices
     for f in nb.MOVE_FUNCS:
+        if f.__name__.endswith("matrix"):
+            cases.append((f.__name__, lambda f=f: f(xm, window=50, min_count=5)))
+            continue
         args = (x, y) if f.__name__ in ("move_cov", "move_corr") else (x,)
         cases.append((f.__name__, lambda f=f, args=args: f(*args, window=50, min_count=5)))
     for f in nb.MOVE_EXP_FUNCS:
         args = (x, y) if f.__name__ in ("move_exp_nancov", "move_exp_nancorr") else (x,)
         cases.append((f.__name__, lambda f=f, args=args: f(*args, alpha=0.1)))
+    for f in nb.MOVE_EXP_MATRIX_FUNCS:
+        cases.append((f.__name__, lambda f=f: f(xm, alpha=0.1)))
+    for f in nb.MATRIX_FUNCS:
+        cases.append((f.__name__, lambda f=f: f(x)))
     for f in nb.GROUPED_FUNCS:
         cases.append((f.__name__, lambda f=f: f(x, lab, num_labels=30, axis=-1)))
     for f in nb.OTHER_FUNCS:
         cases.append((f.__name__, lambda f=f: f(x, limit=7)))
-    for f in getattr(nb, "AGGREGATE_FUNCS", []):
+    for f in nb.AGGREGATION_FUNCS:
         cases.append((f.__name__, lambda f=f: f(x, axis=-1)))
+    cases.append(("factorize", lambda: nb.factorize(lab)[0]))
     return cases
 
 
