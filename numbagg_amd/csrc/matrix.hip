@@ -240,7 +240,8 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
     }
   }
 
-  // Lane l holds rows 4 (l / 16) + c, column l % 16 of each 16 x 16 block.
+  // Lane l holds rows 4 c + l / 16 (c = 0..3), column l % 16 of each 16 x 16
+  // block (the float64 MFMA result layout).
   if (a.splits == 1) {
     T* out = static_cast<T*>(a.out) + b * a.nv * a.nv;
 #pragma unroll
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
       for (int w = 0; w < TW; ++w)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const int64_t i = I * TS + u * 16 + 4 * q + c, j = J * TS + w * 16 + r;
+          const int64_t i = I * TS + u * 16 + 4 * c + q, j = J * TS + w * 16 + r;
           if (i >= a.nv || j >= a.nv || (I == J && i > j)) continue;
           const double Qi = CORR ? acc[4 % NP][u][w][c] : 0.0;
           const double Qj = CORR ? acc[5 % NP][u][w][c] : 0.0;
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
 #pragma unroll
         for (int w = 0; w < TW; ++w)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) part[(p * TS + u * 16 + 4 * q + c) * TS + w * 16 + r] = acc[p][u][w][c];
+          for (int c = 0; c < 4; ++c) part[(p * TS + u * 16 + 4 * c + q) * TS + w * 16 + r] = acc[p][u][w][c];
   }
 }
 
