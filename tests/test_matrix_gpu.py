@@ -96,7 +96,8 @@ def test_static_pandas_goldens(gpu_lib, golden, func, tag):
     got = getattr(nb, func)(golden[f"matrix_array_{tag}"])
     want = golden[f"{func}_{tag}"]
     assert np.array_equal(np.isnan(got), np.isnan(want))
-    assert_allclose(got, want, rtol=1e-12, atol=1e-15)
+    # pandas' own rounding: absolute in the data's scale (variances ~ 1/12)
+    assert_allclose(got, want, rtol=1e-12, atol=1e-13)
 
 
 @pytest.mark.parametrize("func", ["nancorrmatrix", "nancovmatrix"])
