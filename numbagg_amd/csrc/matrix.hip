@@ -55,6 +55,18 @@ __device__ __forceinline__ T mx_nan() {
   return canonical_nan<T>();
 }
 
+// 1/sqrt(v) from the hardware estimate and one Newton step (about 1 ULP of
+// float64), for v within [2^-1000, 2^1000].
+__device__ __forceinline__ double mx_rsqrt(double v) {
+  const double y = __builtin_amdgcn_rsq(v);
+  return y * fma(-(0.5 * v) * y, y, 1.5);
+}
+
+// cov / sqrt(vi vj) for vi, vj > 0.
+__device__ __forceinline__ double mx_corr(double cov, double vv) {
+  return (vv >= 0x1p-1000 && vv <= 0x1p1000) ? cov * mx_rsqrt(vv) : cov / sqrt(vv);
+}
+
 // ---------------------------------------------------------------------------
 // Shifts: the first valid value of each (outer, variable) series within
 // [lo, hi) of its time axis (0 when none is valid). One wave per series.
@@ -203,13 +215,22 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
 
   const int64_t k_lo = split * a.span;
   const int64_t k_hi = k_lo + a.span < a.nobs ? k_lo + a.span : a.nobs;
-  for (int64_t k0 = k_lo; k0 < k_hi; k0 += 16) {
-    const int64_t k = k0 + 4 * q;
-    T vi[TW][4], vj[TW][4];
+  // The next group's loads are issued before this group's MFMAs.
+  T vi[TW][4], vj[TW][4];
 #pragma unroll
-    for (int t = 0; t < TW; ++t) {
-      load4<T, VEC>(irow[t], k, k_hi, iok[t], vi[t]);
-      load4<T, VEC>(jrow[t], k, k_hi, jok[t], vj[t]);
+  for (int t = 0; t < TW; ++t) {
+    load4<T, VEC>(irow[t], k_lo + 4 * q, k_hi, iok[t], vi[t]);
+    load4<T, VEC>(jrow[t], k_lo + 4 * q, k_hi, jok[t], vj[t]);
+  }
+  for (int64_t k0 = k_lo; k0 < k_hi; k0 += 16) {
+    T ni[TW][4], nj[TW][4];
+    const bool more = k0 + 16 < k_hi;
+    if (more) {
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+        load4<T, VEC>(irow[t], k0 + 16 + 4 * q, k_hi, iok[t], ni[t]);
+        load4<T, VEC>(jrow[t], k0 + 16 + 4 * q, k_hi, jok[t], nj[t]);
+      }
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -236,6 +257,15 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
             acc[4][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(qi[u], mj[w], acc[4][u][w], 0, 0, 0);
             acc[5][u][w] = __builtin_amdgcn_mfma_f64_16x16x4f64(mi[u], qj[w], acc[5][u][w], 0, 0, 0);
           }
+        }
+    }
+    if (more) {
+#pragma unroll
+      for (int t = 0; t < TW; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          vi[t][e] = ni[t][e];
+          vj[t][e] = nj[t][e];
         }
     }
   }
@@ -318,7 +348,8 @@ StaticPlan static_plan(int64_t batch, int64_t nv, int64_t nobs, bool corr) {
   p.ntile = (nv + ts - 1) / ts;
   p.tiles = p.ntile * (p.ntile + 1) / 2;
   const int64_t items = batch * p.tiles;
-  int64_t splits = (kStaticTargetWaves + items - 1) / (items > 0 ? items : 1);
+  // Whole rounds of waves: at most the target, so no last round runs a few.
+  int64_t splits = knob("NBG_MATRIX_WAVES", kStaticTargetWaves) / (items > 0 ? items : 1);
   const int64_t max_splits = (nobs + kStaticMinSpan - 1) / kStaticMinSpan;
   if (splits > max_splits) splits = max_splits;
   if (splits < 1) splits = 1;
@@ -425,15 +456,19 @@ __global__ __launch_bounds__(kMxThreads) void mx_move_kernel(MxMoveArgs a) {
     const double dn = (double)n;
     if (CORR) {
       if (n >= mcorr) {
-        const double vi = qi - si * si / dn, vj = qj - sj * sj / dn;
-        if (vi > 0.0 && vj > 0.0) v = (diag ? vi : sp - si * sj / dn) / sqrt(vi * vj);
+        const double inv = 1.0 / dn;
+        const double vi = qi - si * si * inv, vj = qj - sj * sj * inv;
+        if (vi > 0.0 && vj > 0.0) v = diag ? 1.0 : mx_corr(sp - si * sj * inv, vi * vj);
       }
     } else if (n >= a.min_count && n > 1) {
-      v = (sp - si * sj / dn) / (dn - 1.0);
+      v = (dn * sp - si * sj) / (dn * (dn - 1.0));
     }
     out[t * np] = (T)v;
   }
 }
+
+// Waves the moving and exponential kernels aim for (8 per SIMD).
+constexpr int64_t kMoveTargetWaves = 8192;
 
 struct MovePlan {
   int64_t tile_len, tiles, pair_waves, groups, sub_len;
@@ -450,10 +485,12 @@ MovePlan move_plan(int64_t batch, int64_t nobs, int64_t nv, int64_t window) {
     p.groups = kWave / np;
     p.pair_waves = 1;
   }
-  // Time tiles: enough waves to fill the device, but every time range at
-  // least twice the window (the warm-up is repeated per range) and 64 steps.
+  // Time tiles: enough waves to fill the device with several per SIMD, but
+  // every time range at least twice the window (the warm-up is repeated per
+  // range) and 64 steps.
   const int64_t waves_per_tile = batch * p.pair_waves;
-  int64_t tiles = (kStaticTargetWaves + waves_per_tile - 1) / (waves_per_tile > 0 ? waves_per_tile : 1);
+  const int64_t target = knob("NBG_MATRIX_MOVE_WAVES", kMoveTargetWaves);
+  int64_t tiles = (target + waves_per_tile - 1) / (waves_per_tile > 0 ? waves_per_tile : 1);
   int64_t min_range = 2 * window > 64 ? 2 * window : 64;
   int64_t max_tiles = nobs / (min_range * p.groups);
   if (max_tiles < 1) max_tiles = 1;
@@ -581,15 +618,17 @@ __global__ __launch_bounds__(kMxThreads) void mx_exp_chunk_kernel(MxExpArgs a) {
       dp2 *= d * d;
       continue;
     }
-    const double bias = 1.0 - s[4] / (s[3] * s[3]);
-    const bool ok = s[5] >= mw && bias > 0.0 && seen >= 2;
     double v = __builtin_nan("");
-    if (ok) {
-      if (!CORR) {
-        v = (s[2] - s[0] * s[1] / s[3]) / s[3] / bias;
-      } else {
-        const double vi = s[6 % NS] - s[0] * s[0] / s[3], vj = s[7 % NS] - s[1] * s[1] / s[3];
-        if (vi > 0.0 && vj > 0.0) v = (diag ? vi : s[2] - s[0] * s[1] / s[3]) / sqrt(vi * vj);
+    if (s[5] >= mw && seen >= 2) {
+      const double r = 1.0 / s[3];
+      const double bias = 1.0 - s[4] * r * r;
+      if (bias > 0.0) {
+        if (!CORR) {
+          v = (s[2] - s[0] * s[1] * r) * r / bias;
+        } else {
+          const double vi = s[6 % NS] - s[0] * s[0] * r, vj = s[7 % NS] - s[1] * s[1] * r;
+          if (vi > 0.0 && vj > 0.0) v = diag ? 1.0 : mx_corr(s[2] - s[0] * s[1] * r, vi * vj);
+        }
       }
     }
     out[t * np] = (TO)v;
@@ -605,20 +644,62 @@ __global__ __launch_bounds__(kMxThreads) void mx_exp_chunk_kernel(MxExpArgs a) {
   }
 }
 
-// Thread per (batch row, pair): replaces each chunk's local state by the state
-// at the chunk's start.
+// One wave per (batch row, pair): replaces each chunk's local state by the
+// state at the chunk's start. A chunk's effect on the state is the affine map
+// s -> D s + L (D the chunk's decay product, its square for sw2). Each lane
+// composes the maps of a contiguous segment of chunks, a shuffle scan over the
+// wave composes the segments, and each lane then replays its segment from
+// the state its predecessors leave.
 template <bool CORR>
 __global__ __launch_bounds__(kMxThreads) void mx_exp_carry_kernel(MxExpArgs a) {
   constexpr int NS = ExpM<CORR>::NS, REC = ExpM<CORR>::REC;
   const int64_t np = a.nv * a.nv;
-  const int64_t e = (int64_t)blockIdx.x * kMxThreads + threadIdx.x;
-  if (e >= a.batch * np) return;
-  const int64_t b = e / np, qq = e % np;
-  double carry[NS];
+  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  if (wv >= a.batch * np) return;  // whole wave
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t b = wv / np, qq = wv % np;
+  const int64_t seg = (a.chunks + kWave - 1) / kWave;
+  const int64_t c0 = lane * seg < a.chunks ? lane * seg : a.chunks;
+  const int64_t c1 = c0 + seg < a.chunks ? c0 + seg : a.chunks;
+  double D = 1.0, D2 = 1.0, L[NS];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) carry[k] = 0.0;
+  for (int k = 0; k < NS; ++k) L[k] = 0.0;
   int seen = 0;
-  for (int64_t c = 0; c < a.chunks; ++c) {
+  for (int64_t c = c0; c < c1; ++c) {
+    const double* rec = a.state + ((b * a.chunks + c) * np + qq) * REC;
+    const double d = a.decay[(b * a.chunks + c) * 2], d2 = a.decay[(b * a.chunks + c) * 2 + 1];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) L[k] = (k == 4 ? d2 : d) * L[k] + rec[k];
+    D *= d;
+    D2 *= d2;
+    const int ls = (int)rec[NS];
+    seen = seen + ls < 2 ? seen + ls : 2;
+  }
+  // Inclusive scan: lane i ends with the composition of lanes 0..i.
+  for (int off = 1; off < kWave; off <<= 1) {
+    const double pD = __shfl_up(D, off), pD2 = __shfl_up(D2, off);
+    double pL[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) pL[k] = __shfl_up(L[k], off);
+    const int ps = __shfl_up(seen, off);
+    if (lane >= off) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) L[k] = (k == 4 ? D2 : D) * pL[k] + L[k];
+      D *= pD;
+      D2 *= pD2;
+      seen = seen + ps < 2 ? seen + ps : 2;
+    }
+  }
+  // Exclusive: the state entering this lane's segment.
+  double s[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const double v = __shfl_up(L[k], 1);
+    s[k] = lane > 0 ? v : 0.0;
+  }
+  const int vs = __shfl_up(seen, 1);
+  seen = lane > 0 ? vs : 0;
+  for (int64_t c = c0; c < c1; ++c) {
     double* rec = a.state + ((b * a.chunks + c) * np + qq) * REC;
     const double d = a.decay[(b * a.chunks + c) * 2], d2 = a.decay[(b * a.chunks + c) * 2 + 1];
     double loc[NS];
@@ -626,10 +707,10 @@ __global__ __launch_bounds__(kMxThreads) void mx_exp_carry_kernel(MxExpArgs a) {
     for (int k = 0; k < NS; ++k) loc[k] = rec[k];
     const int ls = (int)rec[NS];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) rec[k] = carry[k];
+    for (int k = 0; k < NS; ++k) rec[k] = s[k];
     rec[NS] = (double)seen;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) carry[k] = (k == 4 ? d2 : d) * carry[k] + loc[k];
+    for (int k = 0; k < NS; ++k) s[k] = (k == 4 ? d2 : d) * s[k] + loc[k];
     seen = seen + ls < 2 ? seen + ls : 2;
   }
 }
@@ -649,9 +730,11 @@ ExpPlan exp_plan(int64_t batch, int64_t nobs, int64_t nv, bool corr) {
     p.groups = kWave / np;
     p.pair_waves = 1;
   }
-  // Chunks: enough lanes busy to fill the device, at least 64 steps each.
+  // Chunks: enough lanes busy to fill the device with several waves per SIMD,
+  // at least 64 steps each.
   const int64_t lanes_per_chunk = batch * (p.groups == 1 ? p.pair_waves * kWave : np);
-  int64_t chunks = (kStaticTargetWaves * kWave + lanes_per_chunk - 1) / (lanes_per_chunk > 0 ? lanes_per_chunk : 1);
+  const int64_t target = knob("NBG_MATRIX_MOVE_WAVES", kMoveTargetWaves) * kWave;
+  int64_t chunks = (target + lanes_per_chunk - 1) / (lanes_per_chunk > 0 ? lanes_per_chunk : 1);
   int64_t max_chunks = nobs / 64;
   if (max_chunks < 1) max_chunks = 1;
   if (chunks > max_chunks) chunks = max_chunks;
@@ -742,7 +825,7 @@ int exp_launch(bool corr, const void* x, const void* alpha, int64_t ld_alpha, do
   MxExpArgs a{x, alpha, out, shift, state, decay, ld_alpha, min_weight, batch, nobs, nv,
               p.chunk, p.chunks, p.pair_waves, p.groups};
   const int grid = mx_grid(p.waves);
-  const unsigned carry_grid = (unsigned)((batch * nv * nv + kMxThreads - 1) / kMxThreads);
+  const int carry_grid = mx_grid(batch * nv * nv);
   if (corr) {
     hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, true, false>), dim3(grid), dim3(kMxThreads), 0, s, a);
     hipLaunchKernelGGL((mx_exp_carry_kernel<true>), dim3(carry_grid), dim3(kMxThreads), 0, s, a);
