@@ -38,6 +38,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 matrix, TFLOP/s
 DEV = "cuda"
 METRIC = "GB/s & % HBM peak (move_mean, group_nansum, ffill); ragged-batched vs plain x"
 
@@ -103,9 +104,10 @@ def gen_group(torch, rows, n, num_labels, seed, p_nan=0.05, zipf=None):
 # workloads
 # --------------------------------------------------------------------------
 class Workload:
-    def __init__(self, name, config, nbytes, run, cpu=None, kernel=None):
+    def __init__(self, name, config, nbytes, run, cpu=None, kernel=None, flops=None):
         self.name, self.config, self.bytes, self.run, self.cpu = name, config, nbytes, run, cpu
         self.kernel = kernel or name
+        self.flops = flops  # algorithmic float64 FLOPs of an MFMA-bound workload
 
 
 def tune_key(w):
@@ -263,6 +265,8 @@ def build_extra(torch, nb, wl):
             keys = torch.cat([lab.to(torch.int64) * 7919 - 500_000] * 4).contiguous()
             fcfg = {"shape": [keys.numel()], "dtype": "int64", "uniques": 1000, "sort": True}
             extra.append(Workload("factorize", fcfg, keys.numel() * 16, lambda k=keys: nb.factorize(k, sort=True)))
+    if hasattr(nb, "nancorrmatrix"):
+        extra += matrix_workloads(torch, nb)
     src = [w.copy_src for w in wl if hasattr(w, "copy_src")]
     if src:
         x = src[0]
@@ -271,6 +275,43 @@ def build_extra(torch, nb, wl):
         extra.append(Workload("ffill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda x=x: nb.ffill(x, limit=16)))
         extra.append(Workload("bfill[limit=16]", dict(cfg, limit=16), x.numel() * 8, lambda x=x: nb.bfill(x, limit=16)))
     return extra
+
+
+def matrix_workloads(torch, nb):
+    """Correlation / covariance matrices: static (256 vars, 1M obs) float32 on
+    the float64 matrix cores (2 K P FLOPs per product over the P = V (V + 1) / 2
+    pairs: 6 products for corr, 4 for cov), a batch of 30000 5 x 5 matrices of
+    1000 obs (the reference's (3000, 5, 1000) benchmark shape, 10x), and the
+    moving / exponential matrices of (200000 obs, 32 vars) float32 (outputs of
+    200000 x 32 x 32)."""
+    g = torch.Generator(device=DEV).manual_seed(31)
+
+    def data(shape):
+        x = torch.randn(shape, generator=g, device=DEV, dtype=torch.float32)
+        x[torch.rand(shape, generator=g, device=DEV) < 0.02] = float("nan")
+        return x
+
+    out = []
+    big = data((256, 1_000_000))
+    pairs = 256 * 257 // 2
+    for f, nprod in (("nancorrmatrix", 6), ("nancovmatrix", 4)):
+        out.append(Workload(f"{f}[256x1M]", {"shape": [256, 1_000_000], "dtype": "f32", "nan": 0.02},
+                            big.numel() * 4 + 256 * 256 * 4, (lambda f=f, x=big: getattr(nb, f)(x)),
+                            flops=2 * 1_000_000 * pairs * nprod))
+    small = data((30000, 5, 1000))
+    for f in ("nancorrmatrix", "nancovmatrix"):
+        out.append(Workload(f"{f}[30000x5x1000]", {"shape": [30000, 5, 1000], "dtype": "f32", "nan": 0.02},
+                            small.numel() * 4 + 30000 * 25 * 4, (lambda f=f, x=small: getattr(nb, f)(x))))
+    mv = data((200_000, 32))
+    cfg = {"shape": [200_000, 32], "dtype": "f32", "window": 20}
+    for f in ("move_corrmatrix", "move_covmatrix"):
+        out.append(Workload(f"{f}[w=20]", cfg, mv.numel() * 4 + 200_000 * 32 * 32 * 4,
+                            (lambda f=f, x=mv: getattr(nb, f)(x, window=20))))
+    ecfg = {"shape": [200_000, 32], "dtype": "f32 in, f64 out (Python-float alpha)", "alpha": 0.5}
+    for f in ("move_exp_nancorrmatrix", "move_exp_nancovmatrix"):
+        out.append(Workload(f, ecfg, mv.numel() * 4 + 200_000 * 32 * 32 * 8,
+                            (lambda f=f, x=mv: getattr(nb, f)(x, alpha=0.5))))
+    return out
 
 
 def time_workload(torch, w, steps, stream):
@@ -542,10 +583,18 @@ def main(argv=None):
         for w in extras:
             ms = time_workload(torch, w, max(3, args.steps), stream)
             gbs = w.bytes / (ms * 1e-3) / 1e9
-            all_rec[w.name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
-                               "bytes": w.bytes, "config": w.config}
-            print(f"# all {w.name}: {ms:.3f} ms  {gbs:.0f} GB/s  {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak",
-                  file=sys.stderr, flush=True)
+            if w.flops:
+                tfs = w.flops / (ms * 1e-3) / 1e12
+                all_rec[w.name] = {"ms": round(ms, 4), "TFLOP/s": round(tfs, 2), "bound": "mfma fp64",
+                                   "frac": round(tfs / FP64_PEAK_TFS, 4), "flops": w.flops, "GB/s": round(gbs, 1),
+                                   "bytes": w.bytes, "config": w.config}
+                print(f"# all {w.name}: {ms:.3f} ms  {tfs:.1f} TFLOP/s  {100 * tfs / FP64_PEAK_TFS:.1f}% of FP64 "
+                      f"matrix peak  ({gbs:.0f} GB/s)", file=sys.stderr, flush=True)
+            else:
+                all_rec[w.name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                   "bytes": w.bytes, "config": w.config}
+                print(f"# all {w.name}: {ms:.3f} ms  {gbs:.0f} GB/s  {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak",
+                      file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
     ragged_rec = None
