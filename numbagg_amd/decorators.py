@@ -79,6 +79,43 @@ class _Op:
         if kwargs:
             raise TypeError(f"unsupported keyword arguments: {sorted(kwargs)}")
 
+    @classmethod
+    def _take_out(cls, kwargs: dict):
+        """The gufunc ``out=`` argument the reference forwards with **kwargs
+        (numbagg/decorators.py:341, :414, :487, :731, :807, :1089): an array
+        (or a 1-tuple holding one) that receives the result."""
+        dest = kwargs.pop("out", None)
+        cls._no_extra(kwargs)
+        if isinstance(dest, tuple):
+            if len(dest) != 1:
+                raise ValueError("The 'out' tuple must have exactly one entry")
+            dest = dest[0]
+        if dest is not None and not isinstance(dest, (np.ndarray, torch.Tensor)):
+            raise TypeError("return arrays must be of ArrayType")
+        return dest
+
+    def _deliver(self, res, dest):
+        """Write ``res`` into ``dest`` as NumPy's gufunc output does (exact
+        shape, casting 'same_kind') and return ``dest``."""
+        if dest is None:
+            return res
+        if isinstance(dest, torch.Tensor):
+            r = res if isinstance(res, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(res))
+            if tuple(dest.shape) != tuple(r.shape):
+                raise ValueError(f"output operand with shape {tuple(dest.shape)} does not match the result shape "
+                                 f"{tuple(r.shape)}")
+            dest.copy_(r)
+            return dest
+        r = res.cpu().numpy() if isinstance(res, torch.Tensor) else np.asarray(res)
+        if dest.shape != r.shape:
+            raise ValueError(f"non-broadcastable output operand with shape {dest.shape} doesn't match the "
+                             f"broadcast shape {r.shape}")
+        if not np.can_cast(r.dtype, dest.dtype, casting="same_kind"):
+            raise TypeError(f"Cannot cast ufunc '{self.__name__}' output from {r.dtype!r} to {dest.dtype!r} with "
+                            "casting rule 'same_kind'")
+        np.copyto(dest, r, casting="same_kind")
+        return dest
+
 
 def _single_axis(axis, arr: tuple, func, capital: bool):
     """Axis-tuple rules shared by ndmove/ndmoveexp (decorators.py:318-332, :385-396)."""
@@ -137,7 +174,7 @@ class ndmove(_Op):
         axis: int | tuple[int, ...] = -1,
         **kwargs,
     ):
-        self._no_extra(kwargs)
+        dest = self._take_out(kwargs)
         if min_count is None:
             min_count = window
         elif min_count < 0:
@@ -170,7 +207,7 @@ class ndmove(_Op):
                 _lib.call(self.entry, *args)
         # NumPy places the output's core dimension at `axis` as well.
         res = out.view(*outer, n).movedim(-1, axis)
-        return dev.to_output(res, as_tensor)
+        return self._deliver(dev.to_output(res, as_tensor), dest)
 
 
 class ndmoveexp(_Op):
@@ -194,7 +231,7 @@ class ndmoveexp(_Op):
         axis: int | tuple[int, ...] = -1,
         **kwargs,
     ):
-        self._no_extra(kwargs)
+        dest = self._take_out(kwargs)
         axis = _single_axis(axis, arr, self.func, capital=True)
         if axis is None:
             return arr[0]
@@ -260,7 +297,7 @@ class ndmoveexp(_Op):
         del alpha_keep
         # NumPy places the output's core dimension at `axis` as well.
         res = out.view(*outer, n).movedim(-1, axis)
-        return dev.to_output(res, as_tensor)
+        return self._deliver(dev.to_output(res, as_tensor), dest)
 
 
 class ndfill(_Op):
@@ -273,7 +310,7 @@ class ndfill(_Op):
     """
 
     def __call__(self, arr, *, limit: None | int = None, axis: int = -1, **kwargs):
-        self._no_extra(kwargs)
+        dest = self._take_out(kwargs)
         shape = _shape(arr)
         if limit is None:
             limit = shape[axis]
@@ -285,7 +322,7 @@ class ndfill(_Op):
         as_tensor = dev.is_tensor(arr)
         if not np.issubdtype(dt, np.floating):
             t = dev.to_device(arr)
-            return dev.to_output(t.clone(), as_tensor)
+            return self._deliver(dev.to_output(t.clone(), as_tensor), dest)
         # float16 rows are filled through float32, which round-trips exactly.
         work = np.dtype(np.float32) if dt == np.float16 else dt
         t = dev.to_device(arr, work)
@@ -314,7 +351,7 @@ class ndfill(_Op):
         res = out.view(moved.shape).movedim(-1, axis)
         if work != dt:
             res = res.to(dev.torch_dtype(dt))
-        return dev.to_output(res, as_tensor)
+        return self._deliver(dev.to_output(res, as_tensor), dest)
 
 
 # Group-by strategy codes of the C ABI (nbg_group_*).
@@ -629,7 +666,7 @@ class ndmatrix(_Op):
     def __call__(self, a, **kwargs):
         if "axis" in kwargs or "axes" in kwargs:
             raise TypeError(f"{self.__name__}() got an unexpected keyword argument 'axis'")
-        self._no_extra(kwargs)
+        dest = self._take_out(kwargs)
         if len(_shape(a)) < 2:
             raise ValueError(
                 f"{self.__name__} requires at least a 2D array with shape (..., vars, obs). "
@@ -649,7 +686,7 @@ class ndmatrix(_Op):
                 ws, ws_bytes = dev.workspace(ws_bytes, t.device)
                 _lib.call(self.entry, ctypes.c_int(dev.nbg_code(out)), dev.ptr(x), dev.ptr(out), batch, nv, nobs,
                           dev.ptr(ws), ws_bytes, dev.stream_handle())
-        return dev.to_output(out.view(*outer, nv, nv), as_tensor)
+        return self._deliver(dev.to_output(out.view(*outer, nv, nv), as_tensor), dest)
 
 
 class ndmovematrix(_Op):
@@ -661,7 +698,7 @@ class ndmovematrix(_Op):
     """
 
     def __call__(self, a, window: int, min_count: int | None = None, **kwargs):
-        self._no_extra(kwargs)
+        dest = self._take_out(kwargs)
         shape = _shape(a)
         if len(shape) < 2:
             raise ValueError(f"{self.__name__} requires at least a 2D array with shape (..., obs, vars).")
@@ -686,7 +723,7 @@ class ndmovematrix(_Op):
             ws, ws_bytes = dev.workspace(ws_bytes, t.device)
             _lib.call(self.entry, ctypes.c_int(dev.nbg_code(out)), dev.ptr(x), dev.ptr(out), batch, nobs, nv,
                       int(window), int(min_count), dev.ptr(ws), ws_bytes, dev.stream_handle())
-        return dev.to_output(out.view(*outer, nobs, nv, nv), as_tensor)
+        return self._deliver(dev.to_output(out.view(*outer, nobs, nv, nv), as_tensor), dest)
 
 
 class ndmoveexpmatrix(_Op):
@@ -700,7 +737,7 @@ class ndmoveexpmatrix(_Op):
     """
 
     def __call__(self, a, alpha, min_weight: float = 0, **kwargs):
-        self._no_extra(kwargs)
+        dest = self._take_out(kwargs)
         shape = _shape(a)
         if len(shape) < 2:
             raise ValueError(f"{self.__name__} requires at least a 2D array with shape (..., obs, vars).")
@@ -743,4 +780,4 @@ class ndmoveexpmatrix(_Op):
             _lib.call(self.entry, ctypes.c_int(code_in), ctypes.c_int(dev.nbg_code(out)), dev.ptr(x),
                       dev.ptr(alpha_t), ld_alpha, ctypes.c_double(mw), dev.ptr(out), batch, nobs, nv, dev.ptr(ws),
                       ws_bytes, dev.stream_handle())
-        return dev.to_output(out.view(*outer, nobs, nv, nv), as_tensor)
+        return self._deliver(dev.to_output(out.view(*outer, nobs, nv, nv), as_tensor), dest)

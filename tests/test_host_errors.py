@@ -85,3 +85,31 @@ def test_spin_timeout_increase_raises(monkeypatch):
     with pytest.raises(RuntimeError, match="wait"):
         _device.check_spin_timeouts()
     _device.check_spin_timeouts()  # no new expiry since the last check
+
+
+def test_out_argument_delivery():
+    """out= as the reference's gufuncs take it (numbagg/decorators.py:341, :414,
+    :487, :731, :807, :1089): exact shape, casting 'same_kind', the out array
+    returned; other keywords are rejected."""
+    res = np.arange(6.0).reshape(2, 3)
+    dest = np.empty((2, 3), dtype=np.float32)
+    assert nb.move_mean._deliver(res, dest) is dest
+    np.testing.assert_array_equal(dest, res.astype(np.float32))
+    assert nb.ffill._deliver(res, (dest,)[0]) is dest
+    with pytest.raises(TypeError, match="same_kind"):
+        nb.move_mean._deliver(res, np.empty((2, 3), dtype=np.int64))
+    with pytest.raises(ValueError, match="broadcast shape"):
+        nb.move_mean._deliver(res, np.empty((3, 2)))
+    assert nb.move_mean._take_out({"out": (dest,)}) is dest
+    with pytest.raises(ValueError, match="exactly one"):
+        nb.move_mean._take_out({"out": (dest, dest)})
+    with pytest.raises(TypeError, match="ArrayType"):
+        nb.move_mean._take_out({"out": [1, 2]})
+    with pytest.raises(TypeError, match="unsupported keyword"):
+        nb.move_mean._take_out({"out": dest, "bogus": 1})
+    with pytest.raises(TypeError):
+        nb.nancorrmatrix(np.ones((3, 4)), axis=0)
+    with pytest.raises(ValueError, match="requires at least a 2D array"):
+        nb.move_covmatrix(np.ones(5), window=2)
+    with pytest.raises(ValueError, match="window not in valid range"):
+        nb.move_corrmatrix(np.ones((5, 2)), window=6)

@@ -228,3 +228,29 @@ def test_move_f32_twosum_accumulator(gpu_lib, monkeypatch, func, shape, window, 
     x = walk_array(*shape, seed=window + shape[1] + 5)
     got = getattr(nb, func)(x, window=window, min_count=min_count)
     check_f32(func, got, exact.move(func, x, window, min_count))
+
+
+def test_out_argument(gpu_lib):
+    """out= is filled and returned (numbagg/decorators.py:341, :414, :487)."""
+    import numbagg_amd as nb
+
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((3, 200))
+    want = nb.move_mean(x, window=5)
+    dest = np.empty_like(x)
+    assert nb.move_mean(x, window=5, out=dest) is dest
+    np.testing.assert_array_equal(dest, want)
+    d32 = np.empty(x.shape, dtype=np.float32)
+    assert nb.move_exp_nanmean(x, alpha=0.3, out=d32) is d32
+    np.testing.assert_array_equal(d32, nb.move_exp_nanmean(x, alpha=0.3).astype(np.float32))
+    xt = np.ascontiguousarray(x.T)
+    dm = np.empty((200, 3, 3))
+    assert nb.move_covmatrix(xt, window=10, out=dm) is dm
+    np.testing.assert_array_equal(dm, nb.move_covmatrix(xt, window=10))
+    df = np.empty_like(x)
+    xn = x.copy()
+    xn[:, ::3] = np.nan
+    assert nb.ffill(xn, out=df) is df
+    np.testing.assert_array_equal(df, nb.ffill(xn))
+    with pytest.raises(ValueError):
+        nb.move_mean(x, window=5, out=np.empty((3, 100)))
