@@ -512,13 +512,13 @@ def main(argv=None):
     print(f"# device: {torch.cuda.get_device_name(0)}", file=sys.stderr, flush=True)
     want_all = not args.headline
     wl = build_workloads(torch, nb, lib, want_all)
-    if args.only:
-        keep = set(args.only.split(","))
+    extras = build_extra(torch, nb, wl) if want_all else []
+    keep = set(args.only.split(",")) if args.only else None
+    if keep is not None:
         wl = [w for w in wl if w.name in keep]
+        extras = [w for w in extras if w.name in keep]
     torch.cuda.synchronize()
     print(f"# inputs ready: {[w.name for w in wl]}", file=sys.stderr, flush=True)
-
-    extras = build_extra(torch, nb, wl) if want_all and not args.only else []
     if args.tune:
         autotune(wl + extras)
 
@@ -578,7 +578,7 @@ def main(argv=None):
         print(f"# copy reference: {copy_ms:.3f} ms  {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     all_rec = None
-    if want_all and not args.only:
+    if want_all and extras:
         all_rec = {}
         for w in extras:
             ms = time_workload(torch, w, max(3, args.steps), stream)
@@ -598,14 +598,14 @@ def main(argv=None):
         torch.cuda.empty_cache()
 
     ragged_rec = None
-    if not args.no_ragged:
+    if not args.no_ragged and (keep is None or "ragged" in keep):
         ragged_rec = bench_ragged(torch, nb, max(3, args.steps), judge=args.judge)
         print(f"# ragged: {ragged_rec['ms']:.3f} ms ({ragged_rec['GB/s']:.0f} GB/s), plain launches "
               f"{ragged_rec['plain_launches_ms']:.0f} ms -> {ragged_rec['speedup_vs_plain']}x", file=sys.stderr, flush=True)
 
     ms_step = wall * 1e3 / args.steps
     total_bytes = sum(w.bytes for w in wl)
-    value = total_bytes / (ms_step * 1e-3) / 1e9
+    value = total_bytes / (ms_step * 1e-3) / 1e9 if wl and ms_step > 0 else 0.0
     dom = max(wl, key=lambda w: per[w.name]["ms"]) if wl else None
     dom_ach = per[dom.name]["GB/s"] if dom else None
 
