@@ -114,7 +114,9 @@ def tune_key(w):
     """(function, loop dtype, rows, length, window or label count) of a
     workload's call: the shape class the plain-kernel autotuner records."""
     func = w.name.split("[")[0]
-    shape = w.config.get("shape", [1, 1])
+    shape = list(w.config.get("shape", [1, 1]))
+    if len(shape) < 2:
+        shape = [1] + shape
     rows, n = int(shape[0]), int(shape[1])
     if func.startswith("move_exp_"):
         return func, "float64", rows, n, 0
@@ -154,6 +156,8 @@ def tuned_settings(workloads):
     out = {}
     for w in workloads:
         func, dtype, rows, n, extra = tune_key(w)
+        if plain_tuning.family_of(func) is None:
+            continue
         out[w.name] = db.get(plain_tuning.shape_key(arch, func, dtype, rows, n, extra))
     return {"kernel_versions": db.versions, "entries": len(db.entries), "applied": out}
 

@@ -88,11 +88,17 @@ __global__ __launch_bounds__(kFzThreads) void fz_insert_kernel(const T* __restri
       atomicMin(A.minus_one_first, (unsigned long long)i);
       continue;
     }
+    // The table holds at most half its slots, so a free slot is found. A
+    // slot changes only once (empty -> key) and a first position only
+    // decreases, so plain reads that see an older value are safe: the common
+    // case (the key is present, with an earlier first position) then takes
+    // no atomic, which keeps labels with many repeats free of contention.
     unsigned long long s = mix64(k) & A.mask;
-    for (;;) {  // the table holds at most half its slots, so a free slot is found
-      const unsigned long long old = atomicCAS(&A.keys[s], kEmpty, k);
-      if (old == kEmpty || old == k) {
-        atomicMin(&A.first[s], (unsigned long long)i);
+    for (;;) {
+      unsigned long long cur = A.keys[s];
+      if (cur == kEmpty) cur = atomicCAS(&A.keys[s], kEmpty, k);
+      if (cur == kEmpty || cur == k) {
+        if ((unsigned long long)i < A.first[s]) atomicMin(&A.first[s], (unsigned long long)i);
         break;
       }
       s = (s + 1) & A.mask;
@@ -137,6 +143,44 @@ __global__ void fz_pad_kernel(FzArgs A, int64_t k_total, int by_value) {
       A.skey[p] = kEmpty;
       A.sval[p] = kEmpty - 1;
     }
+  }
+}
+
+// The whole bitonic network in one workgroup's LDS, for lists of at most
+// kFzLdsSort entries (the common case of few uniques): one launch instead of
+// log2(padded) (log2(padded) + 1) / 2.
+constexpr int kFzLdsSort = 4096;
+constexpr int kFzSortThreads = 1024;
+
+__global__ __launch_bounds__(kFzSortThreads) void fz_lds_sort_kernel(unsigned long long* key, unsigned long long* val,
+                                                                      int padded) {
+  __shared__ unsigned long long sk[kFzLdsSort], sv[kFzLdsSort];
+  for (int i = threadIdx.x; i < padded; i += kFzSortThreads) {
+    sk[i] = key[i];
+    sv[i] = val[i];
+  }
+  __syncthreads();
+  for (int k = 2; k <= padded; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < padded; i += kFzSortThreads) {
+        const int p = i ^ j;
+        if (p <= i) continue;
+        const bool up = (i & k) == 0;
+        const unsigned long long a = sk[i], b = sk[p];
+        const bool greater = a > b || (a == b && sv[i] > sv[p]);
+        if (greater == up) {
+          sk[i] = b;
+          sk[p] = a;
+          const unsigned long long t = sv[i];
+          sv[i] = sv[p];
+          sv[p] = t;
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < padded; i += kFzSortThreads) {
+    key[i] = sk[i];
+    val[i] = sv[i];
   }
 }
 
@@ -321,10 +365,14 @@ int nbg_factorize_finish(int dtype, const void* values, int64_t n, int64_t num_u
   do {                                                                                                        \
     hipLaunchKernelGGL((nbg::fz_pad_kernel<T>), dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0, s, a,   \
                        num_uniques, sort ? 1 : 0);                                                            \
-    for (int64_t k = 2; k <= a.padded; k <<= 1)                                                               \
-      for (int64_t j = k >> 1; j > 0; j >>= 1)                                                                \
-        hipLaunchKernelGGL(nbg::fz_bitonic_kernel, dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0, s,  \
-                           a.skey, a.sval, a.padded, j, k);                                                   \
+    if (a.padded <= nbg::kFzLdsSort)                                                                          \
+      hipLaunchKernelGGL(nbg::fz_lds_sort_kernel, dim3(1), dim3(nbg::kFzSortThreads), 0, s, a.skey, a.sval,     \
+                         (int)a.padded);                                                                      \
+    else                                                                                                      \
+      for (int64_t k = 2; k <= a.padded; k <<= 1)                                                             \
+        for (int64_t j = k >> 1; j > 0; j >>= 1)                                                              \
+          hipLaunchKernelGGL(nbg::fz_bitonic_kernel, dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0,   \
+                             s, a.skey, a.sval, a.padded, j, k);                                              \
     hipLaunchKernelGGL((nbg::fz_rank_kernel<T>), dim3(nbg::fz_grid(num_uniques)), dim3(nbg::kFzThreads), 0, s,  \
                        (const T*)values, a, num_uniques, (T*)uniques, minus_one_rank);                        \
     hipLaunchKernelGGL((nbg::fz_codes_kernel<T>), dim3(nbg::fz_grid(n)), dim3(nbg::kFzThreads), 0, s,           \

@@ -51,6 +51,8 @@ GROUP_SUM_FUNCS = {"group_nansum", "group_nanmean"}
 
 
 def family_of(func: str) -> str | None:
+    if func.endswith("matrix"):  # the matrix kernels have no tuning knobs here
+        return None
     if func.startswith("move_exp_"):
         return "exp"
     if func.startswith("move_"):

@@ -68,6 +68,9 @@ def test_candidates_and_families():
     assert plain_tuning.family_of("group_nansum") == "group_sum"
     assert plain_tuning.family_of("group_nanargmax") == "group"
     assert plain_tuning.family_of("ffill") == "fill"
+    assert plain_tuning.family_of("move_corrmatrix") is None
+    assert plain_tuning.family_of("move_exp_nancovmatrix") is None
+    assert plain_tuning.family_of("nansum") is None
     assert len(plain_tuning.candidates("group_sum")) == 3 * 2 * 4
     assert {"NBG_FILL_TWO_PASS": 0} in plain_tuning.candidates("fill")
 
