@@ -40,6 +40,13 @@ HIPCC_FLAGS = [
 ]
 
 
+# Extra flags per source. matrix.hip: MFMA results and accumulators in
+# ordinary VGPRs (gfx950 accepts either form); with the accumulation registers
+# the compiler copied every accumulator in and out of them on each loop
+# iteration.
+SOURCE_FLAGS = {"matrix.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def hipcc() -> str:
     path = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not os.path.exists(path):
@@ -49,7 +56,8 @@ def hipcc() -> str:
 
 def _compile(src: pathlib.Path) -> pathlib.Path:
     obj = OBJ_DIR / (src.stem + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT_DIR / "include"), "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(src.name, []), "-I", str(ROOT_DIR / "include"), "-c", str(src),
+           "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{proc.stderr}")

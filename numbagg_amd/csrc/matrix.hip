@@ -55,6 +55,12 @@ __device__ __forceinline__ T mx_nan() {
   return canonical_nan<T>();
 }
 
+// This wave's index in its workgroup, as a scalar: work-item indices derived
+// from it (and loop bounds derived from those) are then known to be uniform
+// across the wave, so loops branch on scalars and the MFMA accumulators stay
+// in their registers between iterations.
+__device__ __forceinline__ int mx_wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // 1/sqrt(v) from the hardware estimate and one Newton step (about 1 ULP of
 // float64), for v within [2^-1000, 2^1000].
 __device__ __forceinline__ double mx_rsqrt(double v) {
@@ -87,7 +93,7 @@ template <typename T>
 __global__ __launch_bounds__(kMxThreads) void mx_shift_kernel(const T* __restrict__ x, double* __restrict__ shift,
                                                              MxShiftArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t s = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  const int64_t s = (int64_t)blockIdx.x * kMxWaves + mx_wave_id();
   if (s >= a.outer * a.nv) return;  // whole wave
   const int64_t o = s / a.nv, v = s % a.nv;
   const int64_t b = o / a.tiles, tile = o % a.tiles;
@@ -181,7 +187,7 @@ template <typename T, bool CORR, int TW, bool VEC>
 __global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
   constexpr int NP = StaticSpec<CORR>::NPROD;
   constexpr int TS = 16 * TW;
-  const int64_t item = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  const int64_t item = (int64_t)blockIdx.x * kMxWaves + mx_wave_id();
   if (item >= a.batch * a.tiles * a.splits) return;  // whole wave
   const int lane = threadIdx.x & (kWave - 1);
   const int r = lane & 15, q = lane >> 4;
@@ -337,13 +343,16 @@ struct StaticPlan {
 };
 
 // Waves the static kernel aims for (several per SIMD of 256 CUs).
-constexpr int64_t kStaticTargetWaves = 2048;
+constexpr int64_t kStaticTargetWaves = 4096;
 // Fewest observations a split covers (keeps partials small next to the input).
 constexpr int64_t kStaticMinSpan = 4096;
 
 StaticPlan static_plan(int64_t batch, int64_t nv, int64_t nobs, bool corr) {
   StaticPlan p{};
-  p.tw = nv > 16 ? 2 : 1;
+  // 16 x 16 wave tiles by default: a 32 x 32 tile's 96 (corr) float64
+  // accumulators do not fit beside its operands without spills (measured on
+  // MI355X, float32 (256, 1M) corr: 10.9 ms at 32 x 32). Knob NBG_MATRIX_TW.
+  p.tw = knob("NBG_MATRIX_TW", 1) == 2 && nv > 16 ? 2 : 1;
   const int64_t ts = 16 * p.tw;
   p.ntile = (nv + ts - 1) / ts;
   p.tiles = p.ntile * (p.ntile + 1) / 2;
@@ -386,9 +395,46 @@ struct MoveState {
   int64_t n;
 };
 
+// Window sums of one pair: add or remove the observation (xi, xj).
 template <typename T, bool CORR>
+struct PairSums {
+  double si = 0.0, sj = 0.0, sp = 0.0, qi = 0.0, qj = 0.0;
+  int64_t n = 0;
+  __device__ __forceinline__ void add(T xi, T xj, double ci, double cj, double sign) {
+    if (xi == xi && xj == xj) {
+      const double yi = (double)xi - ci, yj = (double)xj - cj;
+      si += sign * yi;
+      sj += sign * yj;
+      sp += sign * (yi * yj);
+      if (CORR) {
+        qi += sign * (yi * yi);
+        qj += sign * (yj * yj);
+      }
+      n += sign > 0.0 ? 1 : -1;
+    }
+  }
+  // The output for the current window, in forms scaled by n so that neither
+  // needs a division by n: corr = (n sp - si sj) / sqrt((n qi - si^2)(n qj -
+  // sj^2)); cov = (n sp - si sj) / (n (n - 1)).
+  __device__ __forceinline__ double value(int64_t min_count, int64_t mcorr, bool diag) const {
+    const double dn = (double)n;
+    if (CORR) {
+      if (n < mcorr) return __builtin_nan("");
+      const double vi = dn * qi - si * si, vj = dn * qj - sj * sj;
+      if (!(vi > 0.0 && vj > 0.0)) return __builtin_nan("");
+      return diag ? 1.0 : mx_corr(dn * sp - si * sj, vi * vj);
+    }
+    if (n < min_count || n < 2) return __builtin_nan("");
+    return (dn * sp - si * sj) / (dn * (dn - 1.0));
+  }
+};
+
+// IT: the index type of element offsets within one batch row's input and
+// output (32-bit when both fit, so addresses are a scalar base plus a 32-bit
+// lane offset).
+template <typename T, bool CORR, typename IT>
 __global__ __launch_bounds__(kMxThreads) void mx_move_kernel(MxMoveArgs a) {
-  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + mx_wave_id();
   if (wv >= a.batch * a.tiles * a.pair_waves) return;  // whole wave
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t pw = wv % a.pair_waves;
@@ -413,57 +459,28 @@ __global__ __launch_bounds__(kMxThreads) void mx_move_kernel(MxMoveArgs a) {
   if (t_end > tile_hi) t_end = tile_hi;
   if (t_start >= t_end) return;
   const int64_t t_w = t_start - a.window + 1 > 0 ? t_start - a.window + 1 : 0;
+  // removals start once a whole window lies after t_w
+  const int64_t t_r = t_w + a.window > t_start ? t_w + a.window : t_start;
   const T* x = static_cast<const T*>(a.x) + b * a.nobs * a.nv;
-  T* out = static_cast<T*>(a.out) + b * a.nobs * np + qq;
+  T* out = static_cast<T*>(a.out) + b * a.nobs * np;
   const double ci = a.shift[(b * a.tiles + tile) * a.nv + i];
   const double cj = a.shift[(b * a.tiles + tile) * a.nv + j];
   const bool diag = i == j;
   const int64_t mcorr = a.min_count > 2 ? a.min_count : 2;
-  double si = 0.0, sj = 0.0, sp = 0.0, qi = 0.0, qj = 0.0;
-  int64_t n = 0;
-  for (int64_t t = t_w; t < t_end; ++t) {
-    {
-      const T xi = x[t * a.nv + i], xj = x[t * a.nv + j];
-      if (xi == xi && xj == xj) {
-        const double yi = (double)xi - ci, yj = (double)xj - cj;
-        si += yi;
-        sj += yj;
-        sp += yi * yj;
-        if (CORR) {
-          qi += yi * yi;
-          qj += yj * yj;
-        }
-        n += 1;
-      }
-    }
-    if (t - a.window >= t_w) {
-      const int64_t to = t - a.window;
-      const T xi = x[to * a.nv + i], xj = x[to * a.nv + j];
-      if (xi == xi && xj == xj) {
-        const double yi = (double)xi - ci, yj = (double)xj - cj;
-        si -= yi;
-        sj -= yj;
-        sp -= yi * yj;
-        if (CORR) {
-          qi -= yi * yi;
-          qj -= yj * yj;
-        }
-        n -= 1;
-      }
-    }
-    if (t < t_start) continue;
-    double v = __builtin_nan("");
-    const double dn = (double)n;
-    if (CORR) {
-      if (n >= mcorr) {
-        const double inv = 1.0 / dn;
-        const double vi = qi - si * si * inv, vj = qj - sj * sj * inv;
-        if (vi > 0.0 && vj > 0.0) v = diag ? 1.0 : mx_corr(sp - si * sj * inv, vi * vj);
-      }
-    } else if (n >= a.min_count && n > 1) {
-      v = (dn * sp - si * sj) / (dn * (dn - 1.0));
-    }
-    out[t * np] = (T)v;
+  const IT step = (IT)a.nv, ostep = (IT)np, back = (IT)(a.window * a.nv);
+  PairSums<T, CORR> st;
+  IT oi = (IT)(t_w * a.nv + i), oj = (IT)(t_w * a.nv + j);
+  int64_t t = t_w;
+  for (; t < t_start; ++t, oi += step, oj += step) st.add(x[oi], x[oj], ci, cj, 1.0);
+  IT oo = (IT)(t_start * np + qq);
+  for (; t < t_r && t < t_end; ++t, oi += step, oj += step, oo += ostep) {
+    st.add(x[oi], x[oj], ci, cj, 1.0);
+    out[oo] = (T)st.value(a.min_count, mcorr, diag);
+  }
+  for (; t < t_end; ++t, oi += step, oj += step, oo += ostep) {
+    st.add(x[oi], x[oj], ci, cj, 1.0);
+    st.add(x[oi - back], x[oj - back], ci, cj, -1.0);
+    out[oo] = (T)st.value(a.min_count, mcorr, diag);
   }
 }
 
@@ -530,11 +547,6 @@ struct MxExpArgs {
   int64_t pair_waves, groups;
 };
 
-template <typename TA>
-__device__ __forceinline__ double mx_alpha(const MxExpArgs& a, int64_t b, int64_t t) {
-  return (double)static_cast<const TA*>(a.alpha)[b * a.ld_alpha + t];
-}
-
 template <bool CORR>
 __device__ __forceinline__ void exp_step(double (&s)[ExpM<CORR>::NS], int& seen, double d, double al, bool valid,
                                          double yi, double yj) {
@@ -578,11 +590,13 @@ __device__ __forceinline__ bool exp_item(const MxExpArgs& a, int64_t wv, int lan
   return b < a.batch && c < a.chunks && qq < np;
 }
 
-template <typename TX, typename TA, bool CORR, bool REPLAY>
+// IT: the index type of element offsets within one batch row (as for
+// mx_move_kernel).
+template <typename TX, typename TA, bool CORR, bool REPLAY, typename IT>
 __global__ __launch_bounds__(kMxThreads) void mx_exp_chunk_kernel(MxExpArgs a) {
   using TO = TA;
   constexpr int NS = ExpM<CORR>::NS, REC = ExpM<CORR>::REC;
-  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + mx_wave_id();
   const int lane = threadIdx.x & (kWave - 1);
   int64_t b, c, qq;
   if (!exp_item(a, wv, lane, b, c, qq)) return;
@@ -606,11 +620,14 @@ __global__ __launch_bounds__(kMxThreads) void mx_exp_chunk_kernel(MxExpArgs a) {
   double dp = 1.0, dp2 = 1.0;
   const double mw = a.min_weight;
   const bool diag = i == j;
-  TO* out = static_cast<TO*>(a.out) + b * a.nobs * np + qq;
-  for (int64_t t = t0; t < t1; ++t) {
-    const double al = mx_alpha<TA>(a, b, t);
+  TO* out = static_cast<TO*>(a.out) + b * a.nobs * np;
+  const TA* alpha = static_cast<const TA*>(a.alpha) + b * a.ld_alpha;
+  const IT step = (IT)a.nv, ostep = (IT)np;
+  IT oi = (IT)(t0 * a.nv + i), oj = (IT)(t0 * a.nv + j), oo = (IT)(t0 * np + qq);
+  for (int64_t t = t0; t < t1; ++t, oi += step, oj += step, oo += ostep) {
+    const double al = (double)alpha[t];
     const double d = 1.0 - al;
-    const TX xi = x[t * a.nv + i], xj = x[t * a.nv + j];
+    const TX xi = x[oi], xj = x[oj];
     const bool valid = xi == xi && xj == xj;
     exp_step<CORR>(s, seen, d, al, valid, valid ? (double)xi - ci : 0.0, valid ? (double)xj - cj : 0.0);
     if (!REPLAY) {
@@ -618,20 +635,21 @@ __global__ __launch_bounds__(kMxThreads) void mx_exp_chunk_kernel(MxExpArgs a) {
       dp2 *= d * d;
       continue;
     }
+    // Forms scaled by sw^2 (sw = s[3]): the bias correction 1 - sw2 / sw^2 is
+    // positive where sw^2 - sw2 is, cov = (s12 sw - s1 s2) / (sw^2 - sw2),
+    // and corr = (s12 sw - s1 s2) / sqrt((s11 sw - s1^2)(s22 sw - s2^2)).
     double v = __builtin_nan("");
-    if (s[5] >= mw && seen >= 2) {
-      const double r = 1.0 / s[3];
-      const double bias = 1.0 - s[4] * r * r;
-      if (bias > 0.0) {
-        if (!CORR) {
-          v = (s[2] - s[0] * s[1] * r) * r / bias;
-        } else {
-          const double vi = s[6 % NS] - s[0] * s[0] * r, vj = s[7 % NS] - s[1] * s[1] * r;
-          if (vi > 0.0 && vj > 0.0) v = diag ? 1.0 : mx_corr(s[2] - s[0] * s[1] * r, vi * vj);
-        }
+    const double sws = s[3] * s[3];
+    if (s[5] >= mw && seen >= 2 && s[4] < sws) {
+      const double cv = s[2] * s[3] - s[0] * s[1];
+      if (!CORR) {
+        v = cv / (sws - s[4]);
+      } else {
+        const double vi = s[6 % NS] * s[3] - s[0] * s[0], vj = s[7 % NS] * s[3] - s[1] * s[1];
+        if (vi > 0.0 && vj > 0.0) v = diag ? 1.0 : mx_corr(cv, vi * vj);
       }
     }
-    out[t * np] = (TO)v;
+    out[oo] = (TO)v;
   }
   if (!REPLAY) {
 #pragma unroll
@@ -654,7 +672,7 @@ template <bool CORR>
 __global__ __launch_bounds__(kMxThreads) void mx_exp_carry_kernel(MxExpArgs a) {
   constexpr int NS = ExpM<CORR>::NS, REC = ExpM<CORR>::REC;
   const int64_t np = a.nv * a.nv;
-  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + (threadIdx.x >> 6);
+  const int64_t wv = (int64_t)blockIdx.x * kMxWaves + mx_wave_id();
   if (wv >= a.batch * np) return;  // whole wave
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t b = wv / np, qq = wv % np;
@@ -806,8 +824,16 @@ int move_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nobs
   MxMoveArgs a{x, out, shift, batch, nobs, nv, window, min_count > 1 ? min_count : 1,
                p.tile_len, p.tiles, p.pair_waves, p.groups, p.sub_len};
   const int grid = mx_grid(batch * p.tiles * p.pair_waves);
-  if (corr) hipLaunchKernelGGL((mx_move_kernel<T, true>), dim3(grid), dim3(kMxThreads), 0, s, a);
-  else hipLaunchKernelGGL((mx_move_kernel<T, false>), dim3(grid), dim3(kMxThreads), 0, s, a);
+  const bool small = nobs * nv * nv < ((int64_t)1 << 31);  // output offsets (the larger) fit 32 bits
+#define NBG_MX_MOVE(C, IT) hipLaunchKernelGGL((mx_move_kernel<T, C, IT>), dim3(grid), dim3(kMxThreads), 0, s, a)
+  if (corr) {
+    if (small) NBG_MX_MOVE(true, uint32_t);
+    else NBG_MX_MOVE(true, int64_t);
+  } else {
+    if (small) NBG_MX_MOVE(false, uint32_t);
+    else NBG_MX_MOVE(false, int64_t);
+  }
+#undef NBG_MX_MOVE
   return check_hip(hipGetLastError(), "matrix launch");
 }
 
@@ -826,15 +852,21 @@ int exp_launch(bool corr, const void* x, const void* alpha, int64_t ld_alpha, do
               p.chunk, p.chunks, p.pair_waves, p.groups};
   const int grid = mx_grid(p.waves);
   const int carry_grid = mx_grid(batch * nv * nv);
+  const bool small = nobs * nv * nv < ((int64_t)1 << 31);  // output offsets (the larger) fit 32 bits
+#define NBG_MX_EXP(C, IT)                                                                                      \
+  do {                                                                                                         \
+    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, C, false, IT>), dim3(grid), dim3(kMxThreads), 0, s, a);   \
+    hipLaunchKernelGGL((mx_exp_carry_kernel<C>), dim3(carry_grid), dim3(kMxThreads), 0, s, a);                \
+    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, C, true, IT>), dim3(grid), dim3(kMxThreads), 0, s, a);    \
+  } while (0)
   if (corr) {
-    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, true, false>), dim3(grid), dim3(kMxThreads), 0, s, a);
-    hipLaunchKernelGGL((mx_exp_carry_kernel<true>), dim3(carry_grid), dim3(kMxThreads), 0, s, a);
-    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, true, true>), dim3(grid), dim3(kMxThreads), 0, s, a);
+    if (small) NBG_MX_EXP(true, uint32_t);
+    else NBG_MX_EXP(true, int64_t);
   } else {
-    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, false, false>), dim3(grid), dim3(kMxThreads), 0, s, a);
-    hipLaunchKernelGGL((mx_exp_carry_kernel<false>), dim3(carry_grid), dim3(kMxThreads), 0, s, a);
-    hipLaunchKernelGGL((mx_exp_chunk_kernel<TX, TA, false, true>), dim3(grid), dim3(kMxThreads), 0, s, a);
+    if (small) NBG_MX_EXP(false, uint32_t);
+    else NBG_MX_EXP(false, int64_t);
   }
+#undef NBG_MX_EXP
   return check_hip(hipGetLastError(), "matrix launch");
 }
 
