@@ -308,31 +308,51 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_kernel(MxStaticArgs a) {
   }
 }
 
-// One thread per output element: sums the partials of its pair (the lower
-// index first) in split order.
+// Adds the splits' partial sums. One workgroup per (batch row, tile, 16 tile
+// positions): 16 lanes per position sum interleaved splits, then lane 0 adds
+// the 16 sums in a fixed order (deterministic), forms the output and writes it
+// to both triangles.
+constexpr int kFinishPos = 16;
+
 template <typename T, bool CORR, int TW>
 __global__ __launch_bounds__(kMxThreads) void mx_static_finish_kernel(MxStaticArgs a) {
   constexpr int NP = StaticSpec<CORR>::NPROD;
   constexpr int TS = 16 * TW;
-  const int64_t e = (int64_t)blockIdx.x * kMxThreads + threadIdx.x;
-  if (e >= a.batch * a.nv * a.nv) return;
-  const int64_t b = e / (a.nv * a.nv);
-  const int64_t i = (e / a.nv) % a.nv, j = e % a.nv;
-  const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-  const int64_t I = lo / TS, J = hi / TS;
-  const int64_t tile = I * a.ntile - I * (I - 1) / 2 + (J - I);
-  const int64_t off = (lo % TS) * TS + hi % TS;
-  double s[NP];
+  constexpr int POS = TS * TS;
+  constexpr int SL = kMxThreads / kFinishPos;  // split lanes per position
+  __shared__ double red[NP][SL][kFinishPos + 1];
+  const int64_t per_tile = POS / kFinishPos;
+  const int64_t bt = blockIdx.x / per_tile;  // batch row x tiles + tile
+  const int pl = threadIdx.x % kFinishPos, sl = threadIdx.x / kFinishPos;
+  const int pos = (int)(blockIdx.x % per_tile) * kFinishPos + pl;
+  double acc[NP];
 #pragma unroll
-  for (int p = 0; p < NP; ++p) s[p] = 0.0;
-  const double* part = a.part + (b * a.tiles + tile) * a.splits * (int64_t)(NP * TS * TS) + off;
-  for (int64_t sp = 0; sp < a.splits; ++sp) {
+  for (int p = 0; p < NP; ++p) acc[p] = 0.0;
+  const double* part = a.part + bt * a.splits * (int64_t)(NP * POS) + pos;
+  for (int64_t sp = sl; sp < a.splits; sp += SL) {
 #pragma unroll
-    for (int p = 0; p < NP; ++p) s[p] += part[p * TS * TS];
-    part += NP * TS * TS;
+    for (int p = 0; p < NP; ++p) acc[p] += part[(sp * NP + p) * POS];
   }
-  const double Qi = CORR ? s[4 % NP] : 0.0, Qj = CORR ? s[5 % NP] : 0.0;
-  static_cast<T*>(a.out)[e] = (T)static_value<CORR>(s[0], s[1], s[2], s[3], Qi, Qj, lo == hi);
+#pragma unroll
+  for (int p = 0; p < NP; ++p) red[p][sl][pl] = acc[p];
+  __syncthreads();
+  if (sl != 0) return;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    double t = 0.0;
+    for (int k = 0; k < SL; ++k) t += red[p][k][pl];
+    acc[p] = t;
+  }
+  const int64_t b = bt / a.tiles, tile = bt % a.tiles;
+  int64_t I, J;
+  tile_of(tile, a.ntile, I, J);
+  const int64_t i = I * TS + pos / TS, j = J * TS + pos % TS;
+  if (i >= a.nv || j >= a.nv || (I == J && i > j)) return;
+  const double Qi = CORR ? acc[4 % NP] : 0.0, Qj = CORR ? acc[5 % NP] : 0.0;
+  const T v = (T)static_value<CORR>(acc[0], acc[1], acc[2], acc[3], Qi, Qj, i == j);
+  T* out = static_cast<T*>(a.out) + b * a.nv * a.nv;
+  out[i * a.nv + j] = v;
+  out[j * a.nv + i] = v;
 }
 
 // Launch plan: wave tile, observation splits, workspace layout.
@@ -753,7 +773,10 @@ ExpPlan exp_plan(int64_t batch, int64_t nobs, int64_t nv, bool corr) {
   const int64_t lanes_per_chunk = batch * (p.groups == 1 ? p.pair_waves * kWave : np);
   const int64_t target = knob("NBG_MATRIX_MOVE_WAVES", kMoveTargetWaves) * kWave;
   int64_t chunks = (target + lanes_per_chunk - 1) / (lanes_per_chunk > 0 ? lanes_per_chunk : 1);
+  // At most 64 chunks per lane of the carry scan's wave: its segments are
+  // walked one chunk after another.
   int64_t max_chunks = nobs / 64;
+  if (max_chunks > kWave * 64) max_chunks = kWave * 64;
   if (max_chunks < 1) max_chunks = 1;
   if (chunks > max_chunks) chunks = max_chunks;
   if (chunks < 1) chunks = 1;
@@ -794,8 +817,8 @@ int static_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nv
     else hipLaunchKernelGGL((mx_static_kernel<T, C, TW, false>), dim3(grid), dim3(kMxThreads), 0, s, a);      \
     if (p.splits > 1)                                                                                          \
       hipLaunchKernelGGL((mx_static_finish_kernel<T, C, TW>),                                                  \
-                         dim3((unsigned)((batch * nv * nv + kMxThreads - 1) / kMxThreads)), dim3(kMxThreads), 0, \
-                         s, a);                                                                                \
+                         dim3((unsigned)(batch * p.tiles * (256 * TW * TW / kFinishPos))), dim3(kMxThreads), 0, s,  \
+                         a);                                                                                   \
   } while (0)
   if (corr) {
     if (p.tw == 2) NBG_MX_STATIC(true, 2);
