@@ -21,7 +21,7 @@ window, through the tile's rolling work).
   the autotuner's timings; inference uses nbg_dense_forward.
 * ``choose_launch`` picks the launch's LDS label slots and per-series
   strategies with the cost model (slot counts whose predicted times are
-  within ``TIE_MARGIN`` of the best are timed once on the batch itself), the
+  within ``TIE_MARGIN`` (10%) of the best are timed once on the batch itself), the
   exhaustive tuner (the fallback, and the judge of the model's regret) or a
   label-count heuristic.
 """
@@ -33,6 +33,7 @@ import hashlib
 import json
 import math
 import pathlib
+import types
 
 import numpy as np
 import torch
@@ -51,10 +52,10 @@ STRATEGIES = (STRATEGY_LDS, STRATEGY_GLOBAL)
 CAPACITIES = (LDS_MAX_LABELS, 4096)
 CONFIGS = tuple((c, st) for c in CAPACITIES for st in STRATEGIES)
 # Slot counts whose predicted time is within this fraction of the best are
-# settled by timing each once on the batch (with two slot counts that costs
-# two launches, far less than the batch statistics the model needs). The model
-# keeps the per-series strategies, the combinatorial part of the choice.
-TIE_MARGIN = 1.0
+# settled by timing each once on the batch (two launches); otherwise the
+# model's ranking decides. The model keeps the per-series strategies, the
+# combinatorial part of the choice.
+TIE_MARGIN = 0.1
 
 
 def kernel_version() -> str:
@@ -99,10 +100,34 @@ def series_chunks(lengths: np.ndarray, max_elements: int = 1 << 27):
         yield s0, len(lengths)
 
 
+# Elements per series the launch choice reads for its statistics (a prefix of
+# each series): the features need label skew and NaN share, not every element.
+SAMPLE_PER_SERIES = 4096
+
+
+def _prefix_sample(rb, values: torch.Tensor, labels: torch.Tensor, sample: int):
+    """(batch view, values, labels) of the first min(length, sample) elements
+    of every series."""
+    slen = np.minimum(np.asarray(rb.lengths, dtype=np.int64), sample)
+    offs = np.concatenate([[0], np.cumsum(slen)]).astype(np.int64)
+    dev_ = values.device
+    n = int(offs[-1])
+    sl = torch.from_numpy(slen).to(dev_)
+    base = torch.repeat_interleave(torch.from_numpy(np.asarray(rb.offsets[:-1], dtype=np.int64)).to(dev_), sl)
+    within = torch.arange(n, device=dev_) - torch.repeat_interleave(torch.from_numpy(offs[:-1]).to(dev_), sl)
+    idx = base + within
+    view = types.SimpleNamespace(nseries=rb.nseries, lengths=slen, offsets=offs, num_labels=rb.num_labels,
+                                 label_offsets=rb.label_offsets)
+    return view, values[idx], labels[idx]
+
+
 def batch_statistics(rb: RaggedMoveMeanGroupMean, values: torch.Tensor, labels: torch.Tensor,
-                     max_elements: int = 1 << 27):
+                     max_elements: int = 1 << 27, sample: int | None = None):
     """Per-series skew (share of the most frequent label) and NaN fraction,
-    computed over chunks of whole series."""
+    computed over chunks of whole series; with ``sample``, over the first
+    ``sample`` elements of each series."""
+    if sample is not None:
+        rb, values, labels = _prefix_sample(rb, values, labels, sample)
     dev_ = values.device
     skew = np.zeros(rb.nseries)
     nanf = np.zeros(rb.nseries)
@@ -434,7 +459,7 @@ def choose_launch(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "mo
 
     if method == "heuristic":
         return LDS_MAX_LABELS, heuristic_strategy(rb.lengths, rb.num_labels)
-    skew, nanf = batch_statistics(rb, values, labels)
+    skew, nanf = batch_statistics(rb, values, labels, sample=SAMPLE_PER_SERIES)
     if method == "model":
         model = model or CostModel.load()
         if model is not None:

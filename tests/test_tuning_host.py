@@ -60,3 +60,19 @@ def test_features_shape_and_logs():
     np.testing.assert_allclose(f[:, 0], [10, 3])
     np.testing.assert_allclose(f[:, 1], [4, 0])
     np.testing.assert_allclose(f[:, 4], np.log2(50), rtol=1e-6)
+
+
+@pytest.mark.parametrize("sample", [1, 16, 100])
+def test_batch_statistics_prefix_sample(sample):
+    from numbagg_amd.tuning import batch_statistics
+
+    rb, vals, labs = _fake_batch([30, 200, 1, 77, 500, 9], [3, 1, 5, 40, 2, 1])
+    skew, nanf = batch_statistics(rb, torch.from_numpy(vals), torch.from_numpy(labs), sample=sample)
+    for s in range(rb.nseries):
+        o0 = rb.offsets[s]
+        o1 = min(rb.offsets[s + 1], o0 + sample)
+        v, lab, k = vals[o0:o1], labs[o0:o1], rb.num_labels[s]
+        ok = (lab >= 0) & (lab < k)
+        top = np.bincount(lab[ok], minlength=k).max() if ok.any() else 0
+        assert skew[s] == pytest.approx(top / (o1 - o0))
+        assert nanf[s] == pytest.approx(np.isnan(v).mean())
