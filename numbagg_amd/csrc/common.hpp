@@ -1,7 +1,9 @@
 // Shared device/host helpers for the numbagg_amd HIP kernels (gfx950 / CDNA4).
 //
-// Everything here is written for 64-wide wavefronts. Kernels are bandwidth-bound
-// streaming scans and reductions; no MFMA is used anywhere in this library.
+// Everything here is written for 64-wide wavefronts. Most kernels are
+// bandwidth-bound streaming scans and reductions; the static correlation /
+// covariance matrices (matrix.hip) are the one dense contraction and run on the
+// float64 matrix cores.
 #pragma once
 
 #include <hip/hip_runtime.h>
