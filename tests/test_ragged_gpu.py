@@ -175,3 +175,43 @@ def test_model_choice_with_measured_tie_break(gpu_lib, monkeypatch):
         mm, gm = rb(v, lab)
         pm, pg = rb.plain(v, lab)
         np.testing.assert_array_equal(_bits(mm), _bits(pm))
+
+
+def test_missing_or_corrupt_model_falls_back_to_exhaustive(gpu_lib, monkeypatch, tmp_path):
+    """A missing, unreadable or stale cost model gives CostModel.load() None and
+    choose_launch(method="model") the exhaustive tuner's choice."""
+    from numbagg_amd import tuning
+    from numbagg_amd.ragged import RaggedMoveMeanGroupMean
+
+    lengths, ks, v, lab = make_batch(12, 100, 20_000, 10, 3000, seed=5)
+    rb = RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50)
+    skew, nanf = tuning.batch_statistics(rb, v, lab, sample=tuning.SAMPLE_PER_SERIES)
+    want = tuning.exhaustive_launch(rb, v, lab, skew, nanf)
+    corrupt = tmp_path / "corrupt.npz"
+    corrupt.write_bytes(b"not a model")
+    stale = tmp_path / "stale.npz"
+    m = tuning.CostModel(seed=1)
+    m.meta = {"arch": "gfx000", "kernel": "0", "configs": [list(c) for c in tuning.CONFIGS]}
+    m.save(stale)
+    calls = []
+    real = tuning.exhaustive_launch
+
+    def spy(*args, **kwargs):
+        calls.append(1)
+        return real(*args, **kwargs)
+
+    monkeypatch.setattr(tuning, "exhaustive_launch", spy)
+    for path in (tmp_path / "missing.npz", corrupt, stale):
+        monkeypatch.setenv("NBG_COST_MODEL", str(path))
+        assert tuning.CostModel.load() is None, path.name
+        n0 = len(calls)
+        cap, st = tuning.choose_launch(rb, v, lab, method="model")
+        assert len(calls) == n0 + 1, path.name  # the exhaustive tuner decided
+        assert cap in tuning.CAPACITIES and st.shape == (rb.nseries,)
+        # a valid launch: the plain launches' results
+        rb.set_strategy(st)
+        rb.set_lds_labels(cap)
+        mm, _ = rb(v, lab)
+        pm, _ = rb.plain(v, lab)
+        np.testing.assert_array_equal(_bits(mm), _bits(pm))
+    assert want[0] in tuning.CAPACITIES

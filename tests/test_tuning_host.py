@@ -76,3 +76,16 @@ def test_batch_statistics_prefix_sample(sample):
         top = np.bincount(lab[ok], minlength=k).max() if ok.any() else 0
         assert skew[s] == pytest.approx(top / (o1 - o0))
         assert nanf[s] == pytest.approx(np.isnan(v).mean())
+
+
+def test_cost_model_load_missing_or_unreadable(tmp_path, monkeypatch):
+    """No GPU needed: a missing or unreadable model file loads as None (the
+    launch choice then falls back to the exhaustive tuner)."""
+    from numbagg_amd.tuning import CostModel
+
+    assert CostModel.load(tmp_path / "missing.npz") is None
+    bad = tmp_path / "bad.npz"
+    bad.write_bytes(b"\x00garbage")
+    assert CostModel.load(bad) is None
+    monkeypatch.setenv("NBG_COST_MODEL", str(bad))
+    assert CostModel.load() is None
