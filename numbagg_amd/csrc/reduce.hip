@@ -297,14 +297,10 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
   if (lane == 0) parts[item] = p;
 }
 
-// Output of one row from its chunk partials (merged in chunk order).
+// The output of one row from its merged partial.
 template <typename T, typename TO, int KIND>
-__global__ void reduce_finalize_kernel(const RPart* __restrict__ parts, int64_t rows, int64_t chunks_per_row,
-                                       int64_t ddof, const T* __restrict__ x, int64_t ld, TO* __restrict__ out) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= rows) return;
-  RPart p = parts[row * chunks_per_row];
-  for (int64_t c = 1; c < chunks_per_row; ++c) merge_part<T, KIND>(p, parts[row * chunks_per_row + c]);
+__device__ __forceinline__ TO part_value(const RPart& p, int64_t row, int64_t ddof, const T* __restrict__ x,
+                                         int64_t ld) {
   TO r;
   if constexpr (KIND == kRSum) {
     if constexpr (!std::is_floating_point<T>::value) {
@@ -339,7 +335,61 @@ __global__ void reduce_finalize_kernel(const RPart* __restrict__ parts, int64_t 
     if (p.w0 == 0ull) r = (TO)canonical_nan<double>();
     else r = (TO)(double)x[row * ld + (int64_t)p.w1];
   }
-  out[row] = r;
+  return r;
+}
+
+// Output of one row from its chunk partials (merged in chunk order), a thread
+// per row.
+template <typename T, typename TO, int KIND>
+__global__ void reduce_finalize_kernel(const RPart* __restrict__ parts, int64_t rows, int64_t chunks_per_row,
+                                       int64_t ddof, const T* __restrict__ x, int64_t ld, TO* __restrict__ out) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  RPart p = parts[row * chunks_per_row];
+  for (int64_t c = 1; c < chunks_per_row; ++c) merge_part<T, KIND>(p, parts[row * chunks_per_row + c]);
+  out[row] = part_value<T, TO, KIND>(p, row, ddof, x, ld);
+}
+
+// The same for rows of many chunks, a workgroup per row: each thread merges a
+// contiguous run of chunks, then a pairwise tree over the threads in order
+// (left before right, so every merge keeps chunk order; deterministic).
+constexpr int kRedFinishThreads = 256;
+
+template <typename T, typename TO, int KIND>
+__global__ __launch_bounds__(kRedFinishThreads) void reduce_finalize_wide_kernel(
+    const RPart* __restrict__ parts, int64_t chunks_per_row, int64_t ddof, const T* __restrict__ x, int64_t ld,
+    TO* __restrict__ out) {
+  __shared__ RPart sp[kRedFinishThreads];
+  __shared__ int hv[kRedFinishThreads];
+  const int64_t row = blockIdx.x;
+  const int t = threadIdx.x;
+  const int64_t per = (chunks_per_row + kRedFinishThreads - 1) / kRedFinishThreads;
+  const int64_t c0 = t * per;
+  const int64_t c1 = c0 + per < chunks_per_row ? c0 + per : chunks_per_row;
+  const RPart* rp = parts + row * chunks_per_row;
+  RPart p{};
+  int have = 0;
+  for (int64_t c = c0; c < c1; ++c) {
+    if (have) merge_part<T, KIND>(p, rp[c]);
+    else p = rp[c];
+    have = 1;
+  }
+  sp[t] = p;
+  hv[t] = have;
+  for (int stride = 1; stride < kRedFinishThreads; stride <<= 1) {
+    __syncthreads();
+    if ((t & (2 * stride - 1)) == 0 && hv[t + stride]) {
+      if (hv[t]) {
+        RPart a = sp[t];
+        merge_part<T, KIND>(a, sp[t + stride]);
+        sp[t] = a;
+      } else {
+        sp[t] = sp[t + stride];
+        hv[t] = 1;
+      }
+    }
+  }
+  if (t == 0) out[row] = part_value<T, TO, KIND>(sp[0], row, ddof, x, ld);
 }
 
 // Chunks per row: about 8 wavefronts per SIMD over the device, chunks of
@@ -400,9 +450,14 @@ int reduce_impl(const T* x, TO* out, int64_t rows, int64_t n, int64_t ld, int64_
     hipLaunchKernelGGL((reduce_kernel<T, KIND, 1, 1>), dim3((unsigned)blocks), dim3(kRedThreads), 0, s, x, n, ld,
                        chunk, cpr, items, parts);
   if (int rc = check_hip(hipGetLastError(), "reduce_kernel launch")) return rc;
-  const int64_t fblocks = (rows + 255) / 256;
-  hipLaunchKernelGGL((reduce_finalize_kernel<T, TO, KIND>), dim3((unsigned)fblocks), dim3(256), 0, s, parts, rows, cpr,
-                     ddof, x, ld, out);
+  if (cpr >= 64 && rows <= 0x7fffffffLL) {
+    hipLaunchKernelGGL((reduce_finalize_wide_kernel<T, TO, KIND>), dim3((unsigned)rows), dim3(kRedFinishThreads), 0,
+                       s, parts, cpr, ddof, x, ld, out);
+  } else {
+    const int64_t fblocks = (rows + 255) / 256;
+    hipLaunchKernelGGL((reduce_finalize_kernel<T, TO, KIND>), dim3((unsigned)fblocks), dim3(256), 0, s, parts, rows,
+                       cpr, ddof, x, ld, out);
+  }
   return check_hip(hipGetLastError(), "reduce_finalize_kernel launch");
 }
 
