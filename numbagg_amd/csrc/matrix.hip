@@ -38,6 +38,7 @@
 // at a chunk's start = decay product of the previous chunk x state at its
 // start + its local state), and a third replays each chunk from its carried
 // state and writes the outputs. Shifts are per variable over the whole series.
+#include <initializer_list>
 #include <type_traits>
 
 #include "common.hpp"
@@ -798,10 +799,19 @@ int mx_grid(int64_t waves) {
   return (int)(blocks < 1 ? 1 : blocks);
 }
 
+// Every grid of a launch fits a 32-bit block count.
+bool grids_fit(std::initializer_list<int64_t> waves) {
+  for (int64_t w : waves)
+    if (w < 0 || (w + kMxWaves - 1) / kMxWaves > 0x7fffffffLL) return false;
+  return true;
+}
+
 template <typename T>
 int static_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nv, int64_t nobs, void* ws,
                   hipStream_t s) {
   const StaticPlan p = static_plan(batch, nv, nobs, corr);
+  if (!grids_fit({batch * nv, batch * p.tiles * p.splits, batch * p.tiles * 64}))
+    return set_error(NBG_ERR_INVALID, "matrix: batch too large for one launch");
   char* w = static_cast<char*>(ws);
   double* shift = reinterpret_cast<double*>(w);
   double* part = reinterpret_cast<double*>(w + align256(p.shift_bytes));
@@ -840,6 +850,8 @@ template <typename T>
 int move_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nobs, int64_t nv, int64_t window,
                 int64_t min_count, void* ws, hipStream_t s) {
   const MovePlan p = move_plan(batch, nobs, nv, window);
+  if (!grids_fit({batch * p.tiles * nv, batch * p.tiles * p.pair_waves}))
+    return set_error(NBG_ERR_INVALID, "matrix: batch too large for one launch");
   double* shift = static_cast<double*>(ws);
   MxShiftArgs sa{batch * p.tiles, nv, p.tiles, p.tile_len, window - 1, nobs, nobs * nv, 1, nv};
   hipLaunchKernelGGL(mx_shift_kernel<T>, dim3(mx_grid(batch * p.tiles * nv)), dim3(kMxThreads), 0, s,
@@ -864,6 +876,8 @@ template <typename TX, typename TA>
 int exp_launch(bool corr, const void* x, const void* alpha, int64_t ld_alpha, double min_weight, void* out,
                int64_t batch, int64_t nobs, int64_t nv, void* ws, hipStream_t s) {
   const ExpPlan p = exp_plan(batch, nobs, nv, corr);
+  if (!grids_fit({batch * nv, p.waves, batch * nv * nv}))
+    return set_error(NBG_ERR_INVALID, "matrix: batch too large for one launch");
   char* w = static_cast<char*>(ws);
   double* shift = reinterpret_cast<double*>(w);
   double* state = reinterpret_cast<double*>(w + align256(p.shift_bytes));

@@ -392,3 +392,34 @@ def test_matrices_stay_on_device(gpu_lib):
     r = nb.nancorrmatrix(x)
     assert isinstance(r, torch.Tensor) and r.is_cuda and r.shape == (3, 4, 4)
     assert_allclose(r.cpu().numpy(), nb.nancorrmatrix(x.cpu().numpy()), rtol=0, atol=0)
+
+
+def test_matrix_edge_shapes(gpu_lib):
+    """One variable, one observation, all-NaN variables, empty batches, ragged
+    observation counts and float64 data at a large level."""
+    import numbagg_amd as nb
+    from oracle import exact
+
+    rng = np.random.default_rng(9)
+    for shape in ((1, 1, 50), (2, 3, 1), (1, 17, 4099), (3, 2, 5)):
+        x = rng.standard_normal(shape)
+        x[..., ::7] = np.nan
+        if shape[1] > 2:
+            x[:, 1] = np.nan
+        for f in ("nancorrmatrix", "nancovmatrix"):
+            _check(getattr(nb, f)(x), exact.matrix(f, x), f == "nancorrmatrix", 1e-12, f"{f} {shape}")
+    assert nb.nancovmatrix(np.zeros((0, 3, 10))).shape == (0, 3, 3)
+    big = rng.standard_normal((1, 2, 3000)) * 1e3 + 1e12
+    _check(nb.nancovmatrix(big), exact.matrix("nancovmatrix", big), False, 1e-12, "large level")
+    for shape, window in (((1, 1, 1), 1), ((1, 7, 1), 3), ((2, 5, 3), 5)):
+        x = rng.standard_normal(shape)
+        x[:, ::2, 0] = np.nan
+        for f in ("move_corrmatrix", "move_covmatrix"):
+            got = getattr(nb, f)(x, window=window, min_count=1)
+            _check(got, exact.matrix(f, x, window=window, min_count=1), f == "move_corrmatrix", 1e-11,
+                   f"{f} {shape}", _pair_counts(x, window))
+        for f in ("move_exp_nancorrmatrix", "move_exp_nancovmatrix"):
+            got = getattr(nb, f)(x, alpha=0.4)
+            _check(got, exact.matrix(f, x, alpha=0.4), f == "move_exp_nancorrmatrix", 1e-11, f"{f} {shape}",
+                   _pair_counts(x))
+    assert nb.move_corrmatrix(np.zeros((0, 4, 2)), window=2).shape == (0, 4, 2, 2)
