@@ -372,9 +372,14 @@ def bench_ragged(torch, nb, steps, judge=False):
     rb = ragged.RaggedMoveMeanGroupMean(lengths, ks, window=50, min_count=50)
     model = tuning.CostModel.load()
     source = "cost model" if model is not None else "exhaustive autotuner (no valid cost model)"
+    # the caller's output buffers (the launches timed for the choice write there)
+    mo0 = torch.empty_like(values)
+    go0 = torch.empty(rb.total_labels, dtype=torch.float32, device=values.device)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    cap, strat = tuning.choose_launch(rb, values, labels, method="model", model=model)
+    cap, strat = tuning.choose_launch(rb, values, labels, method="model", model=model, move_out=mo0, group_out=go0)
     choose_s = time.perf_counter() - t0
+    del mo0, go0
     rb.set_strategy(strat)
     rb.set_lds_labels(cap)
     replay, mo, go = rb.graph(values, labels)

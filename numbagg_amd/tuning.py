@@ -197,13 +197,17 @@ class Autotuner:
         self.db_path.parent.mkdir(parents=True, exist_ok=True)
         self.db_path.write_text(json.dumps({"meta": self.meta, "entries": self.entries}, indent=1))
 
-    def _time(self, rb, values, labels, strategy, lds_labels: int = LDS_MAX_LABELS) -> float:
-        """ns per element of one launch; `strategy` is one value or a per-series array."""
+    def _time(self, rb, values, labels, strategy, lds_labels: int = LDS_MAX_LABELS, move_out=None,
+              group_out=None) -> float:
+        """ns per element of one launch; `strategy` is one value or a per-series
+        array. Outputs go to `move_out` / `group_out` when given (a large batch's
+        fresh output allocation costs more than the launches timed)."""
         st = np.broadcast_to(np.asarray(strategy, dtype=np.uint8), (rb.nseries,))
         rb.set_strategy(np.ascontiguousarray(st))
         rb.set_lds_labels(lds_labels)
-        mo = torch.empty_like(values)
-        go = torch.empty(rb.total_labels, dtype=torch.float32, device=values.device)
+        mo = move_out if move_out is not None else torch.empty_like(values)
+        go = group_out if group_out is not None else torch.empty(rb.total_labels, dtype=torch.float32,
+                                                                 device=values.device)
         rb(values, labels, mo, go)
         stream = torch.cuda.current_stream()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -454,10 +458,12 @@ def exhaustive_launch(rb: RaggedMoveMeanGroupMean, values, labels, skew, nanf, r
     return int(best[1]), best[2]
 
 
-def choose_launch(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "model", model: CostModel | None = None):
+def choose_launch(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "model", model: CostModel | None = None,
+                  move_out=None, group_out=None):
     """(LDS label slots, per-series strategies) for `rb` by "model" (falls back
     to "exhaustive" when no valid model is available), "exhaustive" or
-    "heuristic" (default slots, the label-count rule)."""
+    "heuristic" (default slots, the label-count rule). Launches timed for the
+    choice write into `move_out` / `group_out` when given."""
     from .ragged import heuristic_strategy
 
     if method == "heuristic":
@@ -474,7 +480,7 @@ def choose_launch(rb: RaggedMoveMeanGroupMean, values, labels, method: str = "mo
             if len(close) == 1:
                 return int(close[0][1]), close[0][2]
             tuner = Autotuner(db_path=pathlib.Path("/nonexistent/db.json"), reps=1)
-            timed = [(tuner._time(rb, values, labels, st, cap), cap, st) for _, cap, st in close]
+            timed = [(tuner._time(rb, values, labels, st, cap, move_out, group_out), cap, st) for _, cap, st in close]
             _, cap, st = min(timed, key=lambda c: c[0])
             return int(cap), st
         method = "exhaustive"
