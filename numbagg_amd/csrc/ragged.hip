@@ -41,7 +41,6 @@ namespace {
 // 9.75 ms). Knob: NBG_RAGGED_GRAB.
 constexpr int kRaggedGrab = 16;
 constexpr int kRaggedMaxGrab = 64;  // a grab's tile descriptors are held in LDS
-constexpr int64_t kRaggedHotMinTiles = 16;  // series length (tiles) from which labels 0 and 1 go to registers
 constexpr int64_t kRaggedLdsLabels = 1024;     // default LDS label slots per launch
 constexpr int64_t kRaggedMaxLdsLabels = 8192;  // 96 KB of slots
 
@@ -95,34 +94,6 @@ __device__ __forceinline__ void ragged_rolling(const RaggedArgs& A, int64_t off,
                                                const double* lut, int staged, int full) {
   rolling_tile<float, kMean, true, VEC, KP, LUT>(A.values + off, nullptr, A.move_out + off, n,
                                                  A.window, A.min_count, tile, smem, rs, lut, staged, full);
-}
-
-// Adds the workgroup's register sums of labels 0 and 1 (each thread's, reduced
-// over its wave) into their LDS slots, then waits for the workgroup; the hot
-// registers restart from zero.
-__device__ __forceinline__ void ragged_hot_flush(int64_t nl, double* l_sum, unsigned int* l_cnt, double& h_s0,
-                                                 unsigned int& h_c0, double& h_s1, unsigned int& h_c1) {
-  for (int m = kWave / 2; m > 0; m >>= 1) {
-    h_s0 += __shfl_xor(h_s0, m, kWave);
-    h_c0 += __shfl_xor(h_c0, m, kWave);
-    h_s1 += __shfl_xor(h_s1, m, kWave);
-    h_c1 += __shfl_xor(h_c1, m, kWave);
-  }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    if (h_c0) {
-      atomicAdd(&l_sum[0], h_s0);
-      atomicAdd(&l_cnt[0], h_c0);
-    }
-    if (h_c1 && nl > 1) {
-      atomicAdd(&l_sum[1], h_s1);
-      atomicAdd(&l_cnt[1], h_c1);
-    }
-  }
-  h_s0 = 0.0;
-  h_s1 = 0.0;
-  h_c0 = 0u;
-  h_c1 = 0u;
-  __syncthreads();
 }
 
 // Flushes the LDS label slots [0, nl) of series `s` into its global
@@ -189,14 +160,6 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
     // the grab and are flushed when the series changes or the grab ends.
     int cur = -1;
     int64_t cur_lo = 0, cur_nl = 0;
-    // Labels 0 and 1 (the most frequent under skewed labels) of a long series
-    // are summed in registers when their slots are in LDS: many lanes of a
-    // wave on one LDS address would serialise their atomics. Short series
-    // (a few tiles) keep the plain path: the register sums' extra barrier at
-    // each series change would cost more than it saves there.
-    double h_s0 = 0.0, h_s1 = 0.0;
-    unsigned int h_c0 = 0u, h_c1 = 0u;
-    bool cur_hot = false;
     int staged = 0;  // leading segment elements already in LDS (carried halo + prefetched tile)
     float pa[KP];
     for (int i = 0; i < cnt; ++i) {
@@ -210,10 +173,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
       // fits) and sends the rest to global atomics; strategy 1 is global only.
       const int64_t nl = d.strategy == 0 ? (K < A.lds_labels ? K : A.lds_labels) : 0;
       if (s != cur) {
-        if (cur_nl > 0) {
-          if (cur_hot) ragged_hot_flush(cur_nl, l_sum, l_cnt, h_s0, h_c0, h_s1, h_c1);
-          ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
-        }
+        if (cur_nl > 0) ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
         for (int64_t l = tid; l < nl; l += kMoveThreads) {
           l_sum[l] = 0.0;
           l_cnt[l] = 0u;
@@ -221,7 +181,6 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
         cur = s;
         cur_lo = lo;
         cur_nl = nl;  // the zeroing is ordered before use by rolling_tile's barriers
-        cur_hot = nl > 1 && n >= kRaggedHotMinTiles * (int64_t)TILE;
       }
       const float* xs = A.values + d.off;
       // 16-byte vector accesses need element alignment only (Vec16), so every
@@ -280,13 +239,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
         const float x = la[ehp + j + (j >> 4)];
         const int32_t l = lj[j];
         if (l >= 0 && l < K && !is_nan(x)) {
-          if (cur_hot && l == 0) {
-            h_s0 += (double)x;
-            h_c0 += 1u;
-          } else if (cur_hot && l == 1) {
-            h_s1 += (double)x;
-            h_c1 += 1u;
-          } else if (l < nl) {
+          if (l < nl) {
             atomicAdd(&l_sum[l], (double)x);
             atomicAdd(&l_cnt[l], 1u);
           } else {
@@ -314,10 +267,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
         staged = 0;
       }
     }
-    if (cur_nl > 0) {
-      if (cur_hot) ragged_hot_flush(cur_nl, l_sum, l_cnt, h_s0, h_c0, h_s1, h_c1);
-      ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
-    }
+    if (cur_nl > 0) ragged_flush(A, cur_lo, cur_nl, l_sum, l_cnt);
   }
 }
 
