@@ -22,10 +22,13 @@
 //      just before its chunk; K sequential add/remove steps produce its outputs.
 // For windows too long for LDS the tail and halo are read from global memory.
 //
-// Accumulators: first moments of float32 data for move_sum/mean are FP32
-// TwoSum pairs; the moment kernels (var/std/cov/corr) work on data shifted by a
-// per-tile constant with float64 first and second moments; float64 data uses
-// double-double throughout (TwoProd for squares and cross products).
+// Accumulators: first moments of float32 data for move_sum/mean are float64
+// (exact differences head - tail; FP32 TwoSum pairs with
+// NBG_F32_ACCUMULATOR=twosum); the moment kernels (var/std/cov/corr) work on
+// data shifted by a per-tile constant with float64 first and second moments;
+// float64 data uses double-double throughout (TwoProd for squares and cross
+// products). Outputs of full tiles of the one-input kinds are passed through
+// LDS so that the stores are contiguous across each wave.
 #include <cstdlib>
 #include <cstring>
 

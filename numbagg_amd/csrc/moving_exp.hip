@@ -653,7 +653,43 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
     for (int j = 0; j < K && t0 + e0 + j < n; ++j) rerun(j);
   }
   constexpr int OV = 16 / (int)sizeof(TA);  // outputs per 16-byte store
-  if (full && ((uintptr_t)(ro + t0 + e0)) % 16 == 0) {
+  // A full tile's outputs go through LDS (the staged inputs' region, free once
+  // every thread has re-run its positions), so that each store instruction
+  // writes consecutive 16-byte pieces across the wave instead of one piece
+  // per K * sizeof(TA) bytes. Rounds of XR threads' outputs fit the region;
+  // a thread's K outputs sit at stride K + 1 (conflict-free LDS writes).
+  constexpr int64_t kRegion = ((int64_t)TILE + TILE / 16 + 16) * (int64_t)sizeof(TX) * Sp::NARGS;  // pad16(TILE) + 16 slots
+  constexpr int kFit = (int)(kRegion / ((K + 1) * (int64_t)sizeof(TA)));
+  constexpr int XR = kFit >= kExpThreads ? kExpThreads : (kFit >= kExpThreads / 2 ? kExpThreads / 2 : kExpThreads / 4);
+  static_assert((int64_t)XR * (K + 1) * (int64_t)sizeof(TA) <= kRegion, "output round fits the staged region");
+  static_assert(K % OV == 0, "whole 16-byte pieces per thread");
+  if (full && ((uintptr_t)(ro + t0)) % 16 == 0) {
+    TA* lo = reinterpret_cast<TA*>(smem_raw);
+    char* rot = reinterpret_cast<char*>(ro + t0);
+    struct alignas(16) V { TA v[OV]; };
+    __syncthreads();  // every re-run read of the staged inputs is done
+#pragma unroll
+    for (int r = 0; r < kExpThreads / XR; ++r) {
+      if (r > 0) __syncthreads();  // the previous round's reads are done
+      if (tid / XR == r) {
+        const int b = (tid - r * XR) * (K + 1);
+#pragma unroll
+        for (int j = 0; j < K; ++j) lo[b + j] = res[j];
+      }
+      __syncthreads();
+      // pieces of OV outputs: piece q of the round is thread q / (K / OV)'s
+      // outputs (q % (K / OV)) * OV ..
+      constexpr int kPieces = XR * K / OV;
+#pragma unroll
+      for (int q = tid; q < kPieces; q += kExpThreads) {
+        const int th = q / (K / OV), jo = (q % (K / OV)) * OV;
+        V vo;
+#pragma unroll
+        for (int v = 0; v < OV; ++v) vo.v[v] = lo[th * (K + 1) + jo + v];
+        *reinterpret_cast<V*>(rot + ((uint32_t)(r * XR * K + q * OV) * (uint32_t)sizeof(TA))) = vo;
+      }
+    }
+  } else if (full && ((uintptr_t)(ro + t0 + e0)) % 16 == 0) {
     struct alignas(16) V { TA v[OV]; };
 #pragma unroll
     for (int j = 0; j < K; j += OV) {

@@ -444,6 +444,18 @@ __host__ __device__ inline int64_t rolling_stage_bytes(int items, int64_t window
   return ((pad16(seg) + 16) * (int64_t)sizeof(T) * nargs + 15) / 16 * 16;
 }
 
+// Calls f(std::integral_constant<int, mis>{}) for a wave-uniform mis in
+// [M, 16): one inlined copy of f per value.
+template <int M, typename F>
+__device__ __forceinline__ void with_mis16(int mis, F& f) {
+  if constexpr (M < 15) {
+    if (mis == M) f(std::integral_constant<int, M>{});
+    else with_mis16<M + 1>(mis, f);
+  } else {
+    f(std::integral_constant<int, 15>{});
+  }
+}
+
 // The first non-NaN value among staged segment elements e0 .. e0 + 255 (0 if
 // none): one LDS read and a ballot per 64 elements; wave-uniform result.
 template <typename T>
@@ -464,7 +476,8 @@ __device__ __forceinline__ T first_valid_staged(const T* seg, int e0) {
 // here); callers that know it from 32-bit tile counts keep the test on the
 // scalar unit. Ends with a barrier so the caller may reuse the shared memory
 // for its next tile.
-template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false>
+template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false,
+          bool LDS_OUT = false>
 __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* __restrict__ rb,
                                              T* __restrict__ ro, int64_t n, int64_t window,
                                              int64_t min_count, int64_t tile,
@@ -634,18 +647,40 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     int dc[K];
     double dsum = 0.0;
     int dcnt = 0;
+    // Tail offsets tbase + j + ((mis + j) >> 4) with mis a compile-time
+    // constant (one copy of this pass per misalignment, picked by a uniform
+    // branch) are immediates of the LDS reads, not per-read address sums.
+    auto first_pass = [&](auto mis_c) {
+      constexpr int M = decltype(mis_c)::value;
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      T h, t, u;
-      head(j, h, u);
-      tail(j, t, u);
-      const bool hv = !is_nan(h), tv = !is_nan(t);
-      // NaN -> 0 selected in float32 before widening (one select each)
-      const T h0 = nan_to_zero(h), t0v = nan_to_zero(t);
-      dj[j] = (double)h0 - (double)t0v;
-      dc[j] = (int)hv - (int)tv;
-      dsum += dj[j];
-      dcnt += dc[j];
+      for (int j = 0; j < K; ++j) {
+        T h, u;
+        head(j, h, u);
+        const T t = la[tbase + j + ((M + j) >> 4)];
+        const bool hv = !is_nan(h), tv = !is_nan(t);
+        // NaN -> 0 selected in float32 before widening (one select each)
+        const T h0 = nan_to_zero(h), t0v = nan_to_zero(t);
+        dj[j] = (double)h0 - (double)t0v;
+        dc[j] = (int)hv - (int)tv;
+        dsum += dj[j];
+        dcnt += dc[j];
+      }
+    };
+    if constexpr (TAIL_LDS && K % 16 == 0) {
+      with_mis16<0>(mis, first_pass);
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        T h, t, u;
+        head(j, h, u);
+        tail(j, t, u);
+        const bool hv = !is_nan(h), tv = !is_nan(t);
+        const T h0 = nan_to_zero(h), t0v = nan_to_zero(t);
+        dj[j] = (double)h0 - (double)t0v;
+        dc[j] = (int)hv - (int)tv;
+        dsum += dj[j];
+        dcnt += dc[j];
+      }
     }
     S delta = S::zero();
     delta.c = dcnt;
@@ -744,7 +779,33 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       res[j] = Tr::out(st, min_count, nullptr);
     }
   }
-  if (VEC > 1 && tile_full) {
+  // LDS_OUT: a full tile's outputs go through LDS so that each store
+  // instruction writes consecutive 16-byte pieces across the wave (a thread's
+  // own K outputs are K consecutive positions, so storing them directly
+  // spreads one instruction over 64 separate pieces of K * sizeof(T) bytes).
+  // They take the staged segment's first pad16(TILE) slots: the caller's
+  // carried halo comes from beyond them, and this tile's last reads of its
+  // inputs came before the scan's barrier (sum, mean) or before the barrier
+  // below (the moment kinds re-read them in the output pass). The two-input
+  // kinds (cov, corr) store directly: they are bound by their arithmetic, and
+  // the staging cost registers there (move_cov 1.93 -> 2.50 ms on float32
+  // (256, 2M), window 1000).
+  constexpr bool XPOSE = LDS_OUT && VEC > 1 && K % 16 == 0 && Tr::NARGS == 1;
+  if (XPOSE && tile_full) {
+    if constexpr (Tr::SHIFT) __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) la[tid * (K + K / 16) + j + (j >> 4)] = res[j];  // pad16(tid K + j)
+    __syncthreads();
+    char* rot = reinterpret_cast<char*>(ro + t0);
+#pragma unroll
+    for (int m = 0; m < K / VEC; ++m) {
+      const int e = (m * kMoveThreads + tid) * VEC;  // VEC consecutive slots of one 16-slot group
+      Vec16<T> vo;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) vo.v[v] = la[pad16(e) + v];
+      *reinterpret_cast<Vec16<T>*>(rot + (uint32_t)e * (uint32_t)sizeof(T)) = vo;
+    }
+  } else if (VEC > 1 && tile_full) {
     // full tile (block-uniform test): no per-thread checks, 32-bit offsets
     // from the tile's base
     char* rot = reinterpret_cast<char*>(ro + t0);
@@ -900,8 +961,8 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(mo
   for (int tile = t_begin; tile < t_end; ++tile) {
     const bool more = carry && tile + 1 < t_end;
     if (PF && more) prefetch(tile + 1);
-    rolling_tile<T, KIND, TAIL_LDS, VEC, KP, LUT>(ra, rb, ro, n, window, min_count, tile, smem_raw,
-                                                  rs, lut, staged, tile < full_tiles ? 1 : 0);
+    rolling_tile<T, KIND, TAIL_LDS, VEC, KP, LUT, true>(ra, rb, ro, n, window, min_count, tile, smem_raw,
+                                                        rs, lut, staged, tile < full_tiles ? 1 : 0);
     if (more) {
       // The next tile's segment starts TILE elements later: its first H
       // elements are this segment's last H (source and destination disjoint,
