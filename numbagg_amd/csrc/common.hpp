@@ -221,6 +221,19 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n) {
 __host__ __device__ __forceinline__ int64_t pad16(int64_t e) { return e + (e >> 4); }
 __host__ __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 
+// A 16-byte store of 16 / sizeof(T) values to a 16-byte aligned address,
+// non-temporal: for outputs written once in whole-wave contiguous runs, which
+// then go to HBM without lingering as dirty lines in the caches (their later
+// write-back would take bandwidth from whatever kernel runs next).
+template <typename T>
+__device__ __forceinline__ void store_stream16(T* p, const T (&v)[16 / sizeof(T)]) {
+  typedef T vt __attribute__((ext_vector_type(16 / sizeof(T))));
+  vt x;
+#pragma unroll
+  for (int i = 0; i < (int)(16 / sizeof(T)); ++i) x[i] = v[i];
+  __builtin_nontemporal_store(x, reinterpret_cast<vt*>(p));
+}
+
 // Zero-fill kernel used to reset look-back status words and ticket counters.
 // (A kernel rather than a memset node so the call stays graph-capture safe.)
 __global__ void zero_words_kernel(unsigned long long* p, int64_t nwords);

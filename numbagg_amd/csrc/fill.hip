@@ -378,7 +378,7 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
       VecT<T> t;
 #pragma unroll
       for (int v = 0; v < VEC; ++v) t.v[v] = res[v];
-      *reinterpret_cast<VecT<T>*>(orow + p0 + e0) = t;
+      store_stream16(orow + p0 + e0, t.v);  // whole-wave contiguous runs
     } else {
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {

@@ -686,7 +686,7 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
         V vo;
 #pragma unroll
         for (int v = 0; v < OV; ++v) vo.v[v] = lo[th * (K + 1) + jo + v];
-        *reinterpret_cast<V*>(rot + ((uint32_t)(r * XR * K + q * OV) * (uint32_t)sizeof(TA))) = vo;
+        store_stream16(reinterpret_cast<TA*>(rot + ((uint32_t)(r * XR * K + q * OV) * (uint32_t)sizeof(TA))), vo.v);
       }
     }
   } else if (full && ((uintptr_t)(ro + t0 + e0)) % 16 == 0) {

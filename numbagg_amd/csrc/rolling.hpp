@@ -476,15 +476,20 @@ __device__ __forceinline__ T first_valid_staged(const T* seg, int e0) {
 // here); callers that know it from 32-bit tile counts keep the test on the
 // scalar unit. Ends with a barrier so the caller may reuse the shared memory
 // for its next tile.
+// No work of the caller on a tile's staged inputs.
+struct NoReads {
+  __device__ __forceinline__ void operator()() const {}
+};
+
 template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false,
-          bool LDS_OUT = false>
+          bool LDS_OUT = false, typename Reads = NoReads>
 __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* __restrict__ rb,
                                              T* __restrict__ ro, int64_t n, int64_t window,
                                              int64_t min_count, int64_t tile,
                                              unsigned char* smem_raw,
                                              RollingShared<typename Traits<T, KIND>::S>& rs,
                                              const double* lut = nullptr, int staged = 0,
-                                             int full = -1) {
+                                             int full = -1, Reads reads = Reads{}) {
   using Tr = Traits<T, KIND>;
   using S = typename Tr::S;
   constexpr int K = KP;
@@ -547,6 +552,9 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     }
   }
   __syncthreads();
+  // The caller's own reads of the staged inputs (the ragged kernel's group
+  // pass over the thread's head values), ahead of this tile's evaluation.
+  reads();
 
   // Per-tile shifts for the moment kinds: the first valid value among the
   // tile's first 256 inputs (any constant works; one near the data keeps the
@@ -779,20 +787,19 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       res[j] = Tr::out(st, min_count, nullptr);
     }
   }
-  // LDS_OUT: a full tile's outputs go through LDS so that each store
-  // instruction writes consecutive 16-byte pieces across the wave (a thread's
-  // own K outputs are K consecutive positions, so storing them directly
-  // spreads one instruction over 64 separate pieces of K * sizeof(T) bytes).
-  // They take the staged segment's first pad16(TILE) slots: the caller's
-  // carried halo comes from beyond them, and this tile's last reads of its
-  // inputs came before the scan's barrier (sum, mean) or before the barrier
-  // below (the moment kinds re-read them in the output pass). The two-input
-  // kinds (cov, corr) store directly: they are bound by their arithmetic, and
-  // the staging cost registers there (move_cov 1.93 -> 2.50 ms on float32
-  // (256, 2M), window 1000).
-  constexpr bool XPOSE = LDS_OUT && VEC > 1 && K % 16 == 0 && Tr::NARGS == 1;
+  // LDS_OUT (sum, mean): a full tile's outputs go through LDS so that each
+  // store instruction writes consecutive 16-byte pieces across the wave (a
+  // thread's own K outputs are K consecutive positions, so storing them
+  // directly spreads one instruction over 64 separate pieces of
+  // K * sizeof(T) bytes). They take the staged segment's first pad16(TILE)
+  // slots: the caller's carried halo comes from beyond them, and this tile's
+  // last reads of its inputs (the caller's `reads` included) came before the
+  // scan's barrier. The moment kinds, bound by their arithmetic,
+  // store directly (staging measured no faster for var / std and slower for
+  // cov, whose registers it raises: 1.93 -> 2.50 ms on float32 (256, 2M),
+  // window 1000).
+  constexpr bool XPOSE = LDS_OUT && VEC > 1 && K % 16 == 0 && !Tr::SHIFT;
   if (XPOSE && tile_full) {
-    if constexpr (Tr::SHIFT) __syncthreads();
 #pragma unroll
     for (int j = 0; j < K; ++j) la[tid * (K + K / 16) + j + (j >> 4)] = res[j];  // pad16(tid K + j)
     __syncthreads();
@@ -803,7 +810,7 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       Vec16<T> vo;
 #pragma unroll
       for (int v = 0; v < VEC; ++v) vo.v[v] = la[pad16(e) + v];
-      *reinterpret_cast<Vec16<T>*>(rot + (uint32_t)e * (uint32_t)sizeof(T)) = vo;
+      store_stream16(reinterpret_cast<T*>(rot + (uint32_t)e * (uint32_t)sizeof(T)), vo.v);
     }
   } else if (VEC > 1 && tile_full) {
     // full tile (block-uniform test): no per-thread checks, 32-bit offsets
