@@ -27,8 +27,9 @@
 // NBG_F32_ACCUMULATOR=twosum); the moment kernels (var/std/cov/corr) work on
 // data shifted by a per-tile constant with float64 first and second moments;
 // float64 data uses double-double throughout (TwoProd for squares and cross
-// products). Outputs of full tiles of the one-input kinds are passed through
-// LDS so that the stores are contiguous across each wave.
+// products). Outputs of full move_sum / move_mean tiles pass through LDS so
+// that the stores are contiguous across each wave (and stream past the
+// caches).
 #include <cstdlib>
 #include <cstring>
 

@@ -87,17 +87,13 @@ struct RaggedTile {
   int strategy;  // 0: LDS slots for labels below lds_labels, 1: global only
 };
 
-// move_mean of one tile; `group` (the tile's group pass, which reads the
-// staged inputs) runs once they are staged, before the evaluation, so that a
-// full tile's outputs can then pass through the staged segment's LDS.
-template <int VEC, int KP, bool LUT, typename G>
+template <int VEC, int KP, bool LUT>
 __device__ __forceinline__ void ragged_rolling(const RaggedArgs& A, int64_t off, int64_t n,
                                                int64_t tile, unsigned char* smem,
                                                RollingShared<Traits<float, kMean>::S>& rs,
-                                               const double* lut, int staged, int full, G& group) {
-  rolling_tile<float, kMean, true, VEC, KP, LUT, true, G&>(A.values + off, nullptr, A.move_out + off, n,
-                                                           A.window, A.min_count, tile, smem, rs, lut,
-                                                           staged, full, group);
+                                               const double* lut, int staged, int full) {
+  rolling_tile<float, kMean, true, VEC, KP, LUT>(A.values + off, nullptr, A.move_out + off, n,
+                                                 A.window, A.min_count, tile, smem, rs, lut, staged, full);
 }
 
 // Flushes the LDS label slots [0, nl) of series `s` into its global
@@ -215,52 +211,48 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
           }
         }
       }
-      // group_nanmean accumulation of the thread's KP values, read from the
-      //    staged segment (offsets [H, H + TILE); positions past the series end
-      //    hold NaN and carry label -1 here), with the labels of this thread's
-      //    KP positions (16-byte loads inside the series).
-      auto group = [&]() {
-        const int32_t* lab = A.labels + d.off;
-        int32_t lj[KP];
-        if (c0 + KP <= n) {
-#pragma unroll
-          for (int j = 0; j < KP; j += 4) {
-            const Vec16<int32_t> q = *reinterpret_cast<const Vec16<int32_t>*>(lab + c0 + j);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) lj[j + v] = q.v[v];
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < KP; ++j) lj[j] = c0 + j < n ? lab[c0 + j] : -1;
-        }
-        const int ehp = pad16((int)(c0 - ((t0 - A.window) & ~(int64_t)15)));
-#pragma unroll
-        for (int j = 0; j < KP; ++j) {
-          const float x = la[ehp + j + (j >> 4)];
-          const int32_t l = lj[j];
-          if (l >= 0 && l < K && !is_nan(x)) {
-            if (l < nl) {
-              atomicAdd(&l_sum[l], (double)x);
-              atomicAdd(&l_cnt[l], 1u);
-            } else {
-              unsafeAtomicAdd(&A.gsum[lo + l], (double)x);
-              atomicAdd(&A.gcount[lo + l], 1u);
-            }
-          }
-        }
-      };
-      // 1. move_mean of this tile (identical arithmetic to the plain kernel),
-      //    with the group pass on its staged inputs ahead of the evaluation.
+      // 1. move_mean of this tile (identical arithmetic to the plain kernel).
       if (aligned)
-        ragged_rolling<4, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full, group);
+        ragged_rolling<4, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full);
       else
-        ragged_rolling<1, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full, group);
+        ragged_rolling<1, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full);
+      // Labels of this thread's KP positions (16-byte loads inside the series).
+      const int32_t* lab = A.labels + d.off;
+      int32_t lj[KP];
+      if (c0 + KP <= n) {
+#pragma unroll
+        for (int j = 0; j < KP; j += 4) {
+          const Vec16<int32_t> q = *reinterpret_cast<const Vec16<int32_t>*>(lab + c0 + j);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) lj[j + v] = q.v[v];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < KP; ++j) lj[j] = c0 + j < n ? lab[c0 + j] : -1;
+      }
+      // 2. group_nanmean accumulation of the thread's KP values, read from the
+      //    staged segment (offsets [H, H + TILE); positions past the series end
+      //    hold NaN and carry label -1 here).
+      const int ehp = pad16((int)(c0 - ((t0 - A.window) & ~(int64_t)15)));
+#pragma unroll
+      for (int j = 0; j < KP; ++j) {
+        const float x = la[ehp + j + (j >> 4)];
+        const int32_t l = lj[j];
+        if (l >= 0 && l < K && !is_nan(x)) {
+          if (l < nl) {
+            atomicAdd(&l_sum[l], (double)x);
+            atomicAdd(&l_cnt[l], 1u);
+          } else {
+            unsafeAtomicAdd(&A.gsum[lo + l], (double)x);
+            atomicAdd(&A.gcount[lo + l], 1u);
+          }
+        }
+      }
       if (more) {
         // Carry the halo: segment offsets [TILE, TILE + H) -> [0, H). The
-        // tile's reads of its inputs and its outputs' pass through LDS (below
-        // pad16(TILE)) ended at rolling_tile's closing barrier; this barrier
-        // orders the copy's reads before the prefetched inputs overwrite
-        // [H, H + TILE).
+        // group reads above use offsets [H, H + TILE) only, so the copy may
+        // overlap them; the barrier then orders the copy's reads and the
+        // group reads before the prefetched inputs overwrite [H, H + TILE).
         for (int e = tid; e < H; e += kMoveThreads) la[pad16(e)] = la[pad16(e + TILE)];
         __syncthreads();
 #pragma unroll

@@ -444,18 +444,6 @@ __host__ __device__ inline int64_t rolling_stage_bytes(int items, int64_t window
   return ((pad16(seg) + 16) * (int64_t)sizeof(T) * nargs + 15) / 16 * 16;
 }
 
-// Calls f(std::integral_constant<int, mis>{}) for a wave-uniform mis in
-// [M, 16): one inlined copy of f per value.
-template <int M, typename F>
-__device__ __forceinline__ void with_mis16(int mis, F& f) {
-  if constexpr (M < 15) {
-    if (mis == M) f(std::integral_constant<int, M>{});
-    else with_mis16<M + 1>(mis, f);
-  } else {
-    f(std::integral_constant<int, 15>{});
-  }
-}
-
 // The first non-NaN value among staged segment elements e0 .. e0 + 255 (0 if
 // none): one LDS read and a ballot per 64 elements; wave-uniform result.
 template <typename T>
@@ -476,20 +464,15 @@ __device__ __forceinline__ T first_valid_staged(const T* seg, int e0) {
 // here); callers that know it from 32-bit tile counts keep the test on the
 // scalar unit. Ends with a barrier so the caller may reuse the shared memory
 // for its next tile.
-// No work of the caller on a tile's staged inputs.
-struct NoReads {
-  __device__ __forceinline__ void operator()() const {}
-};
-
 template <typename T, int KIND, bool TAIL_LDS, int VEC, int KP = default_items<T>(), bool LUT = false,
-          bool LDS_OUT = false, typename Reads = NoReads>
+          bool LDS_OUT = false>
 __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* __restrict__ rb,
                                              T* __restrict__ ro, int64_t n, int64_t window,
                                              int64_t min_count, int64_t tile,
                                              unsigned char* smem_raw,
                                              RollingShared<typename Traits<T, KIND>::S>& rs,
                                              const double* lut = nullptr, int staged = 0,
-                                             int full = -1, Reads reads = Reads{}) {
+                                             int full = -1) {
   using Tr = Traits<T, KIND>;
   using S = typename Tr::S;
   constexpr int K = KP;
@@ -552,9 +535,6 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     }
   }
   __syncthreads();
-  // The caller's own reads of the staged inputs (the ragged kernel's group
-  // pass over the thread's head values), ahead of this tile's evaluation.
-  reads();
 
   // Per-tile shifts for the moment kinds: the first valid value among the
   // tile's first 256 inputs (any constant works; one near the data keeps the
@@ -655,40 +635,18 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
     int dc[K];
     double dsum = 0.0;
     int dcnt = 0;
-    // Tail offsets tbase + j + ((mis + j) >> 4) with mis a compile-time
-    // constant (one copy of this pass per misalignment, picked by a uniform
-    // branch) are immediates of the LDS reads, not per-read address sums.
-    auto first_pass = [&](auto mis_c) {
-      constexpr int M = decltype(mis_c)::value;
 #pragma unroll
-      for (int j = 0; j < K; ++j) {
-        T h, u;
-        head(j, h, u);
-        const T t = la[tbase + j + ((M + j) >> 4)];
-        const bool hv = !is_nan(h), tv = !is_nan(t);
-        // NaN -> 0 selected in float32 before widening (one select each)
-        const T h0 = nan_to_zero(h), t0v = nan_to_zero(t);
-        dj[j] = (double)h0 - (double)t0v;
-        dc[j] = (int)hv - (int)tv;
-        dsum += dj[j];
-        dcnt += dc[j];
-      }
-    };
-    if constexpr (TAIL_LDS && K % 16 == 0) {
-      with_mis16<0>(mis, first_pass);
-    } else {
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        T h, t, u;
-        head(j, h, u);
-        tail(j, t, u);
-        const bool hv = !is_nan(h), tv = !is_nan(t);
-        const T h0 = nan_to_zero(h), t0v = nan_to_zero(t);
-        dj[j] = (double)h0 - (double)t0v;
-        dc[j] = (int)hv - (int)tv;
-        dsum += dj[j];
-        dcnt += dc[j];
-      }
+    for (int j = 0; j < K; ++j) {
+      T h, t, u;
+      head(j, h, u);
+      tail(j, t, u);
+      const bool hv = !is_nan(h), tv = !is_nan(t);
+      // NaN -> 0 selected in float32 before widening (one select each)
+      const T h0 = nan_to_zero(h), t0v = nan_to_zero(t);
+      dj[j] = (double)h0 - (double)t0v;
+      dc[j] = (int)hv - (int)tv;
+      dsum += dj[j];
+      dcnt += dc[j];
     }
     S delta = S::zero();
     delta.c = dcnt;
@@ -793,11 +751,11 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   // directly spreads one instruction over 64 separate pieces of
   // K * sizeof(T) bytes). They take the staged segment's first pad16(TILE)
   // slots: the caller's carried halo comes from beyond them, and this tile's
-  // last reads of its inputs (the caller's `reads` included) came before the
-  // scan's barrier. The moment kinds, bound by their arithmetic,
-  // store directly (staging measured no faster for var / std and slower for
-  // cov, whose registers it raises: 1.93 -> 2.50 ms on float32 (256, 2M),
-  // window 1000).
+  // last reads of its inputs came before the scan's barrier. The moment
+  // kinds, bound by their arithmetic, store directly (staging measured no
+  // faster for var / std and slower for cov, whose registers it raises:
+  // 1.93 -> 2.50 ms on float32 (256, 2M), window 1000). The stores stream
+  // past the caches: a (256, 2M) output is far larger than they are.
   constexpr bool XPOSE = LDS_OUT && VEC > 1 && K % 16 == 0 && !Tr::SHIFT;
   if (XPOSE && tile_full) {
 #pragma unroll
