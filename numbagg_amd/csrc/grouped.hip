@@ -451,7 +451,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
           if (r < nr) {
             const TV* vr = vbase + (int64_t)r * A.ld_v;
 #pragma unroll
-            for (int u = 0; u < U; ++u) vv[r][u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+            for (int u = 0; u < U; ++u) vv[r][u] = load_stream(reinterpret_cast<const V*>(vr + base + u * STEP));
           }
         }
 #pragma unroll
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           ll[u] = *reinterpret_cast<const W*>(lab + b0 + u * STEP);
-          vc[u] = *reinterpret_cast<const V*>(vbase + b0 + u * STEP);
+          vc[u] = load_stream(reinterpret_cast<const V*>(vbase + b0 + u * STEP));
         }
       }
       for (int64_t k = 0; k < nfull; ++k) {
@@ -484,12 +484,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
             if (r + 1 < nr) {
               const TV* vr = vbase + (int64_t)(r + 1) * A.ld_v;
 #pragma unroll
-              for (int u = 0; u < U; ++u) vn[u] = *reinterpret_cast<const V*>(vr + base + u * STEP);
+              for (int u = 0; u < U; ++u) vn[u] = load_stream(reinterpret_cast<const V*>(vr + base + u * STEP));
             } else if (k + 1 < nfull) {
 #pragma unroll
               for (int u = 0; u < U; ++u) {
                 lln[u] = *reinterpret_cast<const W*>(lab + base + SUPER + u * STEP);
-                vn[u] = *reinterpret_cast<const V*>(vbase + base + SUPER + u * STEP);
+                vn[u] = load_stream(reinterpret_cast<const V*>(vbase + base + SUPER + u * STEP));
               }
             }
 #pragma unroll

@@ -331,11 +331,11 @@ __global__ __launch_bounds__(kExpThreads) void exp_kernel(ExpArgs A) {
     if (q_vec) {
       struct alignas(16) V { TX v[VEC]; };
       for (int e = tid * VEC; e < TILE; e += kExpThreads * VEC) {
-        V va = *reinterpret_cast<const V*>(ra + q0 + e);
+        V va = load_stream(reinterpret_cast<const V*>(ra + q0 + e));
 #pragma unroll
         for (int v = 0; v < VEC; ++v) la[pad16(e + v)] = va.v[v];
         if (Sp::NARGS == 2) {
-          V vb = *reinterpret_cast<const V*>(rb + q0 + e);
+          V vb = load_stream(reinterpret_cast<const V*>(rb + q0 + e));
 #pragma unroll
           for (int v = 0; v < VEC; ++v) lb[pad16(e + v)] = vb.v[v];
         }

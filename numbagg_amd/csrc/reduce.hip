@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
     for (; e + SUPER <= c1; e += SUPER) {
       V vv[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) vv[u] = *reinterpret_cast<const V*>(xr + e + u * STEP + lane * VEC);
+      for (int u = 0; u < U; ++u) vv[u] = load_stream(reinterpret_cast<const V*>(xr + e + u * STEP + lane * VEC));
       if constexpr (is_var_kind(KIND)) {
         T b[U * VEC];
 #pragma unroll

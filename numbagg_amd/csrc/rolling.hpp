@@ -889,11 +889,11 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(mo
 #pragma unroll
       for (int m = 0; m < NV; ++m) {
         const uint32_t o = (threadIdx.x + (uint32_t)m * kMoveThreads) * 16u;  // bytes
-        const Vec16<T> va = *reinterpret_cast<const Vec16<T>*>(rat + o);
+        const Vec16<T> va = load_stream(reinterpret_cast<const Vec16<T>*>(rat + o));
 #pragma unroll
         for (int v = 0; v < VEC; ++v) pa[m * VEC + v] = va.v[v];
         if constexpr (Tr::NARGS == 2) {
-          const Vec16<T> vb = *reinterpret_cast<const Vec16<T>*>(rbt + o);
+          const Vec16<T> vb = load_stream(reinterpret_cast<const Vec16<T>*>(rbt + o));
 #pragma unroll
           for (int v = 0; v < VEC; ++v) pb[m * VEC + v] = vb.v[v];
         }

@@ -115,7 +115,10 @@ int launch_zero(void* p, size_t bytes, hipStream_t s) {
 __global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src,
                                                           uint4* __restrict__ dst, int64_t n16) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n16) dst[i] = src[i];
+  if (i < n16) {
+    const uint4 v = load_stream(src + i);
+    store_stream16(reinterpret_cast<unsigned int*>(dst + i), *reinterpret_cast<const unsigned int(*)[4]>(&v));
+  }
 }
 
 }  // namespace nbg
