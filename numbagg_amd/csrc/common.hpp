@@ -10,8 +10,6 @@
 #include <stdint.h>
 #include <stddef.h>
 
-#include <type_traits>
-
 #include "../../include/numbagg_amd.h"
 
 namespace nbg {
@@ -223,33 +221,58 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n) {
 __host__ __device__ __forceinline__ int64_t pad16(int64_t e) { return e + (e >> 4); }
 __host__ __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 
+// Integer vectors of SIZE bytes at alignment ALIGN (the alignment of the
+// values they carry: element-aligned 16-byte accesses are legal on gfx9+).
+template <int SIZE, int ALIGN>
+struct NtVec;
+template <>
+struct NtVec<4, 4> {
+  typedef int t;
+};
+template <>
+struct NtVec<8, 4> {
+  typedef int t __attribute__((ext_vector_type(2), aligned(4)));
+};
+template <>
+struct NtVec<8, 8> {
+  typedef int t __attribute__((ext_vector_type(2), aligned(8)));
+};
+template <>
+struct NtVec<16, 4> {
+  typedef int t __attribute__((ext_vector_type(4), aligned(4)));
+};
+template <>
+struct NtVec<16, 8> {
+  typedef int t __attribute__((ext_vector_type(4), aligned(8)));
+};
+template <>
+struct NtVec<16, 16> {
+  typedef int t __attribute__((ext_vector_type(4), aligned(16)));
+};
+
 // A load of a 4-, 8- or 16-byte value (vector of values) that is read once:
 // non-temporal, so the stream does not displace reused data from the caches
 // (measured on MI355X: a 16-byte-per-thread copy of 4 GB runs at 6.5 TB/s
 // with non-temporal loads and stores, 6.26 TB/s with plain ones).
 template <typename S>
 __device__ __forceinline__ S load_stream(const S* p) {
-  static_assert(sizeof(S) == 4 || sizeof(S) == 8 || sizeof(S) == 16, "4, 8 or 16 bytes");
-  typedef int v2i __attribute__((ext_vector_type(2)));
-  typedef int v4i __attribute__((ext_vector_type(4)));
-  using R = typename std::conditional<sizeof(S) == 4, int, typename std::conditional<sizeof(S) == 8, v2i, v4i>::type>::type;
+  using R = typename NtVec<(int)sizeof(S), (int)alignof(S)>::t;
   const R x = __builtin_nontemporal_load(reinterpret_cast<const R*>(p));
   S r;
   __builtin_memcpy(&r, &x, sizeof(S));
   return r;
 }
 
-// A 16-byte store of 16 / sizeof(T) values to a 16-byte aligned address,
+// A 16-byte store of 16 / sizeof(T) values (at the alignment of T),
 // non-temporal: for outputs written once in whole-wave contiguous runs, which
 // then go to HBM without lingering as dirty lines in the caches (their later
 // write-back would take bandwidth from whatever kernel runs next).
 template <typename T>
 __device__ __forceinline__ void store_stream16(T* p, const T (&v)[16 / sizeof(T)]) {
-  typedef T vt __attribute__((ext_vector_type(16 / sizeof(T))));
-  vt x;
-#pragma unroll
-  for (int i = 0; i < (int)(16 / sizeof(T)); ++i) x[i] = v[i];
-  __builtin_nontemporal_store(x, reinterpret_cast<vt*>(p));
+  using R = typename NtVec<16, (int)alignof(T)>::t;
+  R x;
+  __builtin_memcpy(&x, v, 16);
+  __builtin_nontemporal_store(x, reinterpret_cast<R*>(p));
 }
 
 // Zero-fill kernel used to reset look-back status words and ticket counters.

@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
         for (int m = 0; m < NV; ++m) {
           const int64_t p = t0n + ((int64_t)tid + (int64_t)m * kMoveThreads) * 4;
           if (nfull || p + 4 <= n) {
-            const Vec16<float> q = *reinterpret_cast<const Vec16<float>*>(xs + p);
+            const Vec16<float> q = load_stream(reinterpret_cast<const Vec16<float>*>(xs + p));
 #pragma unroll
             for (int v = 0; v < 4; ++v) pa[m * 4 + v] = q.v[v];
           } else {
@@ -216,7 +216,9 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(3)
         ragged_rolling<4, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full);
       else
         ragged_rolling<1, KP, LUT>(A, d.off, n, tile, smem_raw, rs, lut, staged, full);
-      // Labels of this thread's KP positions (16-byte loads inside the series).
+      // Labels of this thread's KP positions (16-byte loads inside the series;
+      // plain loads: the thread's four loads are strided across the wave and
+      // share cache lines, which streaming loads would fetch four times).
       const int32_t* lab = A.labels + d.off;
       int32_t lj[KP];
       if (c0 + KP <= n) {
