@@ -89,3 +89,20 @@ def test_cost_model_load_missing_or_unreadable(tmp_path, monkeypatch):
     assert CostModel.load(bad) is None
     monkeypatch.setenv("NBG_COST_MODEL", str(bad))
     assert CostModel.load() is None
+
+
+def test_launch_candidates_add_the_uniform_launch():
+    """Each close slot count is timed with the model's strategies and, when
+    they are mixed, with every series on the strategy holding most elements."""
+    from numbagg_amd.tuning import STRATEGY_GLOBAL, STRATEGY_LDS, launch_candidates
+
+    lengths = np.array([1000, 10, 10, 500])
+    all_lds = np.full(4, STRATEGY_LDS, np.uint8)
+    mixed = np.array([STRATEGY_LDS, STRATEGY_GLOBAL, STRATEGY_GLOBAL, STRATEGY_LDS], np.uint8)
+    mostly_global = np.array([STRATEGY_GLOBAL, STRATEGY_LDS, STRATEGY_LDS, STRATEGY_GLOBAL], np.uint8)
+    c = launch_candidates([(1.0, 4096, all_lds)], lengths)
+    assert len(c) == 1 and c[0][0] == 4096 and np.array_equal(c[0][1], all_lds)
+    c = launch_candidates([(1.0, 4096, mixed), (1.5, 1024, mostly_global)], lengths)
+    assert [cap for cap, _ in c] == [4096, 4096, 1024, 1024]
+    assert np.array_equal(c[0][1], mixed) and np.array_equal(c[1][1], all_lds)
+    assert np.array_equal(c[2][1], mostly_global) and np.all(c[3][1] == STRATEGY_GLOBAL)
