@@ -123,10 +123,6 @@ struct GArgs {
   const double* means;  // kPDev: per (row, label) mean
   int argmin;           // kPPacked / kPKey / kPKeyIdx: complement keys
   int mom;              // variance finalize: from shifted moments (kPMom) instead of two passes
-  // Label occurrences (positions with a label in [0, num_labels), shared by
-  // every row): the counted passes count each (row, label)'s invalid values,
-  // and a count is the label's occurrences minus them.
-  const unsigned long long* hist;
 };
 
 // Canonical NaN bit patterns: the "no shift yet" marker of kPMom slots.
@@ -308,28 +304,17 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
     } else {
       if ((uint64_t)lbl >= (uint64_t)L1) return;
     }
-    if (!gvalid(v)) {
-      // counted passes count the invalid values (NaN is rare; the valid count
-      // is the label's occurrences minus them)
-      if constexpr ((PASS == kPSum && COUNT) || PASS == kPCount) {
-        const int64_t l = LDS ? (int64_t)(r * (int)L1 + (int)lbl) : (int64_t)r * L1 + (int64_t)lbl;
-        if constexpr (PASS == kPSum) {
-          if (LDS) atomicAdd(&l_c[l], 1u);
-          else atomicAdd(&g_u1[l], 1ull);
-        } else {
-          if (LDS) atomicAdd(&l_u32[l], 1u);
-          else atomicAdd(&g_u0[l], 1ull);
-        }
-      }
-      return;
-    }
-    if constexpr (PASS == kPCount) return;  // a valid value: nothing to add
+    if (!gvalid(v)) return;
     if constexpr (H) {
       if ((uint32_t)lbl < 2u) {  // r is a compile-time index at every call site
         const bool first = lbl == 0;
         if constexpr (PASS == kPSum) {
           h_s[r][0] += first ? (double)v : 0.0;
           h_s[r][1] += first ? 0.0 : (double)v;
+          if (COUNT) {
+            h_c[r][0] += first ? 1u : 0u;
+            h_c[r][1] += first ? 0u : 1u;
+          }
         } else if constexpr (PASS == kPMom) {
           // the thread's first value of the label is its shift
           const TV kc = first ? h_k[r][0] : h_k[r][1];
@@ -361,8 +346,13 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
     // LDS slots fit 32-bit indexing (the LDS budget bounds R * L1)
     const int64_t l = LDS ? (int64_t)(r * (int)L1 + (int)lbl) : (int64_t)r * L1 + (int64_t)lbl;
     if constexpr (PASS == kPSum) {
-      if (LDS) atomicAdd(&l_s[l], (SA)v);  // ds_add_f64 / ds_add_u64
-      else g_atomic_add(&g_s[l], (SA)v);
+      if (LDS) {
+        atomicAdd(&l_s[l], (SA)v);  // ds_add_f64 / ds_add_u64
+        if (COUNT) atomicAdd(&l_c[l], 1u);
+      } else {
+        g_atomic_add(&g_s[l], (SA)v);
+        if (COUNT) atomicAdd(&g_u1[l], 1ull);
+      }
     } else if constexpr (PASS == kPSumSq) {
       const SA sq = (SA)v * (SA)v;  // exact float64 squares of float32 data; wrapping for integers
       if (LDS) atomicAdd(&l_s[l], sq);
@@ -376,6 +366,9 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
         if (LDS) atomicOr(&l_u32[l], 1u);
         else atomicOr(&g_u0[l], 1ull);
       }
+    } else if constexpr (PASS == kPCount) {
+      if (LDS) atomicAdd(&l_u32[l], 1u);
+      else atomicAdd(&g_u0[l], 1ull);
     } else if constexpr (PASS == kPPacked) {
       uint32_t k = okey32(v);
       if (A.argmin) k = ~k;
@@ -565,15 +558,18 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
           double x = h_s[r][h];
+          unsigned int c = h_c[r][h];
 #pragma unroll
           for (int o = kWave / 2; o > 0; o >>= 1) {
             if constexpr (PASS == kPProd) x *= __shfl_xor(x, o);
             else x += __shfl_xor(x, o);
+            if (COUNT) c += __shfl_xor(c, o);
           }
           if (lane_id() == 0 && r < nr && h < L1) {
             const int l = r * (int)L1 + h;
             if constexpr (PASS == kPSum) {
               atomicAdd(&l_s[l], x);
+              if (COUNT) atomicAdd(&l_c[l], c);
             } else if constexpr (PASS == kPSumSq) {
               atomicAdd(&l_s[l], x);
             } else if constexpr (PASS == kPProd) {
@@ -588,10 +584,14 @@ __global__ __launch_bounds__(kGroupThreads) void group_pass_kernel(GArgs A) {
     __syncthreads();
     for (int64_t l = tid; l < Lp; l += kGroupThreads) {
       if constexpr (PASS == kPSum) {
-        if (l_s[l] != SA(0)) g_atomic_add(&g_s[l], l_s[l]);
         if (COUNT) {
-          const unsigned int c = l_c[l];  // invalid values
-          if (c) atomicAdd(&g_u1[l], (unsigned long long)c);
+          const unsigned int c = l_c[l];
+          if (c) {
+            g_atomic_add(&g_s[l], l_s[l]);
+            atomicAdd(&g_u1[l], (unsigned long long)c);
+          }
+        } else if (l_s[l] != SA(0)) {
+          g_atomic_add(&g_s[l], l_s[l]);
         }
       } else if constexpr (PASS == kPSumSq) {
         if (l_s[l] != SA(0)) g_atomic_add(&g_s[l], l_s[l]);
@@ -642,32 +642,6 @@ __global__ void group_init_kernel(unsigned long long* acc0, unsigned long long i
   }
 }
 
-// Label occurrences: hist[l] += positions with label l in [0, L). Per-block
-// LDS counts when L fits (lds != 0), flushed once; global atomics otherwise.
-template <typename TL>
-__global__ __launch_bounds__(256) void group_hist_kernel(const TL* __restrict__ lab, int64_t n, int64_t L,
-                                                         unsigned long long* hist, int lds) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  unsigned int* h = reinterpret_cast<unsigned int*>(smem_raw);
-  if (lds) {
-    for (int64_t l = threadIdx.x; l < L; l += blockDim.x) h[l] = 0u;
-    __syncthreads();
-  }
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const TL l = lab[i];
-    if (l >= 0 && (int64_t)l < L) {
-      if (lds) atomicAdd(&h[l], 1u);
-      else atomicAdd(&hist[l], 1ull);
-    }
-  }
-  if (lds) {
-    __syncthreads();
-    for (int64_t l = threadIdx.x; l < L; l += blockDim.x)
-      if (h[l]) atomicAdd(&hist[l], (unsigned long long)h[l]);
-  }
-}
-
 template <typename TV>
 __device__ __forceinline__ TV missing_value() {
   if (sizeof(TV) == 4 && (TV)0.5 != 0) return (TV)canonical_nan<float>();
@@ -686,10 +660,9 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t row = q / A.num_labels;
-    const int64_t label = q - row * A.num_labels;
     if (means_out) {
       const double s = static_cast<const double*>(A.acc0)[q];
-      const unsigned long long c = A.hist[label] - static_cast<const unsigned long long*>(A.acc1)[q];
+      const unsigned long long c = static_cast<const unsigned long long*>(A.acc1)[q];
       means_out[q] = c ? s / (double)c : 0.0;
       continue;
     }
@@ -698,10 +671,10 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
       r = (TV)static_cast<const SA*>(A.acc0)[q];  // integers: modular narrowing
     } else if constexpr (KIND == kGMean) {
       const double s = static_cast<const double*>(A.acc0)[q];
-      const unsigned long long c = A.hist[label] - static_cast<const unsigned long long*>(A.acc1)[q];
+      const unsigned long long c = static_cast<const unsigned long long*>(A.acc1)[q];
       r = c ? (TV)(s / (double)c) : missing_value<TV>();
     } else if constexpr (KIND == kGCount) {
-      r = (TV)(A.hist[label] - static_cast<const unsigned long long*>(A.acc0)[q]);
+      r = (TV)static_cast<const unsigned long long*>(A.acc0)[q];
     } else if constexpr (KIND == kGArgMax || KIND == kGArgMin || KIND == kGMin || KIND == kGMax) {
       // index of the extreme: packed (key32, ~idx32) words, or the key-index pass
       bool has;
@@ -733,9 +706,7 @@ __global__ void group_finalize_kernel(GArgs A, double* means_out) {
       const unsigned long long i = static_cast<const unsigned long long*>(A.acc0)[q];
       r = i ? vals[row * A.ld_v + (int64_t)(i - 1)] : missing_value<TV>();
     } else {  // var / std from (count, sum of squared deviations)
-      // shifted moments count the values; the two-pass sum counts the invalid ones
-      const unsigned long long c2 = static_cast<const unsigned long long*>(A.acc2)[q];
-      const unsigned long long c = A.mom ? c2 : A.hist[label] - c2;
+      const unsigned long long c = static_cast<const unsigned long long*>(A.acc2)[q];
       double m2 = static_cast<const double*>(A.acc1)[q];
       if (A.mom && c) {
         // shifted moments: M2 = sum (x-Kg)^2 - (sum (x-Kg))^2 / n, kept >= 0
@@ -888,24 +859,6 @@ int run_pass(GArgs a, bool lds, hipStream_t s) {
 }
 
 int launch_init(void* acc0, unsigned long long init0, void* acc1, unsigned long long init1,
-                int64_t count, hipStream_t s);
-
-// Label occurrences into `hist` (L words, cleared here first).
-constexpr int64_t kHistLdsLabels = 8192;  // 32 KB of per-block counts
-template <typename TL>
-int launch_hist(const void* labels, int64_t n, int64_t L, unsigned long long* hist, hipStream_t s) {
-  if (int rc = launch_init(hist, 0ull, nullptr, 0ull, L, s)) return rc;
-  if (n == 0) return NBG_OK;
-  const int lds = L <= kHistLdsLabels ? 1 : 0;
-  int64_t blocks = (n + 256 * 16 - 1) / (256 * 16);  // about 16 labels per thread
-  if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL((group_hist_kernel<TL>), dim3((unsigned)blocks), dim3(256),
-                     lds ? (size_t)L * sizeof(unsigned int) : 0, s, static_cast<const TL*>(labels), n, L, hist,
-                     lds);
-  return check_hip(hipGetLastError(), "group_hist_kernel launch");
-}
-
-int launch_init(void* acc0, unsigned long long init0, void* acc1, unsigned long long init1,
                 int64_t count, hipStream_t s) {
   int64_t blocks = (count + 255) / 256;
   if (blocks > 8192) blocks = 8192;
@@ -946,18 +899,13 @@ int64_t lds_bytes_per_label(int kind, int vdtype) {
   (void)vdtype;
 }
 
-// Workspace layout: acc0 | acc1 | acc2 | acc3 (rows x num_labels x 8 bytes
-// each), then for var / std the label occurrences (num_labels x 8 bytes);
-// mean and count keep them in an accumulator they do not use.
+// Workspace layout: acc0 | acc1 | acc2 | acc3 (rows x num_labels x 8 bytes each).
 size_t group_ws_bytes(int64_t rows, int64_t L) { return (size_t)(4 * rows * L) * 8; }
-size_t group_ws_bytes_kind(int64_t rows, int64_t L, int kind) {
-  return group_ws_bytes(rows, L) + ((kind == kGVar || kind == kGStd) ? (size_t)L * 8 : 0);
-}
 
 template <typename TV, typename TL, int KIND>
 int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) {
   const int64_t L = a.num_labels;
-  const size_t need = group_ws_bytes_kind(a.rows, L, KIND);
+  const size_t need = group_ws_bytes(a.rows, L);
   if (ws == nullptr || ws_bytes < need)
     return set_error(NBG_ERR_INVALID, "group: workspace too small (%zu < %zu)", ws_bytes, need);
   const int64_t per = (int64_t)a.rows * L;
@@ -989,18 +937,14 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
       return launch_finalize<TV, KIND>(a, nullptr, s);
     case kGMean:
       if ((rc = launch_init(acc0, 0ull, acc1, 0ull, per, s))) return rc;
-      if ((rc = launch_hist<TL>(a.labels, a.n, L, acc2, s))) return rc;
       a.acc0 = acc0;
       a.acc1 = acc1;
-      a.hist = acc2;
       if ((rc = run_pass<TV, TL, kPSum>(a, lds, s))) return rc;
       return launch_finalize<TV, KIND>(a, nullptr, s);
     case kGCount:
       if ((rc = launch_init(acc0, 0ull, nullptr, 0ull, per, s))) return rc;
-      if ((rc = launch_hist<TL>(a.labels, a.n, L, acc1, s))) return rc;
       a.acc0 = acc0;
       a.acc1 = nullptr;
-      a.hist = acc1;
       if ((rc = run_pass<TV, TL, kPCount>(a, lds, s))) return rc;
       return launch_finalize<TV, KIND>(a, nullptr, s);
     case kGSumSq:
@@ -1074,17 +1018,15 @@ int group_impl(GArgs a, int strategy, void* ws, size_t ws_bytes, hipStream_t s) 
       // (the usual two-pass correction term) is negligible: with n values
       // per label its share of sum d^2 is about (n 2**-53 |mean| / std)^2.
       if ((rc = launch_init(acc0, 0ull, acc1, 0ull, per, s))) return rc;
-      if ((rc = launch_hist<TL>(a.labels, a.n, L, acc3 + per, s))) return rc;
       a.acc0 = acc0;
       a.acc1 = acc1;
-      a.hist = acc3 + per;
       if ((rc = run_pass<TV, TL, kPSum>(a, lds, s))) return rc;
       double* means = reinterpret_cast<double*>(acc2);
       if ((rc = launch_finalize<TV, KIND>(a, means, s))) return rc;
       GArgs b = a;
       b.means = means;
       b.acc1 = acc3;  // sum of squared deviations
-      b.acc2 = acc1;  // invalid-value counts from the first pass
+      b.acc2 = acc1;  // counts from the first pass
       if ((rc = launch_init(acc3, 0ull, nullptr, 0ull, per, s))) return rc;
       if ((rc = run_pass<TV, TL, kPDev>(b, lds, s))) return rc;
       return launch_finalize<TV, KIND>(b, nullptr, s);
@@ -1195,7 +1137,7 @@ size_t nbg_group_workspace_bytes(int vdtype, int64_t rows, int64_t n, int64_t nu
   (void)n;
   (void)strategy;
   if (rows <= 0 || num_labels <= 0) return 0;
-  return nbg::group_ws_bytes_kind(rows, num_labels, nbg::kGVar);  // the largest of every function's
+  return nbg::group_ws_bytes(rows, num_labels);
 }
 
 #define NBG_GROUP(NAME, KIND)                                                                  \
