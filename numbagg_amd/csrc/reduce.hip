@@ -92,6 +92,22 @@ struct Chan {
   }
 };
 
+// float32 extremes keep 32-bit keys, the best value and a 32-bit index
+// within the chunk per lane (chunks are below 2**31 positions); the 64-bit
+// key and the absolute index are formed once, when the lane's part is taken.
+template <typename T, int KIND>
+constexpr bool small_ext() {
+  return is_ext_kind(KIND) && sizeof(T) == 4 && std::is_floating_point<T>::value;
+}
+
+// Orderable 32-bit key of a float32 value (-0 as +0): larger values give
+// larger keys (complemented for minima); every non-NaN value's key is nonzero.
+__device__ __forceinline__ uint32_t ext_key32(float v, bool minimum) {
+  const uint32_t b = __float_as_uint(v + 0.0f);
+  const uint32_t k = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  return minimum ? ~k : k;
+}
+
 template <typename T, int KIND>
 struct Lane {
   double s = 0.0;                                 // float32 data: float64 sum
@@ -101,6 +117,21 @@ struct Lane {
   unsigned long long cnt = 0ull, nans = 0ull;
   Chan ch = Chan{0.0, 0.0, 0.0};
   unsigned long long key = 0ull, idx = 0ull;
+  uint32_t key32 = 0u, idx32 = 0u;  // small_ext: key (0: none yet), index within the chunk
+  T best = T(0);                    // small_ext: the value of key32
+
+  // small_ext: one element at position rel within the chunk.
+  __device__ __forceinline__ void fold32(T v, uint32_t rel) {
+    if (!is_nan(v)) {
+      const uint32_t k = ext_key32((float)v, KIND == kRMin || KIND == kRArgMin);
+      const bool take = first_wins(KIND) ? k > key32 : k >= key32;
+      if (take) {
+        key32 = k;
+        idx32 = rel;
+        best = v;
+      }
+    }
+  }
 
   // One element at position i (non-variance kinds).
   __device__ __forceinline__ void fold(T v, int64_t i) {
@@ -158,8 +189,16 @@ struct Lane {
     ch.merge((double)nb, mb, m2b);
   }
 
-  __device__ __forceinline__ RPart to_part() const {
+  // c0: the chunk's first position (small_ext indices are relative to it).
+  __device__ __forceinline__ RPart to_part(int64_t c0) const {
     RPart p{0ull, 0ull, 0ull, 0ull};
+    if constexpr (small_ext<T, KIND>()) {
+      if (key32 != 0u) {
+        p.w0 = ext_key(best, KIND == kRMin || KIND == kRArgMin);
+        p.w1 = (unsigned long long)(c0 + (int64_t)idx32);
+      }
+      return p;
+    }
     if constexpr (KIND == kRSum || KIND == kRMean) {
       if constexpr (!std::is_floating_point<T>::value) {
         p.w0 = us;
@@ -264,6 +303,12 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
 #pragma unroll
           for (int j = 0; j < VEC; ++j) b[u * VEC + j] = vv[u].v[j];
         st.template fold_batch<U * VEC>(b);
+      } else if constexpr (small_ext<T, KIND>()) {
+        const uint32_t rel = (uint32_t)(e - c0) + (uint32_t)(lane * VEC);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) st.fold32(vv[u].v[j], rel + (uint32_t)(u * STEP + j));
       } else {
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -277,11 +322,13 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
     if constexpr (is_var_kind(KIND)) {
       const T b[1] = {xr[i]};
       st.template fold_batch<1>(b);
+    } else if constexpr (small_ext<T, KIND>()) {
+      st.fold32(xr[i], (uint32_t)(i - c0));
     } else {
       st.fold(xr[i], i);
     }
   }
-  RPart p = st.to_part();
+  RPart p = st.to_part(c0);
 #pragma unroll
   for (int m = 1; m < kWave; m <<= 1) {
     const RPart q = shfl_xor_part(p, m);
@@ -407,7 +454,9 @@ int64_t reduce_chunk(int64_t rows, int64_t n, int64_t super_elems) {
   const int64_t min_chunk = 16384 / (int64_t)sizeof(T);
   if (chunk < min_chunk) chunk = min_chunk;
   chunk = (chunk + super_elems - 1) / super_elems * super_elems;
-  return chunk;
+  // below 2**31 positions (float32 extremes index within the chunk in 32 bits)
+  const int64_t max_chunk = ((int64_t)1 << 31) / super_elems * super_elems;
+  return chunk < max_chunk ? chunk : max_chunk;
 }
 
 template <typename T>

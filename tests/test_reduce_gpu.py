@@ -278,3 +278,26 @@ def test_repeatable_bits(gpu_lib):
     for f in (nb.nansum, nb.nanmean, nb.nanvar):
         a, b = f(x, axis=-1), f(x, axis=-1)
         assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("func", ["nanmax", "nanmin", "nanargmax", "nanargmin"])
+def test_extreme_ties_signed_zeros_and_infinities(gpu_lib, func, dtype):
+    """Repeated extremes spread over many chunks and lanes (the first index
+    wins, as the reference's strict comparison keeps it), -0.0 beside +0.0
+    and infinities: bit-exact against the C restatement."""
+    import numbagg_amd as nb
+    from oracle import restated
+
+    rng = np.random.default_rng(7)
+    n = 2_000_003
+    x = rng.integers(-3, 4, size=(4, n)).astype(dtype)  # many ties of every value
+    x[rng.random(x.shape) < 0.05] = np.nan
+    x[1] = np.where(x[1] == 0, -0.0, x[1])  # signed zeros: row 1 all -0.0 where 0
+    x[2, rng.integers(0, n, 50)] = np.inf
+    x[2, rng.integers(0, n, 50)] = -np.inf
+    x[3] = rng.choice(np.array([0.0, -0.0, np.nan], dtype=dtype), size=n)  # zeros of both signs only
+    got = getattr(nb, func)(x, axis=-1)
+    want = restated.reduce(func, x)
+    assert got.dtype == want.dtype
+    assert_array_equal(got.view(np.uint8), want.view(np.uint8))
