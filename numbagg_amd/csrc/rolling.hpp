@@ -305,6 +305,16 @@ struct VarTraits {
       return (T)(KIND == kStd ? sqrt(var) : var);
     }
   }
+  // float32 with tables in LDS: rc[c] and rc1[c] hold the refined
+  // reciprocals 1/max(c, 1) and 1/max(c - 1, 1) that out() computes, so the
+  // results are the same.
+  __device__ static T out_tab(const S& s, int64_t mc, const double* rc, const double* rc1) {
+    const int64_t mce = mc > 2 ? mc : 2;
+    const double m1 = s.s1.to_double();
+    double var = (s.s11.to_double() - m1 * m1 * rc[s.c]) * rc1[s.c];
+    var = s.c >= mce ? var : canonical_nan<double>();
+    return (T)(KIND == kStd ? sqrt_for_f32(var) : var);
+  }
 };
 template <typename T>
 struct Traits<T, kVar> : VarTraits<T, kVar> {};
@@ -333,19 +343,26 @@ struct Traits<T, kCorr> {
   using S = MomStats<T, 2>;
   static constexpr int NARGS = 2;
   static constexpr bool SHIFT = true;
+  // float32: the correlation from r = 1/max(c, 1)
+  __device__ static T from_r(const S& s, int64_t mce, double r) {
+    const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
+    const double va = s.s11.to_double() * r - ma * ma;
+    const double vb = s.s22.to_double() * r - mb * mb;
+    const double cv = s.s12.to_double() * r - ma * mb;
+    const double vv = va * vb;
+    const bool in_range = vv > 0x1p-1000 && vv < 0x1p1000;
+    double res = cv * rsqrt_refined(in_range ? vv : 1.0);
+    if (!in_range && vv > 0) res = cv / sqrt(vv);  // extreme scales only
+    return (T)((s.c >= mce && vv > 0) ? res : canonical_nan<double>());
+  }
+  // float32 with the reciprocal table (the value recip forms)
+  __device__ static T out_tab(const S& s, int64_t mc, const double* rc, const double*) {
+    return from_r(s, mc > 1 ? mc : 1, rc[s.c]);
+  }
   __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
     const int64_t mce = mc > 1 ? mc : 1;
     if constexpr (sizeof(T) == 4) {
-      const double r = recip(s.c > 0 ? s.c : 1);
-      const double ma = s.s1.to_double() * r, mb = s.s2.to_double() * r;
-      const double va = s.s11.to_double() * r - ma * ma;
-      const double vb = s.s22.to_double() * r - mb * mb;
-      const double cv = s.s12.to_double() * r - ma * mb;
-      const double vv = va * vb;
-      const bool in_range = vv > 0x1p-1000 && vv < 0x1p1000;
-      double res = cv * rsqrt_refined(in_range ? vv : 1.0);
-      if (!in_range && vv > 0) res = cv / sqrt(vv);  // extreme scales only
-      return (T)((s.c >= mce && vv > 0) ? res : canonical_nan<double>());
+      return from_r(s, mce, recip(s.c > 0 ? s.c : 1));
     } else {
       if (s.c < mce) return canonical_nan<T>();
       const double r = 1.0 / (double)s.c;
@@ -742,7 +759,8 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       if (valid(a, b)) st.add_obs(a, b, ka, kb);
       tail(j, a, b);
       if (valid(a, b)) st.sub_obs(a, b, ka, kb);
-      res[j] = Tr::out(st, min_count, nullptr);
+      if constexpr (LUT) res[j] = Tr::out_tab(st, min_count, lut, lut + window + 1);  // c in [0, window]
+      else res[j] = Tr::out(st, min_count, nullptr);
     }
   }
   // LDS_OUT (sum, mean): a full tile's outputs go through LDS so that each
@@ -798,7 +816,8 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   __syncthreads();
 }
 
-// Largest window for which mean keeps a 1/c table in LDS.
+// Largest window for which mean (and float32 var / std / corr) keep reciprocal
+// tables in LDS.
 constexpr int64_t kMeanLutWindow = 4096;
 
 // lut[c] = 1/c (hardware reciprocal + one Newton step) for c >= mce, NaN for
@@ -808,6 +827,22 @@ __device__ __forceinline__ void fill_recip_lut(double* lut, int64_t window, int6
   for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads)
     lut[c] = c >= mce ? recip((int)c) : canonical_nan<double>();
 }
+
+// Moment tables: rc[c] = 1/max(c, 1) and, with two tables, rc1[c] =
+// 1/max(c - 1, 1) following it (refined reciprocals, as count_recips forms
+// them), c = 0..window.
+__device__ __forceinline__ void fill_moment_luts(double* rc, int64_t window, bool two) {
+  double* rc1 = rc + window + 1;
+  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads) {
+    const CountRecips q = count_recips((int)c);
+    rc[c] = q.r;
+    if (two) rc1[c] = q.r1;
+  }
+}
+
+// Reciprocal tables of (window + 1) doubles a tile with tables uses: 1/c for
+// mean and corr, 1/c and 1/(c - 1) for var and std.
+__host__ __device__ constexpr int lut_tables(int kind) { return (kind == kMean || kind == kCorr) ? 1 : 2; }
 
 // Items per thread for float32 sum/mean: 16 (default) or 32, from
 // NBG_MOVE_ITEMS; a tuning knob (more items amortise the per-tile scan and
@@ -861,7 +896,9 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(mo
   double* lut = nullptr;
   if constexpr (LUT) {
     lut = reinterpret_cast<double*>(smem_raw + rolling_stage_bytes<T>(KP, window, TAIL_LDS, Tr::NARGS));
-    fill_recip_lut(lut, window, min_count > 1 ? min_count : 1);  // ordered by the tile's first barrier
+    // ordered by the tile's first barrier
+    if constexpr (KIND == kMean) fill_recip_lut(lut, window, min_count > 1 ? min_count : 1);
+    else fill_moment_luts(lut, window, lut_tables(KIND) == 2);
   }
   const T* ra = xa + row * ld_a;
   const T* rb = (Tr::NARGS == 2) ? xb + row * ld_b : nullptr;
