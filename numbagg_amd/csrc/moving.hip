@@ -63,15 +63,15 @@ int move_launch_k(const T* a, const T* b, T* out, int64_t rows, int64_t n, int64
   // any element offset or stride take the vector path.
   constexpr int MAXV = 16 / (int)sizeof(T);
   const int v_use = pick_vec_elem<T>(a, Tr::NARGS == 2 ? b : a, out);
-  // reciprocal tables in LDS (float32): 1/c for mean and corr, 1/c and
-  // 1/(c - 1) for var and std, read instead of refined reciprocals per output
-  // (move_var 1.27 -> 1.20 ms, move_std 1.43 -> 1.34, move_corr 2.37 -> 2.33 on
-  // (256, 2M), window 1000). Not cov: its two tables beside two staged inputs
-  // leave room for 2 workgroups per CU instead of 3 (1.93 -> 2.51 ms).
-  constexpr bool LUT_OK = (KIND == kMean || KIND == kVar || KIND == kStd || KIND == kCorr) && sizeof(T) == 4;
+  // a reciprocal table in LDS (float32): 1/c for mean (NaN below the
+  // effective min_count), 1/max(c, 1) for the moment kinds, read instead of
+  // refined reciprocals per output, which also frees registers (move_var 1.26
+  // -> 1.20 ms, move_std 1.41 -> 1.31, move_cov 1.92 -> 1.88 on (256, 2M),
+  // window 1000)
+  constexpr bool LUT_OK = KIND != kSum && KIND < 16 && sizeof(T) == 4;
   const bool use_lut = LUT_OK && mean_uses_lut(window);
   size_t lds = (size_t)rolling_stage_bytes<T>(KP, window, tail_lds, Tr::NARGS);
-  if (use_lut) lds += (size_t)lut_tables(KIND) * (size_t)(window + 1) * sizeof(double);
+  if (use_lut) lds += (size_t)(window + 1) * sizeof(double);
   dim3 grid((unsigned)blocks), block(kMoveThreads);
   // Each next tile of a span is prefetched into registers while the current
   // one is evaluated (more VGPRs, fewer resident waves; measured faster on

@@ -305,13 +305,13 @@ struct VarTraits {
       return (T)(KIND == kStd ? sqrt(var) : var);
     }
   }
-  // float32 with tables in LDS: rc[c] and rc1[c] hold the refined
-  // reciprocals 1/max(c, 1) and 1/max(c - 1, 1) that out() computes, so the
-  // results are the same.
-  __device__ static T out_tab(const S& s, int64_t mc, const double* rc, const double* rc1) {
+  // float32 with the table in LDS: rc[c] holds the refined reciprocal
+  // 1/max(c, 1) that count_recips forms, and rc[max(c - 1, 0)] is its
+  // 1/max(c - 1, 1), so the results are the same.
+  __device__ static T out_tab(const S& s, int64_t mc, const double* rc) {
     const int64_t mce = mc > 2 ? mc : 2;
     const double m1 = s.s1.to_double();
-    double var = (s.s11.to_double() - m1 * m1 * rc[s.c]) * rc1[s.c];
+    double var = (s.s11.to_double() - m1 * m1 * rc[s.c]) * rc[s.c > 0 ? s.c - 1 : 0];
     var = s.c >= mce ? var : canonical_nan<double>();
     return (T)(KIND == kStd ? sqrt_for_f32(var) : var);
   }
@@ -337,6 +337,13 @@ struct Traits<T, kCov> {
       return (T)((s.s12.to_double() - s.s1.to_double() * s.s2.to_double() / c) / (c - 1.0));
     }
   }
+  // float32 with the reciprocal table (as VarTraits::out_tab)
+  __device__ static T out_tab(const S& s, int64_t mc, const double* rc) {
+    const int64_t mce = mc > 2 ? mc : 2;
+    const double cov = (s.s12.to_double() - s.s1.to_double() * s.s2.to_double() * rc[s.c]) *
+                       rc[s.c > 0 ? s.c - 1 : 0];
+    return (T)(s.c >= mce ? cov : canonical_nan<double>());
+  }
 };
 template <typename T>
 struct Traits<T, kCorr> {
@@ -356,7 +363,7 @@ struct Traits<T, kCorr> {
     return (T)((s.c >= mce && vv > 0) ? res : canonical_nan<double>());
   }
   // float32 with the reciprocal table (the value recip forms)
-  __device__ static T out_tab(const S& s, int64_t mc, const double* rc, const double*) {
+  __device__ static T out_tab(const S& s, int64_t mc, const double* rc) {
     return from_r(s, mc > 1 ? mc : 1, rc[s.c]);
   }
   __device__ static T out(const S& s, int64_t mc, const double* = nullptr) {
@@ -759,7 +766,7 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
       if (valid(a, b)) st.add_obs(a, b, ka, kb);
       tail(j, a, b);
       if (valid(a, b)) st.sub_obs(a, b, ka, kb);
-      if constexpr (LUT) res[j] = Tr::out_tab(st, min_count, lut, lut + window + 1);  // c in [0, window]
+      if constexpr (LUT) res[j] = Tr::out_tab(st, min_count, lut);  // c in [0, window]
       else res[j] = Tr::out(st, min_count, nullptr);
     }
   }
@@ -816,8 +823,8 @@ __device__ __forceinline__ void rolling_tile(const T* __restrict__ ra, const T* 
   __syncthreads();
 }
 
-// Largest window for which mean (and float32 var / std / corr) keep reciprocal
-// tables in LDS.
+// Largest window for which mean (and the float32 moment kinds) keep a
+// reciprocal table in LDS.
 constexpr int64_t kMeanLutWindow = 4096;
 
 // lut[c] = 1/c (hardware reciprocal + one Newton step) for c >= mce, NaN for
@@ -828,21 +835,11 @@ __device__ __forceinline__ void fill_recip_lut(double* lut, int64_t window, int6
     lut[c] = c >= mce ? recip((int)c) : canonical_nan<double>();
 }
 
-// Moment tables: rc[c] = 1/max(c, 1) and, with two tables, rc1[c] =
-// 1/max(c - 1, 1) following it (refined reciprocals, as count_recips forms
-// them), c = 0..window.
-__device__ __forceinline__ void fill_moment_luts(double* rc, int64_t window, bool two) {
-  double* rc1 = rc + window + 1;
-  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads) {
-    const CountRecips q = count_recips((int)c);
-    rc[c] = q.r;
-    if (two) rc1[c] = q.r1;
-  }
+// Moment table: rc[c] = 1/max(c, 1) (the refined reciprocal recip forms),
+// c = 0..window; 1/max(c - 1, 1) is rc[max(c - 1, 0)].
+__device__ __forceinline__ void fill_moment_lut(double* rc, int64_t window) {
+  for (int64_t c = threadIdx.x; c <= window; c += kMoveThreads) rc[c] = recip(c > 0 ? (int)c : 1);
 }
-
-// Reciprocal tables of (window + 1) doubles a tile with tables uses: 1/c for
-// mean and corr, 1/c and 1/(c - 1) for var and std.
-__host__ __device__ constexpr int lut_tables(int kind) { return (kind == kMean || kind == kCorr) ? 1 : 2; }
 
 // Items per thread for float32 sum/mean: 16 (default) or 32, from
 // NBG_MOVE_ITEMS; a tuning knob (more items amortise the per-tile scan and
@@ -898,7 +895,7 @@ __global__ __launch_bounds__(kMoveThreads) __attribute__((amdgpu_waves_per_eu(mo
     lut = reinterpret_cast<double*>(smem_raw + rolling_stage_bytes<T>(KP, window, TAIL_LDS, Tr::NARGS));
     // ordered by the tile's first barrier
     if constexpr (KIND == kMean) fill_recip_lut(lut, window, min_count > 1 ? min_count : 1);
-    else fill_moment_luts(lut, window, lut_tables(KIND) == 2);
+    else fill_moment_lut(lut, window);
   }
   const T* ra = xa + row * ld_a;
   const T* rb = (Tr::NARGS == 2) ? xb + row * ld_b : nullptr;
