@@ -120,17 +120,14 @@ struct Lane {
   uint32_t key32 = 0u, idx32 = 0u;  // small_ext: key (0: none yet), index within the chunk
   T best = T(0);                    // small_ext: the value of key32
 
-  // small_ext: one element at position rel within the chunk.
+  // small_ext: one element at position rel within the chunk, by selects (no
+  // branch per element).
   __device__ __forceinline__ void fold32(T v, uint32_t rel) {
-    if (!is_nan(v)) {
-      const uint32_t k = ext_key32((float)v, KIND == kRMin || KIND == kRArgMin);
-      const bool take = first_wins(KIND) ? k > key32 : k >= key32;
-      if (take) {
-        key32 = k;
-        idx32 = rel;
-        best = v;
-      }
-    }
+    const uint32_t k = ext_key32((float)v, KIND == kRMin || KIND == kRArgMin);
+    const bool take = !is_nan(v) && (first_wins(KIND) ? k > key32 : k >= key32);
+    key32 = take ? k : key32;
+    idx32 = take ? rel : idx32;
+    best = take ? v : best;
   }
 
   // One element at position i (non-variance kinds).
