@@ -356,6 +356,96 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_finish_kernel(MxStaticAr
   out[j * a.nv + i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Static matrices of a few variables (V <= kSmallV) over a large batch: one
+// workgroup per batch row on the vector units (a 16 x 16 matrix-core tile
+// would leave at least three quarters of its products unused). Thread t
+// takes pair t % PP (i <= j, row-major; PP = pairs rounded up to a power of
+// two) and every K-th observation of each staged chunk (K = 256 / PP, from
+// t / PP); the chunk's shifted values sit in LDS as float64 (NaN where
+// missing), a row per variable with one spare slot so that the rows fall in
+// different banks. The K partial sums of a pair are added in a fixed order
+// (deterministic) and the output formed as static_value does.
+// ---------------------------------------------------------------------------
+constexpr int kSmallV = 8;
+constexpr int kSmallChunk = 512;  // observations staged per round
+constexpr int64_t kSmallMinBatch = 512;
+
+template <typename T, bool CORR>
+__global__ __launch_bounds__(kMxThreads) void mx_small_kernel(MxStaticArgs a) {
+  constexpr int NP = StaticSpec<CORR>::NPROD;
+  constexpr int RS = kSmallChunk + 1;  // LDS row stride (doubles)
+  __shared__ double ys[kSmallV * RS];
+  __shared__ double red[NP][kMxThreads];
+  const int64_t b = blockIdx.x;
+  const int nv = (int)a.nv;
+  const int npairs = nv * (nv + 1) / 2;
+  int pp = 1;
+  while (pp < npairs) pp <<= 1;
+  const int K = kMxThreads / pp;
+  const int tid = threadIdx.x;
+  const int pair = tid % pp, k = tid / pp;
+  // pair -> (i, j), i <= j, row-major over the upper triangle
+  int i = 0, rem = pair < npairs ? pair : 0;
+  while (rem >= nv - i) {
+    rem -= nv - i;
+    ++i;
+  }
+  const int j = i + rem;
+  const T* x = static_cast<const T*>(a.x) + b * a.nv * a.nobs;
+  const double* shift = a.shift + b * a.nv;
+  double acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+  for (int64_t o0 = 0; o0 < a.nobs; o0 += kSmallChunk) {
+    const int cnt = (int)(a.nobs - o0 < kSmallChunk ? a.nobs - o0 : kSmallChunk);
+    __syncthreads();  // the previous chunk's reads are done
+    for (int e = tid; e < nv * kSmallChunk; e += kMxThreads) {
+      const int v = e / kSmallChunk, o = e - v * kSmallChunk;
+      double y = __builtin_nan("");
+      if (o < cnt) {
+        const T val = x[(int64_t)v * a.nobs + o0 + o];
+        if (val == val) y = (double)val - shift[v];
+      }
+      ys[v * RS + o] = y;
+    }
+    __syncthreads();
+    if (pair < npairs) {
+      const double* yi = ys + i * RS;
+      const double* yj = ys + j * RS;
+      for (int o = k; o < cnt; o += K) {
+        const double a_ = yi[o], b_ = yj[o];
+        const bool both = a_ == a_ && b_ == b_;
+        const double u = both ? a_ : 0.0, w = both ? b_ : 0.0;
+        acc[0] += u * w;
+        acc[1] += u;
+        acc[2] += w;
+        acc[3] += both ? 1.0 : 0.0;
+        if constexpr (CORR) {
+          acc[4 % NP] += u * u;
+          acc[5 % NP] += w * w;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) red[q][tid] = acc[q];
+  __syncthreads();
+  if (tid < npairs) {
+    double t[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      t[q] = 0.0;
+      for (int kk = 0; kk < K; ++kk) t[q] += red[q][kk * pp + tid];
+    }
+    const double Qi = CORR ? t[4 % NP] : 0.0, Qj = CORR ? t[5 % NP] : 0.0;
+    const T v = (T)static_value<CORR>(t[0], t[1], t[2], t[3], Qi, Qj, i == j);
+    T* out = static_cast<T*>(a.out) + b * a.nv * a.nv;
+    out[i * a.nv + j] = v;
+    out[j * a.nv + i] = v;
+  }
+}
+
 // Launch plan: wave tile, observation splits, workspace layout.
 struct StaticPlan {
   int tw;
@@ -819,6 +909,12 @@ int static_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nv
   hipLaunchKernelGGL(mx_shift_kernel<T>, dim3(mx_grid(batch * nv)), dim3(kMxThreads), 0, s,
                      static_cast<const T*>(x), shift, sa);
   MxStaticArgs a{x, out, shift, part, batch, nv, nobs, p.ntile, p.tiles, p.splits, p.span};
+  if (nv <= kSmallV && batch >= kSmallMinBatch && batch <= 0x7fffffffLL &&
+      knob("NBG_MATRIX_SMALL", 1) != 0) {  // a workgroup per batch row on the vector units
+    if (corr) hipLaunchKernelGGL((mx_small_kernel<T, true>), dim3((unsigned)batch), dim3(kMxThreads), 0, s, a);
+    else hipLaunchKernelGGL((mx_small_kernel<T, false>), dim3((unsigned)batch), dim3(kMxThreads), 0, s, a);
+    return check_hip(hipGetLastError(), "matrix launch");
+  }
   const bool vec = ((uintptr_t)x % 16) == 0 && (nobs % (16 / (int64_t)sizeof(T))) == 0;
   const int grid = mx_grid(batch * p.tiles * p.splits);
 #define NBG_MX_STATIC(C, TW)                                                                                   \
