@@ -365,18 +365,21 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_finish_kernel(MxStaticAr
 // t / PP); the chunk's shifted values sit in LDS as float64 (NaN where
 // missing), a row per variable with one spare slot so that the rows fall in
 // different banks. The K partial sums of a pair are added in a fixed order
-// (deterministic) and the output formed as static_value does.
+// (deterministic) and the output formed as static_value does. The LDS (a
+// chunk's rows, then the partial sums in the same space) stays near 12 KB, so
+// many workgroups share a CU: each one's work is short and latency-bound.
 // ---------------------------------------------------------------------------
 constexpr int kSmallV = 8;
-constexpr int kSmallChunk = 512;  // observations staged per round
+constexpr int kSmallChunk = 256;  // observations staged per round
 constexpr int64_t kSmallMinBatch = 512;
 
 template <typename T, bool CORR>
 __global__ __launch_bounds__(kMxThreads) void mx_small_kernel(MxStaticArgs a) {
   constexpr int NP = StaticSpec<CORR>::NPROD;
   constexpr int RS = kSmallChunk + 1;  // LDS row stride (doubles)
-  __shared__ double ys[kSmallV * RS];
-  __shared__ double red[NP][kMxThreads];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* ys = reinterpret_cast<double*>(smem_raw);   // nv rows of RS
+  double* red = reinterpret_cast<double*>(smem_raw);  // NP x kMxThreads, after the last chunk
   const int64_t b = blockIdx.x;
   const int nv = (int)a.nv;
   const int npairs = nv * (nv + 1) / 2;
@@ -428,15 +431,16 @@ __global__ __launch_bounds__(kMxThreads) void mx_small_kernel(MxStaticArgs a) {
       }
     }
   }
+  __syncthreads();  // the last chunk's reads are done: its space takes the partial sums
 #pragma unroll
-  for (int q = 0; q < NP; ++q) red[q][tid] = acc[q];
+  for (int q = 0; q < NP; ++q) red[q * kMxThreads + tid] = acc[q];
   __syncthreads();
   if (tid < npairs) {
     double t[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
       t[q] = 0.0;
-      for (int kk = 0; kk < K; ++kk) t[q] += red[q][kk * pp + tid];
+      for (int kk = 0; kk < K; ++kk) t[q] += red[q * kMxThreads + kk * pp + tid];
     }
     const double Qi = CORR ? t[4 % NP] : 0.0, Qj = CORR ? t[5 % NP] : 0.0;
     const T v = (T)static_value<CORR>(t[0], t[1], t[2], t[3], Qi, Qj, i == j);
@@ -911,8 +915,10 @@ int static_launch(bool corr, const void* x, void* out, int64_t batch, int64_t nv
   MxStaticArgs a{x, out, shift, part, batch, nv, nobs, p.ntile, p.tiles, p.splits, p.span};
   if (nv <= kSmallV && batch >= kSmallMinBatch && batch <= 0x7fffffffLL &&
       knob("NBG_MATRIX_SMALL", 1) != 0) {  // a workgroup per batch row on the vector units
-    if (corr) hipLaunchKernelGGL((mx_small_kernel<T, true>), dim3((unsigned)batch), dim3(kMxThreads), 0, s, a);
-    else hipLaunchKernelGGL((mx_small_kernel<T, false>), dim3((unsigned)batch), dim3(kMxThreads), 0, s, a);
+    const int64_t rows_d = nv * (kSmallChunk + 1), red_d = (corr ? 6 : 4) * (int64_t)kMxThreads;
+    const size_t lds = (size_t)(rows_d > red_d ? rows_d : red_d) * sizeof(double);
+    if (corr) hipLaunchKernelGGL((mx_small_kernel<T, true>), dim3((unsigned)batch), dim3(kMxThreads), lds, s, a);
+    else hipLaunchKernelGGL((mx_small_kernel<T, false>), dim3((unsigned)batch), dim3(kMxThreads), lds, s, a);
     return check_hip(hipGetLastError(), "matrix launch");
   }
   const bool vec = ((uintptr_t)x % 16) == 0 && (nobs % (16 / (int64_t)sizeof(T))) == 0;

@@ -88,10 +88,10 @@ STATIC_SHAPES = [
     (1, 70, 3001),  # 3 x 3 tiles, off-diagonal tiles
     (3, 17, 50_000),  # observation splits, finishing kernel
     (1, 200, 20_000),  # many tiles and splits
-    (600, 5, 700),  # few variables over a large batch: vector-unit kernel, two staged chunks
+    (600, 5, 700),  # few variables over a large batch: vector-unit kernel, staged chunks
     (520, 8, 1024),  # the most variables that kernel takes, whole chunks
     (512, 1, 40),  # one variable
-    (513, 3, 513),  # a chunk and one observation
+    (513, 3, 513),  # whole chunks and one observation
 ]
 
 
