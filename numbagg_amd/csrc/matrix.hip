@@ -370,7 +370,7 @@ __global__ __launch_bounds__(kMxThreads) void mx_static_finish_kernel(MxStaticAr
 // many workgroups share a CU: each one's work is short and latency-bound.
 // ---------------------------------------------------------------------------
 constexpr int kSmallV = 8;
-constexpr int kSmallChunk = 256;  // observations staged per round
+constexpr int kSmallChunk = kMxThreads;  // observations staged per round (one per thread)
 constexpr int64_t kSmallMinBatch = 512;
 
 template <typename T, bool CORR>
@@ -400,18 +400,23 @@ __global__ __launch_bounds__(kMxThreads) void mx_small_kernel(MxStaticArgs a) {
   double acc[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+  // Thread t stages observation t of every variable of a chunk; the next
+  // chunk's values are loaded into registers while the current one is summed.
+  T pre[kSmallV];
+  auto load_chunk = [&](int64_t o0) {
+    const bool in = o0 + tid < a.nobs;
+#pragma unroll
+    for (int v = 0; v < kSmallV; ++v)
+      if (v < nv) pre[v] = in ? x[(int64_t)v * a.nobs + o0 + tid] : mx_nan<T>();
+  };
+  load_chunk(0);
   for (int64_t o0 = 0; o0 < a.nobs; o0 += kSmallChunk) {
     const int cnt = (int)(a.nobs - o0 < kSmallChunk ? a.nobs - o0 : kSmallChunk);
     __syncthreads();  // the previous chunk's reads are done
-    for (int e = tid; e < nv * kSmallChunk; e += kMxThreads) {
-      const int v = e / kSmallChunk, o = e - v * kSmallChunk;
-      double y = __builtin_nan("");
-      if (o < cnt) {
-        const T val = x[(int64_t)v * a.nobs + o0 + o];
-        if (val == val) y = (double)val - shift[v];
-      }
-      ys[v * RS + o] = y;
-    }
+#pragma unroll
+    for (int v = 0; v < kSmallV; ++v)
+      if (v < nv) ys[v * RS + tid] = pre[v] == pre[v] ? (double)pre[v] - shift[v] : __builtin_nan("");
+    if (o0 + kSmallChunk < a.nobs) load_chunk(o0 + kSmallChunk);
     __syncthreads();
     if (pair < npairs) {
       const double* yi = ys + i * RS;
