@@ -495,7 +495,7 @@ StaticPlan static_plan(int64_t batch, int64_t nv, int64_t nobs, bool corr) {
 }
 
 // ---------------------------------------------------------------------------
-// Moving-window matrices: a lane per pair i <= j and time range.
+// Moving-window matrices: a lane per ordered pair and time range.
 // ---------------------------------------------------------------------------
 struct MxMoveArgs {
   const void* x;
@@ -561,24 +561,16 @@ __global__ __launch_bounds__(kMxThreads) void mx_move_kernel(MxMoveArgs a) {
   const int64_t tile = (wv / a.pair_waves) % a.tiles;
   const int64_t b = wv / (a.pair_waves * a.tiles);
   const int64_t np = a.nv * a.nv;
-  // A lane per pair i <= j (row-major over the upper triangle): a pair's
-  // statistic is symmetric in (i, j), bitwise, so it is stored at both.
-  const int64_t npu = a.nv * (a.nv + 1) / 2;
   int64_t qq, g;
   if (a.groups == 1) {
     qq = pw * kWave + lane;
     g = 0;
   } else {
-    qq = lane % npu;
-    g = lane / npu;
+    qq = lane % np;
+    g = lane / np;
   }
-  if (qq >= npu || g >= a.groups) return;
-  int64_t i = 0, rem = qq;
-  while (rem >= a.nv - i) {
-    rem -= a.nv - i;
-    ++i;
-  }
-  const int64_t j = i + rem;
+  if (qq >= np || g >= a.groups) return;
+  const int64_t i = qq / a.nv, j = qq % a.nv;
   const int64_t tile_lo = tile * a.tile_len;
   int64_t tile_hi = tile_lo + a.tile_len;
   if (tile_hi > a.nobs) tile_hi = a.nobs;
@@ -600,19 +592,15 @@ __global__ __launch_bounds__(kMxThreads) void mx_move_kernel(MxMoveArgs a) {
   IT oi = (IT)(t_w * a.nv + i), oj = (IT)(t_w * a.nv + j);
   int64_t t = t_w;
   for (; t < t_start; ++t, oi += step, oj += step) st.add(x[oi], x[oj], ci, cj, 1.0);
-  IT oo = (IT)(t_start * np + i * a.nv + j), ot = (IT)(t_start * np + j * a.nv + i);
-  for (; t < t_r && t < t_end; ++t, oi += step, oj += step, oo += ostep, ot += ostep) {
+  IT oo = (IT)(t_start * np + qq);
+  for (; t < t_r && t < t_end; ++t, oi += step, oj += step, oo += ostep) {
     st.add(x[oi], x[oj], ci, cj, 1.0);
-    const T v = (T)st.value(a.min_count, mcorr, diag);
-    out[oo] = v;
-    if (!diag) out[ot] = v;
+    out[oo] = (T)st.value(a.min_count, mcorr, diag);
   }
-  for (; t < t_end; ++t, oi += step, oj += step, oo += ostep, ot += ostep) {
+  for (; t < t_end; ++t, oi += step, oj += step, oo += ostep) {
     st.add(x[oi], x[oj], ci, cj, 1.0);
     st.add(x[oi - back], x[oj - back], ci, cj, -1.0);
-    const T v = (T)st.value(a.min_count, mcorr, diag);
-    out[oo] = v;
-    if (!diag) out[ot] = v;
+    out[oo] = (T)st.value(a.min_count, mcorr, diag);
   }
 }
 
@@ -626,7 +614,7 @@ struct MovePlan {
 
 MovePlan move_plan(int64_t batch, int64_t nobs, int64_t nv, int64_t window) {
   MovePlan p{};
-  const int64_t np = nv * (nv + 1) / 2;  // pairs i <= j, a lane each
+  const int64_t np = nv * nv;
   if (np >= kWave) {
     p.groups = 1;
     p.pair_waves = (np + kWave - 1) / kWave;
