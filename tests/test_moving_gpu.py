@@ -254,3 +254,23 @@ def test_out_argument(gpu_lib):
     np.testing.assert_array_equal(df, nb.ffill(xn))
     with pytest.raises(ValueError):
         nb.move_mean(x, window=5, out=np.empty((3, 100)))
+
+
+@pytest.mark.parametrize("func", ["move_sum", "move_mean", "move_var", "move_std", "move_corr"])
+def test_element_aligned_rows_match_contiguous(gpu_lib, func):
+    """Rows that start one element past a 16-byte boundary (a sliced device
+    tensor) take the 16-byte vector paths at element alignment (streaming
+    prefetch loads, LDS-staged stores): bitwise the results of an aligned
+    contiguous copy."""
+    import torch
+    import numbagg_amd as nb
+
+    x = torch.from_numpy(walk_array(3, 40_001, seed=21, offset=0.0)).cuda()
+    y = torch.from_numpy(walk_array(3, 40_001, seed=22, offset=0.0)).cuda()
+    xv, yv = x[:, 1:], y[:, 1:]
+    assert xv.data_ptr() % 16 == 4
+    args_v = (xv, yv) if func == "move_corr" else (xv,)
+    args_c = tuple(a.contiguous() for a in args_v)
+    got = getattr(nb, func)(*args_v, window=1000, min_count=500)
+    want = getattr(nb, func)(*args_c, window=1000, min_count=500)
+    assert torch.equal(got.view(torch.int32), want.view(torch.int32))
