@@ -592,15 +592,19 @@ __global__ __launch_bounds__(kMxThreads) void mx_move_kernel(MxMoveArgs a) {
   IT oi = (IT)(t_w * a.nv + i), oj = (IT)(t_w * a.nv + j);
   int64_t t = t_w;
   for (; t < t_start; ++t, oi += step, oj += step) st.add(x[oi], x[oj], ci, cj, 1.0);
+  // Outputs (V^2 per step, most of the traffic) by streaming stores: each
+  // wave writes 64 consecutive elements per step, and the lines need not stay
+  // in the caches (measured on MI355X, (200000, 32) float32, window 20:
+  // move_covmatrix 0.528 -> 0.448 ms, move_corrmatrix 0.60 -> 0.56 ms).
   IT oo = (IT)(t_start * np + qq);
   for (; t < t_r && t < t_end; ++t, oi += step, oj += step, oo += ostep) {
     st.add(x[oi], x[oj], ci, cj, 1.0);
-    out[oo] = (T)st.value(a.min_count, mcorr, diag);
+    __builtin_nontemporal_store((T)st.value(a.min_count, mcorr, diag), &out[oo]);
   }
   for (; t < t_end; ++t, oi += step, oj += step, oo += ostep) {
     st.add(x[oi], x[oj], ci, cj, 1.0);
     st.add(x[oi - back], x[oj - back], ci, cj, -1.0);
-    out[oo] = (T)st.value(a.min_count, mcorr, diag);
+    __builtin_nontemporal_store((T)st.value(a.min_count, mcorr, diag), &out[oo]);
   }
 }
 
@@ -769,7 +773,7 @@ __global__ __launch_bounds__(kMxThreads) void mx_exp_chunk_kernel(MxExpArgs a) {
         if (vi > 0.0 && vj > 0.0) v = diag ? 1.0 : mx_corr(cv, vi * vj);
       }
     }
-    out[oo] = (TO)v;
+    __builtin_nontemporal_store((TO)v, &out[oo]);  // streaming, as the moving kernel's
   }
   if (!REPLAY) {
 #pragma unroll
