@@ -10,13 +10,15 @@
 // Design (hash, then sort the uniques):
 //   1. insert: every value's 64-bit key goes into an open-addressing table
 //      (linear probing, compare-and-swap on an empty marker); each slot also
-//      records the first position of its key (atomic minimum);
+//      records the first position of its key (atomic minimum); each
+//      workgroup gathers its chunk's distinct keys in LDS first;
 //   2. compact: occupied slots append (sort key, slot) pairs to a list;
 //   3. sort: a bitonic network sorts the list by sort key (the order-preserving
 //      image of the value, or the first position), padded to a power of two;
 //   4. rank: each sorted entry writes its rank into its slot and its value into
 //      the uniques;
-//   5. codes: every value finds its slot again and reads the rank.
+//   5. codes: every value finds its slot again and reads the rank (for up to
+//      2048 uniques in a key -> rank table that each workgroup builds in LDS).
 // The empty marker is the all-ones word: no float key can equal it (it is a
 // NaN pattern, and NaN is never inserted); the integer -1 (sign-extended to
 // the same bits) is tracked by a flag outside the table.
@@ -77,9 +79,46 @@ struct FzArgs {
   int64_t padded;                      // list length, a power of two
 };
 
+// Key k (not the int64 -1) seen at position i: into the global table.
+__device__ __forceinline__ void fz_global_insert(const FzArgs& A, unsigned long long k, unsigned long long i) {
+  // The table holds at most half its slots, so a free slot is found. A
+  // slot changes only once (empty -> key) and a first position only
+  // decreases, so plain reads that see an older value are safe: the common
+  // case (the key is present, with an earlier first position) then takes
+  // no atomic, which keeps labels with many repeats free of contention.
+  unsigned long long s = mix64(k) & A.mask;
+  for (;;) {
+    unsigned long long cur = A.keys[s];
+    if (cur == kEmpty) cur = atomicCAS(&A.keys[s], kEmpty, k);
+    if (cur == kEmpty || cur == k) {
+      if (i < A.first[s]) atomicMin(&A.first[s], i);
+      break;
+    }
+    s = (s + 1) & A.mask;
+  }
+}
+
+// Insert pass. Each workgroup first gathers its contiguous chunk's keys in an
+// LDS table (key, first position), then puts each distinct key into the
+// global table once: few uniques then cost one L2 access per key and
+// workgroup instead of two per value. A key that finds no LDS slot within
+// kFzLdsProbe probes (the table holds kFzLdsSlots) goes to the global table
+// directly, so any number of uniques is correct.
+constexpr int kFzInsThreads = 1024;
+constexpr int kFzLdsSlots = 4096;  // 64 KiB of keys and first positions
+constexpr int kFzLdsProbe = 16;
+
 template <typename T>
-__global__ __launch_bounds__(kFzThreads) void fz_insert_kernel(const T* __restrict__ x, FzArgs A) {
-  for (int64_t i = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; i < A.n; i += (int64_t)gridDim.x * kFzThreads) {
+__global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __restrict__ x, FzArgs A, int64_t chunk) {
+  __shared__ unsigned long long lk[kFzLdsSlots], lf[kFzLdsSlots];
+  for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads) {
+    lk[j] = kEmpty;
+    lf[j] = kEmpty;
+  }
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < A.n ? lo + chunk : A.n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kFzInsThreads) {
     bool nan;
     const unsigned long long k = fz_key(x[i], nan);
     if (nan) continue;
@@ -88,22 +127,23 @@ __global__ __launch_bounds__(kFzThreads) void fz_insert_kernel(const T* __restri
       atomicMin(A.minus_one_first, (unsigned long long)i);
       continue;
     }
-    // The table holds at most half its slots, so a free slot is found. A
-    // slot changes only once (empty -> key) and a first position only
-    // decreases, so plain reads that see an older value are safe: the common
-    // case (the key is present, with an earlier first position) then takes
-    // no atomic, which keeps labels with many repeats free of contention.
-    unsigned long long s = mix64(k) & A.mask;
-    for (;;) {
-      unsigned long long cur = A.keys[s];
-      if (cur == kEmpty) cur = atomicCAS(&A.keys[s], kEmpty, k);
+    unsigned int h = (unsigned int)mix64(k) & (kFzLdsSlots - 1);
+    bool placed = false;
+    for (int p = 0; p < kFzLdsProbe; ++p) {
+      unsigned long long cur = lk[h];
+      if (cur == kEmpty) cur = atomicCAS(&lk[h], kEmpty, k);
       if (cur == kEmpty || cur == k) {
-        if ((unsigned long long)i < A.first[s]) atomicMin(&A.first[s], (unsigned long long)i);
+        if ((unsigned long long)i < lf[h]) atomicMin(&lf[h], (unsigned long long)i);
+        placed = true;
         break;
       }
-      s = (s + 1) & A.mask;
+      h = (h + 1) & (kFzLdsSlots - 1);
     }
+    if (!placed) fz_global_insert(A, k, (unsigned long long)i);
   }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads)
+    if (lk[j] != kEmpty) fz_global_insert(A, lk[j], lf[j]);
 }
 
 // Occupied slots append (sort key, slot) to the list (positions from an
@@ -253,6 +293,61 @@ __global__ __launch_bounds__(kFzThreads) void fz_codes_kernel(const T* __restric
   }
 }
 
+// Codes for at most kFzLdsCodes uniques: each workgroup builds an LDS hash
+// table (key -> rank, twice as many slots as uniques) from the sorted list,
+// then looks every value up there instead of in the global table.
+constexpr int kFzLdsCodes = 2048;
+constexpr int kFzCodeSlots = 2 * kFzLdsCodes;
+constexpr int kFzCodeThreads = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kFzCodeThreads) void fz_codes_lds_kernel(const T* __restrict__ x, FzArgs A,
+                                                                     const long long* __restrict__ minus_one_rank,
+                                                                     long long* __restrict__ codes, int64_t k_total) {
+  __shared__ unsigned long long tk[kFzCodeSlots];
+  __shared__ int tr[kFzCodeSlots];
+  if (fz_total(A) != k_total || k_total > kFzLdsCodes) return;
+  for (int j = threadIdx.x; j < kFzCodeSlots; j += kFzCodeThreads) tk[j] = kEmpty;
+  __syncthreads();
+  for (int r = threadIdx.x; r < (int)k_total; r += kFzCodeThreads) {
+    const unsigned long long s = A.sval[r];
+    if (s == kEmpty) continue;  // the int64 -1
+    const unsigned long long k = A.keys[s];
+    // distinct keys, at most half the slots: a free slot is always found
+    for (unsigned int h = (unsigned int)mix64(k) & (kFzCodeSlots - 1);; h = (h + 1) & (kFzCodeSlots - 1)) {
+      if (atomicCAS(&tk[h], kEmpty, k) == kEmpty) {
+        tr[h] = r;
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  const long long m1 = *minus_one_rank;
+  for (int64_t i = (int64_t)blockIdx.x * kFzCodeThreads + threadIdx.x; i < A.n;
+       i += (int64_t)gridDim.x * kFzCodeThreads) {
+    bool nan;
+    const unsigned long long k = fz_key(x[i], nan);
+    long long c = -1;
+    if (!nan) {
+      if (k == kEmpty) {
+        c = m1;
+      } else {
+        // ends at the key or at a free slot (a key not inserted: values
+        // changed since the count)
+        for (unsigned int h = (unsigned int)mix64(k) & (kFzCodeSlots - 1);; h = (h + 1) & (kFzCodeSlots - 1)) {
+          const unsigned long long ks = tk[h];
+          if (ks == k) {
+            c = tr[h];
+            break;
+          }
+          if (ks == kEmpty) break;
+        }
+      }
+    }
+    codes[i] = c;
+  }
+}
+
 __global__ void fz_init_kernel(FzArgs A) {
   const int64_t size = (int64_t)A.mask + 1;
   for (int64_t s = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; s < size; s += (int64_t)gridDim.x * kFzThreads) {
@@ -330,9 +425,18 @@ int nbg_factorize_count(int dtype, const void* values, int64_t n, int sort, void
   hipStream_t s = (hipStream_t)stream;
   const int64_t size = (int64_t)a.mask + 1;
   hipLaunchKernelGGL(nbg::fz_init_kernel, dim3(nbg::fz_grid(size)), dim3(nbg::kFzThreads), 0, s, a);
+  // Insert: at most 512 workgroups (two per CU in 64 KiB of LDS each), each
+  // over a contiguous chunk, a multiple of the workgroup size.
+  int64_t ins_blocks = (n + 16 * nbg::kFzInsThreads - 1) / (16 * nbg::kFzInsThreads);
+  ins_blocks = ins_blocks < 1 ? 1 : (ins_blocks > 512 ? 512 : ins_blocks);
+  int64_t chunk = (n + ins_blocks - 1) / ins_blocks;
+  chunk = (chunk + nbg::kFzInsThreads - 1) / nbg::kFzInsThreads * nbg::kFzInsThreads;
+  if (chunk < nbg::kFzInsThreads) chunk = nbg::kFzInsThreads;
+  ins_blocks = (n + chunk - 1) / chunk;
+  if (ins_blocks < 1) ins_blocks = 1;
 #define NBG_FZ1(T)                                                                                          \
-  hipLaunchKernelGGL((nbg::fz_insert_kernel<T>), dim3(nbg::fz_grid(n)), dim3(nbg::kFzThreads), 0, s,          \
-                     (const T*)values, a);                                                                  \
+  hipLaunchKernelGGL((nbg::fz_insert_kernel<T>), dim3((unsigned)ins_blocks), dim3(nbg::kFzInsThreads), 0, s,  \
+                     (const T*)values, a, chunk);                                                           \
   hipLaunchKernelGGL((nbg::fz_compact_kernel<T>), dim3(nbg::fz_grid(size)), dim3(nbg::kFzThreads), 0, s, a,   \
                      sort ? 1 : 0)
   switch (dtype) {
@@ -361,6 +465,10 @@ int nbg_factorize_finish(int dtype, const void* values, int64_t n, int64_t num_u
   a.padded = nbg::next_pow2(num_uniques > 1 ? num_uniques : 1);
   hipStream_t s = (hipStream_t)stream;
   long long* minus_one_rank = nbg::fz_minus_one_rank(n, ws);
+  // LDS codes: about 16 values per thread, at most 512 workgroups (each
+  // builds its own table)
+  int64_t codes_blocks = (n + 16 * nbg::kFzCodeThreads - 1) / (16 * nbg::kFzCodeThreads);
+  codes_blocks = codes_blocks < 1 ? 1 : (codes_blocks > 512 ? 512 : codes_blocks);
 #define NBG_FZ2(T)                                                                                            \
   do {                                                                                                        \
     hipLaunchKernelGGL((nbg::fz_pad_kernel<T>), dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0, s, a,   \
@@ -375,8 +483,12 @@ int nbg_factorize_finish(int dtype, const void* values, int64_t n, int64_t num_u
                              s, a.skey, a.sval, a.padded, j, k);                                              \
     hipLaunchKernelGGL((nbg::fz_rank_kernel<T>), dim3(nbg::fz_grid(num_uniques)), dim3(nbg::kFzThreads), 0, s,  \
                        (const T*)values, a, num_uniques, (T*)uniques, minus_one_rank);                        \
-    hipLaunchKernelGGL((nbg::fz_codes_kernel<T>), dim3(nbg::fz_grid(n)), dim3(nbg::kFzThreads), 0, s,           \
-                       (const T*)values, a, (const long long*)minus_one_rank, (long long*)codes, num_uniques);             \
+    if (num_uniques <= nbg::kFzLdsCodes)                                                                      \
+      hipLaunchKernelGGL((nbg::fz_codes_lds_kernel<T>), dim3(codes_blocks), dim3(nbg::kFzCodeThreads), 0, s,    \
+                         (const T*)values, a, (const long long*)minus_one_rank, (long long*)codes, num_uniques);  \
+    else                                                                                                      \
+      hipLaunchKernelGGL((nbg::fz_codes_kernel<T>), dim3(nbg::fz_grid(n)), dim3(nbg::kFzThreads), 0, s,         \
+                         (const T*)values, a, (const long long*)minus_one_rank, (long long*)codes, num_uniques);  \
   } while (0)
   switch (dtype) {
     case NBG_F32: NBG_FZ2(float); break;

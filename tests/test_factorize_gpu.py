@@ -2,7 +2,8 @@
 pandas.factorize): bit-exact codes and uniques for every dtype, NaN, signed
 zeros, -1 and extreme integers, empty and all-NaN input, sort on and off, up
 to the (4M,) label vector of the grouped benchmark (1000 uniform or Zipf(1.1)
-labels) and 1M distinct values; the codes then drive a grouped reduction."""
+labels), 2048 and 2050 uniques (either side of the LDS code table) and 1M
+distinct values; the codes then drive a grouped reduction."""
 
 from __future__ import annotations
 
@@ -45,7 +46,12 @@ def test_edge_cases(gpu_lib, sort):
 @pytest.mark.parametrize("sort", [True, False])
 def test_random_labels(gpu_lib, dtype, sort):
     rng = np.random.default_rng(3)
-    for n, k, zipf in ((4_000_000, 1000, False), (4_000_000, 1000, True), (1_000_003, 10**9, False), (5000, 3, False)):
+    # 2k uniques: 2048 fit the LDS code table (one of them the integer -1,
+    # kept outside the hash table), 2050 do not; 1M distinct values overflow
+    # every workgroup's LDS insert table
+    cases = ((4_000_000, 1000, False), (4_000_000, 1000, True), (1_000_003, 10**9, False), (5000, 3, False),
+             (300_000, 1024, False), (300_000, 1025, False))
+    for n, k, zipf in cases:
         if zipf:
             p = np.arange(1, k + 1, dtype=np.float64) ** -1.1
             v = rng.choice(k, size=n, p=p / p.sum()).astype(dtype)
