@@ -66,35 +66,63 @@ __device__ __forceinline__ unsigned long long fz_order(unsigned long long k) {
   }
 }
 
+// One open-addressing table: keys (kEmpty: free), first position of each
+// key, and the key's code.
+struct FzTable {
+  unsigned long long mask;  // size - 1 (a power of two)
+  unsigned long long* keys;
+  unsigned long long* first;
+  long long* rank;
+};
+
+// Two tables: a small one (at most kFzSmallSlots) that is tried first, and
+// the big one of 2n slots, used only when the small one overflows (more than
+// half full, or a probe longer than kFzSmallProbe) — so few uniques do not
+// pay for initialising and scanning 2n slots.
+constexpr int64_t kFzSmallSlots = 65536;
+constexpr int kFzSmallProbe = 64;
+
 struct FzArgs {
   int64_t n;
-  unsigned long long mask;             // table size - 1 (a power of two)
-  unsigned long long* keys;            // table keys (kEmpty: free)
-  unsigned long long* first;           // first position of the key
-  long long* rank;                     // the key's code
-  unsigned int* count;                 // [0] uniques in the table, [1] minus-one flag
+  FzTable big, small;
+  unsigned int* count;                 // [0] uniques in the table, [1] minus-one flag,
+                                       // [2] small-table slots taken, [3] small table overflowed
   unsigned long long* minus_one_first; // first position of an int64 -1
   unsigned long long* skey;            // sort keys (padded list)
   unsigned long long* sval;            // slot of each list entry (kEmpty: the int64 -1)
   int64_t padded;                      // list length, a power of two
 };
 
-// Key k (not the int64 -1) seen at position i: into the global table.
-__device__ __forceinline__ void fz_global_insert(const FzArgs& A, unsigned long long k, unsigned long long i) {
-  // The table holds at most half its slots, so a free slot is found. A
+// The table in use after the insert passes.
+__device__ __forceinline__ FzTable fz_table(const FzArgs& A) { return A.count[3] ? A.big : A.small; }
+
+// Key k (not the int64 -1) seen at position i: into table t (BIG: the 2n-slot
+// table; else the small one, which flags its overflow and gives up).
+template <bool BIG>
+__device__ __forceinline__ void fz_table_insert(const FzArgs& A, const FzTable& t, unsigned long long k,
+                                                unsigned long long i) {
+  // The big table holds at most half its slots, so a free slot is found. A
   // slot changes only once (empty -> key) and a first position only
   // decreases, so plain reads that see an older value are safe: the common
   // case (the key is present, with an earlier first position) then takes
   // no atomic, which keeps labels with many repeats free of contention.
-  unsigned long long s = mix64(k) & A.mask;
-  for (;;) {
-    unsigned long long cur = A.keys[s];
-    if (cur == kEmpty) cur = atomicCAS(&A.keys[s], kEmpty, k);
-    if (cur == kEmpty || cur == k) {
-      if (i < A.first[s]) atomicMin(&A.first[s], i);
-      break;
+  unsigned long long s = mix64(k) & t.mask;
+  for (int p = 0;; ++p) {
+    if (!BIG && p >= kFzSmallProbe) {
+      atomicOr(&A.count[3], 1u);
+      return;
     }
-    s = (s + 1) & A.mask;
+    unsigned long long cur = t.keys[s];
+    if (cur == kEmpty) {
+      cur = atomicCAS(&t.keys[s], kEmpty, k);
+      if (!BIG && cur == kEmpty && atomicAdd(&A.count[2], 1u) + 1 > (unsigned int)((t.mask + 1) / 2))
+        atomicOr(&A.count[3], 1u);
+    }
+    if (cur == kEmpty || cur == k) {
+      if (i < t.first[s]) atomicMin(&t.first[s], i);
+      return;
+    }
+    s = (s + 1) & t.mask;
   }
 }
 
@@ -108,9 +136,13 @@ constexpr int kFzInsThreads = 1024;
 constexpr int kFzLdsSlots = 4096;  // 64 KiB of keys and first positions
 constexpr int kFzLdsProbe = 16;
 
-template <typename T>
+// BIG: the second pass, into the big table, run only when the small table
+// overflowed.
+template <typename T, bool BIG>
 __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __restrict__ x, FzArgs A, int64_t chunk) {
   __shared__ unsigned long long lk[kFzLdsSlots], lf[kFzLdsSlots];
+  if (BIG && !A.count[3]) return;
+  const FzTable t = BIG ? A.big : A.small;
   for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads) {
     lk[j] = kEmpty;
     lf[j] = kEmpty;
@@ -139,23 +171,24 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
       }
       h = (h + 1) & (kFzLdsSlots - 1);
     }
-    if (!placed) fz_global_insert(A, k, (unsigned long long)i);
+    if (!placed) fz_table_insert<BIG>(A, t, k, (unsigned long long)i);
   }
   __syncthreads();
   for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads)
-    if (lk[j] != kEmpty) fz_global_insert(A, lk[j], lf[j]);
+    if (lk[j] != kEmpty) fz_table_insert<BIG>(A, t, lk[j], lf[j]);
 }
 
 // Occupied slots append (sort key, slot) to the list (positions from an
 // atomic counter; the bitonic sort fixes the order).
 template <typename T>
 __global__ __launch_bounds__(kFzThreads) void fz_compact_kernel(FzArgs A, int by_value) {
-  const int64_t size = (int64_t)A.mask + 1;
+  const FzTable t = fz_table(A);
+  const int64_t size = (int64_t)t.mask + 1;
   for (int64_t s = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; s < size; s += (int64_t)gridDim.x * kFzThreads) {
-    const unsigned long long k = A.keys[s];
+    const unsigned long long k = t.keys[s];
     if (k == kEmpty) continue;
     const unsigned int p = atomicAdd(&A.count[0], 1u);
-    A.skey[p] = by_value ? fz_order<T>(k) : A.first[s];
+    A.skey[p] = by_value ? fz_order<T>(k) : t.first[s];
     A.sval[p] = (unsigned long long)s;
   }
 }
@@ -250,6 +283,7 @@ __global__ __launch_bounds__(kFzThreads) void fz_rank_kernel(const T* __restrict
                                                             T* __restrict__ uniques,
                                                             long long* __restrict__ minus_one_rank) {
   if (fz_total(A) != k_total) return;
+  const FzTable t = fz_table(A);
   for (int64_t r = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; r < k_total; r += (int64_t)gridDim.x * kFzThreads) {
     const unsigned long long s = A.sval[r];
     unsigned long long pos;
@@ -257,8 +291,8 @@ __global__ __launch_bounds__(kFzThreads) void fz_rank_kernel(const T* __restrict
       *minus_one_rank = r;
       pos = *A.minus_one_first;
     } else {
-      A.rank[s] = r;
-      pos = A.first[s];
+      t.rank[s] = r;
+      pos = t.first[s];
     }
     uniques[r] = x[pos];
   }
@@ -269,6 +303,7 @@ __global__ __launch_bounds__(kFzThreads) void fz_codes_kernel(const T* __restric
                                                              const long long* __restrict__ minus_one_rank,
                                                              long long* __restrict__ codes, int64_t k_total) {
   if (fz_total(A) != k_total) return;
+  const FzTable t = fz_table(A);
   for (int64_t i = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; i < A.n; i += (int64_t)gridDim.x * kFzThreads) {
     bool nan;
     const unsigned long long k = fz_key(x[i], nan);
@@ -279,10 +314,10 @@ __global__ __launch_bounds__(kFzThreads) void fz_codes_kernel(const T* __restric
       } else {
         // The table is at most half full, so the probe ends at the key or at a
         // free slot (a key not inserted: values changed since the count).
-        for (unsigned long long s = mix64(k) & A.mask;; s = (s + 1) & A.mask) {
-          const unsigned long long ks = A.keys[s];
+        for (unsigned long long s = mix64(k) & t.mask;; s = (s + 1) & t.mask) {
+          const unsigned long long ks = t.keys[s];
           if (ks == k) {
-            c = A.rank[s];
+            c = t.rank[s];
             break;
           }
           if (ks == kEmpty) break;
@@ -307,12 +342,13 @@ __global__ __launch_bounds__(kFzCodeThreads) void fz_codes_lds_kernel(const T* _
   __shared__ unsigned long long tk[kFzCodeSlots];
   __shared__ int tr[kFzCodeSlots];
   if (fz_total(A) != k_total || k_total > kFzLdsCodes) return;
+  const FzTable t = fz_table(A);
   for (int j = threadIdx.x; j < kFzCodeSlots; j += kFzCodeThreads) tk[j] = kEmpty;
   __syncthreads();
   for (int r = threadIdx.x; r < (int)k_total; r += kFzCodeThreads) {
     const unsigned long long s = A.sval[r];
     if (s == kEmpty) continue;  // the int64 -1
-    const unsigned long long k = A.keys[s];
+    const unsigned long long k = t.keys[s];
     // distinct keys, at most half the slots: a free slot is always found
     for (unsigned int h = (unsigned int)mix64(k) & (kFzCodeSlots - 1);; h = (h + 1) & (kFzCodeSlots - 1)) {
       if (atomicCAS(&tk[h], kEmpty, k) == kEmpty) {
@@ -348,15 +384,19 @@ __global__ __launch_bounds__(kFzCodeThreads) void fz_codes_lds_kernel(const T* _
   }
 }
 
+// Clears the small table and the counters (BIG false), or the big table when
+// the small one overflowed (BIG true).
+template <bool BIG>
 __global__ void fz_init_kernel(FzArgs A) {
-  const int64_t size = (int64_t)A.mask + 1;
+  if (BIG && !A.count[3]) return;
+  const FzTable t = BIG ? A.big : A.small;
+  const int64_t size = (int64_t)t.mask + 1;
   for (int64_t s = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; s < size; s += (int64_t)gridDim.x * kFzThreads) {
-    A.keys[s] = kEmpty;
-    A.first[s] = kEmpty;
+    t.keys[s] = kEmpty;
+    t.first[s] = kEmpty;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    A.count[0] = 0u;
-    A.count[1] = 0u;
+  if (!BIG && blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int c = 0; c < 4; ++c) A.count[c] = 0u;
     *A.minus_one_first = kEmpty;
   }
 }
@@ -371,28 +411,42 @@ int64_t next_pow2(int64_t v) {
 // factor at most one half.
 int64_t fz_table_size(int64_t n) { return next_pow2(2 * (n > 512 ? n : 512)); }
 
+int64_t fz_small_size(int64_t n) {
+  const int64_t big = fz_table_size(n);
+  return big < kFzSmallSlots ? big : kFzSmallSlots;
+}
+
 size_t fz_ws_bytes(int64_t n) {
-  const int64_t size = fz_table_size(n);
+  const int64_t size = fz_table_size(n) + fz_small_size(n);
   const int64_t padded = next_pow2(n + 1);
   return (size_t)size * 24 + (size_t)padded * 16 + 32;
 }
 
-// Workspace: keys | first | rank (table size each, 8 bytes) | sort keys |
-// sort values (n + 1 rounded up to a power of two each) | counters.
+// Workspace: keys | first | rank of the big table, then of the small one
+// (table size each, 8 bytes) | sort keys | sort values (n + 1 rounded up to
+// a power of two each) | four 32-bit counters | minus-one first position |
+// minus-one rank.
+FzTable fz_table_at(char*& p, int64_t size) {
+  FzTable t;
+  t.mask = (unsigned long long)(size - 1);
+  t.keys = reinterpret_cast<unsigned long long*>(p);
+  t.first = t.keys + size;
+  t.rank = reinterpret_cast<long long*>(t.first + size);
+  p = reinterpret_cast<char*>(t.rank + size);
+  return t;
+}
+
 FzArgs fz_layout(int64_t n, void* ws) {
   FzArgs a{};
-  const int64_t size = fz_table_size(n);
   const int64_t padded_max = next_pow2(n + 1);
   char* p = static_cast<char*>(ws);
   a.n = n;
-  a.mask = (unsigned long long)(size - 1);
-  a.keys = reinterpret_cast<unsigned long long*>(p);
-  a.first = a.keys + size;
-  a.rank = reinterpret_cast<long long*>(a.first + size);
-  a.skey = reinterpret_cast<unsigned long long*>(a.rank + size);
+  a.big = fz_table_at(p, fz_table_size(n));
+  a.small = fz_table_at(p, fz_small_size(n));
+  a.skey = reinterpret_cast<unsigned long long*>(p);
   a.sval = a.skey + padded_max;
   a.count = reinterpret_cast<unsigned int*>(a.sval + padded_max);
-  a.minus_one_first = reinterpret_cast<unsigned long long*>(a.count + 2);
+  a.minus_one_first = reinterpret_cast<unsigned long long*>(a.count + 4);
   a.padded = padded_max;
   return a;
 }
@@ -423,8 +477,9 @@ int nbg_factorize_count(int dtype, const void* values, int64_t n, int sort, void
     return nbg::set_error(NBG_ERR_INVALID, "nbg_factorize_count: workspace too small");
   nbg::FzArgs a = nbg::fz_layout(n, ws);
   hipStream_t s = (hipStream_t)stream;
-  const int64_t size = (int64_t)a.mask + 1;
-  hipLaunchKernelGGL(nbg::fz_init_kernel, dim3(nbg::fz_grid(size)), dim3(nbg::kFzThreads), 0, s, a);
+  const int64_t size = (int64_t)a.big.mask + 1;
+  const int64_t small = (int64_t)a.small.mask + 1;
+  hipLaunchKernelGGL(nbg::fz_init_kernel<false>, dim3(nbg::fz_grid(small)), dim3(nbg::kFzThreads), 0, s, a);
   // Insert: at most 512 workgroups (two per CU in 64 KiB of LDS each), each
   // over a contiguous chunk, a multiple of the workgroup size.
   int64_t ins_blocks = (n + 16 * nbg::kFzInsThreads - 1) / (16 * nbg::kFzInsThreads);
@@ -434,9 +489,14 @@ int nbg_factorize_count(int dtype, const void* values, int64_t n, int sort, void
   if (chunk < nbg::kFzInsThreads) chunk = nbg::kFzInsThreads;
   ins_blocks = (n + chunk - 1) / chunk;
   if (ins_blocks < 1) ins_blocks = 1;
+  // small table, then (only if it overflowed: the kernels exit at once
+  // otherwise) the big one
 #define NBG_FZ1(T)                                                                                          \
-  hipLaunchKernelGGL((nbg::fz_insert_kernel<T>), dim3((unsigned)ins_blocks), dim3(nbg::kFzInsThreads), 0, s,  \
-                     (const T*)values, a, chunk);                                                           \
+  hipLaunchKernelGGL((nbg::fz_insert_kernel<T, false>), dim3((unsigned)ins_blocks), dim3(nbg::kFzInsThreads), \
+                     0, s, (const T*)values, a, chunk);                                                     \
+  hipLaunchKernelGGL(nbg::fz_init_kernel<true>, dim3(nbg::fz_grid(size)), dim3(nbg::kFzThreads), 0, s, a);    \
+  hipLaunchKernelGGL((nbg::fz_insert_kernel<T, true>), dim3((unsigned)ins_blocks), dim3(nbg::kFzInsThreads),  \
+                     0, s, (const T*)values, a, chunk);                                                     \
   hipLaunchKernelGGL((nbg::fz_compact_kernel<T>), dim3(nbg::fz_grid(size)), dim3(nbg::kFzThreads), 0, s, a,   \
                      sort ? 1 : 0)
   switch (dtype) {
