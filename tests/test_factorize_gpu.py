@@ -48,9 +48,10 @@ def test_random_labels(gpu_lib, dtype, sort):
     rng = np.random.default_rng(3)
     # 2k uniques: 2048 fit the LDS code table (one of them the integer -1,
     # kept outside the hash table), 2050 do not; 1M distinct values overflow
-    # every workgroup's LDS insert table
+    # every workgroup's LDS insert table and the small global table; 30000
+    # uniques fill the small table to 46%
     cases = ((4_000_000, 1000, False), (4_000_000, 1000, True), (1_000_003, 10**9, False), (5000, 3, False),
-             (300_000, 1024, False), (300_000, 1025, False))
+             (300_000, 1024, False), (300_000, 1025, False), (300_000, 15_000, False))
     for n, k, zipf in cases:
         if zipf:
             p = np.arange(1, k + 1, dtype=np.float64) ** -1.1
