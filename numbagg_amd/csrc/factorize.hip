@@ -150,28 +150,42 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < A.n ? lo + chunk : A.n;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kFzInsThreads) {
-    bool nan;
-    const unsigned long long k = fz_key(x[i], nan);
-    if (nan) continue;
-    if (k == kEmpty) {  // int64 -1: outside the table
-      atomicOr(&A.count[1], 1u);
-      atomicMin(A.minus_one_first, (unsigned long long)i);
-      continue;
+  // four values per thread in flight: their loads are issued before the
+  // first one is looked up
+  constexpr int U = 4;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += U * kFzInsThreads) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * kFzInsThreads;
+      v[u] = i < hi ? x[i] : T(0);
     }
-    unsigned int h = (unsigned int)mix64(k) & (kFzLdsSlots - 1);
-    bool placed = false;
-    for (int p = 0; p < kFzLdsProbe; ++p) {
-      unsigned long long cur = lk[h];
-      if (cur == kEmpty) cur = atomicCAS(&lk[h], kEmpty, k);
-      if (cur == kEmpty || cur == k) {
-        if ((unsigned long long)i < lf[h]) atomicMin(&lf[h], (unsigned long long)i);
-        placed = true;
-        break;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * kFzInsThreads;
+      if (i >= hi) continue;
+      bool nan;
+      const unsigned long long k = fz_key(v[u], nan);
+      if (nan) continue;
+      if (k == kEmpty) {  // int64 -1: outside the table
+        atomicOr(&A.count[1], 1u);
+        atomicMin(A.minus_one_first, (unsigned long long)i);
+        continue;
       }
-      h = (h + 1) & (kFzLdsSlots - 1);
+      unsigned int h = (unsigned int)mix64(k) & (kFzLdsSlots - 1);
+      bool placed = false;
+      for (int p = 0; p < kFzLdsProbe; ++p) {
+        unsigned long long cur = lk[h];
+        if (cur == kEmpty) cur = atomicCAS(&lk[h], kEmpty, k);
+        if (cur == kEmpty || cur == k) {
+          if ((unsigned long long)i < lf[h]) atomicMin(&lf[h], (unsigned long long)i);
+          placed = true;
+          break;
+        }
+        h = (h + 1) & (kFzLdsSlots - 1);
+      }
+      if (!placed) fz_table_insert<BIG>(A, t, k, (unsigned long long)i);
     }
-    if (!placed) fz_table_insert<BIG>(A, t, k, (unsigned long long)i);
   }
   __syncthreads();
   for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads)
@@ -219,26 +233,34 @@ __global__ void fz_pad_kernel(FzArgs A, int64_t k_total, int by_value) {
   }
 }
 
-// The whole bitonic network in one workgroup's LDS, for lists of at most
-// kFzLdsSort entries (the common case of few uniques): one launch instead of
-// log2(padded) (log2(padded) + 1) / 2.
+// Bitonic stages in LDS. Each workgroup holds one chunk of kFzLdsSort
+// entries (the whole list when it is shorter) and runs, for every stage size
+// k from k_first to k_last, the steps j = min(k / 2, chunk / 2) .. 1, whose
+// partners lie inside the chunk; the direction comes from the global index,
+// so the chunks together form the same network as the one-step global
+// kernel. (2, padded) on one workgroup sorts lists of up to kFzLdsSort
+// entries in one launch; longer lists run (2, kFzLdsSort) once and then, for
+// each k above it, global steps down to j = kFzLdsSort and one (k, k) launch
+// for the rest.
 constexpr int kFzLdsSort = 4096;
 constexpr int kFzSortThreads = 1024;
 
 __global__ __launch_bounds__(kFzSortThreads) void fz_lds_sort_kernel(unsigned long long* key, unsigned long long* val,
-                                                                      int padded) {
+                                                                      int64_t padded, int64_t k_first, int64_t k_last) {
   __shared__ unsigned long long sk[kFzLdsSort], sv[kFzLdsSort];
-  for (int i = threadIdx.x; i < padded; i += kFzSortThreads) {
-    sk[i] = key[i];
-    sv[i] = val[i];
+  const int len = padded < kFzLdsSort ? (int)padded : kFzLdsSort;
+  const int64_t base = (int64_t)blockIdx.x * len;
+  for (int i = threadIdx.x; i < len; i += kFzSortThreads) {
+    sk[i] = key[base + i];
+    sv[i] = val[base + i];
   }
   __syncthreads();
-  for (int k = 2; k <= padded; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < padded; i += kFzSortThreads) {
+  for (int64_t k = k_first; k <= k_last; k <<= 1)
+    for (int j = (int)((k >> 1) < (len >> 1) ? (k >> 1) : (len >> 1)); j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < len; i += kFzSortThreads) {
         const int p = i ^ j;
         if (p <= i) continue;
-        const bool up = (i & k) == 0;
+        const bool up = ((base + i) & k) == 0;
         const unsigned long long a = sk[i], b = sk[p];
         const bool greater = a > b || (a == b && sv[i] > sv[p]);
         if (greater == up) {
@@ -251,9 +273,9 @@ __global__ __launch_bounds__(kFzSortThreads) void fz_lds_sort_kernel(unsigned lo
       }
       __syncthreads();
     }
-  for (int i = threadIdx.x; i < padded; i += kFzSortThreads) {
-    key[i] = sk[i];
-    val[i] = sv[i];
+  for (int i = threadIdx.x; i < len; i += kFzSortThreads) {
+    key[base + i] = sk[i];
+    val[base + i] = sv[i];
   }
 }
 
@@ -533,14 +555,21 @@ int nbg_factorize_finish(int dtype, const void* values, int64_t n, int64_t num_u
   do {                                                                                                        \
     hipLaunchKernelGGL((nbg::fz_pad_kernel<T>), dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0, s, a,   \
                        num_uniques, sort ? 1 : 0);                                                            \
-    if (a.padded <= nbg::kFzLdsSort)                                                                          \
+    if (a.padded <= nbg::kFzLdsSort) {                                                                        \
       hipLaunchKernelGGL(nbg::fz_lds_sort_kernel, dim3(1), dim3(nbg::kFzSortThreads), 0, s, a.skey, a.sval,     \
-                         (int)a.padded);                                                                      \
-    else                                                                                                      \
-      for (int64_t k = 2; k <= a.padded; k <<= 1)                                                             \
-        for (int64_t j = k >> 1; j > 0; j >>= 1)                                                              \
+                         a.padded, (int64_t)2, a.padded);                                                     \
+    } else {                                                                                                  \
+      const unsigned chunks = (unsigned)(a.padded / nbg::kFzLdsSort);                                         \
+      hipLaunchKernelGGL(nbg::fz_lds_sort_kernel, dim3(chunks), dim3(nbg::kFzSortThreads), 0, s, a.skey,        \
+                         a.sval, a.padded, (int64_t)2, (int64_t)nbg::kFzLdsSort);                             \
+      for (int64_t k = 2 * (int64_t)nbg::kFzLdsSort; k <= a.padded; k <<= 1) {                                \
+        for (int64_t j = k >> 1; j >= nbg::kFzLdsSort; j >>= 1)                                               \
           hipLaunchKernelGGL(nbg::fz_bitonic_kernel, dim3(nbg::fz_grid(a.padded)), dim3(nbg::kFzThreads), 0,   \
                              s, a.skey, a.sval, a.padded, j, k);                                              \
+        hipLaunchKernelGGL(nbg::fz_lds_sort_kernel, dim3(chunks), dim3(nbg::kFzSortThreads), 0, s, a.skey,      \
+                           a.sval, a.padded, k, k);                                                           \
+      }                                                                                                       \
+    }                                                                                                         \
     hipLaunchKernelGGL((nbg::fz_rank_kernel<T>), dim3(nbg::fz_grid(num_uniques)), dim3(nbg::kFzThreads), 0, s,  \
                        (const T*)values, a, num_uniques, (T*)uniques, minus_one_rank);                        \
     if (num_uniques <= nbg::kFzLdsCodes)                                                                      \
