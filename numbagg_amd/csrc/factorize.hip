@@ -141,12 +141,14 @@ constexpr int kFzLdsProbe = 16;
 template <typename T, bool BIG>
 __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __restrict__ x, FzArgs A, int64_t chunk) {
   __shared__ unsigned long long lk[kFzLdsSlots], lf[kFzLdsSlots];
+  __shared__ unsigned int lused, over;
   if (BIG && !A.count[3]) return;
   const FzTable t = BIG ? A.big : A.small;
   for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads) {
     lk[j] = kEmpty;
     lf[j] = kEmpty;
   }
+  if (threadIdx.x == 0) lused = 0u;
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < A.n ? lo + chunk : A.n;
@@ -154,6 +156,14 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
   // first one is looked up
   constexpr int U = 4;
   for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += U * kFzInsThreads) {
+    if (!BIG) {
+      // Once the small table has overflowed its contents are dropped: stop.
+      // One lane per wave reads the flag (lane 0 has the wave's lowest
+      // position, so it is active whenever any lane is).
+      unsigned int f = 0u;
+      if (__lane_id() == 0) f = __atomic_load_n(&A.count[3], __ATOMIC_RELAXED);
+      if (__shfl(f, 0)) break;
+    }
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -174,9 +184,17 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
       }
       unsigned int h = (unsigned int)mix64(k) & (kFzLdsSlots - 1);
       bool placed = false;
-      for (int p = 0; p < kFzLdsProbe; ++p) {
+      // a table three-quarters full (many distinct keys in this chunk) only
+      // looks its keys up near their home slot and claims no more slots
+      const bool claim = __atomic_load_n(&lused, __ATOMIC_RELAXED) < (unsigned int)(kFzLdsSlots * 3 / 4);
+      const int lim = claim ? kFzLdsProbe : 4;
+      for (int p = 0; p < lim; ++p) {
         unsigned long long cur = lk[h];
-        if (cur == kEmpty) cur = atomicCAS(&lk[h], kEmpty, k);
+        if (cur == kEmpty) {
+          if (!claim) break;
+          cur = atomicCAS(&lk[h], kEmpty, k);
+          if (cur == kEmpty) atomicAdd(&lused, 1u);
+        }
         if (cur == kEmpty || cur == k) {
           if ((unsigned long long)i < lf[h]) atomicMin(&lf[h], (unsigned long long)i);
           placed = true;
@@ -187,23 +205,36 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
       if (!placed) fz_table_insert<BIG>(A, t, k, (unsigned long long)i);
     }
   }
+  if (threadIdx.x == 0) over = BIG ? 0u : __atomic_load_n(&A.count[3], __ATOMIC_RELAXED);
   __syncthreads();
+  if (over) return;
   for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads)
     if (lk[j] != kEmpty) fz_table_insert<BIG>(A, t, lk[j], lf[j]);
 }
 
 // Occupied slots append (sort key, slot) to the list (positions from an
-// atomic counter; the bitonic sort fixes the order).
+// atomic counter, one atomic per wave; the bitonic sort fixes the order).
 template <typename T>
 __global__ __launch_bounds__(kFzThreads) void fz_compact_kernel(FzArgs A, int by_value) {
   const FzTable t = fz_table(A);
   const int64_t size = (int64_t)t.mask + 1;
-  for (int64_t s = (int64_t)blockIdx.x * kFzThreads + threadIdx.x; s < size; s += (int64_t)gridDim.x * kFzThreads) {
-    const unsigned long long k = t.keys[s];
-    if (k == kEmpty) continue;
-    const unsigned int p = atomicAdd(&A.count[0], 1u);
-    A.skey[p] = by_value ? fz_order<T>(k) : t.first[s];
-    A.sval[p] = (unsigned long long)s;
+  const int lane = __lane_id();
+  // the workgroup steps together, so every wave has all its lanes here
+  for (int64_t s0 = (int64_t)blockIdx.x * kFzThreads; s0 < size; s0 += (int64_t)gridDim.x * kFzThreads) {
+    const int64_t s = s0 + threadIdx.x;
+    const unsigned long long k = s < size ? t.keys[s] : kEmpty;
+    const bool occ = k != kEmpty;
+    const unsigned long long m = __ballot(occ);
+    if (m == 0ull) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0u;
+    if (lane == leader) base = atomicAdd(&A.count[0], (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    if (occ) {
+      const unsigned int p = base + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+      A.skey[p] = by_value ? fz_order<T>(k) : t.first[s];
+      A.sval[p] = (unsigned long long)s;
+    }
   }
 }
 
