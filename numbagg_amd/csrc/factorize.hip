@@ -98,9 +98,13 @@ __device__ __forceinline__ FzTable fz_table(const FzArgs& A) { return A.count[3]
 
 // Key k (not the int64 -1) seen at position i: into table t (BIG: the 2n-slot
 // table; else the small one, which flags its overflow and gives up).
+// `over` (small table only): the workgroup's LDS copy of the overflow flag,
+// set here when this call finds the overflow, and read at every probe so
+// that a workgroup stops probing a full table as soon as one of its threads
+// has seen it overflow.
 template <bool BIG>
 __device__ __forceinline__ void fz_table_insert(const FzArgs& A, const FzTable& t, unsigned long long k,
-                                                unsigned long long i) {
+                                                unsigned long long i, unsigned int* over) {
   // The big table holds at most half its slots, so a free slot is found. A
   // slot changes only once (empty -> key) and a first position only
   // decreases, so plain reads that see an older value are safe: the common
@@ -108,15 +112,21 @@ __device__ __forceinline__ void fz_table_insert(const FzArgs& A, const FzTable& 
   // no atomic, which keeps labels with many repeats free of contention.
   unsigned long long s = mix64(k) & t.mask;
   for (int p = 0;; ++p) {
-    if (!BIG && p >= kFzSmallProbe) {
-      atomicOr(&A.count[3], 1u);
-      return;
+    if (!BIG) {
+      if (__atomic_load_n(over, __ATOMIC_RELAXED)) return;
+      if (p >= kFzSmallProbe) {
+        atomicOr(&A.count[3], 1u);
+        __atomic_store_n(over, 1u, __ATOMIC_RELAXED);
+        return;
+      }
     }
     unsigned long long cur = t.keys[s];
     if (cur == kEmpty) {
       cur = atomicCAS(&t.keys[s], kEmpty, k);
-      if (!BIG && cur == kEmpty && atomicAdd(&A.count[2], 1u) + 1 > (unsigned int)((t.mask + 1) / 2))
+      if (!BIG && cur == kEmpty && atomicAdd(&A.count[2], 1u) + 1 > (unsigned int)((t.mask + 1) / 2)) {
         atomicOr(&A.count[3], 1u);
+        __atomic_store_n(over, 1u, __ATOMIC_RELAXED);
+      }
     }
     if (cur == kEmpty || cur == k) {
       if (i < t.first[s]) atomicMin(&t.first[s], i);
@@ -148,7 +158,10 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
     lk[j] = kEmpty;
     lf[j] = kEmpty;
   }
-  if (threadIdx.x == 0) lused = 0u;
+  if (threadIdx.x == 0) {
+    lused = 0u;
+    over = 0u;
+  }
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < A.n ? lo + chunk : A.n;
@@ -156,14 +169,8 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
   // first one is looked up
   constexpr int U = 4;
   for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += U * kFzInsThreads) {
-    if (!BIG) {
-      // Once the small table has overflowed its contents are dropped: stop.
-      // One lane per wave reads the flag (lane 0 has the wave's lowest
-      // position, so it is active whenever any lane is).
-      unsigned int f = 0u;
-      if (__lane_id() == 0) f = __atomic_load_n(&A.count[3], __ATOMIC_RELAXED);
-      if (__shfl(f, 0)) break;
-    }
+    // once the small table has overflowed its contents are dropped: stop
+    if (!BIG && __atomic_load_n(&over, __ATOMIC_RELAXED)) break;
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -202,39 +209,65 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
         }
         h = (h + 1) & (kFzLdsSlots - 1);
       }
-      if (!placed) fz_table_insert<BIG>(A, t, k, (unsigned long long)i);
+      if (!placed) fz_table_insert<BIG>(A, t, k, (unsigned long long)i, &over);
     }
   }
-  if (threadIdx.x == 0) over = BIG ? 0u : __atomic_load_n(&A.count[3], __ATOMIC_RELAXED);
+  __syncthreads();
+  if (threadIdx.x == 0 && !BIG && __atomic_load_n(&A.count[3], __ATOMIC_RELAXED)) over = 1u;
   __syncthreads();
   if (over) return;
   for (int j = threadIdx.x; j < kFzLdsSlots; j += kFzInsThreads)
-    if (lk[j] != kEmpty) fz_table_insert<BIG>(A, t, lk[j], lf[j]);
+    if (lk[j] != kEmpty) fz_table_insert<BIG>(A, t, lk[j], lf[j], &over);
 }
 
-// Occupied slots append (sort key, slot) to the list (positions from an
-// atomic counter, one atomic per wave; the bitonic sort fixes the order).
+// Occupied slots append (sort key, slot) to the list; the bitonic sort fixes
+// the order. Each workgroup takes a contiguous range of slots, counts its
+// occupied ones, claims their list positions with one atomic, then writes
+// them in slot order (one atomic per workgroup: the same-address atomics of
+// per-wave claims serialise in L2, 1.5 ms for 4M uniques).
 template <typename T>
 __global__ __launch_bounds__(kFzThreads) void fz_compact_kernel(FzArgs A, int by_value) {
+  __shared__ unsigned int wcount[kFzThreads / 64], red[kFzThreads / 64];
+  __shared__ unsigned int base;
   const FzTable t = fz_table(A);
   const int64_t size = (int64_t)t.mask + 1;
-  const int lane = __lane_id();
+  int64_t per = (size + gridDim.x - 1) / gridDim.x;
+  per = (per + kFzThreads - 1) / kFzThreads * kFzThreads;
+  const int64_t r0 = (int64_t)blockIdx.x * per;
+  const int64_t r1 = r0 + per < size ? r0 + per : size;
+  const int lane = __lane_id(), wave = threadIdx.x / 64;
+  unsigned int c = 0u;
+  for (int64_t s = r0 + threadIdx.x; s < r1; s += kFzThreads) c += t.keys[s] != kEmpty ? 1u : 0u;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+  if (lane == 0) red[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int total = 0u;
+    for (int w = 0; w < kFzThreads / 64; ++w) total += red[w];
+    base = total ? atomicAdd(&A.count[0], total) : 0u;
+  }
+  __syncthreads();
+  unsigned int off = base;
   // the workgroup steps together, so every wave has all its lanes here
-  for (int64_t s0 = (int64_t)blockIdx.x * kFzThreads; s0 < size; s0 += (int64_t)gridDim.x * kFzThreads) {
+  for (int64_t s0 = r0; s0 < r1; s0 += kFzThreads) {
     const int64_t s = s0 + threadIdx.x;
-    const unsigned long long k = s < size ? t.keys[s] : kEmpty;
+    const unsigned long long k = s < r1 ? t.keys[s] : kEmpty;
     const bool occ = k != kEmpty;
     const unsigned long long m = __ballot(occ);
-    if (m == 0ull) continue;
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned int base = 0u;
-    if (lane == leader) base = atomicAdd(&A.count[0], (unsigned int)__popcll(m));
-    base = __shfl(base, leader);
+    if (lane == 0) wcount[wave] = (unsigned int)__popcll(m);
+    __syncthreads();
+    unsigned int before = 0u, all = 0u;
+    for (int w = 0; w < kFzThreads / 64; ++w) {
+      before += w < wave ? wcount[w] : 0u;
+      all += wcount[w];
+    }
     if (occ) {
-      const unsigned int p = base + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+      const unsigned int p = off + before + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
       A.skey[p] = by_value ? fz_order<T>(k) : t.first[s];
       A.sval[p] = (unsigned long long)s;
     }
+    off += all;
+    __syncthreads();  // wcount is rewritten by the next step
   }
 }
 
