@@ -171,6 +171,10 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
   for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += U * kFzInsThreads) {
     // once the small table has overflowed its contents are dropped: stop
     if (!BIG && __atomic_load_n(&over, __ATOMIC_RELAXED)) break;
+    // a table three-quarters full (many distinct keys in this chunk) only
+    // looks keys up near their home slot and claims no more slots
+    const bool claim = __atomic_load_n(&lused, __ATOMIC_RELAXED) < (unsigned int)(kFzLdsSlots * 3 / 4);
+    const int lim = claim ? kFzLdsProbe : 4;
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -191,10 +195,6 @@ __global__ __launch_bounds__(kFzInsThreads) void fz_insert_kernel(const T* __res
       }
       unsigned int h = (unsigned int)mix64(k) & (kFzLdsSlots - 1);
       bool placed = false;
-      // a table three-quarters full (many distinct keys in this chunk) only
-      // looks its keys up near their home slot and claims no more slots
-      const bool claim = __atomic_load_n(&lused, __ATOMIC_RELAXED) < (unsigned int)(kFzLdsSlots * 3 / 4);
-      const int lim = claim ? kFzLdsProbe : 4;
       for (int p = 0; p < lim; ++p) {
         unsigned long long cur = lk[h];
         if (cur == kEmpty) {
