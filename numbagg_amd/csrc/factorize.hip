@@ -319,8 +319,11 @@ __global__ __launch_bounds__(kFzSortThreads) void fz_lds_sort_kernel(unsigned lo
     sv[i] = val[base + i];
   }
   __syncthreads();
-  for (int64_t k = k_first; k <= k_last; k <<= 1)
-    for (int j = (int)((k >> 1) < (len >> 1) ? (k >> 1) : (len >> 1)); j > 0; j >>= 1) {
+  constexpr int M = kFzLdsSort / kFzSortThreads;  // entries per thread
+  for (int64_t k = k_first; k <= k_last; k <<= 1) {
+    const int jtop = (int)((k >> 1) < (len >> 1) ? (k >> 1) : (len >> 1));
+    // strides of a wave or more: through LDS, a barrier per step
+    for (int j = jtop; j >= 64; j >>= 1) {
       for (int i = threadIdx.x; i < len; i += kFzSortThreads) {
         const int p = i ^ j;
         if (p <= i) continue;
@@ -337,6 +340,49 @@ __global__ __launch_bounds__(kFzSortThreads) void fz_lds_sort_kernel(unsigned lo
       }
       __syncthreads();
     }
+    // strides below 64: partners in the same wave, exchanged by lane shuffles
+    // (entry e sits in lane e mod 64), no barriers
+    const int jr = jtop < 32 ? jtop : 32;
+    if (jr >= 1) {
+      unsigned long long rk[M], rv[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int e = threadIdx.x + m * kFzSortThreads;
+        if (e < len) {
+          rk[m] = sk[e];
+          rv[m] = sv[e];
+        }
+      }
+      for (int j = jr; j > 0; j >>= 1) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int e = threadIdx.x + m * kFzSortThreads;
+          if (e < len) {  // len is a power of two: the partner e ^ j is active too
+            const unsigned long long pk = __shfl_xor(rk[m], j), pv = __shfl_xor(rv[m], j);
+            const bool lower = (e & j) == 0;
+            const bool up = ((base + e) & k) == 0;
+            // (a, va): the pair's lower entry, (b, vb): its upper entry
+            const unsigned long long a = lower ? rk[m] : pk, b = lower ? pk : rk[m];
+            const unsigned long long va = lower ? rv[m] : pv, vb = lower ? pv : rv[m];
+            const bool greater = a > b || (a == b && va > vb);
+            if (greater == up) {
+              rk[m] = pk;
+              rv[m] = pv;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int e = threadIdx.x + m * kFzSortThreads;
+        if (e < len) {
+          sk[e] = rk[m];
+          sv[e] = rv[m];
+        }
+      }
+      __syncthreads();
+    }
+  }
   for (int i = threadIdx.x; i < len; i += kFzSortThreads) {
     key[base + i] = sk[i];
     val[base + i] = sv[i];
